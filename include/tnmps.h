@@ -1,0 +1,203 @@
+/*
+ * tnmps.h -- C ABI of the MI355X Schmidt-form (Gamma-Lambda) MPS update path.
+ *
+ * Drop-in boundary for tensor-net's `MPS<Q, C64>` operator API
+ * (tensor-net/lib/mps.rs).  Every entry point below names the reference
+ * method it replaces.  All state (Gamma tensors, Schmidt values, bond
+ * dimensions) lives in device memory; calls are enqueued on the handle's HIP
+ * stream and block only where they return a value to the host.
+ *
+ * Conventions carried over from the reference:
+ *  - complex numbers are interleaved {re, im} doubles (num_complex::Complex64
+ *    is #[repr(C)] {re, im});
+ *  - operators are row-major; for two-qubit operators qubit k is the most
+ *    significant bit of the 4x4 index (gate.rs:750-761);
+ *  - a site index out of range is a silent no-op for unitaries (mps.rs:1301,
+ *    1335) and TN_NONE ("None") for queries (mps.rs:1072, 1142, 1252, 1483);
+ *  - out-of-range sites are dropped from measurement layers, the rest are
+ *    sorted by site and deduplicated keeping the first (mps.rs:2212-2219);
+ *  - one caller per handle at a time (the reference's &mut self); distinct
+ *    handles are independent.
+ */
+#ifndef TNMPS_H
+#define TNMPS_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tn_mps tn_mps;
+
+/* Status codes.  1..5 mirror MPSError (mps.rs:126-154). */
+typedef enum {
+    TN_OK = 0,
+    TN_EMPTY_SYSTEM = 1,                 /* MPSError::EmptySystem */
+    TN_INVALID_QUANTUM_NUMBER = 2,       /* MPSError::InvalidQuantumNumber */
+    TN_UNPHYSICAL_INDEX = 3,             /* MPSError::UnphysicalIndex */
+    TN_STATE_INCOMPATIBLE_SHAPE = 4,     /* MPSError::StateIncompatibleShape */
+    TN_OPERATOR_INCOMPATIBLE_SHAPE = 5,  /* MPSError::OperatorIncompatibleShape */
+    TN_NONE = 6,          /* the reference returns None (index out of range) */
+    TN_ZERO_PROB = 7,     /* a post-selected outcome had probability 0: the
+                             state holds NaN/inf exactly as the reference's
+                             does (mps.rs:1505-1507); reported, not prevented */
+    TN_SAMPLE_RANGE = 8,  /* r >= total probability: the reference panics on
+                             unwrap (mps.rs:1444); the state is projected on
+                             the last outcome with non-zero probability */
+    TN_CAPACITY = 9,      /* a Cutoff rank exceeded the handle's chi_cap */
+    TN_DEVICE = 10,       /* HIP runtime error */
+    TN_INVALID_ARG = 11,  /* null pointer / bad argument at the C boundary */
+    TN_NO_CONVERGENCE = 12 /* SVD hit its sweep limit (result still written) */
+} tn_status;
+
+/* BondDim (mps.rs:157-166).  kind 0 = None (Cutoff at f64::EPSILON),
+ * 1 = Const(chi), 2 = Cutoff(eps). */
+typedef struct {
+    int kind;
+    size_t chi;
+    double eps;
+} tn_trunc;
+
+enum { TN_TRUNC_NONE = 0, TN_TRUNC_CONST = 1, TN_TRUNC_CUTOFF = 2 };
+
+/* Human-readable name of a status code (static string). */
+const char* tn_status_str(tn_status s);
+
+/* Text of the last HIP error seen by the calling thread (static storage). */
+const char* tn_last_error(void);
+
+/* Number of visible HIP devices (0 when none); does not create a context. */
+int tn_device_count(void);
+
+/* --------------------------------------------------------------- lifecycle */
+
+/* MPS::new_qubits (mps.rs:2033-2039 -> MPS::new, mps.rs:370-391): n qubits in
+ * |0...0>.  chi_cap bounds the device allocation per bond; 0 means
+ * trunc.chi for Const, else 256.  stream is a hipStream_t or NULL (the
+ * library creates one).  n == 0 -> TN_EMPTY_SYSTEM. */
+tn_status tn_mps_new_qubits(size_t n, tn_trunc trunc, size_t chi_cap,
+                            void* stream, tn_mps** out);
+/* Drop (frees device memory). */
+void tn_mps_free(tn_mps* h);
+/* MPS::clone (#[derive(Clone)], mps.rs:337): deep copy on the same stream. */
+tn_status tn_mps_clone(const tn_mps* h, tn_mps** out);
+size_t tn_mps_n(const tn_mps* h);
+/* get_trunc / set_trunc (mps.rs:1216-1221). */
+tn_status tn_get_trunc(const tn_mps* h, tn_trunc* out);
+tn_status tn_set_trunc(tn_mps* h, tn_trunc trunc);
+/* Wait for all work enqueued on the handle's stream. */
+tn_status tn_sync(tn_mps* h);
+
+/* ---------------------------------------------------------------- unitaries */
+
+/* MPS::apply_unitary1 (mps.rs:1298-1314).  op: 2x2 row-major, 8 doubles. */
+tn_status tn_apply_u1(tn_mps* h, size_t k, const double* op);
+/* MPS::apply_unitary2 (mps.rs:1332-1387): contract -> gate -> SVD ->
+ * truncate -> split -> renormalise.  op: 4x4 row-major, 32 doubles. */
+tn_status tn_apply_u2(tn_mps* h, size_t k, const double* op);
+/* A sequence of apply_uni calls (mps.rs:2093-2108) as in the unitary loop of
+ * circuit::apply_bilayer (circuit.rs:637).  dims[i] is 2 or 4 (operator
+ * side); ops holds the operators back to back (8 or 32 doubles each).
+ * Consecutive two-qubit gates on disjoint site pairs run in one batched
+ * launch; the result equals applying them one by one in the given order. */
+tn_status tn_apply_unis(tn_mps* h, size_t m, const size_t* ks,
+                        const int* dims, const double* ops);
+
+/* ------------------------------------------------------------- measurement */
+
+/* MPS::prob (mps.rs:1071-1074). */
+tn_status tn_prob(tn_mps* h, size_t k, size_t p, double* out);
+/* MPS::probs (mps.rs:1140-1143): out[2]. */
+tn_status tn_probs(tn_mps* h, size_t k, double* out);
+/* MPS::measure_prob (mps.rs:1479-1489) with the caller's uniform draw r in
+ * [0,1) standing in for rng.gen::<f64>(); projection + full refactor sweep. */
+tn_status tn_measure_prob(tn_mps* h, size_t k, double r, int* outcome,
+                          double* prob);
+/* MPS::measure_postsel_prob (mps.rs:1501-1510). */
+tn_status tn_measure_postsel_prob(tn_mps* h, size_t k, size_t p, double* prob);
+/* MPS::apply_meas_prob_layer (mps.rs:2206-2252).  Entry i measures site
+ * ks[i]: forced[i] = 0/1 is Meas::Proj, -1 is Meas::Rand using draw r[i]
+ * (r may be NULL if no entry is Rand).  Results for the processed entries
+ * (sorted by site, deduplicated, out-of-range dropped) go to out_ks,
+ * out_outcomes, out_probs (capacity m each); *out_count is their number. */
+tn_status tn_meas_layer(tn_mps* h, size_t m, const size_t* ks,
+                        const int* forced, const double* r, size_t* out_count,
+                        size_t* out_ks, int* out_outcomes, double* out_probs);
+/* circuit::apply_bilayer (circuit.rs:624-659): tn_apply_unis then
+ * tn_meas_layer. */
+tn_status tn_apply_bilayer(tn_mps* h, size_t nu, const size_t* uks,
+                           const int* udims, const double* ops, size_t nm,
+                           const size_t* mks, const int* forced,
+                           const double* r, size_t* out_count, size_t* out_ks,
+                           int* out_outcomes, double* out_probs);
+
+/* ------------------------------------------------------------- observables */
+
+/* MPS::entropy_vn (mps.rs:1251-1261). */
+tn_status tn_entropy_vn(tn_mps* h, size_t b, double* out);
+/* MPS::entropy_ry_schmidt (mps.rs:1268-1282). */
+tn_status tn_entropy_ry_schmidt(tn_mps* h, double a, size_t b, double* out);
+/* Schmidt values of bond b (svals[b], mps.rs:352-353): up to cap values. */
+tn_status tn_schmidt(tn_mps* h, size_t b, double* buf, size_t cap,
+                     size_t* len);
+/* bond_dim for every bond (mps.rs:1243-1245): out[n-1]. */
+tn_status tn_bond_dims(tn_mps* h, size_t* out);
+/* max_bond_dim (mps.rs:1227-1229); TN_NONE when n == 1. */
+tn_status tn_max_bond_dim(tn_mps* h, size_t* out);
+/* Copy Gamma[k] (data[k], axes [u_{k-1}, s_k, u_k], row-major, interleaved
+ * complex) into buf (cap doubles); shape[3] receives its dimensions. */
+tn_status tn_get_gamma(tn_mps* h, size_t k, double* buf, size_t cap,
+                       size_t* shape);
+
+/* ------------------------------------------------------ trajectory driver */
+
+/* Run `depth` bilayers (circuit.rs:624-659) on one handle with no host
+ * round trip between layers.  Layer t has nu[t] two-qubit gates at sites
+ * uks[..] with 4x4 operators ops[..] (32 doubles each) and nm[t]
+ * measurements (mks, forced, r as in tn_meas_layer), all packed layer after
+ * layer.  ent_bond: the bond whose entropy_vn is recorded before layer 1 and
+ * after every layer into ent_out[depth+1] (may be NULL).  Outcomes and
+ * probabilities of processed measurements are written to out_outcomes /
+ * out_probs (capacity sum(nm)) in processing order; out_counts[t] receives
+ * the number processed in layer t. */
+tn_status tn_run_circuit(tn_mps* h, size_t depth, const size_t* nu,
+                         const size_t* uks, const double* ops,
+                         const size_t* nm, const size_t* mks,
+                         const int* forced, const double* r, size_t ent_bond,
+                         double* ent_out, size_t* out_counts,
+                         int* out_outcomes, double* out_probs);
+
+/* Counters of the device work done by this handle (for throughput
+ * accounting): number of two-site updates (contract -> SVD -> split) and
+ * the sum over them of Jacobi sweeps. */
+tn_status tn_stats(tn_mps* h, size_t* updates, size_t* sweeps);
+
+/* Device-time accounting of one handle, enabled by tn_set_timing.  Times
+ * are HIP-event spans on the handle's stream: ms_jacobi sums, per Jacobi
+ * sweep, the span from the first to the last step kernel of that sweep.
+ * alg_flops counts, per two-site update with theta of m x n (M = max, N =
+ * min) and contracted bond K: 8mnK (contraction) + 32mn (gate, if any) +
+ * svd part 4(4M^2N + 8MN^2 + 9N^3) (complex R-SVD with U and V). */
+typedef struct {
+    double ms_theta;        /* contraction (+ gate) and W initialisation */
+    double ms_jacobi;       /* Jacobi SVD sweeps */
+    double ms_post;         /* singular values, rank, split, renormalise */
+    double ms_measure;      /* Born probabilities and projection */
+    double alg_flops;       /* algorithmic flops of all updates */
+    double svd_flops;       /* the SVD part of alg_flops */
+    size_t updates;         /* two-site updates (gates and refactors) */
+    size_t sweeps;          /* Jacobi sweeps summed over update batches */
+    size_t jacobi_launches; /* Jacobi step kernels launched */
+} tn_perf;
+
+/* Turn device-time accounting on (1) or off (0).  On adds one stream
+ * synchronisation per update batch. */
+tn_status tn_set_timing(tn_mps* h, int on);
+tn_status tn_get_perf(tn_mps* h, tn_perf* out);
+tn_status tn_reset_perf(tn_mps* h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TNMPS_H */

@@ -1,0 +1,81 @@
+// Device-side data structures shared by the kernels and the host driver of
+// the Schmidt-form (Gamma-Lambda) MPS update path.
+//
+// Layout in HBM (one MPS, "chain"):
+//   gamma : n site slots.  Site k holds Gamma[k] with axes [u_{k-1}, s_k, u_k]
+//           (tensor-net/lib/mps.rs:345-350), row-major and packed by its
+//           current dimensions (dims[k], 2, dims[k+1]); the slot is sized for
+//           the largest dimensions the bond can reach.
+//   lam   : n-1 bond slots of doubles; bond k holds lambda[k] (mps.rs:352).
+//   dims  : n+1 ints; dims[0] = dims[n] = 1, dims[k+1] = dimension of bond k.
+//
+// One two-site update (contract -> SVD -> truncate -> split -> renormalise)
+// runs as a "task".  Its SVD works on X, the contracted theta (m x n, with
+// m = 2 chiL, n = 2 chiR) or its conjugate transpose, whichever has at least
+// as many rows as columns, stored column-major with R = max(m, n) rows and
+// C = min(m, n) columns.  W (C x C, column-major) accumulates the one-sided
+// Jacobi rotations: on exit X W has orthogonal columns.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace tnmps {
+
+constexpr int kBW = 8;          // Jacobi block width (columns)
+constexpr int kPW = 2 * kBW;    // panel width (two blocks)
+constexpr int kMaxSweeps = 40;  // outer Jacobi sweep limit
+constexpr int kNPart = 64;      // blocks of the split/renormalise kernels
+constexpr int kMaxCols = 1024;  // largest C (= min(m, n)) the SVD accepts
+
+// info[] slots of a task (device ints)
+enum {
+  kInfoM = 0, kInfoN, kInfoR, kInfoC, kInfoOrient, kInfoRank, kInfoFlags,
+  kInfoSweeps, kInfoChiL, kInfoChiM, kInfoChiR, kInfoLen = 16
+};
+
+// flag bits in info[kInfoFlags]
+constexpr int kFlagNoConv = 1;     // Jacobi hit kMaxSweeps
+constexpr int kFlagCapacity = 2;   // rank clamped to the bond's allocation
+
+// One two-site update (mps.rs:846-880 local_refactor, or mps.rs:1332-1387
+// apply_unitary2 when has_gate is set).  All pointers are device addresses.
+struct UpdDesc {
+  double2* gk;           // Gamma[k]
+  double2* gk1;          // Gamma[k+1]
+  const double* lamL;    // lambda[k-1]; nullptr when k == 0
+  double* lamM;          // lambda[k], overwritten by the new Schmidt values
+  const double* lamR;    // lambda[k+1]; nullptr when k == n-2
+  int* dims;             // &dims[k]: chiL = dims[0], chiM = dims[1], chiR = dims[2]
+  double2* X;            // SVD working matrix, ldx rows
+  double2* W;            // accumulated right rotation, ldw rows
+  double* sraw;          // column norms of X after convergence (C)
+  double* sval;          // new lambda values, sorted (C)
+  double* fac;           // scale of the right factor rows (C)
+  int* perm;             // sorted column order (C)
+  double* part;          // per-block partial norms (2 * kNPart)
+  double* offs;          // per-sweep max off-diagonal ratio, stride nt
+  int* info;             // kInfoLen ints
+  int ldx, ldw;
+  int has_gate;
+  int tkind;             // 0 None, 1 Const, 2 Cutoff (BondDim, mps.rs:157-166)
+  int tchi;
+  int cap_rank;          // allocation bound on the new bond dimension
+  double teps;
+  double2 gate[16];      // 4x4 row-major, qubit k the most significant bit
+};
+
+// One site measurement or probability query (mps.rs:1027-1134, 1427-1465).
+struct MeasDesc {
+  double2* g;            // Gamma[k]
+  const double* lamL;    // lambda[k-1] or nullptr
+  const double* lamR;    // lambda[k] or nullptr
+  const int* dims;       // &dims[k]: chiL = dims[0], chiR = dims[1]
+  int forced;            // 0/1 forced outcome, -1 sample with r, -2 query only
+  int slot;              // index into the result arrays
+  double r;              // uniform draw for sampling
+  int* out_outcome;
+  double* out_prob;      // outcome probability
+  double* out_probs;     // both probabilities, 2 per slot
+  int* out_flag;         // 1 zero probability, 2 draw out of range
+};
+
+}  // namespace tnmps

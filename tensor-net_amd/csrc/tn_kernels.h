@@ -1,0 +1,20 @@
+// Host-side launchers for the kernels in tn_kernels.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "tn_device.h"
+
+namespace tnmps {
+
+void launch_theta(const UpdDesc* descs, int nt, int gx, hipStream_t s);
+void launch_winit(const UpdDesc* descs, int nt, int gx, hipStream_t s);
+void launch_jacobi_step(const UpdDesc* descs, int nt, int npairs, int sweep, int step,
+                        double tol_rot, double conv_tol, hipStream_t s);
+void launch_finish(const UpdDesc* descs, int nt, int sweeps_run, hipStream_t s);
+void launch_split(const UpdDesc* descs, int nt, hipStream_t s);
+void launch_renorm(const UpdDesc* descs, int nt, hipStream_t s);
+void launch_measure(const MeasDesc* descs, int nm, hipStream_t s);
+void launch_entropy(const double* lam, const int* dimp, double a, double* out, hipStream_t s);
+void launch_u1(double2* g, const int* dims, const double* op8, int gx, hipStream_t s);
+
+}  // namespace tnmps

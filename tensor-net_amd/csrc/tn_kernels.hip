@@ -1,0 +1,645 @@
+// HIP kernels of the Schmidt-form MPS update path, written for CDNA4 (gfx950).
+//
+// Every kernel takes an array of task descriptors and uses blockIdx.y as the
+// task index, so one launch serves one update, the independent gates of a
+// brickwork layer, or the lock-stepped updates of many trajectories.  Bond
+// dimensions are read from device memory: the host never waits for a rank.
+//
+// Kernels (reference lines each one replaces):
+//   k_theta        theta = lambda Gamma lambda Gamma lambda, gate applied
+//                  (mps.rs:628-651, 732-754, 1343-1352)
+//   k_winit        W = I for the Jacobi SVD
+//   k_jacobi_step  one round-robin step of a block one-sided Jacobi SVD
+//                  (stands in for zgesvd, mps.rs:299)
+//   k_finish       singular values, sort, rank rule, renormalisation
+//                  (process_svd, mps.rs:236-293)
+//   k_split        new Gamma[k], Gamma[k+1] with lambda divisions and partial
+//                  norms (mps.rs:852-876, 1357-1380, local_norm 1147-1202)
+//   k_renorm       local_renormalize of both sites (mps.rs:1206-1209)
+//   k_measure      Born probabilities, sampling or forced outcome, projection
+//                  (mps.rs:1027-1134, 1427-1465, 2229-2242)
+//   k_entropy      von Neumann / Renyi entropy of one bond (mps.rs:1251-1282)
+//   k_u1           single-site unitary (mps.rs:1298-1314)
+#include "tn_device.h"
+#include "tn_kernels.h"
+
+#include <cfloat>
+
+namespace tnmps {
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// conj(a) * b
+__device__ __forceinline__ double2 cmulc(double2 a, double2 b) {
+  return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) {
+  return make_double2(a.x + b.x, a.y + b.y);
+}
+__device__ __forceinline__ double2 cscale(double2 a, double s) {
+  return make_double2(a.x * s, a.y * s);
+}
+__device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+__device__ __forceinline__ double cabs2(double2 a) { return a.x * a.x + a.y * a.y; }
+// complex / real as num_complex divides by Complex{re: d, im: 0}
+__device__ __forceinline__ double2 cdivr(double2 a, double d) {
+  const double den = d * d;
+  return make_double2((a.x * d) / den, (a.y * d) / den);
+}
+
+// Block-wide sum of one double; deterministic (fixed tree).  `buf` holds at
+// least blockDim.x / 64 doubles.  Every thread gets the result.
+__device__ double block_sum(double v, double* buf) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) buf[wid] = v;
+  __syncthreads();
+  double s = 0.0;
+  const int nw = (blockDim.x + 63) >> 6;
+  for (int i = 0; i < nw; ++i) s += buf[i];
+  __syncthreads();
+  return s;
+}
+
+__device__ double block_max(double v, double* buf) {
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) buf[wid] = v;
+  __syncthreads();
+  double s = 0.0;
+  const int nw = (blockDim.x + 63) >> 6;
+  for (int i = 0; i < nw; ++i) s = fmax(s, buf[i]);
+  __syncthreads();
+  return s;
+}
+
+// ---------------------------------------------------------------- theta
+// theta'[v,(s,t),w] = sum_{s't'} U[(s,t),(s',t')] lamL_v lamR_w
+//                     sum_u Gk[v,s',u] lamM_u Gk1[u,t',w]
+// written into X as theta (orient 0, m >= n) or theta^H (orient 1).
+// Tile: 4 values of v x 64 values of w per 256-thread block.
+__global__ __launch_bounds__(256) void k_theta(const UpdDesc* __restrict__ descs, int nt) {
+  const UpdDesc& d = descs[blockIdx.y];
+  const int chiL = d.dims[0], chiM = d.dims[1], chiR = d.dims[2];
+  const int m = 2 * chiL, n = 2 * chiR;
+  const int orient = (m >= n) ? 0 : 1;
+  if (blockIdx.x == 0) {
+    if (threadIdx.x == 0) {
+      d.info[kInfoM] = m;
+      d.info[kInfoN] = n;
+      d.info[kInfoR] = orient ? n : m;
+      d.info[kInfoC] = orient ? m : n;
+      d.info[kInfoOrient] = orient;
+      d.info[kInfoFlags] = 0;
+      d.info[kInfoSweeps] = 0;
+      d.info[kInfoChiL] = chiL;
+      d.info[kInfoChiM] = chiM;
+      d.info[kInfoChiR] = chiR;
+    }
+    for (int s = threadIdx.x; s < kMaxSweeps; s += blockDim.x) d.offs[s * nt] = 0.0;
+  }
+  const int nvt = (chiL + 3) / 4, nwt = (chiR + 63) / 64;
+  const int tv = threadIdx.x >> 6, tw = threadIdx.x & 63;
+  for (int tile = blockIdx.x; tile < nvt * nwt; tile += gridDim.x) {
+    const int v = (tile / nwt) * 4 + tv;
+    const int w = (tile % nwt) * 64 + tw;
+    if (v >= chiL || w >= chiR) continue;
+    double2 acc[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) acc[i] = make_double2(0.0, 0.0);
+    const double2* __restrict__ a0 = d.gk + (size_t)(v * 2 + 0) * chiM;
+    const double2* __restrict__ a1 = d.gk + (size_t)(v * 2 + 1) * chiM;
+    for (int u = 0; u < chiM; ++u) {
+      const double lm = d.lamM[u];
+      const double2 b0 = cscale(d.gk1[(size_t)(u * 2 + 0) * chiR + w], lm);
+      const double2 b1 = cscale(d.gk1[(size_t)(u * 2 + 1) * chiR + w], lm);
+      const double2 x0 = a0[u], x1 = a1[u];
+      acc[0] = cadd(acc[0], cmul(x0, b0));  // s'=0 t'=0
+      acc[1] = cadd(acc[1], cmul(x0, b1));  // s'=0 t'=1
+      acc[2] = cadd(acc[2], cmul(x1, b0));  // s'=1 t'=0
+      acc[3] = cadd(acc[3], cmul(x1, b1));  // s'=1 t'=1
+    }
+    const double ll = d.lamL ? d.lamL[v] : 1.0;
+    const double lr = d.lamR ? d.lamR[w] : 1.0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      if (d.lamL) acc[i] = cscale(acc[i], ll);
+      if (d.lamR) acc[i] = cscale(acc[i], lr);
+    }
+    double2 out[4];
+    if (d.has_gate) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        double2 o = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) o = cadd(o, cmul(d.gate[i * 4 + j], acc[j]));
+        out[i] = o;
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) out[i] = acc[i];
+    }
+#pragma unroll
+    for (int st = 0; st < 4; ++st) {
+      const int s = st >> 1, t = st & 1;
+      const int row = v * 2 + s, col = t * chiR + w;
+      if (orient == 0)
+        d.X[(size_t)col * d.ldx + row] = out[st];
+      else
+        d.X[(size_t)row * d.ldx + col] = cconj(out[st]);
+    }
+  }
+}
+
+// ---------------------------------------------------------------- W = I
+__global__ __launch_bounds__(256) void k_winit(const UpdDesc* __restrict__ descs) {
+  const UpdDesc& d = descs[blockIdx.y];
+  const int chiL = d.dims[0], chiR = d.dims[2];
+  const int C = min(2 * chiL, 2 * chiR);
+  const size_t tot = (size_t)C * C;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < tot;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(e % C), j = (int)(e / C);
+    d.W[(size_t)j * d.ldw + i] = make_double2(i == j ? 1.0 : 0.0, 0.0);
+  }
+}
+
+// ---------------------------------------------------------- Jacobi step
+// Block one-sided Jacobi (Hestenes) over column blocks of width kBW.  One
+// launch is one step of the round-robin ("circle") ordering of the nbe
+// blocks (nbe = number of blocks rounded up to even; a ghost block holds no
+// columns).  Block (p, task) owns the pair of column blocks (a, b): it forms
+// the panel Gram matrix G = P^H P (kPW x kPW), diagonalises it with cyclic
+// two-sided Jacobi rotations (G <- Q^H G Q, Wl <- Wl Q), and applies Wl to the
+// panel of X and of W.  The largest relative off-diagonal |G_ij| /
+// sqrt(G_ii G_jj) seen in the sweep goes to offs[sweep]; a task whose
+// previous sweep stayed below conv_tol has converged and does nothing.
+__device__ __forceinline__ int rr_player(int i, int step, int nbe) {
+  return 1 + (i - 1 + step) % (nbe - 1);
+}
+
+__global__ __launch_bounds__(256) void k_jacobi_step(const UpdDesc* __restrict__ descs, int nt,
+                                                     int sweep, int step, double tol_rot,
+                                                     double conv_tol) {
+  const UpdDesc& d = descs[blockIdx.y];
+  if (sweep > 0 && d.offs[(sweep - 1) * nt] <= conv_tol) return;
+  const int R = d.info[kInfoR], C = d.info[kInfoC];
+  const int nb = (C + kBW - 1) / kBW;
+  const int nbe = nb + (nb & 1);
+  if (step >= nbe - 1) return;
+  const int p = blockIdx.x;
+  if (p >= nbe / 2) return;
+  const int ba = (p == 0) ? 0 : rr_player(p, step, nbe);
+  const int bb = rr_player(nbe - 1 - p, step, nbe);
+
+  __shared__ int col[kPW];
+  __shared__ double2 Ps[64][kPW + 1];
+  __shared__ double2 G[kPW][kPW + 1];
+  __shared__ double2 Wl[kPW][kPW + 1];
+  __shared__ double rcs[kPW / 2];
+  __shared__ double2 rsn[kPW / 2];
+  __shared__ int rflag;
+  __shared__ double red[4];
+
+  const int tid = threadIdx.x;
+  if (tid < kPW) {
+    const int c = (tid < kBW) ? ba * kBW + tid : bb * kBW + (tid - kBW);
+    col[tid] = (c < C) ? c : -1;
+  }
+  __syncthreads();
+
+  // ---- Gram matrix, rows staged through LDS 64 at a time
+  const int gi = tid >> 4, gj = tid & 15;
+  double2 acc = make_double2(0.0, 0.0);
+  for (int r0 = 0; r0 < R; r0 += 64) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int e = tid + 256 * q;
+      const int rr = e & 63, cc = e >> 6;
+      const int c = col[cc];
+      Ps[rr][cc] = (c >= 0 && r0 + rr < R) ? d.X[(size_t)c * d.ldx + r0 + rr]
+                                           : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+#pragma unroll 8
+    for (int rr = 0; rr < 64; ++rr) acc = cadd(acc, cmulc(Ps[rr][gi], Ps[rr][gj]));
+    __syncthreads();
+  }
+  G[gi][gj] = acc;
+  __syncthreads();
+
+  // ---- convergence measure of this panel
+  double ratio = 0.0;
+  if (gi < gj) {
+    const double a = G[gi][gi].x, b = G[gj][gj].x;
+    if (a > 0.0 && b > 0.0) ratio = sqrt(cabs2(G[gi][gj])) / sqrt(a * b);
+  }
+  const double off = block_max(ratio, red);
+  if (tid == 0 && off > 0.0)
+    atomicMax((unsigned long long*)&d.offs[sweep * nt], (unsigned long long)__double_as_longlong(off));
+  if (off <= tol_rot) return;  // panel already orthogonal: nothing to apply
+
+  // ---- diagonalise G with cyclic Jacobi; accumulate Wl
+  Wl[gi][gj] = make_double2(gi == gj ? 1.0 : 0.0, 0.0);
+  __syncthreads();
+  for (int isw = 0; isw < 8; ++isw) {
+    if (tid == 0) rflag = 0;
+    __syncthreads();
+    for (int round = 0; round < kPW - 1; ++round) {
+      if (tid < kPW / 2) {
+        const int i1 = (tid == 0) ? 0 : rr_player(tid, round, kPW);
+        const int i2 = rr_player(kPW - 1 - tid, round, kPW);
+        const double a = G[i1][i1].x, b = G[i2][i2].x;
+        const double2 c = G[i1][i2];
+        const double ac = sqrt(cabs2(c));
+        double cs = 1.0;
+        double2 sn = make_double2(0.0, 0.0);
+        if (a > 0.0 && b > 0.0 && ac > tol_rot * sqrt(a * b)) {
+          const double tau = (b - a) / (2.0 * ac);
+          const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+          cs = 1.0 / sqrt(1.0 + t * t);
+          const double s = t * cs;
+          sn = make_double2(s * c.x / ac, s * c.y / ac);  // sn * e^{i phi}
+          rflag = 1;
+        }
+        rcs[tid] = cs;
+        rsn[tid] = sn;
+      }
+      __syncthreads();
+      // Q restricted to pair (i1, i2): Q[i1][i1] = cs, Q[i1][i2] = sn e,
+      // Q[i2][i1] = -conj(sn e), Q[i2][i2] = cs.
+      // Locate partner and role of index x in this round.
+      auto locate = [&](int x, int& pair, int& role, int& partner) {
+        // player x sits at position pos: x == 0 -> pos 0; else solve
+        // x = 1 + (pos - 1 + round) % (kPW - 1)
+        int pos = (x == 0) ? 0 : 1 + ((x - 1 - round) % (kPW - 1) + (kPW - 1)) % (kPW - 1);
+        if (pos < kPW / 2) {
+          pair = pos;
+          role = 0;
+          partner = rr_player(kPW - 1 - pos, round, kPW);
+        } else {
+          pair = kPW - 1 - pos;
+          role = 1;
+          partner = (pair == 0) ? 0 : rr_player(pair, round, kPW);
+        }
+      };
+      int pi, ri, qi, pj, rj, qj;
+      locate(gi, pi, ri, qi);
+      locate(gj, pj, rj, qj);
+      // column j of Q has nonzeros at rows j (diag) and qj (off)
+      const double csj = rcs[pj];
+      const double2 snj = rsn[pj];
+      const double2 Qjj = make_double2(csj, 0.0);
+      const double2 Qoj = (rj == 0) ? make_double2(-snj.x, snj.y)  // Q[i2][i1] = -conj(sn e)
+                                    : snj;                         // Q[i1][i2] = sn e
+      const double csi = rcs[pi];
+      const double2 sni = rsn[pi];
+      const double2 Qii = make_double2(csi, 0.0);
+      const double2 Qoi = (ri == 0) ? make_double2(-sni.x, sni.y) : sni;
+      // G'[i][j] = sum_{i' in {i, qi}} sum_{j' in {j, qj}} conj(Q[i'][i]) G[i'][j'] Q[j'][j]
+      const double2 g_ij = G[gi][gj], g_ioj = G[gi][qj], g_oij = G[qi][gj], g_oioj = G[qi][qj];
+      const double2 t0 = cadd(cmul(g_ij, Qjj), cmul(g_ioj, Qoj));     // row i of G Q, col j
+      const double2 t1 = cadd(cmul(g_oij, Qjj), cmul(g_oioj, Qoj));   // row qi of G Q, col j
+      const double2 gnew = cadd(cmulc(Qii, t0), cmulc(Qoi, t1));
+      const double2 w_ij = Wl[gi][gj], w_ioj = Wl[gi][qj];
+      const double2 wnew = cadd(cmul(w_ij, Qjj), cmul(w_ioj, Qoj));
+      __syncthreads();
+      G[gi][gj] = gnew;
+      Wl[gi][gj] = wnew;
+      __syncthreads();
+    }
+    if (rflag == 0) break;
+    __syncthreads();
+  }
+
+  // ---- apply Wl to the panel of X (rows < R) and of W (rows < C)
+  for (int pass = 0; pass < 2; ++pass) {
+    double2* M = pass == 0 ? d.X : d.W;
+    const size_t ld = pass == 0 ? (size_t)d.ldx : (size_t)d.ldw;
+    const int rows = pass == 0 ? R : C;
+    for (int r = tid; r < rows; r += blockDim.x) {
+      double2 pr[kPW];
+#pragma unroll
+      for (int c = 0; c < kPW; ++c)
+        pr[c] = col[c] >= 0 ? M[(size_t)col[c] * ld + r] : make_double2(0.0, 0.0);
+#pragma unroll
+      for (int j = 0; j < kPW; ++j) {
+        if (col[j] < 0) continue;
+        double2 o = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int c = 0; c < kPW; ++c) o = cadd(o, cmul(pr[c], Wl[c][j]));
+        M[(size_t)col[j] * ld + r] = o;
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------- finish
+// Column norms of X are the singular values (theta = U S V^H).  Sort them
+// descending, apply process_svd (mps.rs:236-293): normalise, pick the rank,
+// renormalise, and record the right-factor scale fac_j = s_j * norm2.
+__global__ __launch_bounds__(1024) void k_finish(const UpdDesc* __restrict__ descs, int nt,
+                                                 int sweeps_run) {
+  const UpdDesc& d = descs[blockIdx.x];
+  const int R = d.info[kInfoR], C = d.info[kInfoC];
+  __shared__ double key[kMaxCols];
+  __shared__ int idx[kMaxCols];
+  __shared__ double red[16];
+  __shared__ int first_bad;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+
+  if (tid == 0) d.info[kInfoSweeps] = sweeps_run;
+
+  for (int j = wid; j < C; j += 16) {
+    double s = 0.0;
+    for (int r = lane; r < R; r += 64) s += cabs2(d.X[(size_t)j * d.ldx + r]);
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (lane == 0) d.sraw[j] = sqrt(s);
+  }
+  __syncthreads();
+  __threadfence_block();
+  int P2 = 1;
+  while (P2 < C) P2 <<= 1;
+  for (int i = tid; i < P2; i += blockDim.x) {
+    key[i] = (i < C) ? d.sraw[i] : -1.0;
+    idx[i] = i;
+  }
+  __syncthreads();
+  // bitonic sort: descending key, ascending index on ties
+  for (int k = 2; k <= P2; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = tid; i < P2; i += blockDim.x) {
+        const int l = i ^ j;
+        if (l > i) {
+          const bool desc = ((i & k) == 0);
+          const double ki = key[i], kl = key[l];
+          const int ii = idx[i], il = idx[l];
+          // "i before l" in the final order
+          const bool i_first = (ki > kl) || (ki == kl && ii < il);
+          if (desc ? !i_first : i_first) {
+            key[i] = kl; key[l] = ki;
+            idx[i] = il; idx[l] = ii;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  // norm of all values (mps.rs:243-248)
+  double v = 0.0;
+  for (int i = tid; i < C; i += blockDim.x) v += key[i] * key[i];
+  const double norm = sqrt(block_sum(v, red));
+  for (int i = tid; i < C; i += blockDim.x) key[i] = key[i] / norm;
+  if (tid == 0) first_bad = C;
+  __syncthreads();
+  // leading count of normal values above the threshold (take_while)
+  double thr;
+  if (d.tkind == 2) thr = fabs(d.teps);
+  else thr = DBL_EPSILON;
+  for (int i = tid; i < C; i += blockDim.x) {
+    const double s = key[i];
+    const bool ok = (s >= DBL_MIN) && isfinite(s) && (s > thr);
+    if (!ok) atomicMin(&first_bad, i);
+  }
+  __syncthreads();
+  const int lead = first_bad;
+  int rank;
+  if (d.tkind == 1) {
+    const int r = d.tchi < 1 ? 1 : d.tchi;
+    rank = r < lead ? r : lead;
+  } else {
+    rank = lead > 1 ? lead : 1;
+  }
+  int flags = 0;
+  if (rank > d.cap_rank) {
+    rank = d.cap_rank;
+    flags |= kFlagCapacity;
+  }
+  if (rank > C) rank = C;
+  double v2 = 0.0;
+  for (int i = tid; i < rank; i += blockDim.x) v2 += key[i] * key[i];
+  const double norm2 = sqrt(block_sum(v2, red));
+  for (int i = tid; i < rank; i += blockDim.x) {
+    const double s = key[i] / norm2;
+    d.sval[i] = s;
+    d.lamM[i] = s;
+    d.fac[i] = s * norm2;
+    d.perm[i] = idx[i];
+  }
+  if (tid == 0) {
+    d.info[kInfoRank] = rank;
+    d.info[kInfoFlags] |= flags;
+    d.dims[1] = rank;
+  }
+}
+
+// -------------------------------------------------------------- split
+// Gamma[k][v,s,j]   = U[(v,s), j] / lamL_v
+// Gamma[k+1][j,t,w] = (fac_j V^H[j, (t,w)]) / s_j / lamR_w
+// plus the partial sums of local_norm for both sites.
+__global__ __launch_bounds__(256) void k_split(const UpdDesc* __restrict__ descs) {
+  const UpdDesc& d = descs[blockIdx.y];
+  const int chiL = d.info[kInfoChiL], chiR = d.info[kInfoChiR];
+  const int orient = d.info[kInfoOrient];
+  const int rank = d.info[kInfoRank];
+  __shared__ double red[4];
+  const size_t nk = (size_t)chiL * 2 * rank, nk1 = (size_t)rank * 2 * chiR;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  double accA = 0.0, accB = 0.0;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < nk; e += stride) {
+    const int j = (int)(e % rank);
+    const int row = (int)(e / rank);  // v*2 + s
+    const int v = row >> 1;
+    const int pj = d.perm[j];
+    double2 u;
+    if (orient == 0) {
+      u = cdivr(d.X[(size_t)pj * d.ldx + row], d.sraw[pj]);
+    } else {
+      u = d.W[(size_t)pj * d.ldw + row];
+    }
+    double wl = 1.0;
+    if (d.lamL) {
+      wl = d.lamL[v];
+      u = cdivr(u, wl);
+    }
+    d.gk[e] = u;
+    const double sj = d.sval[j];
+    accA += cabs2(u) * (wl * wl) * (sj * sj);
+  }
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < nk1; e += stride) {
+    const int j = (int)(e / (2 * (size_t)chiR));
+    const int cidx = (int)(e % (2 * (size_t)chiR));  // t*chiR + w
+    const int w = cidx % chiR;
+    const int pj = d.perm[j];
+    double2 vh;
+    if (orient == 0) {
+      vh = cconj(d.W[(size_t)pj * d.ldw + cidx]);
+    } else {
+      vh = cconj(cdivr(d.X[(size_t)pj * d.ldx + cidx], d.sraw[pj]));
+    }
+    double2 g = cscale(vh, d.fac[j]);
+    const double sj = d.sval[j];
+    g = cdivr(g, sj);
+    double wr = 1.0;
+    if (d.lamR) {
+      wr = d.lamR[w];
+      g = cdivr(g, wr);
+    }
+    d.gk1[e] = g;
+    accB += cabs2(g) * (sj * sj) * (wr * wr);
+  }
+  const double sA = block_sum(accA, red);
+  const double sB = block_sum(accB, red);
+  if (threadIdx.x == 0) {
+    d.part[2 * blockIdx.x + 0] = sA;
+    d.part[2 * blockIdx.x + 1] = sB;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_renorm(const UpdDesc* __restrict__ descs) {
+  const UpdDesc& d = descs[blockIdx.y];
+  const int chiL = d.info[kInfoChiL], chiR = d.info[kInfoChiR];
+  const int rank = d.info[kInfoRank];
+  __shared__ double nrm[2];
+  if (threadIdx.x < 2) {
+    double s = 0.0;
+    for (int b = 0; b < (int)gridDim.x; ++b) s += d.part[2 * b + threadIdx.x];
+    nrm[threadIdx.x] = sqrt(s);
+  }
+  __syncthreads();
+  const double nA = nrm[0], nB = nrm[1];
+  const size_t nk = (size_t)chiL * 2 * rank, nk1 = (size_t)rank * 2 * chiR;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < nk; e += stride)
+    d.gk[e] = cdivr(d.gk[e], nA);
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < nk1; e += stride)
+    d.gk1[e] = cdivr(d.gk1[e], nB);
+}
+
+// ------------------------------------------------------------ measure
+__global__ __launch_bounds__(1024) void k_measure(const MeasDesc* __restrict__ descs) {
+  const MeasDesc& d = descs[blockIdx.x];
+  const int chiL = d.dims[0], chiR = d.dims[1];
+  __shared__ double red[16];
+  __shared__ int s_out;
+  __shared__ double s_prob;
+  double p0 = 0.0, p1 = 0.0;
+  const size_t tot = (size_t)chiL * chiR;
+  for (size_t e = threadIdx.x; e < tot; e += blockDim.x) {
+    const int v = (int)(e / chiR), u = (int)(e % chiR);
+    double wgt = 1.0;
+    if (d.lamL) wgt *= d.lamL[v] * d.lamL[v];
+    if (d.lamR) wgt *= d.lamR[u] * d.lamR[u];
+    p0 += cabs2(d.g[((size_t)v * 2 + 0) * chiR + u]) * wgt;
+    p1 += cabs2(d.g[((size_t)v * 2 + 1) * chiR + u]) * wgt;
+  }
+  p0 = block_sum(p0, red);
+  p1 = block_sum(p1, red);
+  if (threadIdx.x == 0) {
+    int flag = 0, out;
+    double prob;
+    if (d.forced >= 0) {
+      out = d.forced;
+      prob = out == 0 ? p0 : p1;
+    } else {
+      // first j with r < cumulative probability (mps.rs:1441-1446)
+      const double c0 = p0, c1 = p0 + p1;
+      if (d.r < c0) out = 0;
+      else if (d.r < c1) out = 1;
+      else {
+        out = p1 > 0.0 ? 1 : 0;
+        flag |= 2;
+      }
+      prob = out == 0 ? p0 : p1;
+    }
+    if (d.forced != -2 && !(prob > 0.0)) flag |= 1;
+    s_out = out;
+    s_prob = prob;
+    d.out_outcome[d.slot] = out;
+    d.out_prob[d.slot] = prob;
+    d.out_probs[2 * d.slot + 0] = p0;
+    d.out_probs[2 * d.slot + 1] = p1;
+    d.out_flag[d.slot] = flag;
+  }
+  __syncthreads();
+  if (d.forced == -2) return;
+  const int out = s_out;
+  const double renorm = sqrt(s_prob);
+  for (size_t e = threadIdx.x; e < tot; e += blockDim.x) {
+    const int v = (int)(e / chiR), u = (int)(e % chiR);
+    double2* keep = d.g + ((size_t)v * 2 + out) * chiR + u;
+    double2* drop = d.g + ((size_t)v * 2 + (1 - out)) * chiR + u;
+    *keep = cdivr(*keep, renorm);
+    *drop = make_double2(0.0, 0.0);
+  }
+}
+
+// ------------------------------------------------------------ entropy
+// a == 1: -sum s^2 ln s^2; otherwise ln(sum (s^2)^a) / (1 - a).
+__global__ __launch_bounds__(256) void k_entropy(const double* lam, const int* dimp, double a,
+                                                 double* out) {
+  __shared__ double red[4];
+  const int len = *dimp;
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < len; i += blockDim.x) {
+    const double s2 = lam[i] * lam[i];
+    if (a == 1.0) acc += -s2 * log(s2);
+    else acc += pow(s2, a);
+  }
+  const double s = block_sum(acc, red);
+  if (threadIdx.x == 0) *out = (a == 1.0) ? s : log(s) * (1.0 / (1.0 - a));
+}
+
+// ----------------------------------------------------------------- u1
+struct Gate2 { double2 m[4]; };
+__global__ __launch_bounds__(256) void k_u1(double2* g, const int* dims, Gate2 op) {
+  const int chiL = dims[0], chiR = dims[1];
+  const size_t tot = (size_t)chiL * chiR;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < tot;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const int v = (int)(e / chiR), u = (int)(e % chiR);
+    double2* p0 = g + ((size_t)v * 2 + 0) * chiR + u;
+    double2* p1 = g + ((size_t)v * 2 + 1) * chiR + u;
+    const double2 a = *p0, b = *p1;
+    *p0 = cadd(cmul(op.m[0], a), cmul(op.m[1], b));
+    *p1 = cadd(cmul(op.m[2], a), cmul(op.m[3], b));
+  }
+}
+
+// ------------------------------------------------------------- launch
+void launch_theta(const UpdDesc* descs, int nt, int gx, hipStream_t s) {
+  hipLaunchKernelGGL(k_theta, dim3(gx, nt), dim3(256), 0, s, descs, nt);
+}
+void launch_winit(const UpdDesc* descs, int nt, int gx, hipStream_t s) {
+  hipLaunchKernelGGL(k_winit, dim3(gx, nt), dim3(256), 0, s, descs);
+}
+void launch_jacobi_step(const UpdDesc* descs, int nt, int npairs, int sweep, int step,
+                        double tol_rot, double conv_tol, hipStream_t s) {
+  hipLaunchKernelGGL(k_jacobi_step, dim3(npairs, nt), dim3(256), 0, s, descs, nt, sweep, step,
+                     tol_rot, conv_tol);
+}
+void launch_finish(const UpdDesc* descs, int nt, int sweeps_run, hipStream_t s) {
+  hipLaunchKernelGGL(k_finish, dim3(nt), dim3(1024), 0, s, descs, nt, sweeps_run);
+}
+void launch_split(const UpdDesc* descs, int nt, hipStream_t s) {
+  hipLaunchKernelGGL(k_split, dim3(kNPart, nt), dim3(256), 0, s, descs);
+}
+void launch_renorm(const UpdDesc* descs, int nt, hipStream_t s) {
+  hipLaunchKernelGGL(k_renorm, dim3(kNPart, nt), dim3(256), 0, s, descs);
+}
+void launch_measure(const MeasDesc* descs, int nm, hipStream_t s) {
+  hipLaunchKernelGGL(k_measure, dim3(nm), dim3(1024), 0, s, descs);
+}
+void launch_entropy(const double* lam, const int* dimp, double a, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_entropy, dim3(1), dim3(256), 0, s, lam, dimp, a, out);
+}
+void launch_u1(double2* g, const int* dims, const double* op8, int gx, hipStream_t s) {
+  Gate2 op;
+  for (int i = 0; i < 4; ++i) op.m[i] = make_double2(op8[2 * i], op8[2 * i + 1]);
+  hipLaunchKernelGGL(k_u1, dim3(gx), dim3(256), 0, s, g, dims, op);
+}
+
+}  // namespace tnmps

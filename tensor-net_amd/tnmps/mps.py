@@ -1,0 +1,392 @@
+"""Host-side mirror of tensor-net's ``MPS`` operator API over the C ABI.
+
+Method names, argument meaning, return conventions (``None`` for an
+out-of-range index, silent no-op for out-of-range unitaries) and error kinds
+follow ``tensor-net/lib/mps.rs``; every method cites the reference method it
+mirrors.  The state lives on the GPU; all arithmetic runs in the HIP kernels
+of ``libtnmps.so``.
+
+Truncation is given as in the reference's ``Option<BondDim<f64>>``:
+``None``, ``("const", chi)`` or ``("cutoff", eps)``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import _lib as L
+
+
+class MPSError(Exception):
+    """Mirrors ``MPSError`` (mps.rs:126-154) plus the ABI's extra statuses."""
+
+    KINDS = {
+        L.TN_EMPTY_SYSTEM: "EmptySystem",
+        L.TN_INVALID_QUANTUM_NUMBER: "InvalidQuantumNumber",
+        L.TN_UNPHYSICAL_INDEX: "UnphysicalIndex",
+        L.TN_STATE_INCOMPATIBLE_SHAPE: "StateIncompatibleShape",
+        L.TN_OPERATOR_INCOMPATIBLE_SHAPE: "OperatorIncompatibleShape",
+        L.TN_ZERO_PROB: "ZeroProbability",
+        L.TN_SAMPLE_RANGE: "SampleOutOfRange",
+        L.TN_CAPACITY: "Capacity",
+        L.TN_DEVICE: "Device",
+        L.TN_INVALID_ARG: "InvalidArgument",
+        L.TN_NO_CONVERGENCE: "NoConvergence",
+    }
+
+    def __init__(self, code: int, where: str = ""):
+        self.code = code
+        self.kind = self.KINDS.get(code, f"status{code}")
+        msg = f"{where}: {L.status_str(code)}"
+        if code == L.TN_DEVICE:
+            msg += f" ({L.last_error()})"
+        super().__init__(msg)
+
+
+def _check(code: int, where: str) -> None:
+    if code != L.TN_OK:
+        raise MPSError(code, where)
+
+
+def to_trunc(trunc) -> L.tn_trunc:
+    if trunc is None:
+        return L.tn_trunc(L.TRUNC_NONE, 0, 0.0)
+    kind, val = trunc
+    if kind == "const":
+        return L.tn_trunc(L.TRUNC_CONST, int(val), 0.0)
+    if kind == "cutoff":
+        return L.tn_trunc(L.TRUNC_CUTOFF, 0, float(val))
+    raise ValueError(f"unknown truncation {trunc!r}")
+
+
+def from_trunc(t: L.tn_trunc):
+    if t.kind == L.TRUNC_NONE:
+        return None
+    if t.kind == L.TRUNC_CONST:
+        return ("const", int(t.chi))
+    return ("cutoff", float(t.eps))
+
+
+def _op_buf(op, side: int) -> np.ndarray:
+    a = np.ascontiguousarray(np.asarray(op, dtype=np.complex128))
+    if a.shape != (side, side):
+        raise MPSError(L.TN_OPERATOR_INCOMPATIBLE_SHAPE, "operator shape")
+    return a.view(np.float64).reshape(-1)
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _szarr(vals) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(vals, dtype=np.uintp))
+
+
+def _szptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_size_t))
+
+
+def _iptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+class MPS:
+    """Device-resident Schmidt-form MPS of qubits (``MPS<Q, C64>``, mps.rs:338-356)."""
+
+    def __init__(self, handle: int, lib):
+        self._h = C.c_void_p(handle)
+        self._lib = lib
+
+    # ------------------------------------------------------------ lifecycle
+    @classmethod
+    def new_qubits(cls, n: int, trunc=None, chi_cap: int = 0, stream=None) -> "MPS":
+        """``MPS::new_qubits`` (mps.rs:2033-2039): n qubits in |0...0>."""
+        lib = L.load()
+        h = C.c_void_p()
+        _check(lib.tn_mps_new_qubits(int(n), to_trunc(trunc), int(chi_cap),
+                                     C.c_void_p(stream) if stream else None, C.byref(h)),
+               "new_qubits")
+        return cls(h.value, lib)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            self._lib.tn_mps_free(h)
+            self._h = C.c_void_p()
+
+    def free(self) -> None:
+        self.__del__()
+
+    def clone(self) -> "MPS":
+        """``#[derive(Clone)]`` (mps.rs:337)."""
+        h = C.c_void_p()
+        _check(self._lib.tn_mps_clone(self._h, C.byref(h)), "clone")
+        return MPS(h.value, self._lib)
+
+    @property
+    def n(self) -> int:
+        return int(self._lib.tn_mps_n(self._h))
+
+    def get_trunc(self):
+        """``get_trunc`` (mps.rs:1216)."""
+        t = L.tn_trunc()
+        _check(self._lib.tn_get_trunc(self._h, C.byref(t)), "get_trunc")
+        return from_trunc(t)
+
+    def set_trunc(self, trunc) -> "MPS":
+        """``set_trunc`` (mps.rs:1219-1221)."""
+        _check(self._lib.tn_set_trunc(self._h, to_trunc(trunc)), "set_trunc")
+        return self
+
+    def sync(self) -> None:
+        _check(self._lib.tn_sync(self._h), "sync")
+
+    # ------------------------------------------------------------ unitaries
+    def apply_unitary1(self, k: int, op) -> "MPS":
+        """``apply_unitary1`` (mps.rs:1298-1314); out-of-range k is a no-op."""
+        buf = _op_buf(op, 2)
+        _check(self._lib.tn_apply_u1(self._h, int(k), _dptr(buf)), "apply_unitary1")
+        return self
+
+    def apply_unitary2(self, k: int, op) -> "MPS":
+        """``apply_unitary2`` (mps.rs:1332-1387); out-of-range k is a no-op."""
+        buf = _op_buf(op, 4)
+        _check(self._lib.tn_apply_u2(self._h, int(k), _dptr(buf)), "apply_unitary2")
+        return self
+
+    def apply_unis(self, unis: Sequence[Tuple[int, np.ndarray]]) -> "MPS":
+        """The unitary loop of ``circuit::apply_bilayer`` (circuit.rs:637-638)
+        via ``apply_uni`` (mps.rs:2093-2108)."""
+        if not unis:
+            return self
+        ks, dims, bufs = [], [], []
+        for k, mat in unis:
+            a = np.asarray(mat, dtype=np.complex128)
+            if a.shape not in ((2, 2), (4, 4)):
+                raise MPSError(L.TN_OPERATOR_INCOMPATIBLE_SHAPE, "apply_uni")
+            ks.append(int(k))
+            dims.append(a.shape[0])
+            bufs.append(np.ascontiguousarray(a).view(np.float64).reshape(-1))
+        ksa = _szarr(ks)
+        da = np.asarray(dims, dtype=np.int32)
+        ops = np.ascontiguousarray(np.concatenate(bufs))
+        _check(self._lib.tn_apply_unis(self._h, len(ks), _szptr(ksa), _iptr(da), _dptr(ops)),
+               "apply_unis")
+        return self
+
+    # ---------------------------------------------------------- measurement
+    def prob(self, k: int, p: int) -> Optional[float]:
+        """``prob`` (mps.rs:1071-1074)."""
+        out = C.c_double()
+        rc = self._lib.tn_prob(self._h, int(k), int(p), C.byref(out))
+        if rc == L.TN_NONE:
+            return None
+        _check(rc, "prob")
+        return out.value
+
+    def probs(self, k: int) -> Optional[List[float]]:
+        """``probs`` (mps.rs:1140-1143)."""
+        out = np.zeros(2)
+        rc = self._lib.tn_probs(self._h, int(k), _dptr(out))
+        if rc == L.TN_NONE:
+            return None
+        _check(rc, "probs")
+        return [float(out[0]), float(out[1])]
+
+    def measure_prob(self, k: int, r: float) -> Optional[Tuple[int, float]]:
+        """``measure_prob`` (mps.rs:1479-1489); ``r`` is the uniform draw the
+        reference takes from its rng."""
+        o = C.c_int()
+        pr = C.c_double()
+        rc = self._lib.tn_measure_prob(self._h, int(k), float(r), C.byref(o), C.byref(pr))
+        if rc == L.TN_NONE:
+            return None
+        _check(rc, "measure_prob")
+        return int(o.value), pr.value
+
+    def measure_postsel_prob(self, k: int, p: int) -> Optional[float]:
+        """``measure_postsel_prob`` (mps.rs:1501-1510).  A zero-probability
+        outcome raises ``MPSError`` of kind ZeroProbability (the state then
+        holds NaN/inf, as the reference's does)."""
+        pr = C.c_double()
+        rc = self._lib.tn_measure_postsel_prob(self._h, int(k), int(p), C.byref(pr))
+        if rc == L.TN_NONE:
+            return None
+        _check(rc, "measure_postsel_prob")
+        return pr.value
+
+    def apply_meas_prob_layer(self, meas: Sequence[Tuple[int, Optional[int], float]]):
+        """``apply_meas_prob_layer`` (mps.rs:2206-2252).  ``meas`` holds
+        (site, forced outcome or None for ``Meas::Rand``, uniform draw)."""
+        m = len(meas)
+        if m == 0:
+            return []
+        ks = _szarr([x[0] for x in meas])
+        fo = np.asarray([-1 if x[1] is None else int(x[1]) for x in meas], dtype=np.int32)
+        rs = np.asarray([float(x[2]) for x in meas], dtype=np.float64)
+        cnt = C.c_size_t()
+        oks = np.zeros(m, dtype=np.uintp)
+        oo = np.zeros(m, dtype=np.int32)
+        op = np.zeros(m, dtype=np.float64)
+        _check(self._lib.tn_meas_layer(self._h, m, _szptr(ks), _iptr(fo), _dptr(rs),
+                                       C.byref(cnt), _szptr(oks), _iptr(oo), _dptr(op)),
+               "apply_meas_prob_layer")
+        c = cnt.value
+        return [(int(oks[i]), int(oo[i]), float(op[i])) for i in range(c)]
+
+    # ---------------------------------------------------------- observables
+    def entropy_vn(self, b: int) -> Optional[float]:
+        """``entropy_vn`` (mps.rs:1251-1261)."""
+        out = C.c_double()
+        rc = self._lib.tn_entropy_vn(self._h, int(b), C.byref(out))
+        if rc == L.TN_NONE:
+            return None
+        _check(rc, "entropy_vn")
+        return out.value
+
+    def entropy_ry_schmidt(self, a: float, b: int) -> Optional[float]:
+        """``entropy_ry_schmidt`` (mps.rs:1268-1282)."""
+        out = C.c_double()
+        rc = self._lib.tn_entropy_ry_schmidt(self._h, float(a), int(b), C.byref(out))
+        if rc == L.TN_NONE:
+            return None
+        _check(rc, "entropy_ry_schmidt")
+        return out.value
+
+    def bond_dims(self) -> List[int]:
+        n = self.n
+        if n < 2:
+            return []
+        out = np.zeros(n - 1, dtype=np.uintp)
+        _check(self._lib.tn_bond_dims(self._h, _szptr(out)), "bond_dims")
+        return [int(x) for x in out]
+
+    def bond_dim(self, k: int) -> Optional[int]:
+        """``bond_dim`` (mps.rs:1243-1245)."""
+        d = self.bond_dims()
+        return d[k] if 0 <= k < len(d) else None
+
+    def max_bond_dim(self) -> Optional[int]:
+        """``max_bond_dim`` (mps.rs:1227-1229)."""
+        d = self.bond_dims()
+        return max(d) if d else None
+
+    def schmidt(self, b: int) -> Optional[np.ndarray]:
+        """Schmidt values of bond b (``svals[b]``, mps.rs:352-353)."""
+        ln = C.c_size_t()
+        rc = self._lib.tn_schmidt(self._h, int(b), None, 0, C.byref(ln))
+        if rc == L.TN_NONE:
+            return None
+        _check(rc, "schmidt")
+        out = np.zeros(ln.value)
+        _check(self._lib.tn_schmidt(self._h, int(b), _dptr(out), ln.value, C.byref(ln)),
+               "schmidt")
+        return out
+
+    def gamma(self, k: int) -> Optional[np.ndarray]:
+        """Gamma[k] (``data[k]``, axes [u_{k-1}, s_k, u_k])."""
+        shp = np.zeros(3, dtype=np.uintp)
+        rc = self._lib.tn_get_gamma(self._h, int(k), None, 0, _szptr(shp))
+        if rc == L.TN_NONE:
+            return None
+        _check(rc, "gamma")
+        tot = int(shp[0] * shp[1] * shp[2])
+        buf = np.zeros(2 * tot)
+        _check(self._lib.tn_get_gamma(self._h, int(k), _dptr(buf), buf.size, _szptr(shp)),
+               "gamma")
+        return buf.view(np.complex128).reshape(int(shp[0]), int(shp[1]), int(shp[2]))
+
+    def contract(self) -> np.ndarray:
+        """Flat state vector, qubit 0 slowest (``MPS::contract``, mps.rs:794-798).
+        Host-side read-out for small n (tests)."""
+        n = self.n
+        acc = self.gamma(0).reshape(2, -1)
+        for k in range(1, n):
+            lam = self.schmidt(k - 1)
+            g = self.gamma(k) * lam[:, None, None]
+            acc = (acc @ g.reshape(g.shape[0], -1)).reshape(-1, g.shape[2])
+        return acc.reshape(-1)
+
+    def stats(self) -> Tuple[int, int]:
+        u = C.c_size_t()
+        s = C.c_size_t()
+        _check(self._lib.tn_stats(self._h, C.byref(u), C.byref(s)), "stats")
+        return u.value, s.value
+
+    def set_timing(self, on: bool = True) -> None:
+        _check(self._lib.tn_set_timing(self._h, 1 if on else 0), "set_timing")
+
+    def perf(self) -> dict:
+        p = L.tn_perf()
+        _check(self._lib.tn_get_perf(self._h, C.byref(p)), "perf")
+        return {name: getattr(p, name) for name, _ in L.tn_perf._fields_}
+
+    def reset_perf(self) -> None:
+        _check(self._lib.tn_reset_perf(self._h), "reset_perf")
+
+    # ----------------------------------------------------------- circuits
+    def run_circuit(self, layers, forced=None, rs=None, ent_bond: Optional[int] = None):
+        """Run ``apply_bilayer`` (circuit.rs:624-659) for every layer with no
+        host round trip between layers.
+
+        ``layers``: [(unis, meas_sites)] with unis = [(k, 4x4)].  With
+        ``forced`` (per layer [(k, outcome)]) the measurements replay as
+        ``Meas::Proj``; otherwise they are ``Meas::Rand`` and consume the
+        draws ``rs`` in processing order (sorted, deduplicated sites).
+        Returns a dict like the oracle's ``run_trajectory``.
+        """
+        n = self.n
+        depth = len(layers)
+        if ent_bond is None:
+            ent_bond = n // 2 - 1
+        nu, uks, ops, nm, mks, fo, rr = [], [], [], [], [], [], []
+        rit = iter(rs) if rs is not None else None
+        for t, (unis, msites) in enumerate(layers):
+            nu.append(len(unis))
+            for k, mat in unis:
+                uks.append(int(k))
+                ops.append(np.ascontiguousarray(np.asarray(mat, dtype=np.complex128))
+                           .view(np.float64).reshape(-1))
+            if forced is not None:
+                ml = [(int(k), int(o), 0.0) for (k, o) in forced[t]]
+            else:
+                srt = sorted(set(int(k) for k in msites if k < n))
+                rmap = {k: next(rit) for k in srt}
+                ml = [(int(k), -1, float(rmap.get(int(k), 0.0))) for k in msites]
+            nm.append(len(ml))
+            for (k, o, r) in ml:
+                mks.append(k)
+                fo.append(o)
+                rr.append(r)
+        nu_a = _szarr(nu)
+        uks_a = _szarr(uks if uks else [0])
+        ops_a = np.ascontiguousarray(np.concatenate(ops)) if ops else np.zeros(32)
+        nm_a = _szarr(nm)
+        mks_a = _szarr(mks if mks else [0])
+        fo_a = np.asarray(fo if fo else [0], dtype=np.int32)
+        rr_a = np.asarray(rr if rr else [0.0], dtype=np.float64)
+        tot = max(len(mks), 1)
+        ent = np.zeros(depth + 1)
+        cnts = np.zeros(max(depth, 1), dtype=np.uintp)
+        oo = np.zeros(tot, dtype=np.int32)
+        op = np.zeros(tot, dtype=np.float64)
+        _check(self._lib.tn_run_circuit(self._h, depth, _szptr(nu_a), _szptr(uks_a),
+                                        _dptr(ops_a), _szptr(nm_a), _szptr(mks_a),
+                                        _iptr(fo_a), _dptr(rr_a), int(ent_bond), _dptr(ent),
+                                        _szptr(cnts), _iptr(oo), _dptr(op)),
+               "run_circuit")
+        # processed sites per layer: sorted, deduplicated, in range
+        record, probs = [], []
+        pos = 0
+        mo = 0
+        for t in range(depth):
+            sites = sorted(set(k for k in mks[mo:mo + nm[t]] if k < n))
+            mo += nm[t]
+            c = int(cnts[t])
+            assert c == len(sites)
+            record.append([(sites[i], int(oo[pos + i])) for i in range(c)])
+            probs.append([(sites[i], float(op[pos + i])) for i in range(c)])
+            pos += c
+        return {"record": record, "probs": probs, "entropy": list(ent)}
