@@ -29,6 +29,9 @@ extern "C" {
 #endif
 
 typedef struct tn_mps tn_mps;
+/* A HIP stream plus the update workspace shared by the handles made in it.
+ * Handles of one engine can be advanced in lock step (tn_batch_bilayer). */
+typedef struct tn_engine tn_engine;
 
 /* Status codes.  1..5 mirror MPSError (mps.rs:126-154). */
 typedef enum {
@@ -61,6 +64,11 @@ typedef struct {
 
 enum { TN_TRUNC_NONE = 0, TN_TRUNC_CONST = 1, TN_TRUNC_CUTOFF = 2 };
 
+/* Arithmetic of the two-site contraction theta = U (lambda Gamma lambda Gamma
+ * lambda).  The SVD, truncation, lambda renormalisation and every stored
+ * tensor stay FP64 in both modes. */
+enum { TN_PREC_F64 = 0, TN_PREC_F32 = 1 };
+
 /* Human-readable name of a status code (static string). */
 const char* tn_status_str(tn_status s);
 
@@ -78,8 +86,22 @@ int tn_device_count(void);
  * library creates one).  n == 0 -> TN_EMPTY_SYSTEM. */
 tn_status tn_mps_new_qubits(size_t n, tn_trunc trunc, size_t chi_cap,
                             void* stream, tn_mps** out);
-/* Drop (frees device memory). */
+/* Drop (frees device memory).  A handle made by tn_mps_new_qubits frees its
+ * private engine with it. */
 void tn_mps_free(tn_mps* h);
+
+/* An engine for handles whose bonds stay at or below chi_max (1..512), on
+ * `stream` (hipStream_t, or NULL: the library creates one).  No reference
+ * counterpart: the reference runs one MPS per thread; the engine lets many
+ * trajectories share launches.  tn_engine_free defers the release until the
+ * last handle of the engine is freed. */
+tn_status tn_engine_new(size_t chi_max, void* stream, tn_engine** out);
+void tn_engine_free(tn_engine* e);
+/* MPS::new_qubits (mps.rs:2033-2039) inside an engine. */
+tn_status tn_mps_new_qubits_in(tn_engine* e, size_t n, tn_trunc trunc, size_t chi_cap,
+                               tn_mps** out);
+/* Contraction arithmetic of this handle (TN_PREC_F64 default). */
+tn_status tn_set_precision(tn_mps* h, int prec);
 /* MPS::clone (#[derive(Clone)], mps.rs:337): deep copy on the same stream. */
 tn_status tn_mps_clone(const tn_mps* h, tn_mps** out);
 size_t tn_mps_n(const tn_mps* h);
@@ -167,6 +189,23 @@ tn_status tn_run_circuit(tn_mps* h, size_t depth, const size_t* nu,
                          const int* forced, const double* r, size_t ent_bond,
                          double* ent_out, size_t* out_counts,
                          int* out_outcomes, double* out_probs);
+
+/* One bilayer (circuit::apply_bilayer, circuit.rs:624-659) on each of
+ * `count` handles of engine e, in lock step: the gates of all handles run in
+ * shared launches, then the measurement schedules (mps.rs:2206-2252) advance
+ * together, step j of every handle in one launch.  Per handle i, in order:
+ * nu[i] two-qubit gates (uks, ops: 32 doubles each) and nm[i] measurements
+ * (mks, forced, r as in tn_meas_layer), packed handle after handle.
+ * ent_bonds[i] (optional) selects the bond whose entropy_vn goes to
+ * ent_out[i] after the layer; out_counts[i] receives the number of processed
+ * measurements of handle i, whose outcomes and probabilities go, handle after
+ * handle, to out_outcomes / out_probs.  Each handle's result equals running
+ * tn_apply_bilayer on it alone. */
+tn_status tn_batch_bilayer(tn_engine* e, size_t count, tn_mps* const* hs, const size_t* nu,
+                           const size_t* uks, const double* ops, const size_t* nm,
+                           const size_t* mks, const int* forced, const double* r,
+                           const size_t* ent_bonds, double* ent_out, size_t* out_counts,
+                           int* out_outcomes, double* out_probs);
 
 /* Counters of the device work done by this handle (for throughput
  * accounting): number of two-site updates (contract -> SVD -> split) and

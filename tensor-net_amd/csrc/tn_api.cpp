@@ -3,9 +3,14 @@
 // The driver restates the control flow of tensor-net's MPS methods
 // (tensor-net/lib/mps.rs) -- which bonds are refactored in which order, when
 // sites are measured, what is a no-op -- and enqueues the device work for
-// each step on the handle's HIP stream.  All numerical work happens in the
+// each step on the engine's HIP stream.  All numerical work happens in the
 // kernels of tn_kernels.hip; Gamma/lambda never leave the device except when
 // the caller asks for them.
+//
+// An engine owns a stream, the update workspace and the descriptor staging.
+// Handles bound to one engine can be advanced in lock step: the updates and
+// measurements that different handles need at the same point of their
+// schedules run in shared launches.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -16,9 +21,9 @@
 #include <string>
 #include <vector>
 
-#include "tnmps.h"
 #include "tn_device.h"
 #include "tn_kernels.h"
+#include "tnmps.h"
 
 using namespace tnmps;
 
@@ -33,6 +38,12 @@ thread_local std::string g_last_error;
       g_last_error = std::string(#expr) + ": " + hipGetErrorString(e__);          \
       return TN_DEVICE;                                                           \
     }                                                                             \
+  } while (0)
+
+#define TN_TRY(expr)                \
+  do {                              \
+    tn_status s__ = (expr);         \
+    if (s__ != TN_OK) return s__;   \
   } while (0)
 
 double env_double(const char* name, double dflt) {
@@ -54,6 +65,11 @@ struct Tolerances {
   }
 };
 
+const Tolerances& tolerances() {
+  static Tolerances t;
+  return t;
+}
+
 void hfree(void* p) {
   if (p) (void)hipFree(p);
 }
@@ -61,22 +77,87 @@ void hhfree(void* p) {
   if (p) (void)hipHostFree(p);
 }
 
-const Tolerances& tolerances() {
-  static Tolerances t;
-  return t;
+constexpr int kRing = 4;  // staging buffers in flight per descriptor kind
+
+size_t exact_bond_max(size_t n, size_t b) {
+  // min(2^{b+1}, 2^{n-b-1}) without overflow
+  const size_t l = b + 1, r = n - b - 1;
+  const size_t e = std::min(l, r);
+  if (e >= 40) return (size_t)1 << 40;
+  return (size_t)1 << e;
 }
+
+// Pinned host staging + device copy of a descriptor array, in a ring so the
+// host can prepare the next launch while earlier ones are still queued.
+template <typename T>
+struct DescRing {
+  int cap = 0;
+  int pos = 0;
+  T* h[kRing] = {};
+  T* d[kRing] = {};
+  hipEvent_t ev[kRing] = {};
+
+  void release() {
+    for (int i = 0; i < kRing; ++i) {
+      hhfree(h[i]);
+      hfree(d[i]);
+      if (ev[i]) (void)hipEventDestroy(ev[i]);
+      h[i] = nullptr;
+      d[i] = nullptr;
+      ev[i] = nullptr;
+    }
+    cap = 0;
+  }
+  tn_status ensure(int need, hipStream_t s) {
+    if (need <= cap) return TN_OK;
+    TN_HIP(hipStreamSynchronize(s));
+    release();
+    const int c = std::max(need, 64);
+    for (int i = 0; i < kRing; ++i) {
+      TN_HIP(hipHostMalloc(&h[i], sizeof(T) * c, hipHostMallocDefault));
+      TN_HIP(hipMalloc(&d[i], sizeof(T) * c));
+      TN_HIP(hipEventCreateWithFlags(&ev[i], hipEventDisableTiming));
+    }
+    cap = c;
+    return TN_OK;
+  }
+  // Host buffer for the next launch (waits until its previous copy is done).
+  tn_status next(T** host, int* slot) {
+    const int i = pos;
+    pos = (pos + 1) % kRing;
+    TN_HIP(hipEventSynchronize(ev[i]));
+    *host = h[i];
+    *slot = i;
+    return TN_OK;
+  }
+  tn_status upload(int slot, int count, hipStream_t s) {
+    TN_HIP(hipMemcpyAsync(d[slot], h[slot], sizeof(T) * count, hipMemcpyHostToDevice, s));
+    TN_HIP(hipEventRecord(ev[slot], s));
+    return TN_OK;
+  }
+};
+
+// One entropy read-out: lambda of a bond, its length, Renyi order, result slot.
+struct EntReq {
+  const double* lam;
+  const int* dim;
+  double a;
+  int slot;
+};
 
 }  // namespace
 
 // --------------------------------------------------------------------------
-// Engine: workspace for concurrent update tasks plus staging buffers.
+// Engine: stream, update workspace, descriptor staging, result buffers.
 // --------------------------------------------------------------------------
-struct Engine {
+struct tn_engine {
   int dev = 0;
   hipStream_t stream = nullptr;
   bool own_stream = false;
-  int ldx = 0, ldw = 0;  // row capacities of X (R) and W (C)
-  int slots = 0;         // concurrent update tasks the workspace holds
+  size_t chi_max = 0;   // largest bond dimension of any handle served
+  int ldx = 0, ldw = 0; // row capacities of X (R) and W (C)
+  int slots = 0;        // update tasks the workspace holds
+  int max_slots = 0;    // tasks per launch batch (larger batches are chunked)
   double2* X = nullptr;
   double2* W = nullptr;
   double* sraw = nullptr;
@@ -86,62 +167,58 @@ struct Engine {
   double* part = nullptr;
   double* offs = nullptr;
   int* info = nullptr;
-  UpdDesc* d_desc = nullptr;
-  UpdDesc* h_desc = nullptr;
-  double* h_offs = nullptr;
-  hipEvent_t desc_ev = nullptr;
-  // measurement staging and results
-  int mslots = 0;
-  MeasDesc* d_mdesc = nullptr;
-  MeasDesc* h_mdesc = nullptr;
-  hipEvent_t mdesc_ev = nullptr;
+  DescRing<UpdDesc> upd;
+  DescRing<MeasDesc> meas;
+  int mres = 0;  // measurement result slots
   int* d_out = nullptr;
   double* d_prob = nullptr;
   double* d_probs = nullptr;
   int* d_flag = nullptr;
   double* d_scalar = nullptr;  // scalar results (entropies)
   int nscalar = 0;
-  // statistics
-  unsigned long long updates = 0, sweeps = 0;
+  int coop_blocks = 0;         // >0: persistent cooperative Jacobi with this grid cap
+  int* d_sweeps = nullptr;     // sweeps used by the last cooperative SVD batch
+  int* h_sweeps = nullptr;
   bool timing = false;
   hipEvent_t tev[6] = {};
   tn_perf perf{};
   std::vector<int> h_info;
+  unsigned long long updates = 0, sweeps = 0;
+  int users = 0;               // handles bound to this engine
+  bool auto_free = false;      // freed with its last handle
 
-  ~Engine() { release(); }
+  ~tn_engine() { release(); }
 
   void release_ws() {
     hfree(X); hfree(W); hfree(sraw); hfree(sval); hfree(fac); hfree(perm);
-    hfree(part); hfree(offs); hfree(info); hfree(d_desc);
-    hhfree(h_desc);
-    hhfree(h_offs);
-    X = W = nullptr; sraw = sval = fac = part = offs = nullptr; perm = info = nullptr;
-    d_desc = nullptr; h_desc = nullptr; h_offs = nullptr;
+    hfree(part); hfree(offs); hfree(info);
+    X = W = nullptr;
+    sraw = sval = fac = part = offs = nullptr;
+    perm = info = nullptr;
     slots = 0;
   }
-  void release_meas() {
-    hfree(d_mdesc); hfree(d_out); hfree(d_prob); hfree(d_probs); hfree(d_flag);
-    hhfree(h_mdesc);
-    d_mdesc = nullptr; h_mdesc = nullptr; d_out = nullptr; d_prob = d_probs = nullptr;
-    d_flag = nullptr; mslots = 0;
+  void release_res() {
+    hfree(d_out); hfree(d_prob); hfree(d_probs); hfree(d_flag);
+    d_out = nullptr; d_prob = d_probs = nullptr; d_flag = nullptr;
+    mres = 0;
   }
   void release() {
     if (stream) (void)hipStreamSynchronize(stream);
     release_ws();
-    release_meas();
+    release_res();
+    upd.release();
+    meas.release();
     hfree(d_scalar); d_scalar = nullptr; nscalar = 0;
+    hfree(d_sweeps); d_sweeps = nullptr;
+    hhfree(h_sweeps); h_sweeps = nullptr;
     for (hipEvent_t& ev : tev)
       if (ev) (void)hipEventDestroy(ev), ev = nullptr;
-    if (desc_ev) (void)hipEventDestroy(desc_ev);
-    if (mdesc_ev) (void)hipEventDestroy(mdesc_ev);
-    desc_ev = mdesc_ev = nullptr;
     if (own_stream && stream) (void)hipStreamDestroy(stream);
     stream = nullptr;
   }
 
-  tn_status init(int device, void* s, int ldx_, int ldw_) {
-    dev = device;
-    TN_HIP(hipSetDevice(dev));
+  tn_status init(void* s, size_t chi) {
+    TN_HIP(hipGetDevice(&dev));
     if (s) {
       stream = (hipStream_t)s;
       own_stream = false;
@@ -149,11 +226,17 @@ struct Engine {
       TN_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
       own_stream = true;
     }
-    ldx = ldx_;
-    ldw = ldw_;
-    TN_HIP(hipEventCreateWithFlags(&desc_ev, hipEventDisableTiming));
-    TN_HIP(hipEventCreateWithFlags(&mdesc_ev, hipEventDisableTiming));
+    chi_max = chi;
+    ldx = ldw = (int)(2 * chi);
     for (hipEvent_t& ev : tev) TN_HIP(hipEventCreate(&ev));
+    TN_HIP(hipMalloc(&d_sweeps, sizeof(int)));
+    TN_HIP(hipHostMalloc(&h_sweeps, sizeof(int), hipHostMallocDefault));
+    const char* mode = std::getenv("TNMPS_JACOBI");
+    coop_blocks = (mode && std::strcmp(mode, "step") == 0) ? 0 : jacobi_coop_max_blocks();
+    // workspace budget: TNMPS_WS_BYTES (default 24 GiB) bounds the tasks per launch
+    const double budget = env_double("TNMPS_WS_BYTES", 24.0 * (1ull << 30));
+    const double per = 16.0 * ((double)ldx * ldw + (double)ldw * ldw) + 64.0 * ldw + 4096.0;
+    max_slots = (int)std::max(1.0, std::min(8192.0, budget / per));
     return TN_OK;
   }
 
@@ -178,25 +261,20 @@ struct Engine {
     TN_HIP(hipMalloc(&part, sizeof(double) * 2 * kNPart * ns));
     TN_HIP(hipMalloc(&offs, sizeof(double) * kMaxSweeps * ns));
     TN_HIP(hipMalloc(&info, sizeof(int) * kInfoLen * ns));
-    TN_HIP(hipMalloc(&d_desc, sizeof(UpdDesc) * ns));
-    TN_HIP(hipHostMalloc(&h_desc, sizeof(UpdDesc) * ns, hipHostMallocDefault));
-    TN_HIP(hipHostMalloc(&h_offs, sizeof(double) * ns, hipHostMallocDefault));
     slots = ns;
-    return TN_OK;
+    return upd.ensure(ns, stream);
   }
 
-  tn_status ensure_mslots(int need) {
-    if (need <= mslots) return TN_OK;
+  tn_status ensure_results(int need) {
+    if (need <= mres) return TN_OK;
     TN_HIP(hipStreamSynchronize(stream));
-    release_meas();
-    const int ns = std::max(need, 64);
-    TN_HIP(hipMalloc(&d_mdesc, sizeof(MeasDesc) * ns));
-    TN_HIP(hipHostMalloc(&h_mdesc, sizeof(MeasDesc) * ns, hipHostMallocDefault));
+    release_res();
+    const int ns = std::max(need, 256);
     TN_HIP(hipMalloc(&d_out, sizeof(int) * ns));
     TN_HIP(hipMalloc(&d_prob, sizeof(double) * ns));
     TN_HIP(hipMalloc(&d_probs, sizeof(double) * 2 * ns));
     TN_HIP(hipMalloc(&d_flag, sizeof(int) * ns));
-    mslots = ns;
+    mres = ns;
     return TN_OK;
   }
 
@@ -216,10 +294,11 @@ struct Engine {
 // MPS handle
 // --------------------------------------------------------------------------
 struct tn_mps {
-  Engine* eng = nullptr;
+  tn_engine* eng = nullptr;
   size_t n = 0;
   tn_trunc trunc{};
   size_t cap = 0;
+  int prec = TN_PREC_F64;
   std::vector<size_t> bmax;     // allocation bound per bond
   std::vector<size_t> goff;     // element offset of each site slot
   std::vector<size_t> loff;     // element offset of each bond slot
@@ -234,7 +313,10 @@ struct tn_mps {
     hfree(gam);
     hfree(lam);
     hfree(dims);
-    delete eng;
+    if (eng) {
+      eng->users--;
+      if (eng->auto_free && eng->users <= 0) delete eng;
+    }
   }
   size_t bL(size_t k) const { return k == 0 ? 1 : bmax[k - 1]; }
   size_t bR(size_t k) const { return k + 1 >= n ? 1 : bmax[k]; }
@@ -244,38 +326,42 @@ struct tn_mps {
 
 namespace {
 
-size_t exact_bond_max(size_t n, size_t b) {
-  // min(2^{b+1}, 2^{n-b-1}) without overflow
-  const size_t l = b + 1, r = n - b - 1;
-  const size_t e = std::min(l, r);
-  if (e >= 40) return (size_t)1 << 40;
-  return (size_t)1 << e;
-}
-
 size_t rank_bound(const tn_mps* h) {
   if (h->trunc.kind == TN_TRUNC_CONST) return std::max<size_t>(h->trunc.chi, 1);
   return (size_t)-1;
 }
 
-struct UpdReq {
+// One two-site update of one handle.
+struct Task {
+  tn_mps* h;
   size_t k;
   bool has_gate;
   double gate[32];
 };
 
-// Enqueue a batch of independent two-site updates on one handle: bonds must
-// pairwise differ by at least 2 so that no update reads what another writes.
-tn_status run_updates(tn_mps* h, const std::vector<UpdReq>& reqs) {
-  Engine* e = h->eng;
-  const int nt = (int)reqs.size();
-  if (nt == 0) return TN_OK;
-  tn_status st = e->ensure_slots(nt);
-  if (st != TN_OK) return st;
-  TN_HIP(hipEventSynchronize(e->desc_ev));
+// One site measurement (or probability query with forced == -2).
+struct MTask {
+  tn_mps* h;
+  size_t k;
+  int forced;
+  double r;
+  int slot;
+};
+
+// Enqueue up to max_slots independent updates.  Within one handle, bonds
+// must pairwise differ by at least 2 so that no update reads what another
+// writes; different handles are independent.
+tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
+  TN_TRY(e->ensure_slots(nt));
+  UpdDesc* hd = nullptr;
+  int rs = 0;
+  TN_TRY(e->upd.next(&hd, &rs));
   size_t maxC = 1, maxTiles = 1, maxCC = 1;
   for (int t = 0; t < nt; ++t) {
-    const size_t k = reqs[t].k;
-    UpdDesc& d = e->h_desc[t];
+    const Task& q = tasks[t];
+    tn_mps* h = q.h;
+    const size_t k = q.k;
+    UpdDesc& d = hd[t];
     std::memset(&d, 0, sizeof(d));
     d.gk = h->gam + h->goff[k];
     d.gk1 = h->gam + h->goff[k + 1];
@@ -294,27 +380,27 @@ tn_status run_updates(tn_mps* h, const std::vector<UpdReq>& reqs) {
     d.info = e->info + (size_t)t * kInfoLen;
     d.ldx = e->ldx;
     d.ldw = e->ldw;
-    d.has_gate = reqs[t].has_gate ? 1 : 0;
+    d.has_gate = q.has_gate ? 1 : 0;
+    d.prec = h->prec;
     d.tkind = h->trunc.kind;
     d.tchi = (int)std::min<size_t>(h->trunc.chi, 1u << 30);
     d.cap_rank = (int)h->bmax[k];
     d.teps = h->trunc.eps;
-    if (reqs[t].has_gate)
-      for (int i = 0; i < 16; ++i) d.gate[i] = make_double2(reqs[t].gate[2 * i], reqs[t].gate[2 * i + 1]);
+    if (q.has_gate)
+      for (int i = 0; i < 16; ++i) d.gate[i] = make_double2(q.gate[2 * i], q.gate[2 * i + 1]);
     const size_t chiL = h->uL(k), chiR = h->uR(k + 1);
     const size_t C = std::min(2 * chiL, 2 * chiR);
     maxC = std::max(maxC, C);
     maxCC = std::max(maxCC, C * C);
     maxTiles = std::max(maxTiles, ((chiL + 3) / 4) * ((chiR + 63) / 64));
   }
-  TN_HIP(hipMemcpyAsync(e->d_desc, e->h_desc, sizeof(UpdDesc) * nt, hipMemcpyHostToDevice,
-                        e->stream));
-  TN_HIP(hipEventRecord(e->desc_ev, e->stream));
+  TN_TRY(e->upd.upload(rs, nt, e->stream));
+  const UpdDesc* dd = e->upd.d[rs];
   if (e->timing) TN_HIP(hipEventRecord(e->tev[0], e->stream));
   const int gx_theta = (int)std::min<size_t>(maxTiles, 1024);
   const int gx_w = (int)std::min<size_t>((maxCC + 255) / 256, 1024);
-  launch_theta(e->d_desc, nt, gx_theta, e->stream);
-  launch_winit(e->d_desc, nt, gx_w, e->stream);
+  launch_theta(dd, nt, gx_theta, e->stream);
+  launch_winit(dd, nt, gx_w, e->stream);
   TN_HIP(hipGetLastError());
   if (e->timing) TN_HIP(hipEventRecord(e->tev[1], e->stream));
   const int nb = (int)((maxC + kBW - 1) / kBW);
@@ -322,33 +408,53 @@ tn_status run_updates(tn_mps* h, const std::vector<UpdReq>& reqs) {
   const int npairs = nbe / 2, steps = nbe - 1;
   const Tolerances& tol = tolerances();
   int sweeps_run = 0;
-  for (int sw = 0; sw < kMaxSweeps; ++sw) {
+  if (e->coop_blocks > 0) {
+    // all sweeps in one persistent launch; the sweep count stays on the device
+    const int grid = std::min(nt * npairs, e->coop_blocks);
     if (e->timing) TN_HIP(hipEventRecord(e->tev[2], e->stream));
-    for (int s = 0; s < steps; ++s)
-      launch_jacobi_step(e->d_desc, nt, npairs, sw, s, tol.tol_rot, tol.conv_tol, e->stream);
-    TN_HIP(hipGetLastError());
-    if (e->timing) TN_HIP(hipEventRecord(e->tev[3], e->stream));
-    sweeps_run = sw + 1;
-    TN_HIP(hipMemcpyAsync(e->h_offs, e->offs + (size_t)sw * nt, sizeof(double) * nt,
-                          hipMemcpyDeviceToHost, e->stream));
-    TN_HIP(hipStreamSynchronize(e->stream));
+    TN_HIP(launch_jacobi_coop(dd, nt, npairs, steps, kMaxSweeps, tol.tol_rot, tol.conv_tol,
+                              e->d_sweeps, grid, e->stream));
     if (e->timing) {
-      if (sw == 0) e->perf.ms_theta += e->span(0, 1);
+      TN_HIP(hipEventRecord(e->tev[3], e->stream));
+      TN_HIP(hipMemcpyAsync(e->h_sweeps, e->d_sweeps, sizeof(int), hipMemcpyDeviceToHost,
+                            e->stream));
+      TN_HIP(hipStreamSynchronize(e->stream));
+      e->perf.ms_theta += e->span(0, 1);
       e->perf.ms_jacobi += e->span(2, 3);
-      e->perf.jacobi_launches += steps;
+      e->perf.jacobi_launches += 1;
+      sweeps_run = *e->h_sweeps;
     }
-    bool done = true;
-    for (int t = 0; t < nt; ++t)
-      if (!(e->h_offs[t] <= tol.conv_tol)) done = false;
-    if (done) break;
+  } else {
+    // one launch per round-robin step; the host checks convergence per sweep
+    std::vector<double> offs(nt);
+    for (int sw = 0; sw < kMaxSweeps; ++sw) {
+      if (e->timing) TN_HIP(hipEventRecord(e->tev[2], e->stream));
+      for (int s = 0; s < steps; ++s)
+        launch_jacobi_step(dd, nt, npairs, sw, s, tol.tol_rot, tol.conv_tol, e->stream);
+      TN_HIP(hipGetLastError());
+      if (e->timing) TN_HIP(hipEventRecord(e->tev[3], e->stream));
+      sweeps_run = sw + 1;
+      TN_HIP(hipMemcpyAsync(offs.data(), e->offs + (size_t)sw * nt, sizeof(double) * nt,
+                            hipMemcpyDeviceToHost, e->stream));
+      TN_HIP(hipStreamSynchronize(e->stream));
+      if (e->timing) {
+        if (sw == 0) e->perf.ms_theta += e->span(0, 1);
+        e->perf.ms_jacobi += e->span(2, 3);
+        e->perf.jacobi_launches += steps;
+      }
+      bool done = true;
+      for (int t = 0; t < nt; ++t)
+        if (!(offs[t] <= tol.conv_tol)) done = false;
+      if (done) break;
+    }
   }
   if (e->timing) TN_HIP(hipEventRecord(e->tev[4], e->stream));
-  launch_finish(e->d_desc, nt, sweeps_run, e->stream);
-  launch_split(e->d_desc, nt, e->stream);
-  launch_renorm(e->d_desc, nt, e->stream);
+  launch_finish(dd, nt, e->coop_blocks > 0 ? e->d_sweeps : nullptr, sweeps_run, e->stream);
+  launch_split(dd, nt, e->stream);
+  launch_renorm(dd, nt, e->stream);
   TN_HIP(hipGetLastError());
   e->updates += nt;
-  e->sweeps += (unsigned long long)sweeps_run * nt;
+  e->sweeps += (unsigned long long)sweeps_run;
   if (e->timing) {
     TN_HIP(hipEventRecord(e->tev[5], e->stream));
     e->h_info.resize((size_t)nt * kInfoLen);
@@ -364,49 +470,60 @@ tn_status run_updates(tn_mps* h, const std::vector<UpdReq>& reqs) {
       const double M = std::max(m, n), N = std::min(m, n);
       const double fsvd = 4.0 * (4.0 * M * M * N + 8.0 * M * N * N + 9.0 * N * N * N);
       e->perf.svd_flops += fsvd;
-      e->perf.alg_flops += 8.0 * m * n * K + (reqs[t].has_gate ? 32.0 * m * n : 0.0) + fsvd;
+      e->perf.alg_flops += 8.0 * m * n * K + (tasks[t].has_gate ? 32.0 * m * n : 0.0) + fsvd;
     }
   }
-  const size_t rb = rank_bound(h);
   for (int t = 0; t < nt; ++t) {
-    const size_t k = reqs[t].k;
+    tn_mps* h = tasks[t].h;
+    const size_t k = tasks[t].k;
     const size_t chiL = h->uL(k), chiR = h->uR(k + 1);
-    h->ub[k] = std::min({2 * chiL, 2 * chiR, rb, h->bmax[k]});
+    h->ub[k] = std::min({2 * chiL, 2 * chiR, rank_bound(h), h->bmax[k]});
   }
   return TN_OK;
 }
 
-tn_status refactor(tn_mps* h, size_t k) {
-  UpdReq r;
-  r.k = k;
-  r.has_gate = false;
-  return run_updates(h, {r});
+tn_status run_updates(tn_engine* e, const std::vector<Task>& tasks) {
+  const int nt = (int)tasks.size();
+  for (int a = 0; a < nt; a += e->max_slots)
+    TN_TRY(run_chunk(e, tasks.data() + a, std::min(e->max_slots, nt - a)));
+  return TN_OK;
 }
 
-// One site measurement; results land in slot `slot` of the engine's arrays.
-tn_status enqueue_measure(tn_mps* h, size_t k, int forced, double r, int slot) {
-  Engine* e = h->eng;
-  tn_status st = e->ensure_mslots(slot + 1);
-  if (st != TN_OK) return st;
-  TN_HIP(hipEventSynchronize(e->mdesc_ev));
-  MeasDesc& d = e->h_mdesc[0];
-  std::memset(&d, 0, sizeof(d));
-  d.g = h->gam + h->goff[k];
-  d.lamL = k == 0 ? nullptr : h->lam + h->loff[k - 1];
-  d.lamR = (k + 1 >= h->n) ? nullptr : h->lam + h->loff[k];
-  d.dims = h->dims + k;
-  d.forced = forced;
-  d.slot = slot;
-  d.r = r;
-  d.out_outcome = e->d_out;
-  d.out_prob = e->d_prob;
-  d.out_probs = e->d_probs;
-  d.out_flag = e->d_flag;
-  TN_HIP(hipMemcpyAsync(e->d_mdesc, e->h_mdesc, sizeof(MeasDesc), hipMemcpyHostToDevice,
-                        e->stream));
-  TN_HIP(hipEventRecord(e->mdesc_ev, e->stream));
+tn_status refactor(tn_mps* h, size_t k) {
+  Task t;
+  t.h = h;
+  t.k = k;
+  t.has_gate = false;
+  return run_updates(h->eng, {t});
+}
+
+tn_status run_measures(tn_engine* e, const std::vector<MTask>& ms) {
+  const int nm = (int)ms.size();
+  if (nm == 0) return TN_OK;
+  TN_TRY(e->meas.ensure(nm, e->stream));
+  MeasDesc* hd = nullptr;
+  int rs = 0;
+  TN_TRY(e->meas.next(&hd, &rs));
+  for (int i = 0; i < nm; ++i) {
+    const MTask& q = ms[i];
+    tn_mps* h = q.h;
+    MeasDesc& d = hd[i];
+    std::memset(&d, 0, sizeof(d));
+    d.g = h->gam + h->goff[q.k];
+    d.lamL = q.k == 0 ? nullptr : h->lam + h->loff[q.k - 1];
+    d.lamR = (q.k + 1 >= h->n) ? nullptr : h->lam + h->loff[q.k];
+    d.dims = h->dims + q.k;
+    d.forced = q.forced;
+    d.slot = q.slot;
+    d.r = q.r;
+    d.out_outcome = e->d_out;
+    d.out_prob = e->d_prob;
+    d.out_probs = e->d_probs;
+    d.out_flag = e->d_flag;
+  }
+  TN_TRY(e->meas.upload(rs, nm, e->stream));
   if (e->timing) TN_HIP(hipEventRecord(e->tev[0], e->stream));
-  launch_measure(e->d_mdesc, 1, e->stream);
+  launch_measure(e->meas.d[rs], nm, e->stream);
   TN_HIP(hipGetLastError());
   if (e->timing) {
     TN_HIP(hipEventRecord(e->tev[1], e->stream));
@@ -416,132 +533,264 @@ tn_status enqueue_measure(tn_mps* h, size_t k, int forced, double r, int slot) {
   return TN_OK;
 }
 
-tn_status refactor_sweep(tn_mps* h) {
-  // mps.rs:894-897
-  if (h->n < 2) return TN_OK;
-  for (size_t k = 0; k + 1 < h->n; ++k) {
-    tn_status st = refactor(h, k);
-    if (st != TN_OK) return st;
-  }
-  for (size_t k = h->n - 2; k-- > 0;) {
-    tn_status st = refactor(h, k);
-    if (st != TN_OK) return st;
-  }
+tn_status run_entropies(tn_engine* e, const std::vector<EntReq>& es) {
+  for (const EntReq& q : es) launch_entropy(q.lam, q.dim, q.a, e->d_scalar + q.slot, e->stream);
+  TN_HIP(hipGetLastError());
   return TN_OK;
 }
 
-tn_status read_meas(tn_mps* h, int count, int* outcomes, double* probs, double* both,
-                    int* flags) {
-  Engine* e = h->eng;
-  if (count <= 0) return TN_OK;
-  if (outcomes)
-    TN_HIP(hipMemcpyAsync(outcomes, e->d_out, sizeof(int) * count, hipMemcpyDeviceToHost,
-                          e->stream));
-  if (probs)
-    TN_HIP(hipMemcpyAsync(probs, e->d_prob, sizeof(double) * count, hipMemcpyDeviceToHost,
-                          e->stream));
-  if (both)
-    TN_HIP(hipMemcpyAsync(both, e->d_probs, sizeof(double) * 2 * count,
-                          hipMemcpyDeviceToHost, e->stream));
-  if (flags)
-    TN_HIP(hipMemcpyAsync(flags, e->d_flag, sizeof(int) * count, hipMemcpyDeviceToHost,
-                          e->stream));
+tn_status refactor_sweep(tn_mps* h) {
+  // mps.rs:894-897
+  if (h->n < 2) return TN_OK;
+  for (size_t k = 0; k + 1 < h->n; ++k) TN_TRY(refactor(h, k));
+  for (size_t k = h->n - 2; k-- > 0;) TN_TRY(refactor(h, k));
+  return TN_OK;
+}
+
+tn_status read_results(tn_engine* e, int base, int count, int* outcomes, double* probs,
+                       double* both, int* flags) {
+  if (count > 0) {
+    if (outcomes)
+      TN_HIP(hipMemcpyAsync(outcomes, e->d_out + base, sizeof(int) * count,
+                            hipMemcpyDeviceToHost, e->stream));
+    if (probs)
+      TN_HIP(hipMemcpyAsync(probs, e->d_prob + base, sizeof(double) * count,
+                            hipMemcpyDeviceToHost, e->stream));
+    if (both)
+      TN_HIP(hipMemcpyAsync(both, e->d_probs + 2 * base, sizeof(double) * 2 * count,
+                            hipMemcpyDeviceToHost, e->stream));
+    if (flags)
+      TN_HIP(hipMemcpyAsync(flags, e->d_flag + base, sizeof(int) * count,
+                            hipMemcpyDeviceToHost, e->stream));
+  }
   TN_HIP(hipStreamSynchronize(e->stream));
   return TN_OK;
 }
 
-// apply_meas_prob_layer (mps.rs:2206-2252) without reading results back.
-// Results of the processed entries go to engine slots base .. base+count-1.
-tn_status meas_layer_enqueue(tn_mps* h, size_t m, const size_t* ks, const int* forced,
-                             const double* r, int base, std::vector<size_t>* out_ks) {
+// ------------------------------------------------------------ schedules
+// One step of a handle's schedule: a refactor of bond k, or a measurement.
+struct Op {
+  bool meas;
+  size_t k;
+  int forced;
+  double r;
+  int slot;
+};
+
+// The schedule of apply_meas_prob_layer (mps.rs:2206-2252): drop out-of-range
+// sites (:2212-2216), stable sort by site (:2218), dedup keeping the first
+// (:2219), refactor last..k before each measurement (:2225-2228), then the
+// trailing L->R range last..n-1 (:2246-2249) and the R->L sweep over
+// (0..n-2).rev() (:2250).  Result slots are assigned from `slot` upward.
+void meas_schedule(const tn_mps* h, size_t m, const size_t* ks, const int* forced,
+                   const double* r, int* slot, std::vector<Op>* ops, std::vector<size_t>* done) {
   struct M {
     size_t k;
     int f;
     double r;
-    size_t ord;
   };
   std::vector<M> ms;
   for (size_t i = 0; i < m; ++i) {
-    if (ks[i] >= h->n) continue;  // mps.rs:2212-2216
-    ms.push_back({ks[i], forced ? forced[i] : -1, r ? r[i] : 0.0, i});
+    if (ks[i] >= h->n) continue;
+    ms.push_back({ks[i], forced ? forced[i] : -1, r ? r[i] : 0.0});
   }
-  out_ks->clear();
-  if (ms.empty()) return TN_OK;  // mps.rs:2217
+  if (ms.empty()) return;
   std::stable_sort(ms.begin(), ms.end(), [](const M& a, const M& b) { return a.k < b.k; });
   std::vector<M> dd;
   for (const M& x : ms)
-    if (dd.empty() || dd.back().k != x.k) dd.push_back(x);  // dedup_by_key keeps the first
+    if (dd.empty() || dd.back().k != x.k) dd.push_back(x);
   bool have_last = false;
   size_t last = 0;
-  int slot = base;
   for (const M& x : dd) {
     if (have_last)
-      for (size_t j = last; j < x.k; ++j) {  // mps.rs:2225-2228
-        tn_status st = refactor(h, j);
-        if (st != TN_OK) return st;
-      }
-    tn_status st = enqueue_measure(h, x.k, x.f >= 0 ? x.f : -1, x.r, slot++);
-    if (st != TN_OK) return st;
-    out_ks->push_back(x.k);
+      for (size_t j = last; j < x.k; ++j) ops->push_back({false, j, 0, 0.0, 0});
+    ops->push_back({true, x.k, x.f >= 0 ? x.f : -1, x.r, (*slot)++});
+    done->push_back(x.k);
     have_last = true;
     last = x.k;
   }
   if (have_last)
-    for (size_t j = last; j + 1 < h->n; ++j) {  // mps.rs:2246-2249
-      tn_status st = refactor(h, j);
-      if (st != TN_OK) return st;
-    }
+    for (size_t j = last; j + 1 < h->n; ++j) ops->push_back({false, j, 0, 0.0, 0});
   if (h->n >= 2)
-    for (size_t j = h->n - 2; j-- > 0;) {  // mps.rs:2250
-      tn_status st = refactor(h, j);
-      if (st != TN_OK) return st;
-    }
-  return TN_OK;
+    for (size_t j = h->n - 2; j-- > 0;) ops->push_back({false, j, 0, 0.0, 0});
 }
 
-// Unitaries in order; consecutive two-qubit gates on bonds at least 2 apart
-// run as one batch (they touch disjoint sites and bonds).
-tn_status apply_unis_impl(tn_mps* h, size_t m, const size_t* ks, const int* dimsz,
-                          const double* ops) {
-  std::vector<UpdReq> batch;
-  auto flush = [&]() -> tn_status {
-    tn_status st = run_updates(h, batch);
-    batch.clear();
-    return st;
-  };
+// Gate groups of one handle: consecutive two-qubit gates on bonds at least 2
+// apart form one group (they touch disjoint sites and bonds); a one-qubit
+// gate is a group of its own.  Out-of-range gates are dropped (mps.rs:1301,
+// 1335).
+struct GateOp {
+  size_t k;
+  int dim;
+  const double* op;
+};
+
+tn_status gate_groups(const tn_mps* h, size_t m, const size_t* ks, const int* dimsz,
+                      const double* ops, std::vector<std::vector<GateOp>>* groups) {
   size_t off = 0;
+  std::vector<GateOp> cur;
   for (size_t i = 0; i < m; ++i) {
     const int dsz = dimsz ? dimsz[i] : 4;
     const double* op = ops + off;
     if (dsz == 2) {
       off += 8;
-      if (ks[i] >= h->n) continue;  // mps.rs:1301
-      tn_status st = flush();
-      if (st != TN_OK) return st;
-      launch_u1(h->gam + h->goff[ks[i]], h->dims + ks[i], op, 64, h->eng->stream);
-      TN_HIP(hipGetLastError());
+      if (ks[i] >= h->n) continue;
+      if (!cur.empty()) groups->push_back(cur), cur.clear();
+      groups->push_back({{ks[i], 2, op}});
     } else if (dsz == 4) {
       off += 32;
-      if (h->n == 1 || ks[i] >= h->n - 1) continue;  // mps.rs:1335
+      if (h->n == 1 || ks[i] >= h->n - 1) continue;
       bool conflict = false;
-      for (const UpdReq& b : batch)
+      for (const GateOp& b : cur)
         if ((b.k > ks[i] ? b.k - ks[i] : ks[i] - b.k) < 2) conflict = true;
-      if (conflict) {
-        tn_status st = flush();
-        if (st != TN_OK) return st;
-      }
-      UpdReq r;
-      r.k = ks[i];
-      r.has_gate = true;
-      std::memcpy(r.gate, op, sizeof(double) * 32);
-      batch.push_back(r);
+      if (conflict) groups->push_back(cur), cur.clear();
+      cur.push_back({ks[i], 4, op});
     } else {
-      tn_status st = flush();
-      if (st != TN_OK) return st;
       return TN_OPERATOR_INCOMPATIBLE_SHAPE;  // mps.rs:2104
     }
   }
-  return flush();
+  if (!cur.empty()) groups->push_back(cur);
+  return TN_OK;
+}
+
+// Advance several handles of one engine through their gate groups in lock
+// step: group j of every handle runs in one launch batch.
+tn_status exec_gates(tn_engine* e, const std::vector<tn_mps*>& hs,
+                     const std::vector<std::vector<std::vector<GateOp>>>& groups) {
+  size_t depth = 0;
+  for (const auto& g : groups) depth = std::max(depth, g.size());
+  for (size_t j = 0; j < depth; ++j) {
+    std::vector<Task> tasks;
+    for (size_t i = 0; i < hs.size(); ++i) {
+      if (j >= groups[i].size()) continue;
+      for (const GateOp& g : groups[i][j]) {
+        if (g.dim == 2) {
+          launch_u1(hs[i]->gam + hs[i]->goff[g.k], hs[i]->dims + g.k, g.op, 64, e->stream);
+          TN_HIP(hipGetLastError());
+        } else {
+          Task t;
+          t.h = hs[i];
+          t.k = g.k;
+          t.has_gate = true;
+          std::memcpy(t.gate, g.op, sizeof(double) * 32);
+          tasks.push_back(t);
+        }
+      }
+    }
+    TN_TRY(run_updates(e, tasks));
+  }
+  return TN_OK;
+}
+
+// Advance several handles through their measurement schedules in lock step.
+tn_status exec_ops(tn_engine* e, const std::vector<tn_mps*>& hs,
+                   const std::vector<std::vector<Op>>& ops) {
+  size_t depth = 0;
+  for (const auto& o : ops) depth = std::max(depth, o.size());
+  for (size_t j = 0; j < depth; ++j) {
+    std::vector<Task> tasks;
+    std::vector<MTask> ms;
+    for (size_t i = 0; i < hs.size(); ++i) {
+      if (j >= ops[i].size()) continue;
+      const Op& o = ops[i][j];
+      if (o.meas) {
+        ms.push_back({hs[i], o.k, o.forced, o.r, o.slot});
+      } else {
+        Task t;
+        t.h = hs[i];
+        t.k = o.k;
+        t.has_gate = false;
+        tasks.push_back(t);
+      }
+    }
+    TN_TRY(run_measures(e, ms));
+    TN_TRY(run_updates(e, tasks));
+  }
+  return TN_OK;
+}
+
+tn_status check_flags(const std::vector<int>& flags) {
+  for (int f : flags)
+    if (f & 2) return TN_SAMPLE_RANGE;
+  for (int f : flags)
+    if (f & 1) return TN_ZERO_PROB;
+  return TN_OK;
+}
+
+tn_status new_handle(tn_engine* e, size_t n, tn_trunc trunc, size_t chi_cap, tn_mps** out) {
+  size_t cap = chi_cap;
+  if (cap == 0) cap = trunc.kind == TN_TRUNC_CONST ? std::max<size_t>(trunc.chi, 1) : 256;
+  tn_mps* h = new tn_mps();
+  h->n = n;
+  h->trunc = trunc;
+  h->cap = cap;
+  h->bmax.resize(n > 1 ? n - 1 : 0);
+  size_t maxb = 1;
+  for (size_t b = 0; b + 1 < n; ++b) {
+    h->bmax[b] = std::min(cap, exact_bond_max(n, b));
+    maxb = std::max(maxb, h->bmax[b]);
+  }
+  if (maxb > e->chi_max) {
+    g_last_error = "bond allocation above the engine's chi_max";
+    delete h;
+    return TN_INVALID_ARG;
+  }
+  h->goff.resize(n);
+  size_t g = 0;
+  for (size_t k = 0; k < n; ++k) {
+    h->goff[k] = g;
+    g += h->bL(k) * 2 * h->bR(k);
+  }
+  h->gam_elems = g;
+  h->loff.resize(n > 1 ? n - 1 : 0);
+  size_t l = 0;
+  for (size_t b = 0; b + 1 < n; ++b) {
+    h->loff[b] = l;
+    l += h->bmax[b];
+  }
+  h->lam_elems = std::max<size_t>(l, 1);
+  h->ub.assign(n > 1 ? n - 1 : 0, 1);
+  h->eng = e;
+  e->users++;
+  auto fail = [&](hipError_t er) {
+    g_last_error = hipGetErrorString(er);
+    delete h;
+    return TN_DEVICE;
+  };
+  hipError_t er;
+  if ((er = hipMalloc(&h->gam, sizeof(double2) * h->gam_elems)) != hipSuccess) return fail(er);
+  if ((er = hipMalloc(&h->lam, sizeof(double) * h->lam_elems)) != hipSuccess) return fail(er);
+  if ((er = hipMalloc(&h->dims, sizeof(int) * (n + 1))) != hipSuccess) return fail(er);
+  // |0...0>: Gamma[k] = (1,2,1) with [0,0,0] = 1, lambda = [1] (mps.rs:378-388)
+  std::vector<double2> g0(h->gam_elems, make_double2(0.0, 0.0));
+  for (size_t k = 0; k < n; ++k) g0[h->goff[k]] = make_double2(1.0, 0.0);
+  std::vector<double> l0(h->lam_elems, 0.0);
+  for (size_t b = 0; b + 1 < n; ++b) l0[h->loff[b]] = 1.0;
+  std::vector<int> d0(n + 1, 1);
+  if ((er = hipMemcpyAsync(h->gam, g0.data(), sizeof(double2) * h->gam_elems,
+                           hipMemcpyHostToDevice, e->stream)) != hipSuccess)
+    return fail(er);
+  if ((er = hipMemcpyAsync(h->lam, l0.data(), sizeof(double) * h->lam_elems,
+                           hipMemcpyHostToDevice, e->stream)) != hipSuccess)
+    return fail(er);
+  if ((er = hipMemcpyAsync(h->dims, d0.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice,
+                           e->stream)) != hipSuccess)
+    return fail(er);
+  if ((er = hipStreamSynchronize(e->stream)) != hipSuccess) return fail(er);
+  *out = h;
+  return TN_OK;
+}
+
+tn_status check_trunc(tn_trunc t) {
+  return (t.kind < 0 || t.kind > 2) ? TN_INVALID_ARG : TN_OK;
+}
+
+tn_status check_meas_args(size_t m, const int* forced, const double* r) {
+  for (size_t i = 0; i < m; ++i) {
+    if (forced && (forced[i] < -1 || forced[i] > 1)) return TN_INVALID_ARG;
+    if ((!forced || forced[i] < 0) && !r) return TN_INVALID_ARG;
+  }
+  return TN_OK;
 }
 
 }  // namespace
@@ -578,84 +827,68 @@ int tn_device_count(void) {
   return c;
 }
 
+tn_status tn_engine_new(size_t chi_max, void* stream, tn_engine** out) {
+  if (!out) return TN_INVALID_ARG;
+  *out = nullptr;
+  if (chi_max == 0 || 2 * chi_max > (size_t)kMaxCols) {
+    // the singular-value sort of k_finish holds at most kMaxCols values
+    g_last_error = "chi_max must be in 1..512";
+    return TN_INVALID_ARG;
+  }
+  if (tn_device_count() <= 0) {
+    g_last_error = "no HIP device visible";
+    return TN_DEVICE;
+  }
+  tn_engine* e = new tn_engine();
+  tn_status st = e->init(stream, chi_max);
+  if (st != TN_OK) {
+    delete e;
+    return st;
+  }
+  *out = e;
+  return TN_OK;
+}
+
+void tn_engine_free(tn_engine* e) {
+  if (!e) return;
+  if (e->users > 0) {
+    e->auto_free = true;  // the last handle frees it
+    return;
+  }
+  delete e;
+}
+
+tn_status tn_mps_new_qubits_in(tn_engine* e, size_t n, tn_trunc trunc, size_t chi_cap,
+                               tn_mps** out) {
+  if (!e || !out) return TN_INVALID_ARG;
+  *out = nullptr;
+  if (n == 0) return TN_EMPTY_SYSTEM;  // mps.rs:375
+  TN_TRY(check_trunc(trunc));
+  return new_handle(e, n, trunc, chi_cap, out);
+}
+
 tn_status tn_mps_new_qubits(size_t n, tn_trunc trunc, size_t chi_cap, void* stream,
                             tn_mps** out) {
   if (!out) return TN_INVALID_ARG;
   *out = nullptr;
   if (n == 0) return TN_EMPTY_SYSTEM;  // mps.rs:375
-  if (trunc.kind < 0 || trunc.kind > 2) return TN_INVALID_ARG;
+  TN_TRY(check_trunc(trunc));
   if (tn_device_count() <= 0) {
     g_last_error = "no HIP device visible";
     return TN_DEVICE;
   }
-  int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess) dev = 0;
   size_t cap = chi_cap;
   if (cap == 0) cap = trunc.kind == TN_TRUNC_CONST ? std::max<size_t>(trunc.chi, 1) : 256;
-  tn_mps* h = new tn_mps();
-  h->n = n;
-  h->trunc = trunc;
-  h->cap = cap;
-  h->bmax.resize(n > 1 ? n - 1 : 0);
   size_t maxb = 1;
-  for (size_t b = 0; b + 1 < n; ++b) {
-    h->bmax[b] = std::min(cap, exact_bond_max(n, b));
-    maxb = std::max(maxb, h->bmax[b]);
-  }
-  if (2 * maxb > (size_t)kMaxCols) {
-    // the singular-value sort of k_finish holds at most kMaxCols values
-    g_last_error = "bond allocation above 512 (theta wider than 1024 columns)";
-    delete h;
-    return TN_INVALID_ARG;
-  }
-  h->goff.resize(n);
-  size_t g = 0;
-  for (size_t k = 0; k < n; ++k) {
-    h->goff[k] = g;
-    g += h->bL(k) * 2 * h->bR(k);
-  }
-  h->gam_elems = g;
-  h->loff.resize(n > 1 ? n - 1 : 0);
-  size_t l = 0;
-  for (size_t b = 0; b + 1 < n; ++b) {
-    h->loff[b] = l;
-    l += h->bmax[b];
-  }
-  h->lam_elems = std::max<size_t>(l, 1);
-  h->ub.assign(n > 1 ? n - 1 : 0, 1);
-  h->eng = new Engine();
-  tn_status st = h->eng->init(dev, stream, (int)(2 * maxb), (int)(2 * maxb));
+  for (size_t b = 0; b + 1 < n; ++b) maxb = std::max(maxb, std::min(cap, exact_bond_max(n, b)));
+  tn_engine* e = nullptr;
+  TN_TRY(tn_engine_new(maxb, stream, &e));
+  tn_status st = new_handle(e, n, trunc, chi_cap, out);
   if (st != TN_OK) {
-    delete h;
+    delete e;
     return st;
   }
-  auto fail = [&](hipError_t e) {
-    g_last_error = hipGetErrorString(e);
-    delete h;
-    return TN_DEVICE;
-  };
-  hipError_t e;
-  if ((e = hipMalloc(&h->gam, sizeof(double2) * h->gam_elems)) != hipSuccess) return fail(e);
-  if ((e = hipMalloc(&h->lam, sizeof(double) * h->lam_elems)) != hipSuccess) return fail(e);
-  if ((e = hipMalloc(&h->dims, sizeof(int) * (n + 1))) != hipSuccess) return fail(e);
-  // |0...0>: Gamma[k] = (1,2,1) with [0,0,0] = 1, lambda = [1] (mps.rs:378-388)
-  std::vector<double2> g0(h->gam_elems, make_double2(0.0, 0.0));
-  for (size_t k = 0; k < n; ++k) g0[h->goff[k]] = make_double2(1.0, 0.0);
-  std::vector<double> l0(h->lam_elems, 0.0);
-  for (size_t b = 0; b + 1 < n; ++b) l0[h->loff[b]] = 1.0;
-  std::vector<int> d0(n + 1, 1);
-  Engine* en = h->eng;
-  if ((e = hipMemcpyAsync(h->gam, g0.data(), sizeof(double2) * h->gam_elems,
-                          hipMemcpyHostToDevice, en->stream)) != hipSuccess)
-    return fail(e);
-  if ((e = hipMemcpyAsync(h->lam, l0.data(), sizeof(double) * h->lam_elems,
-                          hipMemcpyHostToDevice, en->stream)) != hipSuccess)
-    return fail(e);
-  if ((e = hipMemcpyAsync(h->dims, d0.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice,
-                          en->stream)) != hipSuccess)
-    return fail(e);
-  if ((e = hipStreamSynchronize(en->stream)) != hipSuccess) return fail(e);
-  *out = h;
+  e->auto_free = true;
   return TN_OK;
 }
 
@@ -663,15 +896,20 @@ void tn_mps_free(tn_mps* h) { delete h; }
 
 tn_status tn_mps_clone(const tn_mps* h, tn_mps** out) {
   if (!h || !out) return TN_INVALID_ARG;
-  tn_status st = tn_mps_new_qubits(h->n, h->trunc, h->cap,
-                                   h->eng->own_stream ? nullptr : (void*)h->eng->stream, out);
-  if (st != TN_OK) return st;
-  tn_mps* c = *out;
+  tn_mps* c = nullptr;
+  if (h->eng->auto_free) {
+    TN_TRY(tn_mps_new_qubits(h->n, h->trunc, h->cap,
+                             h->eng->own_stream ? nullptr : (void*)h->eng->stream, &c));
+  } else {
+    TN_TRY(tn_mps_new_qubits_in(h->eng, h->n, h->trunc, h->cap, &c));
+  }
+  c->prec = h->prec;
   TN_HIP(hipStreamSynchronize(h->eng->stream));
   TN_HIP(hipMemcpy(c->gam, h->gam, sizeof(double2) * h->gam_elems, hipMemcpyDeviceToDevice));
   TN_HIP(hipMemcpy(c->lam, h->lam, sizeof(double) * h->lam_elems, hipMemcpyDeviceToDevice));
   TN_HIP(hipMemcpy(c->dims, h->dims, sizeof(int) * (h->n + 1), hipMemcpyDeviceToDevice));
   c->ub = h->ub;
+  *out = c;
   return TN_OK;
 }
 
@@ -684,8 +922,15 @@ tn_status tn_get_trunc(const tn_mps* h, tn_trunc* out) {
 }
 
 tn_status tn_set_trunc(tn_mps* h, tn_trunc trunc) {
-  if (!h || trunc.kind < 0 || trunc.kind > 2) return TN_INVALID_ARG;
+  if (!h) return TN_INVALID_ARG;
+  TN_TRY(check_trunc(trunc));
   h->trunc = trunc;
+  return TN_OK;
+}
+
+tn_status tn_set_precision(tn_mps* h, int prec) {
+  if (!h || (prec != TN_PREC_F64 && prec != TN_PREC_F32)) return TN_INVALID_ARG;
+  h->prec = prec;
   return TN_OK;
 }
 
@@ -695,38 +940,39 @@ tn_status tn_sync(tn_mps* h) {
   return TN_OK;
 }
 
+tn_status tn_apply_unis(tn_mps* h, size_t m, const size_t* ks, const int* dims,
+                        const double* ops) {
+  if (!h || (m && (!ks || !ops))) return TN_INVALID_ARG;
+  std::vector<std::vector<std::vector<GateOp>>> groups(1);
+  TN_TRY(gate_groups(h, m, ks, dims, ops, &groups[0]));
+  return exec_gates(h->eng, {h}, groups);
+}
+
 tn_status tn_apply_u1(tn_mps* h, size_t k, const double* op) {
   if (!h || !op) return TN_INVALID_ARG;
   int d = 2;
-  return apply_unis_impl(h, 1, &k, &d, op);
+  return tn_apply_unis(h, 1, &k, &d, op);
 }
 
 tn_status tn_apply_u2(tn_mps* h, size_t k, const double* op) {
   if (!h || !op) return TN_INVALID_ARG;
   int d = 4;
-  return apply_unis_impl(h, 1, &k, &d, op);
-}
-
-tn_status tn_apply_unis(tn_mps* h, size_t m, const size_t* ks, const int* dims,
-                        const double* ops) {
-  if (!h || (m && (!ks || !ops))) return TN_INVALID_ARG;
-  return apply_unis_impl(h, m, ks, dims, ops);
+  return tn_apply_unis(h, 1, &k, &d, op);
 }
 
 tn_status tn_probs(tn_mps* h, size_t k, double* out) {
   if (!h || !out) return TN_INVALID_ARG;
   if (k >= h->n) return TN_NONE;  // mps.rs:1142
-  tn_status st = enqueue_measure(h, k, -2, 0.0, 0);
-  if (st != TN_OK) return st;
-  return read_meas(h, 1, nullptr, nullptr, out, nullptr);
+  TN_TRY(h->eng->ensure_results(1));
+  TN_TRY(run_measures(h->eng, {{h, k, -2, 0.0, 0}}));
+  return read_results(h->eng, 0, 1, nullptr, nullptr, out, nullptr);
 }
 
 tn_status tn_prob(tn_mps* h, size_t k, size_t p, double* out) {
   if (!h || !out) return TN_INVALID_ARG;
   if (k >= h->n || p >= 2) return TN_NONE;  // mps.rs:1072
   double both[2];
-  tn_status st = tn_probs(h, k, both);
-  if (st != TN_OK) return st;
+  TN_TRY(tn_probs(h, k, both));
   *out = both[p];
   return TN_OK;
 }
@@ -734,66 +980,52 @@ tn_status tn_prob(tn_mps* h, size_t k, size_t p, double* out) {
 tn_status tn_measure_prob(tn_mps* h, size_t k, double r, int* outcome, double* prob) {
   if (!h) return TN_INVALID_ARG;
   if (k >= h->n) return TN_NONE;  // mps.rs:1483
-  tn_status st = enqueue_measure(h, k, -1, r, 0);
-  if (st != TN_OK) return st;
-  st = refactor_sweep(h);
-  if (st != TN_OK) return st;
+  TN_TRY(h->eng->ensure_results(1));
+  TN_TRY(run_measures(h->eng, {{h, k, -1, r, 0}}));
+  TN_TRY(refactor_sweep(h));
   int o = 0, f = 0;
   double pr = 0.0;
-  st = read_meas(h, 1, &o, &pr, nullptr, &f);
-  if (st != TN_OK) return st;
+  TN_TRY(read_results(h->eng, 0, 1, &o, &pr, nullptr, &f));
   if (outcome) *outcome = o;
   if (prob) *prob = pr;
-  if (f & 2) return TN_SAMPLE_RANGE;
-  return TN_OK;
+  return (f & 2) ? TN_SAMPLE_RANGE : TN_OK;
 }
 
 tn_status tn_measure_postsel_prob(tn_mps* h, size_t k, size_t p, double* prob) {
   if (!h) return TN_INVALID_ARG;
   if (k >= h->n || p >= 2) return TN_NONE;  // mps.rs:1504
-  tn_status st = enqueue_measure(h, k, (int)p, 0.0, 0);
-  if (st != TN_OK) return st;
-  st = refactor_sweep(h);
-  if (st != TN_OK) return st;
+  TN_TRY(h->eng->ensure_results(1));
+  TN_TRY(run_measures(h->eng, {{h, k, (int)p, 0.0, 0}}));
+  TN_TRY(refactor_sweep(h));
   int f = 0;
   double pr = 0.0;
-  st = read_meas(h, 1, nullptr, &pr, nullptr, &f);
-  if (st != TN_OK) return st;
+  TN_TRY(read_results(h->eng, 0, 1, nullptr, &pr, nullptr, &f));
   if (prob) *prob = pr;
-  if (f & 1) return TN_ZERO_PROB;
-  return TN_OK;
+  return (f & 1) ? TN_ZERO_PROB : TN_OK;
 }
 
 tn_status tn_meas_layer(tn_mps* h, size_t m, const size_t* ks, const int* forced,
                         const double* r, size_t* out_count, size_t* out_ks,
                         int* out_outcomes, double* out_probs) {
   if (!h || (m && !ks)) return TN_INVALID_ARG;
-  for (size_t i = 0; i < m; ++i) {
-    if (forced && (forced[i] < -1 || forced[i] > 1)) return TN_INVALID_ARG;
-    if ((!forced || forced[i] < 0) && !r) return TN_INVALID_ARG;
-  }
-  tn_status st = h->eng->ensure_mslots((int)std::max<size_t>(m, 1));
-  if (st != TN_OK) return st;
+  TN_TRY(check_meas_args(m, forced, r));
+  TN_TRY(h->eng->ensure_results((int)std::max<size_t>(m, 1)));
+  std::vector<std::vector<Op>> ops(1);
   std::vector<size_t> done;
-  st = meas_layer_enqueue(h, m, ks, forced, r, 0, &done);
-  if (st != TN_OK) return st;
+  int slot = 0;
+  meas_schedule(h, m, ks, forced, r, &slot, &ops[0], &done);
+  TN_TRY(exec_ops(h->eng, {h}, ops));
   const int cnt = (int)done.size();
-  std::vector<int> flags(cnt);
-  std::vector<int> outs(cnt);
+  std::vector<int> flags(cnt), outs(cnt);
   std::vector<double> probs(cnt);
-  st = read_meas(h, cnt, outs.data(), probs.data(), nullptr, flags.data());
-  if (st != TN_OK) return st;
+  TN_TRY(read_results(h->eng, 0, cnt, outs.data(), probs.data(), nullptr, flags.data()));
   if (out_count) *out_count = done.size();
   for (int i = 0; i < cnt; ++i) {
     if (out_ks) out_ks[i] = done[i];
     if (out_outcomes) out_outcomes[i] = outs[i];
     if (out_probs) out_probs[i] = probs[i];
   }
-  for (int i = 0; i < cnt; ++i)
-    if (flags[i] & 2) return TN_SAMPLE_RANGE;
-  for (int i = 0; i < cnt; ++i)
-    if (flags[i] & 1) return TN_ZERO_PROB;
-  return TN_OK;
+  return check_flags(flags);
 }
 
 tn_status tn_apply_bilayer(tn_mps* h, size_t nu, const size_t* uks, const int* udims,
@@ -801,17 +1033,15 @@ tn_status tn_apply_bilayer(tn_mps* h, size_t nu, const size_t* uks, const int* u
                            const double* r, size_t* out_count, size_t* out_ks,
                            int* out_outcomes, double* out_probs) {
   if (!h) return TN_INVALID_ARG;
-  tn_status st = tn_apply_unis(h, nu, uks, udims, ops);
-  if (st != TN_OK) return st;
+  TN_TRY(tn_apply_unis(h, nu, uks, udims, ops));
   return tn_meas_layer(h, nm, mks, forced, r, out_count, out_ks, out_outcomes, out_probs);
 }
 
 tn_status tn_entropy_ry_schmidt(tn_mps* h, double a, size_t b, double* out) {
   if (!h || !out) return TN_INVALID_ARG;
   if (h->n < 2 || b >= h->n - 1) return TN_NONE;  // mps.rs:1252
-  Engine* e = h->eng;
-  tn_status st = e->ensure_scalars(1);
-  if (st != TN_OK) return st;
+  tn_engine* e = h->eng;
+  TN_TRY(e->ensure_scalars(1));
   launch_entropy(h->lam + h->loff[b], h->dims + b + 1, a, e->d_scalar, e->stream);
   TN_HIP(hipGetLastError());
   TN_HIP(hipMemcpyAsync(out, e->d_scalar, sizeof(double), hipMemcpyDeviceToHost, e->stream));
@@ -838,8 +1068,7 @@ tn_status tn_max_bond_dim(tn_mps* h, size_t* out) {
   if (!h || !out) return TN_INVALID_ARG;
   if (h->n < 2) return TN_NONE;
   std::vector<size_t> d(h->n - 1);
-  tn_status st = tn_bond_dims(h, d.data());
-  if (st != TN_OK) return st;
+  TN_TRY(tn_bond_dims(h, d.data()));
   *out = *std::max_element(d.begin(), d.end());
   return TN_OK;
 }
@@ -848,13 +1077,11 @@ tn_status tn_schmidt(tn_mps* h, size_t b, double* buf, size_t cap, size_t* len) 
   if (!h || !len) return TN_INVALID_ARG;
   if (h->n < 2 || b >= h->n - 1) return TN_NONE;
   std::vector<size_t> d(h->n - 1);
-  tn_status st = tn_bond_dims(h, d.data());
-  if (st != TN_OK) return st;
+  TN_TRY(tn_bond_dims(h, d.data()));
   *len = d[b];
   const size_t cnt = std::min(cap, d[b]);
-  if (cnt && buf) {
+  if (cnt && buf)
     TN_HIP(hipMemcpy(buf, h->lam + h->loff[b], sizeof(double) * cnt, hipMemcpyDeviceToHost));
-  }
   return TN_OK;
 }
 
@@ -880,46 +1107,102 @@ tn_status tn_run_circuit(tn_mps* h, size_t depth, const size_t* nu, const size_t
                          const int* forced, const double* r, size_t ent_bond, double* ent_out,
                          size_t* out_counts, int* out_outcomes, double* out_probs) {
   if (!h) return TN_INVALID_ARG;
-  Engine* e = h->eng;
+  tn_engine* e = h->eng;
   size_t total_m = 0;
   for (size_t t = 0; t < depth; ++t) total_m += nm[t];
-  tn_status st = e->ensure_mslots((int)std::max<size_t>(total_m, 1));
-  if (st != TN_OK) return st;
-  st = e->ensure_scalars((int)depth + 1);
-  if (st != TN_OK) return st;
+  TN_TRY(check_meas_args(total_m, forced, r));
+  TN_TRY(e->ensure_results((int)std::max<size_t>(total_m, 1)));
+  TN_TRY(e->ensure_scalars((int)depth + 1));
   const bool want_ent = ent_out && h->n >= 2 && ent_bond < h->n - 1;
-  if (want_ent)
-    launch_entropy(h->lam + h->loff[ent_bond], h->dims + ent_bond + 1, 1.0, e->d_scalar,
-                   e->stream);
+  const double* lamb = want_ent ? h->lam + h->loff[ent_bond] : nullptr;
+  const int* dimb = want_ent ? h->dims + ent_bond + 1 : nullptr;
+  if (want_ent) TN_TRY(run_entropies(e, {{lamb, dimb, 1.0, 0}}));
   size_t uo = 0, mo = 0;
   int slot = 0;
-  std::vector<size_t> done;
   for (size_t t = 0; t < depth; ++t) {
     std::vector<int> dsz(nu[t], 4);
-    st = apply_unis_impl(h, nu[t], uks + uo, dsz.data(), ops + 32 * uo);
-    if (st != TN_OK) return st;
+    std::vector<std::vector<std::vector<GateOp>>> groups(1);
+    TN_TRY(gate_groups(h, nu[t], uks + uo, dsz.data(), ops + 32 * uo, &groups[0]));
+    TN_TRY(exec_gates(e, {h}, groups));
     uo += nu[t];
-    st = meas_layer_enqueue(h, nm[t], mks + mo, forced ? forced + mo : nullptr,
-                            r ? r + mo : nullptr, slot, &done);
-    if (st != TN_OK) return st;
+    std::vector<std::vector<Op>> mops(1);
+    std::vector<size_t> done;
+    meas_schedule(h, nm[t], mks + mo, forced ? forced + mo : nullptr, r ? r + mo : nullptr,
+                  &slot, &mops[0], &done);
+    TN_TRY(exec_ops(e, {h}, mops));
     mo += nm[t];
     if (out_counts) out_counts[t] = done.size();
-    slot += (int)done.size();
-    if (want_ent)
-      launch_entropy(h->lam + h->loff[ent_bond], h->dims + ent_bond + 1, 1.0,
-                     e->d_scalar + t + 1, e->stream);
+    if (want_ent) TN_TRY(run_entropies(e, {{lamb, dimb, 1.0, (int)t + 1}}));
   }
-  TN_HIP(hipGetLastError());
   std::vector<int> flags(slot);
-  st = read_meas(h, slot, out_outcomes, out_probs, nullptr, flags.data());
-  if (st != TN_OK) return st;
-  if (want_ent) {
+  TN_TRY(read_results(e, 0, slot, out_outcomes, out_probs, nullptr, flags.data()));
+  if (want_ent)
     TN_HIP(hipMemcpy(ent_out, e->d_scalar, sizeof(double) * (depth + 1), hipMemcpyDeviceToHost));
+  return check_flags(flags);
+}
+
+tn_status tn_batch_bilayer(tn_engine* e, size_t count, tn_mps* const* hs, const size_t* nu,
+                           const size_t* uks, const double* ops, const size_t* nm,
+                           const size_t* mks, const int* forced, const double* r,
+                           const size_t* ent_bonds, double* ent_out, size_t* out_counts,
+                           int* out_outcomes, double* out_probs) {
+  if (!e || (count && (!hs || !nu || !nm))) return TN_INVALID_ARG;
+  std::vector<tn_mps*> hv(hs, hs + count);
+  for (tn_mps* h : hv)
+    if (!h || h->eng != e) return TN_INVALID_ARG;
+  size_t total_m = 0;
+  for (size_t i = 0; i < count; ++i) total_m += nm[i];
+  TN_TRY(check_meas_args(total_m, forced, r));
+  TN_TRY(e->ensure_results((int)std::max<size_t>(total_m, 1)));
+  TN_TRY(e->ensure_scalars((int)std::max<size_t>(count, 1)));
+  // gates: group j of every handle in one launch batch
+  std::vector<std::vector<std::vector<GateOp>>> groups(count);
+  size_t uo = 0;
+  for (size_t i = 0; i < count; ++i) {
+    std::vector<int> dsz(nu[i], 4);
+    TN_TRY(gate_groups(hv[i], nu[i], uks + uo, dsz.data(), ops + 32 * uo, &groups[i]));
+    uo += nu[i];
   }
-  for (int i = 0; i < slot; ++i)
-    if (flags[i] & 2) return TN_SAMPLE_RANGE;
-  for (int i = 0; i < slot; ++i)
-    if (flags[i] & 1) return TN_ZERO_PROB;
+  TN_TRY(exec_gates(e, hv, groups));
+  // measurement schedules in lock step
+  std::vector<std::vector<Op>> mops(count);
+  std::vector<size_t> counts(count);
+  size_t mo = 0;
+  int slot = 0;
+  for (size_t i = 0; i < count; ++i) {
+    std::vector<size_t> done;
+    meas_schedule(hv[i], nm[i], mks + mo, forced ? forced + mo : nullptr, r ? r + mo : nullptr,
+                  &slot, &mops[i], &done);
+    counts[i] = done.size();
+    mo += nm[i];
+  }
+  TN_TRY(exec_ops(e, hv, mops));
+  std::vector<EntReq> es;
+  if (ent_out && ent_bonds) {
+    for (size_t i = 0; i < count; ++i) {
+      tn_mps* h = hv[i];
+      if (h->n >= 2 && ent_bonds[i] < h->n - 1)
+        es.push_back({h->lam + h->loff[ent_bonds[i]], h->dims + ent_bonds[i] + 1, 1.0, (int)i});
+    }
+    TN_TRY(run_entropies(e, es));
+  }
+  std::vector<int> flags(slot);
+  TN_TRY(read_results(e, 0, slot, out_outcomes, out_probs, nullptr, flags.data()));
+  if (out_counts)
+    for (size_t i = 0; i < count; ++i) out_counts[i] = counts[i];
+  if (ent_out && ent_bonds) {
+    std::vector<double> tmp(count, 0.0);
+    TN_HIP(hipMemcpy(tmp.data(), e->d_scalar, sizeof(double) * count, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < count; ++i)
+      ent_out[i] = (hv[i]->n >= 2 && ent_bonds[i] < hv[i]->n - 1) ? tmp[i] : NAN;
+  }
+  return check_flags(flags);
+}
+
+tn_status tn_stats(tn_mps* h, size_t* updates, size_t* sweeps) {
+  if (!h) return TN_INVALID_ARG;
+  if (updates) *updates = (size_t)h->eng->updates;
+  if (sweeps) *sweeps = (size_t)h->eng->sweeps;
   return TN_OK;
 }
 
@@ -939,13 +1222,6 @@ tn_status tn_get_perf(tn_mps* h, tn_perf* out) {
 tn_status tn_reset_perf(tn_mps* h) {
   if (!h) return TN_INVALID_ARG;
   h->eng->perf = tn_perf{};
-  return TN_OK;
-}
-
-tn_status tn_stats(tn_mps* h, size_t* updates, size_t* sweeps) {
-  if (!h) return TN_INVALID_ARG;
-  if (updates) *updates = (size_t)h->eng->updates;
-  if (sweeps) *sweeps = (size_t)h->eng->sweeps;
   return TN_OK;
 }
 

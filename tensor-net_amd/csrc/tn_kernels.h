@@ -10,7 +10,14 @@ void launch_theta(const UpdDesc* descs, int nt, int gx, hipStream_t s);
 void launch_winit(const UpdDesc* descs, int nt, int gx, hipStream_t s);
 void launch_jacobi_step(const UpdDesc* descs, int nt, int npairs, int sweep, int step,
                         double tol_rot, double conv_tol, hipStream_t s);
-void launch_finish(const UpdDesc* descs, int nt, int sweeps_run, hipStream_t s);
+hipError_t launch_jacobi_coop(const UpdDesc* descs, int nt, int npairs, int steps,
+                              int max_sweeps, double tol_rot, double conv_tol, int* sweeps_out,
+                              int grid, hipStream_t s);
+// Largest co-resident grid of k_jacobi_coop on the current device (0 when
+// cooperative launch is unsupported).
+int jacobi_coop_max_blocks();
+void launch_finish(const UpdDesc* descs, int nt, const int* sweeps_dev, int sweeps_run,
+                   hipStream_t s);
 void launch_split(const UpdDesc* descs, int nt, hipStream_t s);
 void launch_renorm(const UpdDesc* descs, int nt, hipStream_t s);
 void launch_measure(const MeasDesc* descs, int nm, hipStream_t s);

@@ -23,7 +23,11 @@
 #include "tn_device.h"
 #include "tn_kernels.h"
 
+#include <hip/hip_cooperative_groups.h>
+
 #include <cfloat>
+
+namespace cg = cooperative_groups;
 
 namespace tnmps {
 
@@ -77,6 +81,67 @@ __device__ double block_max(double v, double* buf) {
 }
 
 // ---------------------------------------------------------------- theta
+template <typename C2, typename R>
+__device__ __forceinline__ C2 tc(double2 a) {
+  C2 r;
+  r.x = (R)a.x;
+  r.y = (R)a.y;
+  return r;
+}
+template <typename C2, typename R>
+__device__ __forceinline__ C2 tmad(C2 acc, C2 a, C2 b) {
+  acc.x += a.x * b.x - a.y * b.y;
+  acc.y += a.x * b.y + a.y * b.x;
+  return acc;
+}
+
+// All four (s,t) values of theta'[v,(s,t),w] in arithmetic C2/R (FP64 or
+// FP32 with FP32 accumulation); the result is widened to double.
+template <typename C2, typename R>
+__device__ __forceinline__ void theta_point(const UpdDesc& d, int v, int w, int chiM, int chiR,
+                                            double2 out[4]) {
+  C2 acc[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) acc[i].x = acc[i].y = (R)0;
+  const double2* __restrict__ a0 = d.gk + (size_t)(v * 2 + 0) * chiM;
+  const double2* __restrict__ a1 = d.gk + (size_t)(v * 2 + 1) * chiM;
+  for (int u = 0; u < chiM; ++u) {
+    const R lm = (R)d.lamM[u];
+    C2 b0 = tc<C2, R>(d.gk1[(size_t)(u * 2 + 0) * chiR + w]);
+    C2 b1 = tc<C2, R>(d.gk1[(size_t)(u * 2 + 1) * chiR + w]);
+    b0.x *= lm; b0.y *= lm;
+    b1.x *= lm; b1.y *= lm;
+    const C2 x0 = tc<C2, R>(a0[u]), x1 = tc<C2, R>(a1[u]);
+    acc[0] = tmad<C2, R>(acc[0], x0, b0);  // s'=0 t'=0
+    acc[1] = tmad<C2, R>(acc[1], x0, b1);  // s'=0 t'=1
+    acc[2] = tmad<C2, R>(acc[2], x1, b0);  // s'=1 t'=0
+    acc[3] = tmad<C2, R>(acc[3], x1, b1);  // s'=1 t'=1
+  }
+  if (d.lamL) {
+    const R ll = (R)d.lamL[v];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) { acc[i].x *= ll; acc[i].y *= ll; }
+  }
+  if (d.lamR) {
+    const R lr = (R)d.lamR[w];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) { acc[i].x *= lr; acc[i].y *= lr; }
+  }
+  if (d.has_gate) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      C2 o;
+      o.x = o.y = (R)0;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) o = tmad<C2, R>(o, tc<C2, R>(d.gate[i * 4 + j]), acc[j]);
+      out[i] = make_double2((double)o.x, (double)o.y);
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) out[i] = make_double2((double)acc[i].x, (double)acc[i].y);
+  }
+}
+
 // theta'[v,(s,t),w] = sum_{s't'} U[(s,t),(s',t')] lamL_v lamR_w
 //                     sum_u Gk[v,s',u] lamM_u Gk1[u,t',w]
 // written into X as theta (orient 0, m >= n) or theta^H (orient 1).
@@ -107,41 +172,11 @@ __global__ __launch_bounds__(256) void k_theta(const UpdDesc* __restrict__ descs
     const int v = (tile / nwt) * 4 + tv;
     const int w = (tile % nwt) * 64 + tw;
     if (v >= chiL || w >= chiR) continue;
-    double2 acc[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) acc[i] = make_double2(0.0, 0.0);
-    const double2* __restrict__ a0 = d.gk + (size_t)(v * 2 + 0) * chiM;
-    const double2* __restrict__ a1 = d.gk + (size_t)(v * 2 + 1) * chiM;
-    for (int u = 0; u < chiM; ++u) {
-      const double lm = d.lamM[u];
-      const double2 b0 = cscale(d.gk1[(size_t)(u * 2 + 0) * chiR + w], lm);
-      const double2 b1 = cscale(d.gk1[(size_t)(u * 2 + 1) * chiR + w], lm);
-      const double2 x0 = a0[u], x1 = a1[u];
-      acc[0] = cadd(acc[0], cmul(x0, b0));  // s'=0 t'=0
-      acc[1] = cadd(acc[1], cmul(x0, b1));  // s'=0 t'=1
-      acc[2] = cadd(acc[2], cmul(x1, b0));  // s'=1 t'=0
-      acc[3] = cadd(acc[3], cmul(x1, b1));  // s'=1 t'=1
-    }
-    const double ll = d.lamL ? d.lamL[v] : 1.0;
-    const double lr = d.lamR ? d.lamR[w] : 1.0;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      if (d.lamL) acc[i] = cscale(acc[i], ll);
-      if (d.lamR) acc[i] = cscale(acc[i], lr);
-    }
     double2 out[4];
-    if (d.has_gate) {
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        double2 o = make_double2(0.0, 0.0);
-#pragma unroll
-        for (int j = 0; j < 4; ++j) o = cadd(o, cmul(d.gate[i * 4 + j], acc[j]));
-        out[i] = o;
-      }
-    } else {
-#pragma unroll
-      for (int i = 0; i < 4; ++i) out[i] = acc[i];
-    }
+    if (d.prec == 1)
+      theta_point<float2, float>(d, v, w, chiM, chiR, out);
+    else
+      theta_point<double2, double>(d, v, w, chiM, chiR, out);
 #pragma unroll
     for (int st = 0; st < 4; ++st) {
       const int s = st >> 1, t = st & 1;
@@ -168,92 +203,135 @@ __global__ __launch_bounds__(256) void k_winit(const UpdDesc* __restrict__ descs
 }
 
 // ---------------------------------------------------------- Jacobi step
-// Block one-sided Jacobi (Hestenes) over column blocks of width kBW.  One
-// launch is one step of the round-robin ("circle") ordering of the nbe
-// blocks (nbe = number of blocks rounded up to even; a ghost block holds no
-// columns).  Block (p, task) owns the pair of column blocks (a, b): it forms
-// the panel Gram matrix G = P^H P (kPW x kPW), diagonalises it with cyclic
-// two-sided Jacobi rotations (G <- Q^H G Q, Wl <- Wl Q), and applies Wl to the
-// panel of X and of W.  The largest relative off-diagonal |G_ij| /
-// sqrt(G_ii G_jj) seen in the sweep goes to offs[sweep]; a task whose
-// previous sweep stayed below conv_tol has converged and does nothing.
+// Block one-sided Jacobi (Hestenes) over column blocks of width kBW.  A sweep
+// visits every pair of blocks once, in the round-robin ("circle") ordering of
+// the nbe blocks (nbe = number of blocks rounded up to even; a ghost block
+// holds no columns); one step holds nbe/2 disjoint pairs.  For its pair (a, b)
+// a workgroup forms the panel Gram matrix G = P^H P (kPW x kPW), diagonalises
+// it with cyclic two-sided Jacobi rotations (G <- Q^H G Q, Wl <- Wl Q), and
+// applies Wl to the panel of X and of W.  The largest relative off-diagonal
+// |G_ij| / sqrt(G_ii G_jj) seen in a sweep goes to offs[sweep]; a task whose
+// previous sweep stayed at or below conv_tol has converged.
 __device__ __forceinline__ int rr_player(int i, int step, int nbe) {
   return 1 + (i - 1 + step) % (nbe - 1);
 }
 
-__global__ __launch_bounds__(256) void k_jacobi_step(const UpdDesc* __restrict__ descs, int nt,
-                                                     int sweep, int step, double tol_rot,
-                                                     double conv_tol) {
-  const UpdDesc& d = descs[blockIdx.y];
-  if (sweep > 0 && d.offs[(sweep - 1) * nt] <= conv_tol) return;
+struct PairSmem {
+  double2 Ps[64][kPW + 1];   // 64-row chunk of the panel
+  double2 G[kPW][kPW + 1];
+  double2 Wl[kPW][kPW + 1];
+  double2 part[4][kPW][kPW + 1];  // per-wave partial Gram
+  double rcs[kPW / 2];
+  double2 rsn[kPW / 2];
+  int col[kPW];
+  double red[4];
+};
+
+// Process block pair p of step `step` of sweep `sweep` for task d.
+// 256 threads.  Returns normally in every case (no barrier is skipped by a
+// subset of the block).
+__device__ void jacobi_pair(const UpdDesc& d, int nt, int sweep, int step, int p,
+                            double tol_rot, PairSmem& sm) {
   const int R = d.info[kInfoR], C = d.info[kInfoC];
   const int nb = (C + kBW - 1) / kBW;
   const int nbe = nb + (nb & 1);
-  if (step >= nbe - 1) return;
-  const int p = blockIdx.x;
-  if (p >= nbe / 2) return;
+  if (step >= nbe - 1 || p >= nbe / 2) return;  // uniform across the block
   const int ba = (p == 0) ? 0 : rr_player(p, step, nbe);
   const int bb = rr_player(nbe - 1 - p, step, nbe);
-
-  __shared__ int col[kPW];
-  __shared__ double2 Ps[64][kPW + 1];
-  __shared__ double2 G[kPW][kPW + 1];
-  __shared__ double2 Wl[kPW][kPW + 1];
-  __shared__ double rcs[kPW / 2];
-  __shared__ double2 rsn[kPW / 2];
-  __shared__ int rflag;
-  __shared__ double red[4];
-
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   if (tid < kPW) {
     const int c = (tid < kBW) ? ba * kBW + tid : bb * kBW + (tid - kBW);
-    col[tid] = (c < C) ? c : -1;
+    sm.col[tid] = (c < C) ? c : -1;
   }
   __syncthreads();
 
-  // ---- Gram matrix, rows staged through LDS 64 at a time
-  const int gi = tid >> 4, gj = tid & 15;
-  double2 acc = make_double2(0.0, 0.0);
+  // ---- Gram matrix.  Thread = (row slice rs in 0..15, 4x4 tile ti,tj).
+  // Row slice rs takes rows rs, rs+16, ... of each 64-row chunk; the 16
+  // slices are reduced by shuffles (within a wave) and LDS (across waves).
+  const int tile = tid & 15, rs = tid >> 4;
+  const int ti = tile >> 2, tj = tile & 3;
+  double2 acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = make_double2(0.0, 0.0);
   for (int r0 = 0; r0 < R; r0 += 64) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int e = tid + 256 * q;
       const int rr = e & 63, cc = e >> 6;
-      const int c = col[cc];
-      Ps[rr][cc] = (c >= 0 && r0 + rr < R) ? d.X[(size_t)c * d.ldx + r0 + rr]
-                                           : make_double2(0.0, 0.0);
+      const int c = sm.col[cc];
+      sm.Ps[rr][cc] = (c >= 0 && r0 + rr < R) ? d.X[(size_t)c * d.ldx + r0 + rr]
+                                              : make_double2(0.0, 0.0);
     }
     __syncthreads();
-#pragma unroll 8
-    for (int rr = 0; rr < 64; ++rr) acc = cadd(acc, cmulc(Ps[rr][gi], Ps[rr][gj]));
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int rr = rs + 16 * k;
+      double2 xi[4], xj[4];
+#pragma unroll
+      for (int a = 0; a < 4; ++a) xi[a] = sm.Ps[rr][ti * 4 + a];
+#pragma unroll
+      for (int b = 0; b < 4; ++b) xj[b] = sm.Ps[rr][tj * 4 + b];
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = cadd(acc[a][b], cmulc(xi[a], xj[b]));
+    }
     __syncthreads();
   }
-  G[gi][gj] = acc;
+  // reduce the 4 row slices inside each wave (lanes tile + 16 * s)
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      double2 v = acc[a][b];
+      v.x += __shfl_xor(v.x, 16, 64);
+      v.y += __shfl_xor(v.y, 16, 64);
+      v.x += __shfl_xor(v.x, 32, 64);
+      v.y += __shfl_xor(v.y, 32, 64);
+      acc[a][b] = v;
+    }
+  if (lane < 16) {
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) sm.part[wid][ti * 4 + a][tj * 4 + b] = acc[a][b];
+  }
+  __syncthreads();
+  const int gi = tid >> 4, gj = tid & 15;
+  {
+    double2 g = sm.part[0][gi][gj];
+    g = cadd(g, sm.part[1][gi][gj]);
+    g = cadd(g, sm.part[2][gi][gj]);
+    g = cadd(g, sm.part[3][gi][gj]);
+    sm.G[gi][gj] = g;
+  }
   __syncthreads();
 
   // ---- convergence measure of this panel
   double ratio = 0.0;
   if (gi < gj) {
-    const double a = G[gi][gi].x, b = G[gj][gj].x;
-    if (a > 0.0 && b > 0.0) ratio = sqrt(cabs2(G[gi][gj])) / sqrt(a * b);
+    const double a = sm.G[gi][gi].x, b = sm.G[gj][gj].x;
+    if (a > 0.0 && b > 0.0) ratio = sqrt(cabs2(sm.G[gi][gj])) / sqrt(a * b);
   }
-  const double off = block_max(ratio, red);
+  const double off = block_max(ratio, sm.red);
   if (tid == 0 && off > 0.0)
-    atomicMax((unsigned long long*)&d.offs[sweep * nt], (unsigned long long)__double_as_longlong(off));
-  if (off <= tol_rot) return;  // panel already orthogonal: nothing to apply
+    atomicMax((unsigned long long*)&d.offs[sweep * nt],
+              (unsigned long long)__double_as_longlong(off));
+  if (off <= tol_rot) return;  // panel already orthogonal (uniform)
 
   // ---- diagonalise G with cyclic Jacobi; accumulate Wl
-  Wl[gi][gj] = make_double2(gi == gj ? 1.0 : 0.0, 0.0);
+  sm.Wl[gi][gj] = make_double2(gi == gj ? 1.0 : 0.0, 0.0);
   __syncthreads();
   for (int isw = 0; isw < 8; ++isw) {
-    if (tid == 0) rflag = 0;
-    __syncthreads();
+    int rotated = 0;
     for (int round = 0; round < kPW - 1; ++round) {
       if (tid < kPW / 2) {
         const int i1 = (tid == 0) ? 0 : rr_player(tid, round, kPW);
         const int i2 = rr_player(kPW - 1 - tid, round, kPW);
-        const double a = G[i1][i1].x, b = G[i2][i2].x;
-        const double2 c = G[i1][i2];
+        const double a = sm.G[i1][i1].x, b = sm.G[i2][i2].x;
+        const double2 c = sm.G[i1][i2];
         const double ac = sqrt(cabs2(c));
         double cs = 1.0;
         double2 sn = make_double2(0.0, 0.0);
@@ -262,20 +340,18 @@ __global__ __launch_bounds__(256) void k_jacobi_step(const UpdDesc* __restrict__
           const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
           cs = 1.0 / sqrt(1.0 + t * t);
           const double s = t * cs;
-          sn = make_double2(s * c.x / ac, s * c.y / ac);  // sn * e^{i phi}
-          rflag = 1;
+          sn = make_double2(s * c.x / ac, s * c.y / ac);  // sn e^{i phi}
+          rotated = 1;
         }
-        rcs[tid] = cs;
-        rsn[tid] = sn;
+        sm.rcs[tid] = cs;
+        sm.rsn[tid] = sn;
       }
       __syncthreads();
       // Q restricted to pair (i1, i2): Q[i1][i1] = cs, Q[i1][i2] = sn e,
       // Q[i2][i1] = -conj(sn e), Q[i2][i2] = cs.
-      // Locate partner and role of index x in this round.
       auto locate = [&](int x, int& pair, int& role, int& partner) {
-        // player x sits at position pos: x == 0 -> pos 0; else solve
-        // x = 1 + (pos - 1 + round) % (kPW - 1)
-        int pos = (x == 0) ? 0 : 1 + ((x - 1 - round) % (kPW - 1) + (kPW - 1)) % (kPW - 1);
+        const int pos =
+            (x == 0) ? 0 : 1 + ((x - 1 - round) % (kPW - 1) + (kPW - 1)) % (kPW - 1);
         if (pos < kPW / 2) {
           pair = pos;
           role = 0;
@@ -289,51 +365,116 @@ __global__ __launch_bounds__(256) void k_jacobi_step(const UpdDesc* __restrict__
       int pi, ri, qi, pj, rj, qj;
       locate(gi, pi, ri, qi);
       locate(gj, pj, rj, qj);
-      // column j of Q has nonzeros at rows j (diag) and qj (off)
-      const double csj = rcs[pj];
-      const double2 snj = rsn[pj];
+      const double csj = sm.rcs[pj];
+      const double2 snj = sm.rsn[pj];
       const double2 Qjj = make_double2(csj, 0.0);
-      const double2 Qoj = (rj == 0) ? make_double2(-snj.x, snj.y)  // Q[i2][i1] = -conj(sn e)
-                                    : snj;                         // Q[i1][i2] = sn e
-      const double csi = rcs[pi];
-      const double2 sni = rsn[pi];
+      const double2 Qoj = (rj == 0) ? make_double2(-snj.x, snj.y) : snj;
+      const double csi = sm.rcs[pi];
+      const double2 sni = sm.rsn[pi];
       const double2 Qii = make_double2(csi, 0.0);
       const double2 Qoi = (ri == 0) ? make_double2(-sni.x, sni.y) : sni;
-      // G'[i][j] = sum_{i' in {i, qi}} sum_{j' in {j, qj}} conj(Q[i'][i]) G[i'][j'] Q[j'][j]
-      const double2 g_ij = G[gi][gj], g_ioj = G[gi][qj], g_oij = G[qi][gj], g_oioj = G[qi][qj];
-      const double2 t0 = cadd(cmul(g_ij, Qjj), cmul(g_ioj, Qoj));     // row i of G Q, col j
-      const double2 t1 = cadd(cmul(g_oij, Qjj), cmul(g_oioj, Qoj));   // row qi of G Q, col j
+      const double2 g_ij = sm.G[gi][gj], g_ioj = sm.G[gi][qj];
+      const double2 g_oij = sm.G[qi][gj], g_oioj = sm.G[qi][qj];
+      const double2 t0 = cadd(cmul(g_ij, Qjj), cmul(g_ioj, Qoj));
+      const double2 t1 = cadd(cmul(g_oij, Qjj), cmul(g_oioj, Qoj));
       const double2 gnew = cadd(cmulc(Qii, t0), cmulc(Qoi, t1));
-      const double2 w_ij = Wl[gi][gj], w_ioj = Wl[gi][qj];
+      const double2 w_ij = sm.Wl[gi][gj], w_ioj = sm.Wl[gi][qj];
       const double2 wnew = cadd(cmul(w_ij, Qjj), cmul(w_ioj, Qoj));
       __syncthreads();
-      G[gi][gj] = gnew;
-      Wl[gi][gj] = wnew;
+      sm.G[gi][gj] = gnew;
+      sm.Wl[gi][gj] = wnew;
       __syncthreads();
     }
-    if (rflag == 0) break;
-    __syncthreads();
+    if (!__syncthreads_or(rotated)) break;
   }
 
-  // ---- apply Wl to the panel of X (rows < R) and of W (rows < C)
+  // ---- apply Wl to the panel of X (rows < R) and of W (rows < C).
+  // Thread = (row lane 0..63, output group jg 0..3 of 4 columns); rows are
+  // staged through LDS 64 at a time.
+  const int rl = tid & 63, jg = tid >> 6;
   for (int pass = 0; pass < 2; ++pass) {
     double2* M = pass == 0 ? d.X : d.W;
     const size_t ld = pass == 0 ? (size_t)d.ldx : (size_t)d.ldw;
     const int rows = pass == 0 ? R : C;
-    for (int r = tid; r < rows; r += blockDim.x) {
-      double2 pr[kPW];
+    for (int r0 = 0; r0 < rows; r0 += 64) {
 #pragma unroll
-      for (int c = 0; c < kPW; ++c)
-        pr[c] = col[c] >= 0 ? M[(size_t)col[c] * ld + r] : make_double2(0.0, 0.0);
-#pragma unroll
-      for (int j = 0; j < kPW; ++j) {
-        if (col[j] < 0) continue;
-        double2 o = make_double2(0.0, 0.0);
-#pragma unroll
-        for (int c = 0; c < kPW; ++c) o = cadd(o, cmul(pr[c], Wl[c][j]));
-        M[(size_t)col[j] * ld + r] = o;
+      for (int q = 0; q < 4; ++q) {
+        const int e = tid + 256 * q;
+        const int rr = e & 63, cc = e >> 6;
+        const int c = sm.col[cc];
+        sm.Ps[rr][cc] = (c >= 0 && r0 + rr < rows) ? M[(size_t)c * ld + r0 + rr]
+                                                   : make_double2(0.0, 0.0);
       }
+      __syncthreads();
+      const int r = r0 + rl;
+      double2 o[4];
+#pragma unroll
+      for (int b = 0; b < 4; ++b) o[b] = make_double2(0.0, 0.0);
+#pragma unroll
+      for (int c = 0; c < kPW; ++c) {
+        const double2 x = sm.Ps[rl][c];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) o[b] = cadd(o[b], cmul(x, sm.Wl[c][jg * 4 + b]));
+      }
+      if (r < rows) {
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+          const int c = sm.col[jg * 4 + b];
+          if (c >= 0) M[(size_t)c * ld + r] = o[b];
+        }
+      }
+      __syncthreads();
     }
+  }
+}
+
+// One launch per step (used when cooperative launch is not available).
+__global__ __launch_bounds__(256, 2) void k_jacobi_step(const UpdDesc* __restrict__ descs, int nt,
+                                                     int sweep, int step, double tol_rot,
+                                                     double conv_tol) {
+  __shared__ PairSmem sm;
+  const UpdDesc& d = descs[blockIdx.y];
+  if (sweep > 0 && d.offs[(sweep - 1) * nt] <= conv_tol) return;
+  jacobi_pair(d, nt, sweep, step, blockIdx.x, tol_rot, sm);
+}
+
+// Persistent cooperative kernel: all sweeps and steps of a batch of SVDs in
+// one launch.  Work item = (task, pair); a grid barrier separates steps.
+// Every block runs the same sequence of grid barriers: the loops have
+// grid-uniform bounds and the convergence decision after each sweep is
+// computed by every block from the same (final) per-sweep maxima.
+__global__ __launch_bounds__(256, 2) void k_jacobi_coop(const UpdDesc* __restrict__ descs, int nt,
+                                                     int npairs, int steps, int max_sweeps,
+                                                     double tol_rot, double conv_tol,
+                                                     int* sweeps_out) {
+  __shared__ PairSmem sm;
+  __shared__ int s_done;
+  cg::grid_group grid = cg::this_grid();
+  const int items = nt * npairs;
+  for (int sw = 0; sw < max_sweeps; ++sw) {
+    for (int st = 0; st < steps; ++st) {
+      for (int it = blockIdx.x; it < items; it += gridDim.x) {
+        const int t = it / npairs, p = it - t * npairs;
+        const UpdDesc& d = descs[t];
+        if (sw > 0 && d.offs[(sw - 1) * nt] <= conv_tol) continue;
+        jacobi_pair(d, nt, sw, st, p, tol_rot, sm);
+      }
+      grid.sync();
+    }
+    if (threadIdx.x == 0) {
+      int done = 1;
+      for (int t = 0; t < nt; ++t)
+        if (!(descs[t].offs[sw * nt] <= conv_tol)) {
+          done = 0;
+          break;
+        }
+      s_done = done;
+      if (blockIdx.x == 0) *sweeps_out = sw + 1;
+    }
+    __syncthreads();
+    const int done = s_done;
+    __syncthreads();
+    if (done) break;
   }
 }
 
@@ -342,7 +483,7 @@ __global__ __launch_bounds__(256) void k_jacobi_step(const UpdDesc* __restrict__
 // descending, apply process_svd (mps.rs:236-293): normalise, pick the rank,
 // renormalise, and record the right-factor scale fac_j = s_j * norm2.
 __global__ __launch_bounds__(1024) void k_finish(const UpdDesc* __restrict__ descs, int nt,
-                                                 int sweeps_run) {
+                                                 const int* sweeps_dev, int sweeps_run) {
   const UpdDesc& d = descs[blockIdx.x];
   const int R = d.info[kInfoR], C = d.info[kInfoC];
   __shared__ double key[kMaxCols];
@@ -351,7 +492,7 @@ __global__ __launch_bounds__(1024) void k_finish(const UpdDesc* __restrict__ des
   __shared__ int first_bad;
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
 
-  if (tid == 0) d.info[kInfoSweeps] = sweeps_run;
+  if (tid == 0) d.info[kInfoSweeps] = sweeps_dev ? *sweeps_dev : sweeps_run;
 
   for (int j = wid; j < C; j += 16) {
     double s = 0.0;
@@ -621,8 +762,28 @@ void launch_jacobi_step(const UpdDesc* descs, int nt, int npairs, int sweep, int
   hipLaunchKernelGGL(k_jacobi_step, dim3(npairs, nt), dim3(256), 0, s, descs, nt, sweep, step,
                      tol_rot, conv_tol);
 }
-void launch_finish(const UpdDesc* descs, int nt, int sweeps_run, hipStream_t s) {
-  hipLaunchKernelGGL(k_finish, dim3(nt), dim3(1024), 0, s, descs, nt, sweeps_run);
+hipError_t launch_jacobi_coop(const UpdDesc* descs, int nt, int npairs, int steps,
+                              int max_sweeps, double tol_rot, double conv_tol, int* sweeps_out,
+                              int grid, hipStream_t s) {
+  void* args[] = {(void*)&descs, (void*)&nt, (void*)&npairs, (void*)&steps, (void*)&max_sweeps,
+                  (void*)&tol_rot, (void*)&conv_tol, (void*)&sweeps_out};
+  return hipLaunchCooperativeKernel((const void*)k_jacobi_coop, dim3(grid), dim3(256), args, 0,
+                                    s);
+}
+int jacobi_coop_max_blocks() {
+  int dev = 0, per_cu = 0;
+  hipDeviceProp_t prop;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return 0;
+  if (!prop.cooperativeLaunch) return 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_jacobi_coop, 256, 0) !=
+      hipSuccess)
+    return 0;
+  return per_cu * prop.multiProcessorCount;
+}
+void launch_finish(const UpdDesc* descs, int nt, const int* sweeps_dev, int sweeps_run,
+                   hipStream_t s) {
+  hipLaunchKernelGGL(k_finish, dim3(nt), dim3(1024), 0, s, descs, nt, sweeps_dev, sweeps_run);
 }
 void launch_split(const UpdDesc* descs, int nt, hipStream_t s) {
   hipLaunchKernelGGL(k_split, dim3(kNPart, nt), dim3(256), 0, s, descs);

@@ -28,6 +28,7 @@ TN_INVALID_ARG = 11
 TN_NO_CONVERGENCE = 12
 
 TRUNC_NONE, TRUNC_CONST, TRUNC_CUTOFF = 0, 1, 2
+PREC_F64, PREC_F32 = 0, 1
 
 
 class tn_trunc(C.Structure):
@@ -56,6 +57,12 @@ SIGNATURES = {
     "tn_device_count": (C.c_int, []),
     "tn_mps_new_qubits": (C.c_int, [_sz, tn_trunc, _sz, C.c_void_p, _hp]),
     "tn_mps_free": (None, [_h]),
+    "tn_engine_new": (C.c_int, [_sz, C.c_void_p, _hp]),
+    "tn_engine_free": (None, [_h]),
+    "tn_mps_new_qubits_in": (C.c_int, [_h, _sz, tn_trunc, _sz, _hp]),
+    "tn_set_precision": (C.c_int, [_h, C.c_int]),
+    "tn_batch_bilayer": (C.c_int, [_h, _sz, _hp, _szp, _szp, _dp, _szp, _szp, _ip, _dp, _szp,
+                                   _dp, _szp, _ip, _dp]),
     "tn_mps_clone": (C.c_int, [_h, _hp]),
     "tn_mps_n": (_sz, [_h]),
     "tn_get_trunc": (C.c_int, [_h, C.POINTER(tn_trunc)]),

@@ -92,12 +92,30 @@ def _iptr(a: np.ndarray):
     return a.ctypes.data_as(C.POINTER(C.c_int))
 
 
+def _pack_layer(n, unis, msites, forced_layer=None, rit=None):
+    """Flatten one bilayer: gate sites/ops and measurement sites/outcomes/draws.
+    With ``forced_layer`` ([(k, o)]) the measurements are ``Meas::Proj``;
+    otherwise ``Meas::Rand`` consuming ``rit`` in processing order (sorted,
+    deduplicated, in-range sites)."""
+    uks = [int(k) for k, _ in unis]
+    ops = [np.ascontiguousarray(np.asarray(m, dtype=np.complex128)).view(np.float64).reshape(-1)
+           for _, m in unis]
+    if forced_layer is not None:
+        ml = [(int(k), int(o), 0.0) for (k, o) in forced_layer]
+    else:
+        srt = sorted(set(int(k) for k in msites if k < n))
+        rmap = {k: next(rit) for k in srt}
+        ml = [(int(k), -1, float(rmap.get(int(k), 0.0))) for k in msites]
+    return uks, ops, ml
+
+
 class MPS:
     """Device-resident Schmidt-form MPS of qubits (``MPS<Q, C64>``, mps.rs:338-356)."""
 
-    def __init__(self, handle: int, lib):
+    def __init__(self, handle: int, lib, engine: "Engine" = None):
         self._h = C.c_void_p(handle)
         self._lib = lib
+        self._engine = engine  # keeps a shared engine alive while in use
 
     # ------------------------------------------------------------ lifecycle
     @classmethod
@@ -142,6 +160,13 @@ class MPS:
 
     def sync(self) -> None:
         _check(self._lib.tn_sync(self._h), "sync")
+
+    def set_precision(self, prec: str) -> "MPS":
+        """Contraction arithmetic: "f64" (default) or "f32" (FP32 with FP32
+        accumulation); SVD, truncation and renormalisation stay FP64."""
+        code = {"f64": L.PREC_F64, "f32": L.PREC_F32}[prec]
+        _check(self._lib.tn_set_precision(self._h, code), "set_precision")
+        return self
 
     # ------------------------------------------------------------ unitaries
     def apply_unitary1(self, k: int, op) -> "MPS":
@@ -390,3 +415,86 @@ class MPS:
             probs.append([(sites[i], float(op[pos + i])) for i in range(c)])
             pos += c
         return {"record": record, "probs": probs, "entropy": list(ent)}
+
+
+class Engine:
+    """A HIP stream plus the update workspace shared by many trajectories.
+
+    Trajectories made with :meth:`new_qubits` can be advanced together with
+    :meth:`batch_bilayer`: their gates and the steps of their measurement
+    schedules run in shared launches.  No reference counterpart (the reference
+    runs one MPS per rayon thread, workspace/src/haar_coev_probs.rs:162-203).
+    """
+
+    def __init__(self, chi_max: int, stream=None):
+        self._lib = L.load()
+        h = C.c_void_p()
+        _check(self._lib.tn_engine_new(int(chi_max), C.c_void_p(stream) if stream else None,
+                                       C.byref(h)), "engine_new")
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            self._lib.tn_engine_free(h)
+            self._h = C.c_void_p()
+
+    def new_qubits(self, n: int, trunc=None, chi_cap: int = 0) -> MPS:
+        h = C.c_void_p()
+        _check(self._lib.tn_mps_new_qubits_in(self._h, int(n), to_trunc(trunc), int(chi_cap),
+                                              C.byref(h)), "new_qubits_in")
+        return MPS(h.value, self._lib, engine=self)
+
+    def batch_bilayer(self, states: Sequence[MPS], layers, forced=None, draws=None,
+                      ent_bonds=None):
+        """One bilayer on every state, in lock step (``tn_batch_bilayer``).
+
+        ``layers[i] = (unis, meas_sites)`` for state i; ``forced[i]`` is its
+        [(k, outcome)] record (``Meas::Proj``) or ``draws[i]`` an iterator of
+        uniform draws (``Meas::Rand``).  Returns per state a tuple
+        (record [(k, outcome)], probs [(k, p)], entropy at ent_bonds[i]).
+        """
+        cnt = len(states)
+        nu, uks, ops, nm, mks, fo, rr, sites = [], [], [], [], [], [], [], []
+        for i, st in enumerate(states):
+            unis, msites = layers[i]
+            u, o, ml = _pack_layer(st.n, unis, msites,
+                                   forced[i] if forced is not None else None,
+                                   draws[i] if draws is not None else None)
+            nu.append(len(u))
+            uks += u
+            ops += o
+            nm.append(len(ml))
+            for (k, f, r) in ml:
+                mks.append(k)
+                fo.append(f)
+                rr.append(r)
+            sites.append(sorted(set(k for k, _, _ in ml if k < st.n)))
+        hs = (C.c_void_p * cnt)(*[st._h.value for st in states])
+        nu_a, nm_a = _szarr(nu), _szarr(nm)
+        uks_a = _szarr(uks if uks else [0])
+        ops_a = np.ascontiguousarray(np.concatenate(ops)) if ops else np.zeros(32)
+        mks_a = _szarr(mks if mks else [0])
+        fo_a = np.asarray(fo if fo else [0], dtype=np.int32)
+        rr_a = np.asarray(rr if rr else [0.0], dtype=np.float64)
+        eb = _szarr(ent_bonds if ent_bonds is not None else [st.n // 2 - 1 for st in states])
+        ent = np.zeros(cnt)
+        counts = np.zeros(cnt, dtype=np.uintp)
+        tot = max(len(mks), 1)
+        oo = np.zeros(tot, dtype=np.int32)
+        op = np.zeros(tot)
+        _check(self._lib.tn_batch_bilayer(self._h, cnt, C.cast(hs, C.POINTER(C.c_void_p)),
+                                          _szptr(nu_a), _szptr(uks_a), _dptr(ops_a),
+                                          _szptr(nm_a), _szptr(mks_a), _iptr(fo_a),
+                                          _dptr(rr_a), _szptr(eb), _dptr(ent), _szptr(counts),
+                                          _iptr(oo), _dptr(op)),
+               "batch_bilayer")
+        out, pos = [], 0
+        for i in range(cnt):
+            c = int(counts[i])
+            assert c == len(sites[i])
+            rec = [(sites[i][j], int(oo[pos + j])) for j in range(c)]
+            prs = [(sites[i][j], float(op[pos + j])) for j in range(c)]
+            pos += c
+            out.append((rec, prs, float(ent[i])))
+        return out

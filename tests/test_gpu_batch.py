@@ -1,0 +1,118 @@
+"""Lock-step batching of many trajectories on one engine, the FP32
+contraction mode, and the two Jacobi launch modes.
+
+Each trajectory of a batch must give the oracle's results (same tolerances
+as tests/test_gpu_parity.py) and the same results as when run alone.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from helpers import assert_spectrum_close
+from oracle.mps_oracle import run_trajectory
+from tnmps import MPS, Engine
+from tnmps.workload import haar_circuit
+
+pytestmark = pytest.mark.gpu
+
+
+def close(a, b, rtol=1e-10, atol=1e-12):
+    return abs(a - b) <= atol + rtol * abs(b)
+
+
+def _specs():
+    # (n, depth, p, chi, seed): uneven sizes and bond dimensions in one batch
+    return [(8, 8, 0.2, 16, 1), (12, 10, 0.1, 32, 2), (10, 10, 0.3, 8, 3),
+            (16, 8, 0.05, 32, 4), (6, 6, 0.5, 4, 5), (12, 12, 0.15, 16, 6)]
+
+
+def test_batch_matches_oracle_and_solo(gpu):
+    specs = _specs()
+    circuits = [haar_circuit(n, d, p, s) for (n, d, p, _, s) in specs]
+    refs = [run_trajectory(n, c.layers, ("const", chi), rs=c.draws)
+            for (n, _, _, chi, _), c in zip(specs, circuits)]
+    eng = Engine(32)
+    states = [eng.new_qubits(n, ("const", chi)) for (n, _, _, chi, _) in specs]
+    depth = max(d for (_, d, _, _, _) in specs)
+    recs = [[] for _ in specs]
+    probs = [[] for _ in specs]
+    ents = [[] for _ in specs]
+    for t in range(depth):
+        idx = [i for i, (_, d, _, _, _) in enumerate(specs) if t < d]
+        res = eng.batch_bilayer([states[i] for i in idx],
+                                [circuits[i].layers[t] for i in idx],
+                                forced=[refs[i]["record"][t] for i in idx])
+        for i, (rec, prs, ent) in zip(idx, res):
+            recs[i].append(rec)
+            probs[i].append(prs)
+            ents[i].append(ent)
+    for i, (n, d, p, chi, s) in enumerate(specs):
+        ref = refs[i]
+        assert recs[i] == ref["record"]
+        for a, b in zip(ents[i], ref["entropy"][1:]):
+            assert close(a, b), f"traj {i} entropy {a} vs {b}"
+        for la, lb in zip(probs[i], ref["probs"]):
+            for (ka, pa), (kb, pb) in zip(la, lb):
+                assert ka == kb and close(pa, pb)
+        assert states[i].bond_dims() == ref["state"].bond_dims()
+        for b in range(n - 1):
+            assert_spectrum_close(states[i].schmidt(b), ref["state"].L[b], msg=f"traj {i} bond {b}")
+        # the same trajectory alone, on its own engine
+        solo = MPS.new_qubits(n, ("const", chi))
+        got = solo.run_circuit(circuits[i].layers, forced=ref["record"])
+        for b in range(n - 1):
+            assert_spectrum_close(solo.schmidt(b), states[i].schmidt(b), rtol=1e-12,
+                                  msg=f"solo vs batch traj {i} bond {b}")
+        for a, c in zip(got["entropy"][1:], ents[i]):
+            assert close(a, c, rtol=1e-12)
+
+
+def test_batch_sampled_draws(gpu):
+    """Meas::Rand in a batch consumes each trajectory's own draws."""
+    specs = [(10, 6, 0.3, 16, 11), (8, 6, 0.3, 8, 12)]
+    circuits = [haar_circuit(n, d, p, s) for (n, d, p, _, s) in specs]
+    refs = [run_trajectory(n, c.layers, ("const", chi), rs=c.draws)
+            for (n, _, _, chi, _), c in zip(specs, circuits)]
+    eng = Engine(16)
+    states = [eng.new_qubits(n, ("const", chi)) for (n, _, _, chi, _) in specs]
+    its = [iter(c.draws) for c in circuits]
+    recs = [[], []]
+    for t in range(6):
+        res = eng.batch_bilayer(states, [c.layers[t] for c in circuits], draws=its)
+        for i in range(2):
+            recs[i].append(res[i][0])
+    assert recs[0] == refs[0]["record"] and recs[1] == refs[1]["record"]
+
+
+def test_fp32_contraction_error(gpu):
+    """TN_PREC_F32: theta in FP32 with FP32 accumulation, SVD in FP64.  The
+    entropy error against the FP64 oracle stays at the FP32 level."""
+    n, depth, p, chi = 12, 12, 0.1, 16
+    c = haar_circuit(n, depth, p, 77)
+    ref = run_trajectory(n, c.layers, ("const", chi), rs=c.draws)
+    st = MPS.new_qubits(n, ("const", chi)).set_precision("f32")
+    got = st.run_circuit(c.layers, forced=ref["record"])
+    err = max(abs(a - b) for a, b in zip(got["entropy"], ref["entropy"]))
+    assert err < 1e-4, err
+    assert err > 0.0  # the FP32 path really ran
+    s0 = max(float(np.max(np.abs(st.schmidt(b)[: len(ref["state"].L[b])]
+                                 - ref["state"].L[b][: len(st.schmidt(b))])))
+             for b in range(n - 1))
+    assert s0 < 1e-4
+
+
+def test_step_and_cooperative_jacobi_agree(gpu, monkeypatch):
+    """The per-step launch mode and the persistent cooperative kernel give the
+    same spectra."""
+    n, depth, p, chi = 12, 10, 0.1, 32
+    c = haar_circuit(n, depth, p, 31)
+    ref = run_trajectory(n, c.layers, ("const", chi), rs=c.draws)
+    a = MPS.new_qubits(n, ("const", chi))
+    a.run_circuit(c.layers, forced=ref["record"])
+    monkeypatch.setenv("TNMPS_JACOBI", "step")
+    b = MPS.new_qubits(n, ("const", chi))
+    b.run_circuit(c.layers, forced=ref["record"])
+    for bd in range(n - 1):
+        assert_spectrum_close(a.schmidt(bd), ref["state"].L[bd])
+        assert_spectrum_close(b.schmidt(bd), ref["state"].L[bd])
