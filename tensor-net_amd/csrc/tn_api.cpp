@@ -54,11 +54,12 @@ double env_double(const char* name, double dflt) {
 
 // Jacobi tolerances.  A pair of columns is rotated when |g_ij| exceeds
 // tol_rot * sqrt(g_ii g_jj); the SVD has converged after the first sweep whose
-// largest such ratio is at most conv_tol (Jacobi converges quadratically, so
-// the residual after that sweep is of order conv_tol^2).
+// largest such ratio is at most conv_tol.  That sweep still rotates every pair
+// above tol_rot, and Jacobi converges quadratically, so the residual after it
+// is of order conv_tol^2 = 1e-16.
 struct Tolerances {
   double tol_rot = 1e-15;
-  double conv_tol = 1e-10;
+  double conv_tol = 1e-8;
   Tolerances() {
     tol_rot = env_double("TNMPS_TOL_ROT", tol_rot);
     conv_tol = env_double("TNMPS_CONV_TOL", conv_tol);
@@ -165,8 +166,11 @@ struct tn_engine {
   double* fac = nullptr;
   int* perm = nullptr;
   double* part = nullptr;
+  double2* gpart = nullptr;
+  size_t gpart_per = 0;  // elements of gpart per slot
   double* offs = nullptr;
   int* info = nullptr;
+  int cus = 0;           // compute units of the device
   DescRing<UpdDesc> upd;
   DescRing<MeasDesc> meas;
   int mres = 0;  // measurement result slots
@@ -191,7 +195,8 @@ struct tn_engine {
 
   void release_ws() {
     hfree(X); hfree(W); hfree(sraw); hfree(sval); hfree(fac); hfree(perm);
-    hfree(part); hfree(offs); hfree(info);
+    hfree(part); hfree(offs); hfree(info); hfree(gpart);
+    gpart = nullptr;
     X = W = nullptr;
     sraw = sval = fac = part = offs = nullptr;
     perm = info = nullptr;
@@ -231,8 +236,15 @@ struct tn_engine {
     for (hipEvent_t& ev : tev) TN_HIP(hipEventCreate(&ev));
     TN_HIP(hipMalloc(&d_sweeps, sizeof(int)));
     TN_HIP(hipHostMalloc(&h_sweeps, sizeof(int), hipHostMallocDefault));
+    // Jacobi launch mode: per-step launches (default) or, with
+    // TNMPS_JACOBI=coop, one persistent cooperative launch per SVD batch
     const char* mode = std::getenv("TNMPS_JACOBI");
-    coop_blocks = (mode && std::strcmp(mode, "step") == 0) ? 0 : jacobi_coop_max_blocks();
+    coop_blocks = (mode && std::strcmp(mode, "coop") == 0) ? jacobi_coop_max_blocks() : 0;
+    hipDeviceProp_t prop;
+    TN_HIP(hipGetDeviceProperties(&prop, dev));
+    cus = prop.multiProcessorCount;
+    const int nbmax = (ldw + kBW - 1) / kBW;
+    gpart_per = (size_t)((nbmax + 1) / 2 + 1) * kMaxSplit * kPW * kPW;
     // workspace budget: TNMPS_WS_BYTES (default 24 GiB) bounds the tasks per launch
     const double budget = env_double("TNMPS_WS_BYTES", 24.0 * (1ull << 30));
     const double per = 16.0 * ((double)ldx * ldw + (double)ldw * ldw) + 64.0 * ldw + 4096.0;
@@ -261,6 +273,7 @@ struct tn_engine {
     TN_HIP(hipMalloc(&part, sizeof(double) * 2 * kNPart * ns));
     TN_HIP(hipMalloc(&offs, sizeof(double) * kMaxSweeps * ns));
     TN_HIP(hipMalloc(&info, sizeof(int) * kInfoLen * ns));
+    TN_HIP(hipMalloc(&gpart, sizeof(double2) * gpart_per * ns));
     slots = ns;
     return upd.ensure(ns, stream);
   }
@@ -356,7 +369,7 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
   UpdDesc* hd = nullptr;
   int rs = 0;
   TN_TRY(e->upd.next(&hd, &rs));
-  size_t maxC = 1, maxTiles = 1, maxCC = 1;
+  size_t maxC = 1, maxR = 1, maxTiles = 1, maxCC = 1;
   for (int t = 0; t < nt; ++t) {
     const Task& q = tasks[t];
     tn_mps* h = q.h;
@@ -376,6 +389,7 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
     d.fac = e->fac + (size_t)t * e->ldw;
     d.perm = e->perm + (size_t)t * e->ldw;
     d.part = e->part + (size_t)t * 2 * kNPart;
+    d.gpart = e->gpart + (size_t)t * e->gpart_per;
     d.offs = e->offs + t;
     d.info = e->info + (size_t)t * kInfoLen;
     d.ldx = e->ldx;
@@ -391,6 +405,7 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
     const size_t chiL = h->uL(k), chiR = h->uR(k + 1);
     const size_t C = std::min(2 * chiL, 2 * chiR);
     maxC = std::max(maxC, C);
+    maxR = std::max(maxR, std::max(2 * chiL, 2 * chiR));
     maxCC = std::max(maxCC, C * C);
     maxTiles = std::max(maxTiles, ((chiL + 3) / 4) * ((chiR + 63) / 64));
   }
@@ -425,12 +440,24 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
       sweeps_run = *e->h_sweeps;
     }
   } else {
-    // one launch per round-robin step; the host checks convergence per sweep
+    // one launch per round-robin step; the host checks convergence per sweep.
+    // Few pairs (a lone large SVD) leave most CUs idle: split each pair's
+    // rows over S workgroups (gram kernel, then solve + apply kernel).
+    int S = 1;
+    while (S < kMaxSplit && (size_t)nt * npairs * S * 2 <= (size_t)2 * e->cus &&
+           maxR >= (size_t)128 * S)
+      S *= 2;
     std::vector<double> offs(nt);
     for (int sw = 0; sw < kMaxSweeps; ++sw) {
       if (e->timing) TN_HIP(hipEventRecord(e->tev[2], e->stream));
-      for (int s = 0; s < steps; ++s)
-        launch_jacobi_step(dd, nt, npairs, sw, s, tol.tol_rot, tol.conv_tol, e->stream);
+      for (int s = 0; s < steps; ++s) {
+        if (S == 1) {
+          launch_jacobi_step(dd, nt, npairs, sw, s, tol.tol_rot, tol.conv_tol, e->stream);
+        } else {
+          launch_jacobi_gram(dd, nt, npairs, S, sw, s, tol.conv_tol, e->stream);
+          launch_jacobi_apply(dd, nt, npairs, S, sw, s, tol.tol_rot, tol.conv_tol, e->stream);
+        }
+      }
       TN_HIP(hipGetLastError());
       if (e->timing) TN_HIP(hipEventRecord(e->tev[3], e->stream));
       sweeps_run = sw + 1;
@@ -440,7 +467,7 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
       if (e->timing) {
         if (sw == 0) e->perf.ms_theta += e->span(0, 1);
         e->perf.ms_jacobi += e->span(2, 3);
-        e->perf.jacobi_launches += steps;
+        e->perf.jacobi_launches += (size_t)steps * (S == 1 ? 1 : 2);
       }
       bool done = true;
       for (int t = 0; t < nt; ++t)

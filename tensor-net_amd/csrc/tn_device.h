@@ -25,6 +25,7 @@ constexpr int kPW = 2 * kBW;    // panel width (two blocks)
 constexpr int kMaxSweeps = 40;  // outer Jacobi sweep limit
 constexpr int kNPart = 64;      // blocks of the split/renormalise kernels
 constexpr int kMaxCols = 1024;  // largest C (= min(m, n)) the SVD accepts
+constexpr int kMaxSplit = 8;    // workgroups per block pair in the row-split step
 
 // info[] slots of a task (device ints)
 enum {
@@ -52,6 +53,7 @@ struct UpdDesc {
   double* fac;           // scale of the right factor rows (C)
   int* perm;             // sorted column order (C)
   double* part;          // per-block partial norms (2 * kNPart)
+  double2* gpart;        // row-split partial Gram matrices [pair][slice][kPW*kPW]
   double* offs;          // per-sweep max off-diagonal ratio, stride nt
   int* info;             // kInfoLen ints
   int ldx, ldw;

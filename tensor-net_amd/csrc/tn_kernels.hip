@@ -227,27 +227,30 @@ struct PairSmem {
   double red[4];
 };
 
-// Process block pair p of step `step` of sweep `sweep` for task d.
-// 256 threads.  Returns normally in every case (no barrier is skipped by a
-// subset of the block).
-__device__ void jacobi_pair(const UpdDesc& d, int nt, int sweep, int step, int p,
-                            double tol_rot, PairSmem& sm) {
-  const int R = d.info[kInfoR], C = d.info[kInfoC];
+// Columns of block pair p at round-robin step `step` into sm.col (-1 for
+// columns beyond C).  False (for the whole block) when the pair does not
+// exist for this task's C.
+__device__ bool pair_cols(int C, int step, int p, PairSmem& sm) {
   const int nb = (C + kBW - 1) / kBW;
   const int nbe = nb + (nb & 1);
-  if (step >= nbe - 1 || p >= nbe / 2) return;  // uniform across the block
+  if (step >= nbe - 1 || p >= nbe / 2) return false;
   const int ba = (p == 0) ? 0 : rr_player(p, step, nbe);
   const int bb = rr_player(nbe - 1 - p, step, nbe);
-  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int tid = threadIdx.x;
   if (tid < kPW) {
     const int c = (tid < kBW) ? ba * kBW + tid : bb * kBW + (tid - kBW);
     sm.col[tid] = (c < C) ? c : -1;
   }
   __syncthreads();
+  return true;
+}
 
-  // ---- Gram matrix.  Thread = (row slice rs in 0..15, 4x4 tile ti,tj).
-  // Row slice rs takes rows rs, rs+16, ... of each 64-row chunk; the 16
-  // slices are reduced by shuffles (within a wave) and LDS (across waves).
+// sm.G = P^H P over rows [r_begin, r_end) of the panel of X.
+// Thread = (row slice rs in 0..15, 4x4 tile ti,tj).  Row slice rs takes rows
+// rs, rs+16, ... of each 64-row chunk; the 16 slices are reduced by shuffles
+// (within a wave) and LDS (across waves).
+__device__ void panel_gram(const UpdDesc& d, int r_begin, int r_end, PairSmem& sm) {
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   const int tile = tid & 15, rs = tid >> 4;
   const int ti = tile >> 2, tj = tile & 3;
   double2 acc[4][4];
@@ -255,14 +258,14 @@ __device__ void jacobi_pair(const UpdDesc& d, int nt, int sweep, int step, int p
   for (int a = 0; a < 4; ++a)
 #pragma unroll
     for (int b = 0; b < 4; ++b) acc[a][b] = make_double2(0.0, 0.0);
-  for (int r0 = 0; r0 < R; r0 += 64) {
+  for (int r0 = r_begin; r0 < r_end; r0 += 64) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int e = tid + 256 * q;
       const int rr = e & 63, cc = e >> 6;
       const int c = sm.col[cc];
-      sm.Ps[rr][cc] = (c >= 0 && r0 + rr < R) ? d.X[(size_t)c * d.ldx + r0 + rr]
-                                              : make_double2(0.0, 0.0);
+      sm.Ps[rr][cc] = (c >= 0 && r0 + rr < r_end) ? d.X[(size_t)c * d.ldx + r0 + rr]
+                                                  : make_double2(0.0, 0.0);
     }
     __syncthreads();
 #pragma unroll
@@ -280,7 +283,6 @@ __device__ void jacobi_pair(const UpdDesc& d, int nt, int sweep, int step, int p
     }
     __syncthreads();
   }
-  // reduce the 4 row slices inside each wave (lanes tile + 16 * s)
 #pragma unroll
   for (int a = 0; a < 4; ++a)
 #pragma unroll
@@ -300,28 +302,35 @@ __device__ void jacobi_pair(const UpdDesc& d, int nt, int sweep, int step, int p
   }
   __syncthreads();
   const int gi = tid >> 4, gj = tid & 15;
-  {
-    double2 g = sm.part[0][gi][gj];
-    g = cadd(g, sm.part[1][gi][gj]);
-    g = cadd(g, sm.part[2][gi][gj]);
-    g = cadd(g, sm.part[3][gi][gj]);
-    sm.G[gi][gj] = g;
-  }
+  double2 g = sm.part[0][gi][gj];
+  g = cadd(g, sm.part[1][gi][gj]);
+  g = cadd(g, sm.part[2][gi][gj]);
+  g = cadd(g, sm.part[3][gi][gj]);
+  sm.G[gi][gj] = g;
   __syncthreads();
+}
 
-  // ---- convergence measure of this panel
+// From the panel Gram matrix in sm.G: record the largest relative
+// off-diagonal (when `record`), and unless the panel is already orthogonal
+// diagonalise G with cyclic two-sided Jacobi rotations, accumulating the
+// rotation in sm.Wl.  Returns false (for the whole block) when there is
+// nothing to apply.  A function of sm.G alone: blocks holding the same G
+// compute bitwise the same Wl.
+__device__ bool panel_solve(const UpdDesc& d, int nt, int sweep, bool record, double tol_rot,
+                            PairSmem& sm) {
+  const int tid = threadIdx.x;
+  const int gi = tid >> 4, gj = tid & 15;
   double ratio = 0.0;
   if (gi < gj) {
     const double a = sm.G[gi][gi].x, b = sm.G[gj][gj].x;
     if (a > 0.0 && b > 0.0) ratio = sqrt(cabs2(sm.G[gi][gj])) / sqrt(a * b);
   }
   const double off = block_max(ratio, sm.red);
-  if (tid == 0 && off > 0.0)
+  if (record && tid == 0 && off > 0.0)
     atomicMax((unsigned long long*)&d.offs[sweep * nt],
               (unsigned long long)__double_as_longlong(off));
-  if (off <= tol_rot) return;  // panel already orthogonal (uniform)
+  if (off <= tol_rot) return false;
 
-  // ---- diagonalise G with cyclic Jacobi; accumulate Wl
   sm.Wl[gi][gj] = make_double2(gi == gj ? 1.0 : 0.0, 0.0);
   __syncthreads();
   for (int isw = 0; isw < 8; ++isw) {
@@ -387,45 +396,108 @@ __device__ void jacobi_pair(const UpdDesc& d, int nt, int sweep, int step, int p
     }
     if (!__syncthreads_or(rotated)) break;
   }
+  return true;
+}
 
-  // ---- apply Wl to the panel of X (rows < R) and of W (rows < C).
-  // Thread = (row lane 0..63, output group jg 0..3 of 4 columns); rows are
-  // staged through LDS 64 at a time.
+// Rows [a, b) of the panel of M (leading dimension ld) <- panel * Wl.
+// Thread = (row lane 0..63, output group jg 0..3 of 4 columns); rows are
+// staged through LDS 64 at a time.
+__device__ void panel_apply(double2* M, size_t ld, int a, int b, PairSmem& sm) {
+  const int tid = threadIdx.x;
   const int rl = tid & 63, jg = tid >> 6;
-  for (int pass = 0; pass < 2; ++pass) {
-    double2* M = pass == 0 ? d.X : d.W;
-    const size_t ld = pass == 0 ? (size_t)d.ldx : (size_t)d.ldw;
-    const int rows = pass == 0 ? R : C;
-    for (int r0 = 0; r0 < rows; r0 += 64) {
+  for (int r0 = a; r0 < b; r0 += 64) {
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int e = tid + 256 * q;
-        const int rr = e & 63, cc = e >> 6;
-        const int c = sm.col[cc];
-        sm.Ps[rr][cc] = (c >= 0 && r0 + rr < rows) ? M[(size_t)c * ld + r0 + rr]
-                                                   : make_double2(0.0, 0.0);
-      }
-      __syncthreads();
-      const int r = r0 + rl;
-      double2 o[4];
-#pragma unroll
-      for (int b = 0; b < 4; ++b) o[b] = make_double2(0.0, 0.0);
-#pragma unroll
-      for (int c = 0; c < kPW; ++c) {
-        const double2 x = sm.Ps[rl][c];
-#pragma unroll
-        for (int b = 0; b < 4; ++b) o[b] = cadd(o[b], cmul(x, sm.Wl[c][jg * 4 + b]));
-      }
-      if (r < rows) {
-#pragma unroll
-        for (int b = 0; b < 4; ++b) {
-          const int c = sm.col[jg * 4 + b];
-          if (c >= 0) M[(size_t)c * ld + r] = o[b];
-        }
-      }
-      __syncthreads();
+    for (int q = 0; q < 4; ++q) {
+      const int e = tid + 256 * q;
+      const int rr = e & 63, cc = e >> 6;
+      const int c = sm.col[cc];
+      sm.Ps[rr][cc] = (c >= 0 && r0 + rr < b) ? M[(size_t)c * ld + r0 + rr]
+                                              : make_double2(0.0, 0.0);
     }
+    __syncthreads();
+    const int r = r0 + rl;
+    double2 o[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) o[k] = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int c = 0; c < kPW; ++c) {
+      const double2 x = sm.Ps[rl][c];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) o[k] = cadd(o[k], cmul(x, sm.Wl[c][jg * 4 + k]));
+    }
+    if (r < b) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int c = sm.col[jg * 4 + k];
+        if (c >= 0) M[(size_t)c * ld + r] = o[k];
+      }
+    }
+    __syncthreads();
   }
+}
+
+// Process block pair p of step `step` of sweep `sweep` for task d with one
+// workgroup.  Returns normally in every case (no barrier is skipped by a
+// subset of the block).
+__device__ void jacobi_pair(const UpdDesc& d, int nt, int sweep, int step, int p,
+                            double tol_rot, PairSmem& sm) {
+  const int R = d.info[kInfoR], C = d.info[kInfoC];
+  if (!pair_cols(C, step, p, sm)) return;
+  panel_gram(d, 0, R, sm);
+  if (!panel_solve(d, nt, sweep, true, tol_rot, sm)) return;
+  panel_apply(d.X, (size_t)d.ldx, 0, R, sm);
+  panel_apply(d.W, (size_t)d.ldw, 0, C, sm);
+}
+
+// Row range of slice q of S over n rows (slices are multiples of 64 rows).
+__device__ __forceinline__ void split_rows(int n, int S, int q, int& a, int& b) {
+  const int per = (((n + S - 1) / S) + 63) / 64 * 64;
+  a = min(n, q * per);
+  b = min(n, a + per);
+}
+
+// Row-split step, part 1: workgroup (p, q) writes the Gram matrix of rows
+// slice q of pair p's panel to gpart[p][q].
+__global__ __launch_bounds__(256, 2) void k_jacobi_gram(const UpdDesc* __restrict__ descs,
+                                                        int nt, int sweep, int step, int S,
+                                                        double conv_tol) {
+  __shared__ PairSmem sm;
+  const UpdDesc& d = descs[blockIdx.y];
+  if (sweep > 0 && d.offs[(sweep - 1) * nt] <= conv_tol) return;
+  const int R = d.info[kInfoR], C = d.info[kInfoC];
+  const int p = blockIdx.x / S, q = blockIdx.x - (blockIdx.x / S) * S;
+  if (!pair_cols(C, step, p, sm)) return;
+  int a, b;
+  split_rows(R, S, q, a, b);
+  panel_gram(d, a, b, sm);
+  const int gi = threadIdx.x >> 4, gj = threadIdx.x & 15;
+  d.gpart[((size_t)p * S + q) * (kPW * kPW) + gi * kPW + gj] = sm.G[gi][gj];
+}
+
+// Row-split step, part 2: every workgroup of pair p sums the S partial Gram
+// matrices in the same order, so all of them solve the same G into the same
+// Wl, and applies it to its own slice of the rows of X and of W.
+__global__ __launch_bounds__(256, 2) void k_jacobi_apply(const UpdDesc* __restrict__ descs,
+                                                         int nt, int sweep, int step, int S,
+                                                         double tol_rot, double conv_tol) {
+  __shared__ PairSmem sm;
+  const UpdDesc& d = descs[blockIdx.y];
+  if (sweep > 0 && d.offs[(sweep - 1) * nt] <= conv_tol) return;
+  const int R = d.info[kInfoR], C = d.info[kInfoC];
+  const int p = blockIdx.x / S, q = blockIdx.x - (blockIdx.x / S) * S;
+  if (!pair_cols(C, step, p, sm)) return;
+  const int gi = threadIdx.x >> 4, gj = threadIdx.x & 15;
+  double2 g = make_double2(0.0, 0.0);
+  for (int s = 0; s < S; ++s)
+    g = cadd(g, d.gpart[((size_t)p * S + s) * (kPW * kPW) + gi * kPW + gj]);
+  sm.G[gi][gj] = g;
+  __syncthreads();
+  if (!panel_solve(d, nt, sweep, q == 0, tol_rot, sm)) return;
+  int a, b;
+  split_rows(R, S, q, a, b);
+  panel_apply(d.X, (size_t)d.ldx, a, b, sm);
+  split_rows(C, S, q, a, b);
+  panel_apply(d.W, (size_t)d.ldw, a, b, sm);
 }
 
 // One launch per step (used when cooperative launch is not available).
@@ -780,6 +852,16 @@ int jacobi_coop_max_blocks() {
       hipSuccess)
     return 0;
   return per_cu * prop.multiProcessorCount;
+}
+void launch_jacobi_gram(const UpdDesc* descs, int nt, int npairs, int S, int sweep, int step,
+                        double conv_tol, hipStream_t s) {
+  hipLaunchKernelGGL(k_jacobi_gram, dim3(npairs * S, nt), dim3(256), 0, s, descs, nt, sweep, step,
+                     S, conv_tol);
+}
+void launch_jacobi_apply(const UpdDesc* descs, int nt, int npairs, int S, int sweep, int step,
+                         double tol_rot, double conv_tol, hipStream_t s) {
+  hipLaunchKernelGGL(k_jacobi_apply, dim3(npairs * S, nt), dim3(256), 0, s, descs, nt, sweep,
+                     step, S, tol_rot, conv_tol);
 }
 void launch_finish(const UpdDesc* descs, int nt, const int* sweeps_dev, int sweeps_run,
                    hipStream_t s) {

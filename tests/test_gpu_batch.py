@@ -103,14 +103,14 @@ def test_fp32_contraction_error(gpu):
 
 
 def test_step_and_cooperative_jacobi_agree(gpu, monkeypatch):
-    """The per-step launch mode and the persistent cooperative kernel give the
-    same spectra."""
+    """The per-step launch mode (default; row-split when few pairs) and the
+    persistent cooperative kernel (TNMPS_JACOBI=coop) give the same spectra."""
     n, depth, p, chi = 12, 10, 0.1, 32
     c = haar_circuit(n, depth, p, 31)
     ref = run_trajectory(n, c.layers, ("const", chi), rs=c.draws)
     a = MPS.new_qubits(n, ("const", chi))
     a.run_circuit(c.layers, forced=ref["record"])
-    monkeypatch.setenv("TNMPS_JACOBI", "step")
+    monkeypatch.setenv("TNMPS_JACOBI", "coop")
     b = MPS.new_qubits(n, ("const", chi))
     b.run_circuit(c.layers, forced=ref["record"])
     for bd in range(n - 1):

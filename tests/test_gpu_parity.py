@@ -270,3 +270,17 @@ def test_clone_is_independent(gpu):
     cp.measure_postsel_prob(0, 1)
     np.testing.assert_allclose(st.probs(1), [0.5, 0.5], atol=1e-14)
     np.testing.assert_allclose(cp.probs(1), [0.0, 1.0], atol=1e-14)
+
+
+def test_large_bond_single_update_row_split(gpu):
+    """A lone update with theta up to 128 x 128 runs the row-split Jacobi step
+    (several workgroups per block pair); spectra match the oracle."""
+    n = 14
+    seq = _gate_seq(n, 120, 41)
+    g = MPS.new_qubits(n, ("const", 64))
+    o = VidalMPS(n, ("const", 64))
+    _apply_both(seq, g, o)
+    compare_states(g, o, what="row split")
+    for k, p in [(3, 0), (7, 1)]:
+        assert close(g.measure_postsel_prob(k, p), o.measure_postsel_prob(k, p))
+    compare_states(g, o, what="row split after sweeps")
