@@ -126,7 +126,8 @@ def main() -> int:
         "avg_launch_us": round(perf["ms_jacobi"] * 1e3 / max(perf["jacobi_launches"], 1), 3),
         "launches": perf["jacobi_launches"],
     }
-    breakdown = {k: round(perf[k], 3) for k in ("ms_theta", "ms_jacobi", "ms_post", "ms_measure")}
+    breakdown = {k: round(perf[k], 3) for k in ("ms_theta", "ms_jacobi", "ms_post", "ms_measure",
+                                                "ms_host_launch")}
     breakdown["sweeps_per_update_batch"] = round(perf["sweeps"] / max(perf["updates"], 1), 3)
     breakdown["alg_tflops_whole_update"] = round(perf["alg_flops"] / wall / 1e12, 4)
 

@@ -228,6 +228,7 @@ typedef struct {
     size_t updates;         /* two-site updates (gates and refactors) */
     size_t sweeps;          /* Jacobi sweeps summed over update batches */
     size_t jacobi_launches; /* Jacobi step kernels launched */
+    double ms_host_launch;  /* host time spent enqueueing Jacobi step kernels */
 } tn_perf;
 
 /* Turn device-time accounting on (1) or off (0).  On adds one stream

@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -57,12 +58,16 @@ double env_double(const char* name, double dflt) {
 // largest such ratio is at most conv_tol.  That sweep still rotates every pair
 // above tol_rot, and Jacobi converges quadratically, so the residual after it
 // is of order conv_tol^2 = 1e-16.
+// Inner sweeps: cyclic Jacobi sweeps over a panel's Gram matrix per visit;
+// convergence is judged on the fresh Gram matrix of the next outer sweep.
 struct Tolerances {
-  double tol_rot = 1e-15;
+  double tol_rot = 1e-14;
   double conv_tol = 1e-8;
+  int inner_sweeps = 1;
   Tolerances() {
     tol_rot = env_double("TNMPS_TOL_ROT", tol_rot);
     conv_tol = env_double("TNMPS_CONV_TOL", conv_tol);
+    inner_sweeps = std::max(1, (int)env_double("TNMPS_INNER_SWEEPS", inner_sweeps));
   }
 };
 
@@ -171,6 +176,7 @@ struct tn_engine {
   double* offs = nullptr;
   int* info = nullptr;
   int cus = 0;           // compute units of the device
+  int split_max = kMaxSplit;  // cap on workgroups per block pair (TNMPS_SPLIT_MAX)
   DescRing<UpdDesc> upd;
   DescRing<MeasDesc> meas;
   int mres = 0;  // measurement result slots
@@ -184,6 +190,7 @@ struct tn_engine {
   int* d_sweeps = nullptr;     // sweeps used by the last cooperative SVD batch
   int* h_sweeps = nullptr;
   bool timing = false;
+  double perf_host_launch_ms = 0.0;  // host time spent enqueueing Jacobi steps
   hipEvent_t tev[6] = {};
   tn_perf perf{};
   std::vector<int> h_info;
@@ -243,6 +250,7 @@ struct tn_engine {
     hipDeviceProp_t prop;
     TN_HIP(hipGetDeviceProperties(&prop, dev));
     cus = prop.multiProcessorCount;
+    split_max = std::max(1, std::min(kMaxSplit, (int)env_double("TNMPS_SPLIT_MAX", kMaxSplit)));
     const int nbmax = (ldw + kBW - 1) / kBW;
     gpart_per = (size_t)((nbmax + 1) / 2 + 1) * kMaxSplit * kPW * kPW;
     // workspace budget: TNMPS_WS_BYTES (default 24 GiB) bounds the tasks per launch
@@ -395,6 +403,7 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
     d.ldx = e->ldx;
     d.ldw = e->ldw;
     d.has_gate = q.has_gate ? 1 : 0;
+    d.inner_sweeps = tolerances().inner_sweeps;
     d.prec = h->prec;
     d.tkind = h->trunc.kind;
     d.tchi = (int)std::min<size_t>(h->trunc.chi, 1u << 30);
@@ -444,12 +453,13 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
     // Few pairs (a lone large SVD) leave most CUs idle: split each pair's
     // rows over S workgroups (gram kernel, then solve + apply kernel).
     int S = 1;
-    while (S < kMaxSplit && (size_t)nt * npairs * S * 2 <= (size_t)2 * e->cus &&
+    while (S < e->split_max && (size_t)nt * npairs * S * 2 <= (size_t)2 * e->cus &&
            maxR >= (size_t)128 * S)
       S *= 2;
     std::vector<double> offs(nt);
     for (int sw = 0; sw < kMaxSweeps; ++sw) {
       if (e->timing) TN_HIP(hipEventRecord(e->tev[2], e->stream));
+      const auto h0 = std::chrono::steady_clock::now();
       for (int s = 0; s < steps; ++s) {
         if (S == 1) {
           launch_jacobi_step(dd, nt, npairs, sw, s, tol.tol_rot, tol.conv_tol, e->stream);
@@ -459,6 +469,8 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
         }
       }
       TN_HIP(hipGetLastError());
+      e->perf_host_launch_ms +=
+          std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count();
       if (e->timing) TN_HIP(hipEventRecord(e->tev[3], e->stream));
       sweeps_run = sw + 1;
       TN_HIP(hipMemcpyAsync(offs.data(), e->offs + (size_t)sw * nt, sizeof(double) * nt,
@@ -1243,12 +1255,14 @@ tn_status tn_set_timing(tn_mps* h, int on) {
 tn_status tn_get_perf(tn_mps* h, tn_perf* out) {
   if (!h || !out) return TN_INVALID_ARG;
   *out = h->eng->perf;
+  out->ms_host_launch = h->eng->perf_host_launch_ms;
   return TN_OK;
 }
 
 tn_status tn_reset_perf(tn_mps* h) {
   if (!h) return TN_INVALID_ARG;
   h->eng->perf = tn_perf{};
+  h->eng->perf_host_launch_ms = 0.0;
   return TN_OK;
 }
 

@@ -58,6 +58,7 @@ struct UpdDesc {
   int* info;             // kInfoLen ints
   int ldx, ldw;
   int has_gate;
+  int inner_sweeps;      // inner Jacobi sweeps per panel visit
   int prec;              // 0: FP64 contraction, 1: FP32 contraction
   int tkind;             // 0 None, 1 Const, 2 Cutoff (BondDim, mps.rs:157-166)
   int tchi;

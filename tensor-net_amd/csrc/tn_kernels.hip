@@ -216,13 +216,19 @@ __device__ __forceinline__ int rr_player(int i, int step, int nbe) {
   return 1 + (i - 1 + step) % (nbe - 1);
 }
 
+// Columns whose squared norm is below kTinyNorm2 (norm 1e-17, under
+// f64::EPSILON times a unit-norm theta, so below every truncation threshold of
+// the rank rule) take part in no rotation and in no convergence measure.
+constexpr double kTinyNorm2 = 1e-34;
+
 struct PairSmem {
   double2 Ps[64][kPW + 1];   // 64-row chunk of the panel
   double2 G[kPW][kPW + 1];
   double2 Wl[kPW][kPW + 1];
   double2 part[4][kPW][kPW + 1];  // per-wave partial Gram
-  double rcs[kPW / 2];
-  double2 rsn[kPW / 2];
+  double2 Gb[2][kPW][kPW + 1];    // inner Jacobi: double-buffered G
+  double2 Wb[2][kPW][kPW + 1];    // and rotation
+  int rr_partner[kPW - 1][kPW];   // round-robin partner of each player
   int col[kPW];
   double red[4];
 };
@@ -310,12 +316,53 @@ __device__ void panel_gram(const UpdDesc& d, int r_begin, int r_end, PairSmem& s
   __syncthreads();
 }
 
+// 1/sqrt(x) and 1/x from the hardware approximations refined by two Newton
+// steps each (quadratic convergence to double precision): cheaper than the
+// IEEE-exact division and square-root sequences on the rotation critical path.
+__device__ __forceinline__ double rsq_nr(double x) {
+  double r = __builtin_amdgcn_rsq(x);
+  r = r * (1.5 - 0.5 * x * r * r);
+  r = r * (1.5 - 0.5 * x * r * r);
+  return r;
+}
+__device__ __forceinline__ double rcp_nr(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = r * (2.0 - x * r);
+  r = r * (2.0 - x * r);
+  return r;
+}
+
+// Jacobi rotation zeroing the off-diagonal of [[a, c], [conj(c), b]]:
+// Q = [[cs, sn], [-conj(sn), cs]] with sn = s e^{i arg c} (Golub-Van Loan
+// symmetric Schur 2x2 with the phase of c factored out).  Returns false
+// (identity) when the pair is already orthogonal to tol or a column is
+// negligible.
+__device__ __forceinline__ bool rot2(double a, double b, double2 c, double tol, double& cs,
+                                     double2& sn) {
+  const double c2 = cabs2(c);
+  if (!(a > kTinyNorm2 && b > kTinyNorm2 && c2 > tol * tol * a * b)) {
+    cs = 1.0;
+    sn = make_double2(0.0, 0.0);
+    return false;
+  }
+  const double rc = rsq_nr(c2);  // 1/|c|
+  const double tau = (b - a) * 0.5 * rc;
+  const double x = 1.0 + tau * tau;
+  const double t = (tau >= 0.0 ? 1.0 : -1.0) * rcp_nr(fabs(tau) + x * rsq_nr(x));
+  cs = rsq_nr(1.0 + t * t);
+  const double f = t * cs * rc;
+  sn = make_double2(f * c.x, f * c.y);
+  return true;
+}
+
 // From the panel Gram matrix in sm.G: record the largest relative
 // off-diagonal (when `record`), and unless the panel is already orthogonal
-// diagonalise G with cyclic two-sided Jacobi rotations, accumulating the
-// rotation in sm.Wl.  Returns false (for the whole block) when there is
-// nothing to apply.  A function of sm.G alone: blocks holding the same G
-// compute bitwise the same Wl.
+// diagonalise G with cyclic two-sided Jacobi rotations (round-robin order,
+// kPW/2 disjoint rotations per round), accumulating the rotation in Wl.
+// Every thread computes the two rotations its entry needs, so a round costs
+// one barrier (G and Wl are double-buffered).  On a true return the rotation
+// is in sm.Wl.  A function of sm.G alone: blocks holding the same G compute
+// bitwise the same Wl.
 __device__ bool panel_solve(const UpdDesc& d, int nt, int sweep, bool record, double tol_rot,
                             PairSmem& sm) {
   const int tid = threadIdx.x;
@@ -323,7 +370,7 @@ __device__ bool panel_solve(const UpdDesc& d, int nt, int sweep, bool record, do
   double ratio = 0.0;
   if (gi < gj) {
     const double a = sm.G[gi][gi].x, b = sm.G[gj][gj].x;
-    if (a > 0.0 && b > 0.0) ratio = sqrt(cabs2(sm.G[gi][gj])) / sqrt(a * b);
+    if (a > kTinyNorm2 && b > kTinyNorm2) ratio = sqrt(cabs2(sm.G[gi][gj])) / sqrt(a * b);
   }
   const double off = block_max(ratio, sm.red);
   if (record && tid == 0 && off > 0.0)
@@ -331,71 +378,53 @@ __device__ bool panel_solve(const UpdDesc& d, int nt, int sweep, bool record, do
               (unsigned long long)__double_as_longlong(off));
   if (off <= tol_rot) return false;
 
-  sm.Wl[gi][gj] = make_double2(gi == gj ? 1.0 : 0.0, 0.0);
+  // round-robin tables: partner of player x in round r
+  if (tid < (kPW - 1) * kPW) {
+    const int r = tid / kPW, x = tid - r * kPW;
+    const int pos = (x == 0) ? 0 : 1 + ((x - 1 - r) % (kPW - 1) + (kPW - 1)) % (kPW - 1);
+    const int other = kPW - 1 - pos;
+    sm.rr_partner[r][x] = (other == 0) ? 0 : rr_player(other, r, kPW);
+  }
+  sm.Gb[0][gi][gj] = sm.G[gi][gj];
+  sm.Wb[0][gi][gj] = make_double2(gi == gj ? 1.0 : 0.0, 0.0);
   __syncthreads();
-  for (int isw = 0; isw < 8; ++isw) {
+  int cur = 0;
+  for (int isw = 0; isw < d.inner_sweeps; ++isw) {
     int rotated = 0;
     for (int round = 0; round < kPW - 1; ++round) {
-      if (tid < kPW / 2) {
-        const int i1 = (tid == 0) ? 0 : rr_player(tid, round, kPW);
-        const int i2 = rr_player(kPW - 1 - tid, round, kPW);
-        const double a = sm.G[i1][i1].x, b = sm.G[i2][i2].x;
-        const double2 c = sm.G[i1][i2];
-        const double ac = sqrt(cabs2(c));
-        double cs = 1.0;
-        double2 sn = make_double2(0.0, 0.0);
-        if (a > 0.0 && b > 0.0 && ac > tol_rot * sqrt(a * b)) {
-          const double tau = (b - a) / (2.0 * ac);
-          const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-          cs = 1.0 / sqrt(1.0 + t * t);
-          const double s = t * cs;
-          sn = make_double2(s * c.x / ac, s * c.y / ac);  // sn e^{i phi}
-          rotated = 1;
-        }
-        sm.rcs[tid] = cs;
-        sm.rsn[tid] = sn;
+      const int qi = sm.rr_partner[round][gi], qj = sm.rr_partner[round][gj];
+      // rotation of the pair holding row index gi, and of the pair holding gj;
+      // in each pair the lower index plays i1
+      double csi, csj;
+      double2 sni, snj;
+      {
+        const int i1 = min(gi, qi), i2 = max(gi, qi);
+        rotated |= rot2(sm.Gb[cur][i1][i1].x, sm.Gb[cur][i2][i2].x, sm.Gb[cur][i1][i2], tol_rot,
+                        csi, sni);
+        const int j1 = min(gj, qj), j2 = max(gj, qj);
+        rotated |= rot2(sm.Gb[cur][j1][j1].x, sm.Gb[cur][j2][j2].x, sm.Gb[cur][j1][j2], tol_rot,
+                        csj, snj);
       }
-      __syncthreads();
-      // Q restricted to pair (i1, i2): Q[i1][i1] = cs, Q[i1][i2] = sn e,
-      // Q[i2][i1] = -conj(sn e), Q[i2][i2] = cs.
-      auto locate = [&](int x, int& pair, int& role, int& partner) {
-        const int pos =
-            (x == 0) ? 0 : 1 + ((x - 1 - round) % (kPW - 1) + (kPW - 1)) % (kPW - 1);
-        if (pos < kPW / 2) {
-          pair = pos;
-          role = 0;
-          partner = rr_player(kPW - 1 - pos, round, kPW);
-        } else {
-          pair = kPW - 1 - pos;
-          role = 1;
-          partner = (pair == 0) ? 0 : rr_player(pair, round, kPW);
-        }
-      };
-      int pi, ri, qi, pj, rj, qj;
-      locate(gi, pi, ri, qi);
-      locate(gj, pj, rj, qj);
-      const double csj = sm.rcs[pj];
-      const double2 snj = sm.rsn[pj];
+      // column j of Q: Q[j][j] = cs; off entry Q[qj][j] = -conj(sn) when j is
+      // the lower index of its pair, sn when it is the upper one
       const double2 Qjj = make_double2(csj, 0.0);
-      const double2 Qoj = (rj == 0) ? make_double2(-snj.x, snj.y) : snj;
-      const double csi = sm.rcs[pi];
-      const double2 sni = sm.rsn[pi];
+      const double2 Qoj = (gj < qj) ? make_double2(-snj.x, snj.y) : snj;
       const double2 Qii = make_double2(csi, 0.0);
-      const double2 Qoi = (ri == 0) ? make_double2(-sni.x, sni.y) : sni;
-      const double2 g_ij = sm.G[gi][gj], g_ioj = sm.G[gi][qj];
-      const double2 g_oij = sm.G[qi][gj], g_oioj = sm.G[qi][qj];
+      const double2 Qoi = (gi < qi) ? make_double2(-sni.x, sni.y) : sni;
+      const double2 g_ij = sm.Gb[cur][gi][gj], g_ioj = sm.Gb[cur][gi][qj];
+      const double2 g_oij = sm.Gb[cur][qi][gj], g_oioj = sm.Gb[cur][qi][qj];
       const double2 t0 = cadd(cmul(g_ij, Qjj), cmul(g_ioj, Qoj));
       const double2 t1 = cadd(cmul(g_oij, Qjj), cmul(g_oioj, Qoj));
-      const double2 gnew = cadd(cmulc(Qii, t0), cmulc(Qoi, t1));
-      const double2 w_ij = sm.Wl[gi][gj], w_ioj = sm.Wl[gi][qj];
-      const double2 wnew = cadd(cmul(w_ij, Qjj), cmul(w_ioj, Qoj));
-      __syncthreads();
-      sm.G[gi][gj] = gnew;
-      sm.Wl[gi][gj] = wnew;
+      sm.Gb[cur ^ 1][gi][gj] = cadd(cmulc(Qii, t0), cmulc(Qoi, t1));
+      sm.Wb[cur ^ 1][gi][gj] =
+          cadd(cmul(sm.Wb[cur][gi][gj], Qjj), cmul(sm.Wb[cur][gi][qj], Qoj));
+      cur ^= 1;
       __syncthreads();
     }
     if (!__syncthreads_or(rotated)) break;
   }
+  sm.Wl[gi][gj] = sm.Wb[cur][gi][gj];
+  __syncthreads();
   return true;
 }
 
