@@ -177,6 +177,7 @@ struct tn_engine {
   int* info = nullptr;
   int cus = 0;           // compute units of the device
   int split_max = kMaxSplit;  // cap on workgroups per block pair (TNMPS_SPLIT_MAX)
+  int bw_env = 0;             // forced Jacobi block width (TNMPS_BW), 0 = automatic
   DescRing<UpdDesc> upd;
   DescRing<MeasDesc> meas;
   int mres = 0;  // measurement result slots
@@ -251,8 +252,13 @@ struct tn_engine {
     TN_HIP(hipGetDeviceProperties(&prop, dev));
     cus = prop.multiProcessorCount;
     split_max = std::max(1, std::min(kMaxSplit, (int)env_double("TNMPS_SPLIT_MAX", kMaxSplit)));
-    const int nbmax = (ldw + kBW - 1) / kBW;
-    gpart_per = (size_t)((nbmax + 1) / 2 + 1) * kMaxSplit * kPW * kPW;
+    size_t gp = 0;
+    for (int bw : {kBW, kBWWide}) {
+      const size_t nbmax = (size_t)(ldw + bw - 1) / bw;
+      gp = std::max(gp, (nbmax / 2 + 1) * kMaxSplit * (size_t)(4 * bw * bw));
+    }
+    gpart_per = gp;
+    bw_env = (int)env_double("TNMPS_BW", 0);
     // workspace budget: TNMPS_WS_BYTES (default 24 GiB) bounds the tasks per launch
     const double budget = env_double("TNMPS_WS_BYTES", 24.0 * (1ull << 30));
     const double per = 16.0 * ((double)ldx * ldw + (double)ldw * ldw) + 64.0 * ldw + 4096.0;
@@ -427,7 +433,11 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
   launch_winit(dd, nt, gx_w, e->stream);
   TN_HIP(hipGetLastError());
   if (e->timing) TN_HIP(hipEventRecord(e->tev[1], e->stream));
-  const int nb = (int)((maxC + kBW - 1) / kBW);
+  // Block width 8 (measured faster than 16 even for a lone 512-column SVD:
+  // the 32x32 inner solve costs more than the halved step count saves);
+  // TNMPS_BW=16 selects the wide panels.
+  int bw = (e->bw_env == kBWWide && e->coop_blocks <= 0) ? kBWWide : kBW;
+  const int nb = (int)((maxC + bw - 1) / bw);
   const int nbe = nb + (nb & 1);
   const int npairs = nbe / 2, steps = nbe - 1;
   const Tolerances& tol = tolerances();
@@ -462,10 +472,11 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
       const auto h0 = std::chrono::steady_clock::now();
       for (int s = 0; s < steps; ++s) {
         if (S == 1) {
-          launch_jacobi_step(dd, nt, npairs, sw, s, tol.tol_rot, tol.conv_tol, e->stream);
+          launch_jacobi_step(dd, nt, npairs, bw, sw, s, tol.tol_rot, tol.conv_tol, e->stream);
         } else {
-          launch_jacobi_gram(dd, nt, npairs, S, sw, s, tol.conv_tol, e->stream);
-          launch_jacobi_apply(dd, nt, npairs, S, sw, s, tol.tol_rot, tol.conv_tol, e->stream);
+          launch_jacobi_gram(dd, nt, npairs, bw, S, sw, s, tol.conv_tol, e->stream);
+          launch_jacobi_apply(dd, nt, npairs, bw, S, sw, s, tol.tol_rot, tol.conv_tol,
+                              e->stream);
         }
       }
       TN_HIP(hipGetLastError());

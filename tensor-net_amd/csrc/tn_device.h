@@ -20,8 +20,8 @@
 
 namespace tnmps {
 
-constexpr int kBW = 8;          // Jacobi block width (columns)
-constexpr int kPW = 2 * kBW;    // panel width (two blocks)
+constexpr int kBW = 8;          // Jacobi block width for batched work (columns)
+constexpr int kBWWide = 16;     // Jacobi block width for lone large SVDs
 constexpr int kMaxSweeps = 40;  // outer Jacobi sweep limit
 constexpr int kNPart = 64;      // blocks of the split/renormalise kernels
 constexpr int kMaxCols = 1024;  // largest C (= min(m, n)) the SVD accepts

@@ -8,13 +8,14 @@ namespace tnmps {
 
 void launch_theta(const UpdDesc* descs, int nt, int gx, hipStream_t s);
 void launch_winit(const UpdDesc* descs, int nt, int gx, hipStream_t s);
-void launch_jacobi_step(const UpdDesc* descs, int nt, int npairs, int sweep, int step,
+// Jacobi step with block width bw (8 or 16): one workgroup per block pair.
+void launch_jacobi_step(const UpdDesc* descs, int nt, int npairs, int bw, int sweep, int step,
                         double tol_rot, double conv_tol, hipStream_t s);
 // Row-split step: S workgroups per block pair (gram, then solve + apply).
-void launch_jacobi_gram(const UpdDesc* descs, int nt, int npairs, int S, int sweep, int step,
-                        double conv_tol, hipStream_t s);
-void launch_jacobi_apply(const UpdDesc* descs, int nt, int npairs, int S, int sweep, int step,
-                         double tol_rot, double conv_tol, hipStream_t s);
+void launch_jacobi_gram(const UpdDesc* descs, int nt, int npairs, int bw, int S, int sweep,
+                        int step, double conv_tol, hipStream_t s);
+void launch_jacobi_apply(const UpdDesc* descs, int nt, int npairs, int bw, int S, int sweep,
+                         int step, double tol_rot, double conv_tol, hipStream_t s);
 hipError_t launch_jacobi_coop(const UpdDesc* descs, int nt, int npairs, int steps,
                               int max_sweeps, double tol_rot, double conv_tol, int* sweeps_out,
                               int grid, hipStream_t s);

@@ -203,15 +203,17 @@ __global__ __launch_bounds__(256) void k_winit(const UpdDesc* __restrict__ descs
 }
 
 // ---------------------------------------------------------- Jacobi step
-// Block one-sided Jacobi (Hestenes) over column blocks of width kBW.  A sweep
-// visits every pair of blocks once, in the round-robin ("circle") ordering of
-// the nbe blocks (nbe = number of blocks rounded up to even; a ghost block
-// holds no columns); one step holds nbe/2 disjoint pairs.  For its pair (a, b)
-// a workgroup forms the panel Gram matrix G = P^H P (kPW x kPW), diagonalises
-// it with cyclic two-sided Jacobi rotations (G <- Q^H G Q, Wl <- Wl Q), and
-// applies Wl to the panel of X and of W.  The largest relative off-diagonal
-// |G_ij| / sqrt(G_ii G_jj) seen in a sweep goes to offs[sweep]; a task whose
-// previous sweep stayed at or below conv_tol has converged.
+// Block one-sided Jacobi (Hestenes) over column blocks of width BW (8 or 16).
+// A sweep visits every pair of blocks once, in the round-robin ("circle")
+// ordering of the nbe blocks (nbe = number of blocks rounded up to even; a
+// ghost block holds no columns); one step holds nbe/2 disjoint pairs.  For its
+// pair (a, b) a workgroup forms the panel Gram matrix G = P^H P (PW x PW,
+// PW = 2 BW), diagonalises it with cyclic two-sided Jacobi rotations
+// (G <- Q^H G Q, Wl <- Wl Q), and applies Wl to the panel of X and of W.  The
+// largest relative off-diagonal |G_ij| / sqrt(G_ii G_jj) seen in a sweep goes
+// to offs[sweep]; a task whose previous sweep stayed at or below conv_tol has
+// converged.  BW = 16 halves the steps per sweep; it serves lone large SVDs,
+// whose time is set by the chain of steps rather than by the flops.
 __device__ __forceinline__ int rr_player(int i, int step, int nbe) {
   return 1 + (i - 1 + step) % (nbe - 1);
 }
@@ -221,62 +223,82 @@ __device__ __forceinline__ int rr_player(int i, int step, int nbe) {
 // the rank rule) take part in no rotation and in no convergence measure.
 constexpr double kTinyNorm2 = 1e-34;
 
+template <int BW>
 struct PairSmem {
-  double2 Ps[64][kPW + 1];   // 64-row chunk of the panel
-  double2 G[kPW][kPW + 1];
-  double2 Wl[kPW][kPW + 1];
-  double2 part[4][kPW][kPW + 1];  // per-wave partial Gram
-  double2 Gb[2][kPW][kPW + 1];    // inner Jacobi: double-buffered G
-  double2 Wb[2][kPW][kPW + 1];    // and rotation
-  int rr_partner[kPW - 1][kPW];   // round-robin partner of each player
-  int col[kPW];
+  static constexpr int PW = 2 * BW, LD = PW + 1;
+  double2 Ps[64][LD];           // 64-row chunk of the panel
+  double2 G[PW][LD];            // panel Gram matrix
+  double2 Wl[PW][LD];           // its accumulated rotation
+  double2 scratch[4 * PW * LD]; // per-wave partial Gram, then Gb[2] + Wb[2]
+  int rr_partner[PW - 1][PW];   // round-robin partner of each player
+  int rr_pair[PW - 1][PW];      // index of the player's pair in the round
+  double rcs[PW / 2];           // rotation of each pair of the round
+  double2 rsn[PW / 2];
+  int col[PW];
   double red[4];
+  __device__ double2& part(int w, int i, int j) { return scratch[(w * PW + i) * LD + j]; }
+  __device__ double2& Gb(int b, int i, int j) { return scratch[(b * PW + i) * LD + j]; }
+  __device__ double2& Wb(int b, int i, int j) { return scratch[((2 + b) * PW + i) * LD + j]; }
 };
 
 // Columns of block pair p at round-robin step `step` into sm.col (-1 for
 // columns beyond C).  False (for the whole block) when the pair does not
 // exist for this task's C.
-__device__ bool pair_cols(int C, int step, int p, PairSmem& sm) {
-  const int nb = (C + kBW - 1) / kBW;
+template <int BW>
+__device__ bool pair_cols(int C, int step, int p, PairSmem<BW>& sm) {
+  constexpr int PW = 2 * BW;
+  const int nb = (C + BW - 1) / BW;
   const int nbe = nb + (nb & 1);
   if (step >= nbe - 1 || p >= nbe / 2) return false;
   const int ba = (p == 0) ? 0 : rr_player(p, step, nbe);
   const int bb = rr_player(nbe - 1 - p, step, nbe);
   const int tid = threadIdx.x;
-  if (tid < kPW) {
-    const int c = (tid < kBW) ? ba * kBW + tid : bb * kBW + (tid - kBW);
+  if (tid < PW) {
+    const int c = (tid < BW) ? ba * BW + tid : bb * BW + (tid - BW);
     sm.col[tid] = (c < C) ? c : -1;
   }
   __syncthreads();
   return true;
 }
 
-// sm.G = P^H P over rows [r_begin, r_end) of the panel of X.
-// Thread = (row slice rs in 0..15, 4x4 tile ti,tj).  Row slice rs takes rows
-// rs, rs+16, ... of each 64-row chunk; the 16 slices are reduced by shuffles
-// (within a wave) and LDS (across waves).
-__device__ void panel_gram(const UpdDesc& d, int r_begin, int r_end, PairSmem& sm) {
+// Stage rows [r0, min(r0 + 64, r_end)) of the panel of M into sm.Ps (zeros
+// past r_end or for absent columns).
+template <int BW>
+__device__ __forceinline__ void stage_rows(const double2* __restrict__ M, size_t ld, int r0,
+                                           int r_end, PairSmem<BW>& sm) {
+  constexpr int PW = 2 * BW;
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int q = 0; q < PW / 4; ++q) {
+    const int e = tid + 256 * q;
+    const int rr = e & 63, cc = e >> 6;
+    const int c = sm.col[cc];
+    sm.Ps[rr][cc] = (c >= 0 && r0 + rr < r_end) ? M[(size_t)c * ld + r0 + rr]
+                                                : make_double2(0.0, 0.0);
+  }
+}
+
+// sm.G = P^H P over rows [r_begin, r_end) of the panel of X.  Thread = (row
+// slice rs, 4x4 tile of G); a row slice takes every RS-th row of each
+// 64-row chunk.  Slices are reduced by shuffles (inside a wave) and through
+// LDS (across the four waves).
+template <int BW>
+__device__ void panel_gram(const UpdDesc& d, int r_begin, int r_end, PairSmem<BW>& sm) {
+  constexpr int PW = 2 * BW, TPR = PW / 4, NTILE = TPR * TPR, RS = 256 / NTILE;
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-  const int tile = tid & 15, rs = tid >> 4;
-  const int ti = tile >> 2, tj = tile & 3;
+  const int tile = tid % NTILE, rs = tid / NTILE;
+  const int ti = tile / TPR, tj = tile % TPR;
   double2 acc[4][4];
 #pragma unroll
   for (int a = 0; a < 4; ++a)
 #pragma unroll
     for (int b = 0; b < 4; ++b) acc[a][b] = make_double2(0.0, 0.0);
   for (int r0 = r_begin; r0 < r_end; r0 += 64) {
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int e = tid + 256 * q;
-      const int rr = e & 63, cc = e >> 6;
-      const int c = sm.col[cc];
-      sm.Ps[rr][cc] = (c >= 0 && r0 + rr < r_end) ? d.X[(size_t)c * d.ldx + r0 + rr]
-                                                  : make_double2(0.0, 0.0);
-    }
+    stage_rows<BW>(d.X, (size_t)d.ldx, r0, r_end, sm);
     __syncthreads();
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const int rr = rs + 16 * k;
+#pragma unroll 2
+    for (int k = 0; k < 64 / RS; ++k) {
+      const int rr = rs + RS * k;
       double2 xi[4], xj[4];
 #pragma unroll
       for (int a = 0; a < 4; ++a) xi[a] = sm.Ps[rr][ti * 4 + a];
@@ -289,30 +311,34 @@ __device__ void panel_gram(const UpdDesc& d, int r_begin, int r_end, PairSmem& s
     }
     __syncthreads();
   }
+  // lanes tile, tile + NTILE, ... of one wave hold the same tile
 #pragma unroll
   for (int a = 0; a < 4; ++a)
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
       double2 v = acc[a][b];
-      v.x += __shfl_xor(v.x, 16, 64);
-      v.y += __shfl_xor(v.y, 16, 64);
-      v.x += __shfl_xor(v.x, 32, 64);
-      v.y += __shfl_xor(v.y, 32, 64);
+#pragma unroll
+      for (int off = NTILE; off < 64; off <<= 1) {
+        v.x += __shfl_xor(v.x, off, 64);
+        v.y += __shfl_xor(v.y, off, 64);
+      }
       acc[a][b] = v;
     }
-  if (lane < 16) {
+  if (lane < NTILE) {
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
-      for (int b = 0; b < 4; ++b) sm.part[wid][ti * 4 + a][tj * 4 + b] = acc[a][b];
+      for (int b = 0; b < 4; ++b) sm.part(wid, ti * 4 + a, tj * 4 + b) = acc[a][b];
   }
   __syncthreads();
-  const int gi = tid >> 4, gj = tid & 15;
-  double2 g = sm.part[0][gi][gj];
-  g = cadd(g, sm.part[1][gi][gj]);
-  g = cadd(g, sm.part[2][gi][gj]);
-  g = cadd(g, sm.part[3][gi][gj]);
-  sm.G[gi][gj] = g;
+  for (int e = tid; e < PW * PW; e += 256) {
+    const int gi = e / PW, gj = e % PW;
+    double2 g = sm.part(0, gi, gj);
+    g = cadd(g, sm.part(1, gi, gj));
+    g = cadd(g, sm.part(2, gi, gj));
+    g = cadd(g, sm.part(3, gi, gj));
+    sm.G[gi][gj] = g;
+  }
   __syncthreads();
 }
 
@@ -358,19 +384,23 @@ __device__ __forceinline__ bool rot2(double a, double b, double2 c, double tol, 
 // From the panel Gram matrix in sm.G: record the largest relative
 // off-diagonal (when `record`), and unless the panel is already orthogonal
 // diagonalise G with cyclic two-sided Jacobi rotations (round-robin order,
-// kPW/2 disjoint rotations per round), accumulating the rotation in Wl.
-// Every thread computes the two rotations its entry needs, so a round costs
-// one barrier (G and Wl are double-buffered).  On a true return the rotation
-// is in sm.Wl.  A function of sm.G alone: blocks holding the same G compute
-// bitwise the same Wl.
+// PW/2 disjoint rotations per round; in each pair the lower index plays the
+// first role), accumulating the rotation in sm.Wl.  Returns false (for the
+// whole block) when there is nothing to apply.  A function of sm.G alone:
+// blocks holding the same G compute bitwise the same Wl.
+template <int BW>
 __device__ bool panel_solve(const UpdDesc& d, int nt, int sweep, bool record, double tol_rot,
-                            PairSmem& sm) {
+                            PairSmem<BW>& sm) {
+  constexpr int PW = 2 * BW;
   const int tid = threadIdx.x;
-  const int gi = tid >> 4, gj = tid & 15;
   double ratio = 0.0;
-  if (gi < gj) {
-    const double a = sm.G[gi][gi].x, b = sm.G[gj][gj].x;
-    if (a > kTinyNorm2 && b > kTinyNorm2) ratio = sqrt(cabs2(sm.G[gi][gj])) / sqrt(a * b);
+  for (int e = tid; e < PW * PW; e += 256) {
+    const int gi = e / PW, gj = e % PW;
+    if (gi < gj) {
+      const double a = sm.G[gi][gi].x, b = sm.G[gj][gj].x;
+      if (a > kTinyNorm2 && b > kTinyNorm2)
+        ratio = fmax(ratio, sqrt(cabs2(sm.G[gi][gj])) / sqrt(a * b));
+    }
   }
   const double off = block_max(ratio, sm.red);
   if (record && tid == 0 && off > 0.0)
@@ -378,86 +408,92 @@ __device__ bool panel_solve(const UpdDesc& d, int nt, int sweep, bool record, do
               (unsigned long long)__double_as_longlong(off));
   if (off <= tol_rot) return false;
 
-  // round-robin tables: partner of player x in round r
-  if (tid < (kPW - 1) * kPW) {
-    const int r = tid / kPW, x = tid - r * kPW;
-    const int pos = (x == 0) ? 0 : 1 + ((x - 1 - r) % (kPW - 1) + (kPW - 1)) % (kPW - 1);
-    const int other = kPW - 1 - pos;
-    sm.rr_partner[r][x] = (other == 0) ? 0 : rr_player(other, r, kPW);
+  for (int e = tid; e < (PW - 1) * PW; e += 256) {
+    const int r = e / PW, x = e % PW;
+    const int pos = (x == 0) ? 0 : 1 + ((x - 1 - r) % (PW - 1) + (PW - 1)) % (PW - 1);
+    const int other = PW - 1 - pos;
+    sm.rr_partner[r][x] = (other == 0) ? 0 : rr_player(other, r, PW);
+    sm.rr_pair[r][x] = pos < PW / 2 ? pos : other;
   }
-  sm.Gb[0][gi][gj] = sm.G[gi][gj];
-  sm.Wb[0][gi][gj] = make_double2(gi == gj ? 1.0 : 0.0, 0.0);
+  for (int e = tid; e < PW * PW; e += 256) {
+    const int gi = e / PW, gj = e % PW;
+    sm.Gb(0, gi, gj) = sm.G[gi][gj];
+    sm.Wb(0, gi, gj) = make_double2(gi == gj ? 1.0 : 0.0, 0.0);
+  }
   __syncthreads();
   int cur = 0;
   for (int isw = 0; isw < d.inner_sweeps; ++isw) {
     int rotated = 0;
-    for (int round = 0; round < kPW - 1; ++round) {
-      const int qi = sm.rr_partner[round][gi], qj = sm.rr_partner[round][gj];
-      // rotation of the pair holding row index gi, and of the pair holding gj;
-      // in each pair the lower index plays i1
-      double csi, csj;
-      double2 sni, snj;
-      {
-        const int i1 = min(gi, qi), i2 = max(gi, qi);
-        rotated |= rot2(sm.Gb[cur][i1][i1].x, sm.Gb[cur][i2][i2].x, sm.Gb[cur][i1][i2], tol_rot,
-                        csi, sni);
-        const int j1 = min(gj, qj), j2 = max(gj, qj);
-        rotated |= rot2(sm.Gb[cur][j1][j1].x, sm.Gb[cur][j2][j2].x, sm.Gb[cur][j1][j2], tol_rot,
-                        csj, snj);
+    for (int round = 0; round < PW - 1; ++round) {
+      if (tid < PW / 2) {
+        const int p1 = (tid == 0) ? 0 : rr_player(tid, round, PW);
+        const int p2 = rr_player(PW - 1 - tid, round, PW);
+        const int i1 = min(p1, p2), i2 = max(p1, p2);
+        double cs;
+        double2 sn;
+        rotated |= rot2(sm.Gb(cur, i1, i1).x, sm.Gb(cur, i2, i2).x, sm.Gb(cur, i1, i2), tol_rot,
+                        cs, sn);
+        sm.rcs[tid] = cs;
+        sm.rsn[tid] = sn;
       }
-      // column j of Q: Q[j][j] = cs; off entry Q[qj][j] = -conj(sn) when j is
-      // the lower index of its pair, sn when it is the upper one
-      const double2 Qjj = make_double2(csj, 0.0);
-      const double2 Qoj = (gj < qj) ? make_double2(-snj.x, snj.y) : snj;
-      const double2 Qii = make_double2(csi, 0.0);
-      const double2 Qoi = (gi < qi) ? make_double2(-sni.x, sni.y) : sni;
-      const double2 g_ij = sm.Gb[cur][gi][gj], g_ioj = sm.Gb[cur][gi][qj];
-      const double2 g_oij = sm.Gb[cur][qi][gj], g_oioj = sm.Gb[cur][qi][qj];
-      const double2 t0 = cadd(cmul(g_ij, Qjj), cmul(g_ioj, Qoj));
-      const double2 t1 = cadd(cmul(g_oij, Qjj), cmul(g_oioj, Qoj));
-      sm.Gb[cur ^ 1][gi][gj] = cadd(cmulc(Qii, t0), cmulc(Qoi, t1));
-      sm.Wb[cur ^ 1][gi][gj] =
-          cadd(cmul(sm.Wb[cur][gi][gj], Qjj), cmul(sm.Wb[cur][gi][qj], Qoj));
+      __syncthreads();
+      for (int e = tid; e < PW * PW; e += 256) {
+        const int gi = e / PW, gj = e % PW;
+        const int qi = sm.rr_partner[round][gi], qj = sm.rr_partner[round][gj];
+        const int pi = sm.rr_pair[round][gi], pj = sm.rr_pair[round][gj];
+        // column j of Q: Q[j][j] = cs; off entry Q[qj][j] = -conj(sn) when j
+        // is the lower index of its pair, sn when it is the upper one
+        const double2 snj = sm.rsn[pj], sni = sm.rsn[pi];
+        const double2 Qjj = make_double2(sm.rcs[pj], 0.0);
+        const double2 Qoj = (gj < qj) ? make_double2(-snj.x, snj.y) : snj;
+        const double2 Qii = make_double2(sm.rcs[pi], 0.0);
+        const double2 Qoi = (gi < qi) ? make_double2(-sni.x, sni.y) : sni;
+        const double2 g_ij = sm.Gb(cur, gi, gj), g_ioj = sm.Gb(cur, gi, qj);
+        const double2 g_oij = sm.Gb(cur, qi, gj), g_oioj = sm.Gb(cur, qi, qj);
+        const double2 t0 = cadd(cmul(g_ij, Qjj), cmul(g_ioj, Qoj));
+        const double2 t1 = cadd(cmul(g_oij, Qjj), cmul(g_oioj, Qoj));
+        sm.Gb(cur ^ 1, gi, gj) = cadd(cmulc(Qii, t0), cmulc(Qoi, t1));
+        sm.Wb(cur ^ 1, gi, gj) =
+            cadd(cmul(sm.Wb(cur, gi, gj), Qjj), cmul(sm.Wb(cur, gi, qj), Qoj));
+      }
       cur ^= 1;
       __syncthreads();
     }
     if (!__syncthreads_or(rotated)) break;
   }
-  sm.Wl[gi][gj] = sm.Wb[cur][gi][gj];
+  for (int e = tid; e < PW * PW; e += 256) {
+    const int gi = e / PW, gj = e % PW;
+    sm.Wl[gi][gj] = sm.Wb(cur, gi, gj);
+  }
   __syncthreads();
   return true;
 }
 
 // Rows [a, b) of the panel of M (leading dimension ld) <- panel * Wl.
-// Thread = (row lane 0..63, output group jg 0..3 of 4 columns); rows are
+// Thread = (row lane 0..63, output group jg 0..3 of PW/4 columns); rows are
 // staged through LDS 64 at a time.
-__device__ void panel_apply(double2* M, size_t ld, int a, int b, PairSmem& sm) {
+template <int BW>
+__device__ void panel_apply(double2* M, size_t ld, int a, int b, PairSmem<BW>& sm) {
+  constexpr int PW = 2 * BW, NJ = PW / 4;
   const int tid = threadIdx.x;
   const int rl = tid & 63, jg = tid >> 6;
   for (int r0 = a; r0 < b; r0 += 64) {
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int e = tid + 256 * q;
-      const int rr = e & 63, cc = e >> 6;
-      const int c = sm.col[cc];
-      sm.Ps[rr][cc] = (c >= 0 && r0 + rr < b) ? M[(size_t)c * ld + r0 + rr]
-                                              : make_double2(0.0, 0.0);
-    }
+    stage_rows<BW>(M, ld, r0, b, sm);
     __syncthreads();
     const int r = r0 + rl;
-    double2 o[4];
+    double2 o[NJ];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) o[k] = make_double2(0.0, 0.0);
-#pragma unroll
-    for (int c = 0; c < kPW; ++c) {
+    for (int k = 0; k < NJ; ++k) o[k] = make_double2(0.0, 0.0);
+#pragma unroll 4
+    for (int c = 0; c < PW; ++c) {
       const double2 x = sm.Ps[rl][c];
 #pragma unroll
-      for (int k = 0; k < 4; ++k) o[k] = cadd(o[k], cmul(x, sm.Wl[c][jg * 4 + k]));
+      for (int k = 0; k < NJ; ++k) o[k] = cadd(o[k], cmul(x, sm.Wl[c][jg * NJ + k]));
     }
     if (r < b) {
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const int c = sm.col[jg * 4 + k];
+      for (int k = 0; k < NJ; ++k) {
+        const int c = sm.col[jg * NJ + k];
         if (c >= 0) M[(size_t)c * ld + r] = o[k];
       }
     }
@@ -468,14 +504,15 @@ __device__ void panel_apply(double2* M, size_t ld, int a, int b, PairSmem& sm) {
 // Process block pair p of step `step` of sweep `sweep` for task d with one
 // workgroup.  Returns normally in every case (no barrier is skipped by a
 // subset of the block).
+template <int BW>
 __device__ void jacobi_pair(const UpdDesc& d, int nt, int sweep, int step, int p,
-                            double tol_rot, PairSmem& sm) {
+                            double tol_rot, PairSmem<BW>& sm) {
   const int R = d.info[kInfoR], C = d.info[kInfoC];
-  if (!pair_cols(C, step, p, sm)) return;
-  panel_gram(d, 0, R, sm);
-  if (!panel_solve(d, nt, sweep, true, tol_rot, sm)) return;
-  panel_apply(d.X, (size_t)d.ldx, 0, R, sm);
-  panel_apply(d.W, (size_t)d.ldw, 0, C, sm);
+  if (!pair_cols<BW>(C, step, p, sm)) return;
+  panel_gram<BW>(d, 0, R, sm);
+  if (!panel_solve<BW>(d, nt, sweep, true, tol_rot, sm)) return;
+  panel_apply<BW>(d.X, (size_t)d.ldx, 0, R, sm);
+  panel_apply<BW>(d.W, (size_t)d.ldw, 0, C, sm);
 }
 
 // Row range of slice q of S over n rows (slices are multiples of 64 rows).
@@ -487,56 +524,61 @@ __device__ __forceinline__ void split_rows(int n, int S, int q, int& a, int& b) 
 
 // Row-split step, part 1: workgroup (p, q) writes the Gram matrix of rows
 // slice q of pair p's panel to gpart[p][q].
-__global__ __launch_bounds__(256, 2) void k_jacobi_gram(const UpdDesc* __restrict__ descs,
-                                                        int nt, int sweep, int step, int S,
-                                                        double conv_tol) {
-  __shared__ PairSmem sm;
+template <int BW>
+__global__ __launch_bounds__(256, BW == 8 ? 2 : 1) void k_jacobi_gram(const UpdDesc* __restrict__ descs, int nt,
+                                                     int sweep, int step, int S,
+                                                     double conv_tol) {
+  constexpr int PW = 2 * BW;
+  __shared__ PairSmem<BW> sm;
   const UpdDesc& d = descs[blockIdx.y];
   if (sweep > 0 && d.offs[(sweep - 1) * nt] <= conv_tol) return;
   const int R = d.info[kInfoR], C = d.info[kInfoC];
   const int p = blockIdx.x / S, q = blockIdx.x - (blockIdx.x / S) * S;
-  if (!pair_cols(C, step, p, sm)) return;
+  if (!pair_cols<BW>(C, step, p, sm)) return;
   int a, b;
   split_rows(R, S, q, a, b);
-  panel_gram(d, a, b, sm);
-  const int gi = threadIdx.x >> 4, gj = threadIdx.x & 15;
-  d.gpart[((size_t)p * S + q) * (kPW * kPW) + gi * kPW + gj] = sm.G[gi][gj];
+  panel_gram<BW>(d, a, b, sm);
+  for (int e = threadIdx.x; e < PW * PW; e += 256)
+    d.gpart[((size_t)p * S + q) * (PW * PW) + e] = sm.G[e / PW][e % PW];
 }
 
 // Row-split step, part 2: every workgroup of pair p sums the S partial Gram
 // matrices in the same order, so all of them solve the same G into the same
 // Wl, and applies it to its own slice of the rows of X and of W.
-__global__ __launch_bounds__(256, 2) void k_jacobi_apply(const UpdDesc* __restrict__ descs,
-                                                         int nt, int sweep, int step, int S,
-                                                         double tol_rot, double conv_tol) {
-  __shared__ PairSmem sm;
+template <int BW>
+__global__ __launch_bounds__(256, BW == 8 ? 2 : 1) void k_jacobi_apply(const UpdDesc* __restrict__ descs, int nt,
+                                                      int sweep, int step, int S,
+                                                      double tol_rot, double conv_tol) {
+  constexpr int PW = 2 * BW;
+  __shared__ PairSmem<BW> sm;
   const UpdDesc& d = descs[blockIdx.y];
   if (sweep > 0 && d.offs[(sweep - 1) * nt] <= conv_tol) return;
   const int R = d.info[kInfoR], C = d.info[kInfoC];
   const int p = blockIdx.x / S, q = blockIdx.x - (blockIdx.x / S) * S;
-  if (!pair_cols(C, step, p, sm)) return;
-  const int gi = threadIdx.x >> 4, gj = threadIdx.x & 15;
-  double2 g = make_double2(0.0, 0.0);
-  for (int s = 0; s < S; ++s)
-    g = cadd(g, d.gpart[((size_t)p * S + s) * (kPW * kPW) + gi * kPW + gj]);
-  sm.G[gi][gj] = g;
+  if (!pair_cols<BW>(C, step, p, sm)) return;
+  for (int e = threadIdx.x; e < PW * PW; e += 256) {
+    double2 g = make_double2(0.0, 0.0);
+    for (int s = 0; s < S; ++s) g = cadd(g, d.gpart[((size_t)p * S + s) * (PW * PW) + e]);
+    sm.G[e / PW][e % PW] = g;
+  }
   __syncthreads();
-  if (!panel_solve(d, nt, sweep, q == 0, tol_rot, sm)) return;
+  if (!panel_solve<BW>(d, nt, sweep, q == 0, tol_rot, sm)) return;
   int a, b;
   split_rows(R, S, q, a, b);
-  panel_apply(d.X, (size_t)d.ldx, a, b, sm);
+  panel_apply<BW>(d.X, (size_t)d.ldx, a, b, sm);
   split_rows(C, S, q, a, b);
-  panel_apply(d.W, (size_t)d.ldw, a, b, sm);
+  panel_apply<BW>(d.W, (size_t)d.ldw, a, b, sm);
 }
 
-// One launch per step (used when cooperative launch is not available).
-__global__ __launch_bounds__(256, 2) void k_jacobi_step(const UpdDesc* __restrict__ descs, int nt,
+// One launch per step, one workgroup per block pair.
+template <int BW>
+__global__ __launch_bounds__(256, BW == 8 ? 2 : 1) void k_jacobi_step(const UpdDesc* __restrict__ descs, int nt,
                                                      int sweep, int step, double tol_rot,
                                                      double conv_tol) {
-  __shared__ PairSmem sm;
+  __shared__ PairSmem<BW> sm;
   const UpdDesc& d = descs[blockIdx.y];
   if (sweep > 0 && d.offs[(sweep - 1) * nt] <= conv_tol) return;
-  jacobi_pair(d, nt, sweep, step, blockIdx.x, tol_rot, sm);
+  jacobi_pair<BW>(d, nt, sweep, step, blockIdx.x, tol_rot, sm);
 }
 
 // Persistent cooperative kernel: all sweeps and steps of a batch of SVDs in
@@ -544,11 +586,11 @@ __global__ __launch_bounds__(256, 2) void k_jacobi_step(const UpdDesc* __restric
 // Every block runs the same sequence of grid barriers: the loops have
 // grid-uniform bounds and the convergence decision after each sweep is
 // computed by every block from the same (final) per-sweep maxima.
-__global__ __launch_bounds__(256, 2) void k_jacobi_coop(const UpdDesc* __restrict__ descs, int nt,
+__global__ __launch_bounds__(256) void k_jacobi_coop(const UpdDesc* __restrict__ descs, int nt,
                                                      int npairs, int steps, int max_sweeps,
                                                      double tol_rot, double conv_tol,
                                                      int* sweeps_out) {
-  __shared__ PairSmem sm;
+  __shared__ PairSmem<8> sm;
   __shared__ int s_done;
   cg::grid_group grid = cg::this_grid();
   const int items = nt * npairs;
@@ -558,7 +600,7 @@ __global__ __launch_bounds__(256, 2) void k_jacobi_coop(const UpdDesc* __restric
         const int t = it / npairs, p = it - t * npairs;
         const UpdDesc& d = descs[t];
         if (sw > 0 && d.offs[(sw - 1) * nt] <= conv_tol) continue;
-        jacobi_pair(d, nt, sw, st, p, tol_rot, sm);
+        jacobi_pair<8>(d, nt, sw, st, p, tol_rot, sm);
       }
       grid.sync();
     }
@@ -858,10 +900,32 @@ void launch_theta(const UpdDesc* descs, int nt, int gx, hipStream_t s) {
 void launch_winit(const UpdDesc* descs, int nt, int gx, hipStream_t s) {
   hipLaunchKernelGGL(k_winit, dim3(gx, nt), dim3(256), 0, s, descs);
 }
-void launch_jacobi_step(const UpdDesc* descs, int nt, int npairs, int sweep, int step,
+void launch_jacobi_step(const UpdDesc* descs, int nt, int npairs, int bw, int sweep, int step,
                         double tol_rot, double conv_tol, hipStream_t s) {
-  hipLaunchKernelGGL(k_jacobi_step, dim3(npairs, nt), dim3(256), 0, s, descs, nt, sweep, step,
-                     tol_rot, conv_tol);
+  if (bw == 16)
+    hipLaunchKernelGGL(k_jacobi_step<16>, dim3(npairs, nt), dim3(256), 0, s, descs, nt, sweep,
+                       step, tol_rot, conv_tol);
+  else
+    hipLaunchKernelGGL(k_jacobi_step<8>, dim3(npairs, nt), dim3(256), 0, s, descs, nt, sweep,
+                       step, tol_rot, conv_tol);
+}
+void launch_jacobi_gram(const UpdDesc* descs, int nt, int npairs, int bw, int S, int sweep,
+                        int step, double conv_tol, hipStream_t s) {
+  if (bw == 16)
+    hipLaunchKernelGGL(k_jacobi_gram<16>, dim3(npairs * S, nt), dim3(256), 0, s, descs, nt,
+                       sweep, step, S, conv_tol);
+  else
+    hipLaunchKernelGGL(k_jacobi_gram<8>, dim3(npairs * S, nt), dim3(256), 0, s, descs, nt,
+                       sweep, step, S, conv_tol);
+}
+void launch_jacobi_apply(const UpdDesc* descs, int nt, int npairs, int bw, int S, int sweep,
+                         int step, double tol_rot, double conv_tol, hipStream_t s) {
+  if (bw == 16)
+    hipLaunchKernelGGL(k_jacobi_apply<16>, dim3(npairs * S, nt), dim3(256), 0, s, descs, nt,
+                       sweep, step, S, tol_rot, conv_tol);
+  else
+    hipLaunchKernelGGL(k_jacobi_apply<8>, dim3(npairs * S, nt), dim3(256), 0, s, descs, nt,
+                       sweep, step, S, tol_rot, conv_tol);
 }
 hipError_t launch_jacobi_coop(const UpdDesc* descs, int nt, int npairs, int steps,
                               int max_sweeps, double tol_rot, double conv_tol, int* sweeps_out,
@@ -881,16 +945,6 @@ int jacobi_coop_max_blocks() {
       hipSuccess)
     return 0;
   return per_cu * prop.multiProcessorCount;
-}
-void launch_jacobi_gram(const UpdDesc* descs, int nt, int npairs, int S, int sweep, int step,
-                        double conv_tol, hipStream_t s) {
-  hipLaunchKernelGGL(k_jacobi_gram, dim3(npairs * S, nt), dim3(256), 0, s, descs, nt, sweep, step,
-                     S, conv_tol);
-}
-void launch_jacobi_apply(const UpdDesc* descs, int nt, int npairs, int S, int sweep, int step,
-                         double tol_rot, double conv_tol, hipStream_t s) {
-  hipLaunchKernelGGL(k_jacobi_apply, dim3(npairs * S, nt), dim3(256), 0, s, descs, nt, sweep,
-                     step, S, tol_rot, conv_tol);
 }
 void launch_finish(const UpdDesc* descs, int nt, const int* sweeps_dev, int sweeps_run,
                    hipStream_t s) {
