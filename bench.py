@@ -25,6 +25,10 @@ for _p in (REPO, REPO / "tensor-net_amd"):
         sys.path.insert(0, str(_p))
 
 METRIC = "gate+SVD updates/s @chi=256 FP64 vs mixed; trajectories/s + p99 latency"
+# HBM bytes per dispatch of the dominant Jacobi kernels on this workload, from
+# rocprofv3 FETCH_SIZE / WRITE_SIZE runs scaled by a 1 GiB copy calibration
+# (profiles/jacobi_single_traj_chi256_hbm_traffic.md).
+TRAFFIC_BYTES = {"k_jacobi_apply<8>": 251.3 * 1024, "k_jacobi_gram<8>": 82.7 * 1024}
 FP64_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
 
@@ -73,7 +77,21 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=10546)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB (calibrates "
+                         "the FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+
+        nbytes = args.calibrate_copy << 20
+        src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes": nbytes, "copies": 3}), flush=True)
+        return 0
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the MPS path shards by trajectory and this build "
               "runs on a single device (use --gpus 1)", file=sys.stderr)
@@ -115,16 +133,21 @@ def main() -> int:
     # flops of the SVDs it performs
     jac_s = perf["ms_jacobi"] / 1e3
     achieved = perf["svd_flops"] / jac_s / 1e12 if jac_s > 0 else 0.0
+    # the Jacobi step kernels (row-split gram + solve/apply, or one-workgroup
+    # steps) are credited with the algorithmic R-SVD flops of the SVDs they
+    # perform, over the HIP-event span of the Jacobi sweeps on the library's
+    # stream (dispatch gaps between dependent launches included)
     roof = {
         "bound": "mfma",
-        "kernel": "k_jacobi_step",
+        "kernel": "jacobi steps: k_jacobi_apply<8> + k_jacobi_gram<8> (+ k_jacobi_step<8>)",
         "achieved": round(achieved, 4),
         "peak": FP64_PEAK_TFLOPS,
         "unit": "TFLOP/s",
         "frac": round(achieved / FP64_PEAK_TFLOPS, 5),
-        "traffic": None,
+        "traffic": round((TRAFFIC_BYTES["k_jacobi_apply<8>"] + TRAFFIC_BYTES["k_jacobi_gram<8>"]) / 2),
         "avg_launch_us": round(perf["ms_jacobi"] * 1e3 / max(perf["jacobi_launches"], 1), 3),
         "launches": perf["jacobi_launches"],
+        "svd_flops": perf["svd_flops"],
     }
     breakdown = {k: round(perf[k], 3) for k in ("ms_theta", "ms_jacobi", "ms_post", "ms_measure",
                                                 "ms_host_launch")}
