@@ -8,6 +8,17 @@ refactor sweeps) once the bond dimension has saturated; the warmup layers
 grow the state from |0...0> to saturation.  The metric is two-site updates
 per second (contract -> SVD -> truncate -> split, gates and refactors alike).
 
+Other workloads (``--workload``), each printing its own JSON line:
+  batched  1024 trajectories of N=32, depth 32, p on an 11-point grid
+           0.05..0.30, Const(128), advanced in lock step on one engine;
+           a window of layers after warmup is timed.
+  mixed    N=48 trajectories with Const(512): the FP32-contraction path
+           against the FP64 path on the same outcome record, per-layer
+           Schmidt-value and entropy error and both update rates.
+  serving  Poisson arrivals of trajectory requests (N in {16,32,64,128},
+           p ~ U[0, 0.4], chi_max in {64,128,256}, depth N, FP32 contraction)
+           served by continuous batching; trajectories/s and p50/p99 latency.
+
 Prints one JSON line (stdout); progress goes to stderr.
 """
 from __future__ import annotations
@@ -66,17 +77,203 @@ def cpu_baseline(st, n, trunc, layer, budget_s):
     return count / el, count, el
 
 
+def run_batched(args) -> dict:
+    """Lock-step batch: per-trajectory p from an 11-point grid, uneven bond
+    dimensions across the batch; updates/s over the timed layers."""
+    import numpy as np
+
+    from tnmps import Engine
+    from tnmps.workload import haar_circuit_fast
+
+    n, depth, chi, T = args.n or 32, args.depth or 32, args.chi or 128, args.traj or 1024
+    W, K = args.warmup, args.steps
+    pgrid = np.linspace(0.05, 0.30, 11)
+    t_gen = time.perf_counter()
+    circs = [haar_circuit_fast(n, depth, float(pgrid[i % 11]), args.seed + 7 * i) for i in range(T)]
+    log(f"generated {T} circuits in {time.perf_counter() - t_gen:.1f}s")
+    eng = Engine(chi)
+    states = [eng.new_qubits(n, ("const", chi)) for _ in range(T)]
+    draws = [iter(c.draws) for c in circs]
+    for t in range(W):
+        eng.batch_bilayer(states, [c.layers[t] for c in circs], draws=draws)
+        log(f"warmup layer {t + 1}/{W}: max bond {max(s.max_bond_dim() for s in states[:8])}")
+    eng.sync()
+    eng.set_timing(True)
+    eng.reset_perf()
+    u0, _ = eng.stats()
+    t0 = time.perf_counter()
+    ents = []
+    for t in range(W, W + K):
+        res = eng.batch_bilayer(states, [c.layers[t] for c in circs], draws=draws)
+        ents.append(float(np.mean([r[2] for r in res])))
+        log(f"timed layer {t - W + 1}/{K} ({time.perf_counter() - t0:.1f}s)")
+    eng.sync()
+    wall = time.perf_counter() - t0
+    eng.set_timing(False)
+    u1, _ = eng.stats()
+    perf = eng.perf()
+    updates = u1 - u0
+    jac_s = perf["ms_jacobi"] / 1e3
+    achieved = perf["svd_flops"] / jac_s / 1e12 if jac_s > 0 else 0.0
+    return {
+        "metric": "two-site updates/s (batched lock-step trajectories)",
+        "value": round(updates / wall, 2), "unit": "two-site updates/s", "n_gpus": 1,
+        "steps": K, "warmup": W, "ms_per_step": round(wall / K * 1e3, 2),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{T} trajectories N={n} depth={depth} p in linspace(0.05,0.30,11) "
+                               f"Const({chi}) in lock step; step = one layer of every trajectory",
+                   "trajectories": T, "n_qubits": n, "chi_max": chi, "updates_timed": updates,
+                   "trajectory_layers_per_s": round(T * K / wall, 2),
+                   "trajectories_per_s_equiv": round(T * K / wall / depth, 3),
+                   "mean_half_chain_entropy_last_layer": round(ents[-1], 6)},
+        "roofline": {"bound": "mfma", "kernel": "jacobi step kernels",
+                     "achieved": round(achieved, 4), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / FP64_PEAK_TFLOPS, 5), "traffic": None,
+                     "alg_tflops_whole_update": round(perf["alg_flops"] / wall / 1e12, 4)},
+        "breakdown": {k: round(perf[k], 2) for k in ("ms_theta", "ms_jacobi", "ms_post",
+                                                     "ms_measure", "ms_host_launch")},
+    }
+
+
+def run_mixed(args) -> dict:
+    """FP32 contraction vs FP64 on the same outcome record: per-layer error of
+    the half-chain entropy and of the Schmidt values, and both update rates."""
+    import numpy as np
+
+    from tnmps import Engine
+    from tnmps.workload import haar_circuit_fast
+
+    n, chi, T = args.n or 48, args.chi or 512, args.traj or 4
+    depth = args.depth or (args.warmup + args.steps)
+    circs = [haar_circuit_fast(n, depth, 0.15, args.seed + 11 * i) for i in range(T)]
+    out = {}
+    records = None
+    for prec in ("f64", "f32"):
+        eng = Engine(chi)
+        states = [eng.new_qubits(n, ("const", chi)) for _ in range(T)]
+        for s in states:
+            s.set_precision(prec)
+        eng.set_timing(True)
+        eng.reset_perf()
+        u0, _ = eng.stats()
+        t0 = time.perf_counter()
+        ents, recs, lams = [], [[] for _ in range(T)], []
+        draws = [iter(c.draws) for c in circs]
+        for t in range(depth):
+            if records is None:
+                res = eng.batch_bilayer(states, [c.layers[t] for c in circs], draws=draws)
+            else:
+                res = eng.batch_bilayer(states, [c.layers[t] for c in circs],
+                                        forced=[records[i][t] for i in range(T)])
+            for i, r in enumerate(res):
+                recs[i].append(r[0])
+            ents.append([r[2] for r in res])
+            lams.append([s.schmidt(n // 2 - 1) for s in states])
+            log(f"{prec} layer {t + 1}/{depth} ({time.perf_counter() - t0:.1f}s), "
+                f"max bond {max(s.max_bond_dim() for s in states)}")
+        eng.sync()
+        wall = time.perf_counter() - t0
+        u1, _ = eng.stats()
+        out[prec] = {"ents": np.array(ents), "lams": lams, "rate": (u1 - u0) / wall,
+                     "wall": wall, "perf": eng.perf()}
+        if records is None:
+            records = recs
+        del states, eng
+    e64, e32 = out["f64"]["ents"], out["f32"]["ents"]
+    ent_err = np.max(np.abs(e32 - e64), axis=1)
+    lam_err = []
+    for t in range(depth):
+        m = 0.0
+        for a, b in zip(out["f32"]["lams"][t], out["f64"]["lams"][t]):
+            k = min(len(a), len(b))
+            m = max(m, float(np.max(np.abs(a[:k] - b[:k]))) if k else 0.0)
+        lam_err.append(m)
+    return {
+        "metric": "two-site updates/s, FP32 contraction vs FP64 (mixed precision)",
+        "value": round(out["f32"]["rate"], 3), "unit": "two-site updates/s", "n_gpus": 1,
+        "steps": depth, "warmup": 0, "ms_per_step": round(out["f32"]["wall"] / depth * 1e3, 2),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 contraction, f64 SVD", "data": "synthetic",
+        "config": {"workload": f"{T} trajectories N={n} depth={depth} p=0.15 Const({chi}); the FP32 "
+                               f"run replays the FP64 run's outcome record",
+                   "fp64_updates_per_s": round(out["f64"]["rate"], 3),
+                   "entropy_abs_err_per_layer": [float(f"{x:.3e}") for x in ent_err],
+                   "schmidt_abs_err_per_layer": [float(f"{x:.3e}") for x in lam_err],
+                   "max_entropy_err": float(f"{float(np.max(ent_err)):.3e}")},
+        "roofline": None,
+    }
+
+
+def run_serving(args) -> dict:
+    """Poisson stream of trajectory requests served by continuous batching."""
+    import numpy as np
+
+    from tnmps.serve import Request, Server, percentile
+    from tnmps.workload import haar_circuit_fast
+
+    rng = np.random.default_rng(args.seed)
+    ns = [int(x) for x in args.serve_n.split(",")]
+    chis = [int(x) for x in args.serve_chi.split(",")]
+    rate, dur = args.rate, args.duration
+    reqs, t = [], 0.0
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t > dur:
+            break
+        n = int(rng.choice(ns))
+        p = float(rng.uniform(0.0, 0.4))
+        chi = int(rng.choice(chis))
+        c = haar_circuit_fast(n, n, p, int(rng.integers(1 << 30)))
+        reqs.append(Request(rid=len(reqs), n=n, p=p, chi=chi, layers=c.layers, draws=c.draws,
+                            arrival=t, prec="f32"))
+    log(f"{len(reqs)} requests over {dur}s (rate {rate}/s)")
+    srv = Server(chi_max=max(chis), hbm_budget=args.hbm_budget * 1e9)
+    u0, _ = srv.engine.stats()
+    t0 = time.perf_counter()
+    srv.run(reqs)
+    wall = time.perf_counter() - t0
+    u1, _ = srv.engine.stats()
+    lat = [r.latency for r in reqs if r.latency is not None]
+    return {
+        "metric": "trajectories/s + p50/p99 latency (continuous batching)",
+        "value": round(len(lat) / wall, 4), "unit": "trajectories/s", "n_gpus": 1,
+        "steps": srv.ticks, "warmup": 0, "ms_per_step": round(wall / max(srv.ticks, 1) * 1e3, 2),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 contraction, f64 SVD", "data": "synthetic",
+        "config": {"workload": f"Poisson arrivals {rate}/s for {dur}s, N in {ns}, p~U[0,0.4], "
+                               f"chi_max in {chis}, depth=N, Haar brickwork + Z measurements",
+                   "requests": len(reqs), "completed": len(lat),
+                   "p50_latency_s": round(percentile(lat, 50), 3),
+                   "p99_latency_s": round(percentile(lat, 99), 3),
+                   "mean_latency_s": round(float(np.mean(lat)), 3) if lat else None,
+                   "updates_per_s": round((u1 - u0) / wall, 2),
+                   "trajectory_layers_per_s": round(srv.layers_done / wall, 2),
+                   "peak_resident_bytes": srv.peak_held, "wall_s": round(wall, 2)},
+        "roofline": None,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=12)
-    ap.add_argument("--n", type=int, default=64)
-    ap.add_argument("--chi", type=int, default=256)
+    ap.add_argument("--n", type=int, default=0)
+    ap.add_argument("--chi", type=int, default=0)
     ap.add_argument("--p", type=float, default=0.1)
     ap.add_argument("--seed", type=int, default=10546)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", default="single",
+                    choices=["single", "batched", "mixed", "serving"])
+    ap.add_argument("--traj", type=int, default=0, help="trajectories (batched, mixed)")
+    ap.add_argument("--depth", type=int, default=0, help="circuit depth (batched, mixed)")
+    ap.add_argument("--rate", type=float, default=2.0, help="serving: mean arrivals per s")
+    ap.add_argument("--duration", type=float, default=20.0, help="serving: arrival window s")
+    ap.add_argument("--serve-n", default="16,32,64,128")
+    ap.add_argument("--serve-chi", default="64,128,256")
+    ap.add_argument("--hbm-budget", type=float, default=64.0, help="serving: GB of MPS state")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (calibrates "
                          "the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -97,10 +294,15 @@ def main() -> int:
               "runs on a single device (use --gpus 1)", file=sys.stderr)
         return 2
 
+    if args.workload != "single":
+        fn = {"batched": run_batched, "mixed": run_mixed, "serving": run_serving}[args.workload]
+        print(json.dumps(fn(args)), flush=True)
+        return 0
+
     from tnmps import MPS
     from tnmps.workload import haar_circuit
 
-    n, chi, p = args.n, args.chi, args.p
+    n, chi, p = args.n or 64, args.chi or 256, args.p
     W, K = args.warmup, args.steps
     trunc = ("const", chi)
     c = haar_circuit(n, W + K + 1, p, args.seed)

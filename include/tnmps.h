@@ -237,6 +237,14 @@ tn_status tn_set_timing(tn_mps* h, int on);
 tn_status tn_get_perf(tn_mps* h, tn_perf* out);
 tn_status tn_reset_perf(tn_mps* h);
 
+/* The same counters and accounting for a whole engine (all its handles). */
+tn_status tn_engine_stats(tn_engine* e, size_t* updates, size_t* sweeps);
+tn_status tn_engine_set_timing(tn_engine* e, int on);
+tn_status tn_engine_get_perf(tn_engine* e, tn_perf* out);
+tn_status tn_engine_reset_perf(tn_engine* e);
+/* Wait for all work enqueued on the engine's stream. */
+tn_status tn_engine_sync(tn_engine* e);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1249,6 +1249,40 @@ tn_status tn_batch_bilayer(tn_engine* e, size_t count, tn_mps* const* hs, const 
   return check_flags(flags);
 }
 
+tn_status tn_engine_stats(tn_engine* e, size_t* updates, size_t* sweeps) {
+  if (!e) return TN_INVALID_ARG;
+  if (updates) *updates = (size_t)e->updates;
+  if (sweeps) *sweeps = (size_t)e->sweeps;
+  return TN_OK;
+}
+
+tn_status tn_engine_set_timing(tn_engine* e, int on) {
+  if (!e) return TN_INVALID_ARG;
+  TN_HIP(hipStreamSynchronize(e->stream));
+  e->timing = on != 0;
+  return TN_OK;
+}
+
+tn_status tn_engine_get_perf(tn_engine* e, tn_perf* out) {
+  if (!e || !out) return TN_INVALID_ARG;
+  *out = e->perf;
+  out->ms_host_launch = e->perf_host_launch_ms;
+  return TN_OK;
+}
+
+tn_status tn_engine_reset_perf(tn_engine* e) {
+  if (!e) return TN_INVALID_ARG;
+  e->perf = tn_perf{};
+  e->perf_host_launch_ms = 0.0;
+  return TN_OK;
+}
+
+tn_status tn_engine_sync(tn_engine* e) {
+  if (!e) return TN_INVALID_ARG;
+  TN_HIP(hipStreamSynchronize(e->stream));
+  return TN_OK;
+}
+
 tn_status tn_stats(tn_mps* h, size_t* updates, size_t* sweeps) {
   if (!h) return TN_INVALID_ARG;
   if (updates) *updates = (size_t)h->eng->updates;

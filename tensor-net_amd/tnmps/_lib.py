@@ -90,6 +90,11 @@ SIGNATURES = {
     "tn_set_timing": (C.c_int, [_h, C.c_int]),
     "tn_get_perf": (C.c_int, [_h, C.POINTER(tn_perf)]),
     "tn_reset_perf": (C.c_int, [_h]),
+    "tn_engine_stats": (C.c_int, [_h, _szp, _szp]),
+    "tn_engine_set_timing": (C.c_int, [_h, C.c_int]),
+    "tn_engine_get_perf": (C.c_int, [_h, C.POINTER(tn_perf)]),
+    "tn_engine_reset_perf": (C.c_int, [_h]),
+    "tn_engine_sync": (C.c_int, [_h]),
 }
 
 _LIB = None

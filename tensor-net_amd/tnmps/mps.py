@@ -439,6 +439,27 @@ class Engine:
             self._lib.tn_engine_free(h)
             self._h = C.c_void_p()
 
+    def stats(self) -> Tuple[int, int]:
+        """(two-site updates, Jacobi sweeps) done by all handles of the engine."""
+        u = C.c_size_t()
+        s = C.c_size_t()
+        _check(self._lib.tn_engine_stats(self._h, C.byref(u), C.byref(s)), "engine_stats")
+        return u.value, s.value
+
+    def set_timing(self, on: bool = True) -> None:
+        _check(self._lib.tn_engine_set_timing(self._h, 1 if on else 0), "engine_set_timing")
+
+    def perf(self) -> dict:
+        p = L.tn_perf()
+        _check(self._lib.tn_engine_get_perf(self._h, C.byref(p)), "engine_perf")
+        return {name: getattr(p, name) for name, _ in L.tn_perf._fields_}
+
+    def reset_perf(self) -> None:
+        _check(self._lib.tn_engine_reset_perf(self._h), "engine_reset_perf")
+
+    def sync(self) -> None:
+        _check(self._lib.tn_engine_sync(self._h), "engine_sync")
+
     def new_qubits(self, n: int, trunc=None, chi_cap: int = 0) -> MPS:
         h = C.c_void_p()
         _check(self._lib.tn_mps_new_qubits_in(self._h, int(n), to_trunc(trunc), int(chi_cap),

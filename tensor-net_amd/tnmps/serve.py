@@ -1,0 +1,144 @@
+"""Continuous-batching trajectory server over one engine.
+
+Each request is one trajectory: an ``MPS`` of ``n`` qubits with
+``Const(chi)`` truncation driven through ``depth`` bilayers
+(``circuit::apply_bilayer``, circuit.rs:624-659) of its own circuit, with
+``Meas::Rand`` outcomes drawn from its own uniform stream, as the
+reference's trajectory drivers do one MPS per thread
+(workspace/src/haar_coev_probs.rs:33-83, 162-203).
+
+The server keeps every admitted trajectory resident on the device and, per
+tick, advances all of them by one layer in one lock-step ``batch_bilayer``
+call, so their gates and sweep steps share launches.  Requests that have
+arrived are admitted in arrival order whenever the HBM held by resident
+trajectories stays within the budget (slots are counted in bytes of Gamma
+and lambda); finished trajectories leave at once and free their memory.
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+from typing import Callable, List, Optional, Sequence
+
+import numpy as np
+
+from .mps import Engine, MPS
+
+
+def mps_bytes(n: int, chi: int) -> int:
+    """Device bytes of an n-qubit MPS with bonds capped at chi (the library's
+    allocation: Gamma slots sized for the largest reachable bond dimension)."""
+    b = [min(chi, 2 ** min(k + 1, n - k - 1, 40)) for k in range(n - 1)]
+    g = 0
+    for k in range(n):
+        left = 1 if k == 0 else b[k - 1]
+        right = 1 if k == n - 1 else b[k]
+        g += left * 2 * right
+    return 16 * g + 8 * sum(b) + 4 * (n + 1)
+
+
+@dataclass
+class Request:
+    """One trajectory request: its circuit, draws and arrival time (s)."""
+    rid: int
+    n: int
+    p: float
+    chi: int
+    layers: list                 # [(unis, meas_sites)] of length depth
+    draws: Sequence[float]
+    arrival: float = 0.0
+    prec: str = "f64"
+    # filled by the server
+    admitted: Optional[float] = None
+    done: Optional[float] = None
+    record: list = field(default_factory=list)
+    probs: list = field(default_factory=list)
+    entropy: list = field(default_factory=list)
+
+    @property
+    def depth(self) -> int:
+        return len(self.layers)
+
+    @property
+    def latency(self) -> Optional[float]:
+        return None if self.done is None else self.done - self.arrival
+
+
+class Server:
+    """Admit, advance in lock step, retire.  ``hbm_budget`` bounds the bytes of
+    the resident trajectories' state; ``max_active`` bounds their number."""
+
+    def __init__(self, chi_max: int = 256, hbm_budget: float = 64e9, max_active: int = 1024):
+        self.engine = Engine(chi_max)
+        self.chi_max = chi_max
+        self.hbm_budget = hbm_budget
+        self.max_active = max_active
+        self.ticks = 0
+        self.layers_done = 0
+        self.peak_held = 0.0   # largest bytes resident while more than one was active
+
+    def run(self, requests: List[Request], clock: Callable[[], float] = time.perf_counter,
+            realtime: bool = True) -> List[Request]:
+        """Serve ``requests`` (arrival times relative to the start).  With
+        ``realtime`` the server waits for arrivals on the wall clock;
+        otherwise every request counts as arrived at its time stamp as soon
+        as the server is idle or at a tick boundary past it."""
+        pending = sorted(requests, key=lambda r: (r.arrival, r.rid))
+        active: list = []           # [(req, state, draws iterator, next layer)]
+        held = 0.0
+        t0 = clock()
+        qi = 0
+        while qi < len(pending) or active:
+            now = clock() - t0
+            if not active and qi < len(pending) and pending[qi].arrival > now:
+                if realtime:
+                    time.sleep(min(pending[qi].arrival - now, 0.05))
+                    continue
+                now = pending[qi].arrival
+            # admit in arrival order while memory and slots allow
+            while (qi < len(pending) and pending[qi].arrival <= now + 1e-12
+                   and len(active) < self.max_active):
+                r = pending[qi]
+                need = mps_bytes(r.n, r.chi)
+                if active and held + need > self.hbm_budget:
+                    break
+                st = self.engine.new_qubits(r.n, ("const", r.chi))
+                if r.prec != "f64":
+                    st.set_precision(r.prec)
+                r.admitted = clock() - t0
+                r.entropy = [0.0]  # product state: entropy before layer 1
+                active.append([r, st, iter(r.draws), 0])
+                held += need
+                if len(active) > 1:
+                    self.peak_held = max(self.peak_held, held)
+                qi += 1
+            if not active:
+                continue
+            # one lock-step layer for every resident trajectory
+            res = self.engine.batch_bilayer(
+                [a[1] for a in active],
+                [a[0].layers[a[3]] for a in active],
+                draws=[a[2] for a in active],
+                ent_bonds=[a[0].n // 2 - 1 for a in active])
+            self.ticks += 1
+            self.layers_done += len(active)
+            t_layer = clock() - t0
+            keep = []
+            for a, (rec, prs, ent) in zip(active, res):
+                r = a[0]
+                r.record.append(rec)
+                r.probs.append(prs)
+                r.entropy.append(ent)
+                a[3] += 1
+                if a[3] >= r.depth:
+                    r.done = t_layer
+                    held -= mps_bytes(r.n, r.chi)
+                    a[1].free()
+                else:
+                    keep.append(a)
+            active = keep
+        return requests
+
+
+def percentile(xs: Sequence[float], q: float) -> float:
+    return float(np.percentile(np.asarray(xs), q)) if len(xs) else float("nan")

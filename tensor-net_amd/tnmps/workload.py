@@ -29,6 +29,16 @@ def haar4(rng: np.random.Generator) -> np.ndarray:
     return q / (d / np.abs(d))[None, :]
 
 
+def haar4_batch(rng: np.random.Generator, count: int) -> np.ndarray:
+    """``count`` Haar 4x4 unitaries at once (same construction as ``haar4``,
+    batched QR): shape (count, 4, 4)."""
+    x = rng.standard_normal(count * 32).reshape(count, 4, 4, 2)
+    z = x[..., 0] + 1j * x[..., 1]
+    q, r = np.linalg.qr(z)
+    d = np.diagonal(r, axis1=1, axis2=2)
+    return q / (d / np.abs(d))[:, None, :]
+
+
 def brickwork_sites(n: int, offs: bool) -> List[int]:
     return list(range(int(offs), n - 1, 2))
 
@@ -59,5 +69,24 @@ def haar_circuit(n: int, depth: int, p: float, seed: int) -> Circuit:
     for t in range(depth):
         unis = [(k, haar4(ru)) for k in brickwork_sites(n, t % 2 == 1)]
         c.layers.append((unis, uniform_meas(n, p, rm)))
+    c.draws = list(rd.random(n * depth + 1))
+    return c
+
+
+def haar_circuit_fast(n: int, depth: int, p: float, seed: int) -> Circuit:
+    """``haar_circuit`` with batched unitary generation (its own streams:
+    the unitaries are not those of ``haar_circuit`` for the same seed)."""
+    ru = np.random.default_rng(seed)
+    rm = np.random.default_rng(seed + 1)
+    rd = np.random.default_rng(seed + 2)
+    c = Circuit(n=n, depth=depth, p=p, seed=seed)
+    sites = [brickwork_sites(n, t % 2 == 1) for t in range(depth)]
+    mats = haar4_batch(ru, sum(len(s) for s in sites))
+    pos = 0
+    for t in range(depth):
+        unis = [(k, mats[pos + i]) for i, k in enumerate(sites[t])]
+        pos += len(sites[t])
+        meas = list(np.nonzero(rm.random(n) < p)[0])
+        c.layers.append((unis, [int(k) for k in meas]))
     c.draws = list(rd.random(n * depth + 1))
     return c
