@@ -622,6 +622,8 @@ __global__ __launch_bounds__(256) void k_jacobi_coop(const UpdDesc* __restrict__
 }
 
 // ------------------------------------------------------------- finish
+constexpr double kF32NoiseFloor = 9.5367431640625e-07;  // 2^-20
+
 // Column norms of X are the singular values (theta = U S V^H).  Sort them
 // descending, apply process_svd (mps.rs:236-293): normalise, pick the rank,
 // renormalise, and record the right-factor scale fac_j = s_j * norm2.
@@ -683,6 +685,10 @@ __global__ __launch_bounds__(1024) void k_finish(const UpdDesc* __restrict__ des
   double thr;
   if (d.tkind == 2) thr = fabs(d.teps);
   else thr = DBL_EPSILON;
+  // With an FP32 contraction, normalised singular values below the FP32
+  // round-off of theta (2^-20 relative) are noise, not Schmidt values:
+  // they are not counted as kept values of the rank rule.
+  if (d.prec == 1) thr = fmax(thr, kF32NoiseFloor);
   for (int i = tid; i < C; i += blockDim.x) {
     const double s = key[i];
     const bool ok = (s >= DBL_MIN) && isfinite(s) && (s > thr);
