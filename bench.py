@@ -228,7 +228,7 @@ def run_serving(args) -> dict:
         reqs.append(Request(rid=len(reqs), n=n, p=p, chi=chi, layers=c.layers, draws=c.draws,
                             arrival=t, prec="f32"))
     log(f"{len(reqs)} requests over {dur}s (rate {rate}/s)")
-    srv = Server(chi_max=max(chis), hbm_budget=args.hbm_budget * 1e9)
+    srv = Server(chi_max=max(chis), hbm_budget=args.hbm_budget * 1e9, log=log)
     u0, _ = srv.engine.stats()
     t0 = time.perf_counter()
     srv.run(reqs)

@@ -173,6 +173,10 @@ struct tn_engine {
   double* part = nullptr;
   double2* gpart = nullptr;
   size_t gpart_per = 0;  // elements of gpart per slot
+  unsigned* pcnt = nullptr;
+  int pcnt_per = 0;      // pair counters per slot
+  int* d_abort = nullptr;
+  int fused_blocks = 0;  // >0: fused row-split steps with grids up to this size
   double* offs = nullptr;
   int* info = nullptr;
   int cus = 0;           // compute units of the device
@@ -203,8 +207,9 @@ struct tn_engine {
 
   void release_ws() {
     hfree(X); hfree(W); hfree(sraw); hfree(sval); hfree(fac); hfree(perm);
-    hfree(part); hfree(offs); hfree(info); hfree(gpart);
+    hfree(part); hfree(offs); hfree(info); hfree(gpart); hfree(pcnt);
     gpart = nullptr;
+    pcnt = nullptr;
     X = W = nullptr;
     sraw = sval = fac = part = offs = nullptr;
     perm = info = nullptr;
@@ -224,6 +229,7 @@ struct tn_engine {
     hfree(d_scalar); d_scalar = nullptr; nscalar = 0;
     hfree(d_sweeps); d_sweeps = nullptr;
     hhfree(h_sweeps); h_sweeps = nullptr;
+    hfree(d_abort); d_abort = nullptr;
     for (hipEvent_t& ev : tev)
       if (ev) (void)hipEventDestroy(ev), ev = nullptr;
     if (own_stream && stream) (void)hipStreamDestroy(stream);
@@ -258,6 +264,12 @@ struct tn_engine {
       gp = std::max(gp, (nbmax / 2 + 1) * kMaxSplit * (size_t)(4 * bw * bw));
     }
     gpart_per = gp;
+    pcnt_per = (ldw + kBW - 1) / kBW / 2 + 2;
+    TN_HIP(hipMalloc(&d_abort, sizeof(int)));
+    TN_HIP(hipMemset(d_abort, 0, sizeof(int)));
+    // fused row-split steps (one cooperative launch per step) with TNMPS_FUSED=1;
+    // measured slower than the two-launch step, so off by default
+    fused_blocks = (env_double("TNMPS_FUSED", 0) != 0) ? jacobi_fused_max_blocks() : 0;
     bw_env = (int)env_double("TNMPS_BW", 0);
     // workspace budget: TNMPS_WS_BYTES (default 24 GiB) bounds the tasks per launch
     const double budget = env_double("TNMPS_WS_BYTES", 24.0 * (1ull << 30));
@@ -288,6 +300,7 @@ struct tn_engine {
     TN_HIP(hipMalloc(&offs, sizeof(double) * kMaxSweeps * ns));
     TN_HIP(hipMalloc(&info, sizeof(int) * kInfoLen * ns));
     TN_HIP(hipMalloc(&gpart, sizeof(double2) * gpart_per * ns));
+    TN_HIP(hipMalloc(&pcnt, sizeof(unsigned) * pcnt_per * ns));
     slots = ns;
     return upd.ensure(ns, stream);
   }
@@ -404,6 +417,9 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
     d.perm = e->perm + (size_t)t * e->ldw;
     d.part = e->part + (size_t)t * 2 * kNPart;
     d.gpart = e->gpart + (size_t)t * e->gpart_per;
+    d.pcnt = e->pcnt + (size_t)t * e->pcnt_per;
+    d.pcnt_cap = e->pcnt_per;
+    d.abort_flag = e->d_abort;
     d.offs = e->offs + t;
     d.info = e->info + (size_t)t * kInfoLen;
     d.ldx = e->ldx;
@@ -466,6 +482,8 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
     while (S < e->split_max && (size_t)nt * npairs * S * 2 <= (size_t)2 * e->cus &&
            maxR >= (size_t)128 * S)
       S *= 2;
+    const bool fused = S > 1 && bw == kBW && e->fused_blocks > 0 &&
+                       (size_t)nt * npairs * S <= (size_t)e->fused_blocks;
     std::vector<double> offs(nt);
     for (int sw = 0; sw < kMaxSweeps; ++sw) {
       if (e->timing) TN_HIP(hipEventRecord(e->tev[2], e->stream));
@@ -473,6 +491,9 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
       for (int s = 0; s < steps; ++s) {
         if (S == 1) {
           launch_jacobi_step(dd, nt, npairs, bw, sw, s, tol.tol_rot, tol.conv_tol, e->stream);
+        } else if (fused) {
+          TN_HIP(launch_jacobi_fused(dd, nt, npairs, S, sw, s, tol.tol_rot, tol.conv_tol,
+                                     e->stream));
         } else {
           launch_jacobi_gram(dd, nt, npairs, bw, S, sw, s, tol.conv_tol, e->stream);
           launch_jacobi_apply(dd, nt, npairs, bw, S, sw, s, tol.tol_rot, tol.conv_tol,
@@ -490,7 +511,7 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
       if (e->timing) {
         if (sw == 0) e->perf.ms_theta += e->span(0, 1);
         e->perf.ms_jacobi += e->span(2, 3);
-        e->perf.jacobi_launches += (size_t)steps * (S == 1 ? 1 : 2);
+        e->perf.jacobi_launches += (size_t)steps * ((S == 1 || fused) ? 1 : 2);
       }
       bool done = true;
       for (int t = 0; t < nt; ++t)

@@ -53,10 +53,13 @@ struct UpdDesc {
   double* fac;           // scale of the right factor rows (C)
   int* perm;             // sorted column order (C)
   double* part;          // per-block partial norms (2 * kNPart)
-  double2* gpart;        // row-split partial Gram matrices [pair][slice][kPW*kPW]
+  double2* gpart;        // row-split partial Gram matrices [pair][slice][PW*PW]
+  unsigned* pcnt;        // per-pair arrival counters of the fused row-split step
+  int* abort_flag;       // set when a pair barrier exceeds its spin limit
   double* offs;          // per-sweep max off-diagonal ratio, stride nt
   int* info;             // kInfoLen ints
   int ldx, ldw;
+  int pcnt_cap;          // entries of pcnt
   int has_gate;
   int inner_sweeps;      // inner Jacobi sweeps per panel visit
   int prec;              // 0: FP64 contraction, 1: FP32 contraction

@@ -16,6 +16,11 @@ void launch_jacobi_gram(const UpdDesc* descs, int nt, int npairs, int bw, int S,
                         int step, double conv_tol, hipStream_t s);
 void launch_jacobi_apply(const UpdDesc* descs, int nt, int npairs, int bw, int S, int sweep,
                          int step, double tol_rot, double conv_tol, hipStream_t s);
+// Fused row-split step with block width 8 (cooperative launch; pair barrier).
+hipError_t launch_jacobi_fused(const UpdDesc* descs, int nt, int npairs, int S, int sweep,
+                               int step, double tol_rot, double conv_tol, hipStream_t s);
+// Largest grid of k_jacobi_fused that is co-resident (0: no cooperative launch).
+int jacobi_fused_max_blocks();
 hipError_t launch_jacobi_coop(const UpdDesc* descs, int nt, int npairs, int steps,
                               int max_sweeps, double tol_rot, double conv_tol, int* sweeps_out,
                               int grid, hipStream_t s);

@@ -194,6 +194,8 @@ __global__ __launch_bounds__(256) void k_winit(const UpdDesc* __restrict__ descs
   const UpdDesc& d = descs[blockIdx.y];
   const int chiL = d.dims[0], chiR = d.dims[2];
   const int C = min(2 * chiL, 2 * chiR);
+  if (blockIdx.x == 0)
+    for (int i = threadIdx.x; i < d.pcnt_cap; i += blockDim.x) d.pcnt[i] = 0u;
   const size_t tot = (size_t)C * C;
   for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < tot;
        e += (size_t)gridDim.x * blockDim.x) {
@@ -570,6 +572,67 @@ __global__ __launch_bounds__(256, BW == 8 ? 2 : 1) void k_jacobi_apply(const Upd
   panel_apply<BW>(d.W, (size_t)d.ldw, a, b, sm);
 }
 
+// Fused row-split step (cooperative launch: every workgroup of the grid is
+// resident): workgroup (p, q) writes the Gram matrix of its row slice, meets
+// the other S-1 workgroups of pair p at a pair barrier (an arrival counter in
+// device memory; epoch = the task's step count so far), then sums the S
+// partials in the same order as its partners, solves and applies to its
+// rows.  The spin is bounded: past kSpinLimit polls the workgroup raises
+// abort_flag and every later wait returns at once, so a failed barrier ends
+// the kernel with wrong results instead of hanging it.
+constexpr unsigned kSpinLimit = 1u << 22;
+
+__device__ __forceinline__ void pair_barrier(unsigned* cnt, unsigned target, int* abort_flag) {
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    atomicAdd(cnt, 1u);
+    unsigned spins = 0;
+    while (__hip_atomic_load(cnt, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
+      __builtin_amdgcn_s_sleep(1);
+      if (++spins > kSpinLimit) {
+        atomicExch(abort_flag, 1);
+        break;
+      }
+    }
+  }
+  __syncthreads();
+  __threadfence();
+}
+
+template <int BW>
+__global__ __launch_bounds__(256, BW == 8 ? 2 : 1) void k_jacobi_fused(
+    const UpdDesc* __restrict__ descs, int nt, int sweep, int step, int S, double tol_rot,
+    double conv_tol) {
+  constexpr int PW = 2 * BW;
+  __shared__ PairSmem<BW> sm;
+  const UpdDesc& d = descs[blockIdx.y];
+  if (sweep > 0 && d.offs[(sweep - 1) * nt] <= conv_tol) return;
+  const int R = d.info[kInfoR], C = d.info[kInfoC];
+  const int p = blockIdx.x / S, q = blockIdx.x - (blockIdx.x / S) * S;
+  if (!pair_cols<BW>(C, step, p, sm)) return;
+  const int nb = (C + BW - 1) / BW;
+  const int nbe = nb + (nb & 1);
+  const unsigned epoch = (unsigned)sweep * (unsigned)(nbe - 1) + (unsigned)step;
+  int a, b;
+  split_rows(R, S, q, a, b);
+  panel_gram<BW>(d, a, b, sm);
+  for (int e = threadIdx.x; e < PW * PW; e += 256)
+    d.gpart[((size_t)p * S + q) * (PW * PW) + e] = sm.G[e / PW][e % PW];
+  pair_barrier(&d.pcnt[p], (unsigned)S * (epoch + 1u), d.abort_flag);
+  for (int e = threadIdx.x; e < PW * PW; e += 256) {
+    double2 g = make_double2(0.0, 0.0);
+    for (int s = 0; s < S; ++s) g = cadd(g, d.gpart[((size_t)p * S + s) * (PW * PW) + e]);
+    sm.G[e / PW][e % PW] = g;
+  }
+  __syncthreads();
+  if (!panel_solve<BW>(d, nt, sweep, q == 0, tol_rot, sm)) return;
+  panel_apply<BW>(d.X, (size_t)d.ldx, a, b, sm);
+  split_rows(C, S, q, a, b);
+  panel_apply<BW>(d.W, (size_t)d.ldw, a, b, sm);
+}
+
 // One launch per step, one workgroup per block pair.
 template <int BW>
 __global__ __launch_bounds__(256, BW == 8 ? 2 : 1) void k_jacobi_step(const UpdDesc* __restrict__ descs, int nt,
@@ -932,6 +995,24 @@ void launch_jacobi_apply(const UpdDesc* descs, int nt, int npairs, int bw, int S
   else
     hipLaunchKernelGGL(k_jacobi_apply<8>, dim3(npairs * S, nt), dim3(256), 0, s, descs, nt,
                        sweep, step, S, tol_rot, conv_tol);
+}
+hipError_t launch_jacobi_fused(const UpdDesc* descs, int nt, int npairs, int S, int sweep,
+                               int step, double tol_rot, double conv_tol, hipStream_t s) {
+  void* args[] = {(void*)&descs, (void*)&nt, (void*)&sweep, (void*)&step, (void*)&S,
+                  (void*)&tol_rot, (void*)&conv_tol};
+  return hipLaunchCooperativeKernel((const void*)k_jacobi_fused<8>, dim3(npairs * S, nt),
+                                    dim3(256), args, 0, s);
+}
+int jacobi_fused_max_blocks() {
+  int dev = 0, per_cu = 0;
+  hipDeviceProp_t prop;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return 0;
+  if (!prop.cooperativeLaunch) return 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_jacobi_fused<8>, 256,
+                                                   0) != hipSuccess)
+    return 0;
+  return per_cu * prop.multiProcessorCount;
 }
 hipError_t launch_jacobi_coop(const UpdDesc* descs, int nt, int npairs, int steps,
                               int max_sweeps, double tol_rot, double conv_tol, int* sweeps_out,

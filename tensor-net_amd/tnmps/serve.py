@@ -68,7 +68,8 @@ class Server:
     """Admit, advance in lock step, retire.  ``hbm_budget`` bounds the bytes of
     the resident trajectories' state; ``max_active`` bounds their number."""
 
-    def __init__(self, chi_max: int = 256, hbm_budget: float = 64e9, max_active: int = 1024):
+    def __init__(self, chi_max: int = 256, hbm_budget: float = 64e9, max_active: int = 1024,
+                 log: Optional[Callable[[str], None]] = None):
         self.engine = Engine(chi_max)
         self.chi_max = chi_max
         self.hbm_budget = hbm_budget
@@ -76,6 +77,7 @@ class Server:
         self.ticks = 0
         self.layers_done = 0
         self.peak_held = 0.0   # largest bytes resident while more than one was active
+        self.log = log
 
     def run(self, requests: List[Request], clock: Callable[[], float] = time.perf_counter,
             realtime: bool = True) -> List[Request]:
@@ -122,6 +124,8 @@ class Server:
                 ent_bonds=[a[0].n // 2 - 1 for a in active])
             self.ticks += 1
             self.layers_done += len(active)
+            if self.log and self.ticks % 10 == 0:
+                self.log(f"tick {self.ticks}: {len(active)} active, {len(pending) - qi} queued")
             t_layer = clock() - t0
             keep = []
             for a, (rec, prs, ent) in zip(active, res):
@@ -134,6 +138,9 @@ class Server:
                     r.done = t_layer
                     held -= mps_bytes(r.n, r.chi)
                     a[1].free()
+                    if self.log:
+                        self.log(f"t={t_layer:.1f}s request {r.rid} (n={r.n}, chi={r.chi}) done, "
+                                 f"latency {r.latency:.2f}s")
                 else:
                     keep.append(a)
             active = keep
