@@ -229,6 +229,7 @@ typedef struct {
     size_t sweeps;          /* Jacobi sweeps summed over update batches */
     size_t jacobi_launches; /* Jacobi step kernels launched */
     double ms_host_launch;  /* host time spent enqueueing Jacobi step kernels */
+    size_t factored;        /* updates that decomposed one factor of theta only */
 } tn_perf;
 
 /* Turn device-time accounting on (1) or off (0).  On adds one stream

@@ -182,6 +182,7 @@ struct tn_engine {
   int cus = 0;           // compute units of the device
   int split_max = kMaxSplit;  // cap on workgroups per block pair (TNMPS_SPLIT_MAX)
   int bw_env = 0;             // forced Jacobi block width (TNMPS_BW), 0 = automatic
+  bool factored = true;       // factored updates where a factor is isometric (TNMPS_FACTORED)
   DescRing<UpdDesc> upd;
   DescRing<MeasDesc> meas;
   int mres = 0;  // measurement result slots
@@ -271,6 +272,7 @@ struct tn_engine {
     // measured slower than the two-launch step, so off by default
     fused_blocks = (env_double("TNMPS_FUSED", 0) != 0) ? jacobi_fused_max_blocks() : 0;
     bw_env = (int)env_double("TNMPS_BW", 0);
+    factored = env_double("TNMPS_FACTORED", 1) != 0;
     // workspace budget: TNMPS_WS_BYTES (default 24 GiB) bounds the tasks per launch
     const double budget = env_double("TNMPS_WS_BYTES", 24.0 * (1ull << 30));
     const double per = 16.0 * ((double)ldx * ldw + (double)ldw * ldw) + 64.0 * ldw + 4096.0;
@@ -346,6 +348,12 @@ struct tn_mps {
   double* lam = nullptr;
   int* dims = nullptr;
   std::vector<size_t> ub;       // host-side upper bound of each bond dimension
+  // Per site: kFreshLeft when lambda[k-1] Gamma[k] has orthonormal columns
+  // (the site was last written as the left factor of an update at bond k),
+  // kFreshRight when Gamma[k] lambda[k] has orthonormal rows (last written as
+  // the right factor of an update at bond k-1).  A projection or a one-site
+  // operator clears both; the product state has both.
+  std::vector<unsigned char> fresh;
   size_t gam_elems = 0, lam_elems = 0;
 
   ~tn_mps() {
@@ -369,6 +377,19 @@ namespace {
 size_t rank_bound(const tn_mps* h) {
   if (h->trunc.kind == TN_TRUNC_CONST) return std::max<size_t>(h->trunc.chi, 1);
   return (size_t)-1;
+}
+
+constexpr unsigned char kFreshLeft = 1, kFreshRight = 2;
+
+// Factored mode of a refactor of bond k (no gate): 1 when site k is
+// left-fresh, 2 when site k+1 is right-fresh, the one with fewer SVD rows
+// when both are; 0 for the full theta.
+int factor_mode(const tn_engine* e, const tn_mps* h, size_t k) {
+  if (!e->factored) return 0;
+  const bool fl = (h->fresh[k] & kFreshLeft) != 0;
+  const bool fr = (h->fresh[k + 1] & kFreshRight) != 0;
+  if (fl && fr) return h->uR(k + 1) <= h->uL(k) ? 1 : 2;
+  return fl ? 1 : (fr ? 2 : 0);
 }
 
 // One two-site update of one handle.
@@ -397,12 +418,15 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
   int rs = 0;
   TN_TRY(e->upd.next(&hd, &rs));
   size_t maxC = 1, maxR = 1, maxTiles = 1, maxCC = 1;
+  bool any_factored = false;
   for (int t = 0; t < nt; ++t) {
     const Task& q = tasks[t];
     tn_mps* h = q.h;
     const size_t k = q.k;
     UpdDesc& d = hd[t];
     std::memset(&d, 0, sizeof(d));
+    d.mode = q.has_gate ? 0 : factor_mode(e, h, k);
+    any_factored = any_factored || d.mode != 0;
     d.gk = h->gam + h->goff[k];
     d.gk1 = h->gam + h->goff[k + 1];
     d.lamL = k == 0 ? nullptr : h->lam + h->loff[k - 1];
@@ -422,6 +446,9 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
     d.abort_flag = e->d_abort;
     d.offs = e->offs + t;
     d.info = e->info + (size_t)t * kInfoLen;
+    // the factored product goes to the second half of X's columns; the SVD
+    // of a factored update uses C = K <= chi_max = ldw / 2 of them
+    d.T = d.X + (size_t)(e->ldw / 2) * e->ldx;
     d.ldx = e->ldx;
     d.ldw = e->ldw;
     d.has_gate = q.has_gate ? 1 : 0;
@@ -434,7 +461,10 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
     if (q.has_gate)
       for (int i = 0; i < 16; ++i) d.gate[i] = make_double2(q.gate[2 * i], q.gate[2 * i + 1]);
     const size_t chiL = h->uL(k), chiR = h->uR(k + 1);
-    const size_t C = std::min(2 * chiL, 2 * chiR);
+    // a factored request has C = K when K < min(m, n), else the full
+    // C = min(m, n) <= K: either way C <= min(ub[k], m, n)
+    size_t C = std::min(2 * chiL, 2 * chiR);
+    if (d.mode != 0) C = std::min(C, h->ub[k]);
     maxC = std::max(maxC, C);
     maxR = std::max(maxR, std::max(2 * chiL, 2 * chiR));
     maxCC = std::max(maxCC, C * C);
@@ -521,6 +551,7 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
   }
   if (e->timing) TN_HIP(hipEventRecord(e->tev[4], e->stream));
   launch_finish(dd, nt, e->coop_blocks > 0 ? e->d_sweeps : nullptr, sweeps_run, e->stream);
+  if (any_factored) launch_fmul(dd, nt, (int)maxR, (int)maxC, e->stream);
   launch_split(dd, nt, e->stream);
   launch_renorm(dd, nt, e->stream);
   TN_HIP(hipGetLastError());
@@ -536,12 +567,18 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
     e->perf.updates += nt;
     e->perf.sweeps += sweeps_run;
     for (int t = 0; t < nt; ++t) {
+      // flops of the SVD posed (R x C; LAPACK-style count of a complex SVD
+      // with both factors) plus the contraction or the factor product
       const int* inf = e->h_info.data() + (size_t)t * kInfoLen;
       const double m = inf[kInfoM], n = inf[kInfoN], K = inf[kInfoChiM];
-      const double M = std::max(m, n), N = std::min(m, n);
+      const double M = inf[kInfoR], N = inf[kInfoC], rank = inf[kInfoRank];
       const double fsvd = 4.0 * (4.0 * M * M * N + 8.0 * M * N * N + 9.0 * N * N * N);
       e->perf.svd_flops += fsvd;
-      e->perf.alg_flops += 8.0 * m * n * K + (tasks[t].has_gate ? 32.0 * m * n : 0.0) + fsvd;
+      const int mode = inf[kInfoMode];
+      if (mode != 0) e->perf.factored++;
+      const double fpre = mode == 0 ? 8.0 * m * n * K + (tasks[t].has_gate ? 32.0 * m * n : 0.0)
+                                    : 8.0 * (mode == 1 ? m : n) * K * rank;
+      e->perf.alg_flops += fpre + fsvd;
     }
   }
   for (int t = 0; t < nt; ++t) {
@@ -549,6 +586,8 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
     const size_t k = tasks[t].k;
     const size_t chiL = h->uL(k), chiR = h->uR(k + 1);
     h->ub[k] = std::min({2 * chiL, 2 * chiR, rank_bound(h), h->bmax[k]});
+    h->fresh[k] = kFreshLeft;
+    h->fresh[k + 1] = kFreshRight;
   }
   return TN_OK;
 }
@@ -585,6 +624,7 @@ tn_status run_measures(tn_engine* e, const std::vector<MTask>& ms) {
     d.lamR = (q.k + 1 >= h->n) ? nullptr : h->lam + h->loff[q.k];
     d.dims = h->dims + q.k;
     d.forced = q.forced;
+    if (q.forced != -2) h->fresh[q.k] = 0;  // projected
     d.slot = q.slot;
     d.r = q.r;
     d.out_outcome = e->d_out;
@@ -737,6 +777,7 @@ tn_status exec_gates(tn_engine* e, const std::vector<tn_mps*>& hs,
       for (const GateOp& g : groups[i][j]) {
         if (g.dim == 2) {
           launch_u1(hs[i]->gam + hs[i]->goff[g.k], hs[i]->dims + g.k, g.op, 64, e->stream);
+          hs[i]->fresh[g.k] = 0;  // the operator need not be unitary
           TN_HIP(hipGetLastError());
         } else {
           Task t;
@@ -821,6 +862,7 @@ tn_status new_handle(tn_engine* e, size_t n, tn_trunc trunc, size_t chi_cap, tn_
   }
   h->lam_elems = std::max<size_t>(l, 1);
   h->ub.assign(n > 1 ? n - 1 : 0, 1);
+  h->fresh.assign(n, kFreshLeft | kFreshRight);
   h->eng = e;
   e->users++;
   auto fail = [&](hipError_t er) {
@@ -980,6 +1022,7 @@ tn_status tn_mps_clone(const tn_mps* h, tn_mps** out) {
   TN_HIP(hipMemcpy(c->lam, h->lam, sizeof(double) * h->lam_elems, hipMemcpyDeviceToDevice));
   TN_HIP(hipMemcpy(c->dims, h->dims, sizeof(int) * (h->n + 1), hipMemcpyDeviceToDevice));
   c->ub = h->ub;
+  c->fresh = h->fresh;
   *out = c;
   return TN_OK;
 }

@@ -10,11 +10,19 @@
 //   dims  : n+1 ints; dims[0] = dims[n] = 1, dims[k+1] = dimension of bond k.
 //
 // One two-site update (contract -> SVD -> truncate -> split -> renormalise)
-// runs as a "task".  Its SVD works on X, the contracted theta (m x n, with
-// m = 2 chiL, n = 2 chiR) or its conjugate transpose, whichever has at least
-// as many rows as columns, stored column-major with R = max(m, n) rows and
-// C = min(m, n) columns.  W (C x C, column-major) accumulates the one-sided
-// Jacobi rotations: on exit X W has orthogonal columns.
+// runs as a "task".  theta = A M with A = lamL Gamma[k] (m x K) and
+// M = lamM Gamma[k+1] lamR (K x n), m = 2 chiL, K = chiM, n = 2 chiR.
+// In the full mode the SVD works on X, the contracted theta or its conjugate
+// transpose, whichever has at least as many rows as columns, stored
+// column-major with R = max(m, n) rows and C = min(m, n) columns.  In the
+// factored modes one factor is known to be an isometry (its site was last
+// written from the singular vectors of an update and not touched since), and
+// K < min(m, n): the SVD of theta is that of the other factor, so X is
+//   mode 1 (lamL Gamma[k] has orthonormal columns):   X = M^H (n x K)
+//   mode 2 (Gamma[k+1] lamR has orthonormal rows):    X = A lamM (m x K)
+// and the isometric factor is multiplied into the new Gamma afterwards.
+// W (C x C, column-major) accumulates the one-sided Jacobi rotations: on
+// exit X W has orthogonal columns.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -30,7 +38,7 @@ constexpr int kMaxSplit = 8;    // workgroups per block pair in the row-split st
 // info[] slots of a task (device ints)
 enum {
   kInfoM = 0, kInfoN, kInfoR, kInfoC, kInfoOrient, kInfoRank, kInfoFlags,
-  kInfoSweeps, kInfoChiL, kInfoChiM, kInfoChiR, kInfoLen = 16
+  kInfoSweeps, kInfoChiL, kInfoChiM, kInfoChiR, kInfoMode, kInfoLen = 16
 };
 
 // flag bits in info[kInfoFlags]
@@ -58,9 +66,12 @@ struct UpdDesc {
   int* abort_flag;       // set when a pair barrier exceeds its spin limit
   double* offs;          // per-sweep max off-diagonal ratio, stride nt
   int* info;             // kInfoLen ints
+  double2* T;            // factored modes: product of the kept singular vectors
+                         // with the isometric factor (ldx rows)
   int ldx, ldw;
   int pcnt_cap;          // entries of pcnt
   int has_gate;
+  int mode;              // requested: 0 full, 1 left factor isometric, 2 right
   int inner_sweeps;      // inner Jacobi sweeps per panel visit
   int prec;              // 0: FP64 contraction, 1: FP32 contraction
   int tkind;             // 0 None, 1 Const, 2 Cutoff (BondDim, mps.rs:157-166)

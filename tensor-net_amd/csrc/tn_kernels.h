@@ -29,6 +29,9 @@ hipError_t launch_jacobi_coop(const UpdDesc* descs, int nt, int npairs, int step
 int jacobi_coop_max_blocks();
 void launch_finish(const UpdDesc* descs, int nt, const int* sweeps_dev, int sweeps_run,
                    hipStream_t s);
+// Factored updates: T = kept singular vectors x isometric factor; grid bounds
+// maxX >= max(m, n) and maxRank >= rank over the tasks.
+void launch_fmul(const UpdDesc* descs, int nt, int maxX, int maxRank, hipStream_t s);
 void launch_split(const UpdDesc* descs, int nt, hipStream_t s);
 void launch_renorm(const UpdDesc* descs, int nt, hipStream_t s);
 void launch_measure(const MeasDesc* descs, int nm, hipStream_t s);

@@ -13,6 +13,8 @@
 //                  (stands in for zgesvd, mps.rs:299)
 //   k_finish       singular values, sort, rank rule, renormalisation
 //                  (process_svd, mps.rs:236-293)
+//   k_fmul         factored updates: kept singular vectors times the
+//                  isometric factor
 //   k_split        new Gamma[k], Gamma[k+1] with lambda divisions and partial
 //                  norms (mps.rs:852-876, 1357-1380, local_norm 1147-1202)
 //   k_renorm       local_renormalize of both sites (mps.rs:1206-1209)
@@ -142,22 +144,50 @@ __device__ __forceinline__ void theta_point(const UpdDesc& d, int v, int w, int 
   }
 }
 
+// Shape of a task's SVD problem.  The factored modes apply when requested
+// (no gate) and K < min(m, n); otherwise the full theta is decomposed.  The
+// host bounds C by min(K, m, n) for a factored request: C never exceeds it.
+struct TaskShape {
+  int m, n, K, mode, R, C, orient;
+};
+__device__ __forceinline__ TaskShape task_shape(const UpdDesc& d) {
+  TaskShape t;
+  t.m = 2 * d.dims[0];
+  t.K = d.dims[1];
+  t.n = 2 * d.dims[2];
+  t.mode = (d.mode != 0 && !d.has_gate && t.K < min(t.m, t.n)) ? d.mode : 0;
+  if (t.mode == 1) {
+    t.R = t.n; t.C = t.K; t.orient = 1;
+  } else if (t.mode == 2) {
+    t.R = t.m; t.C = t.K; t.orient = 0;
+  } else {
+    t.orient = (t.m >= t.n) ? 0 : 1;
+    t.R = max(t.m, t.n);
+    t.C = min(t.m, t.n);
+  }
+  return t;
+}
+
 // theta'[v,(s,t),w] = sum_{s't'} U[(s,t),(s',t')] lamL_v lamR_w
 //                     sum_u Gk[v,s',u] lamM_u Gk1[u,t',w]
 // written into X as theta (orient 0, m >= n) or theta^H (orient 1).
-// Tile: 4 values of v x 64 values of w per 256-thread block.
+// Tile: 4 values of v x 64 values of w per 256-thread block.  In the
+// factored modes X is filled elementwise instead (no contraction):
+//   mode 1: X[(t,w), u] = conj(lamM_u Gk1[u,t,w] lamR_w)
+//   mode 2: X[(v,s), u] = lamL_v Gk[v,s,u] lamM_u
 __global__ __launch_bounds__(256) void k_theta(const UpdDesc* __restrict__ descs, int nt) {
   const UpdDesc& d = descs[blockIdx.y];
   const int chiL = d.dims[0], chiM = d.dims[1], chiR = d.dims[2];
-  const int m = 2 * chiL, n = 2 * chiR;
-  const int orient = (m >= n) ? 0 : 1;
+  const TaskShape ts = task_shape(d);
+  const int orient = ts.orient;
   if (blockIdx.x == 0) {
     if (threadIdx.x == 0) {
-      d.info[kInfoM] = m;
-      d.info[kInfoN] = n;
-      d.info[kInfoR] = orient ? n : m;
-      d.info[kInfoC] = orient ? m : n;
+      d.info[kInfoM] = ts.m;
+      d.info[kInfoN] = ts.n;
+      d.info[kInfoR] = ts.R;
+      d.info[kInfoC] = ts.C;
       d.info[kInfoOrient] = orient;
+      d.info[kInfoMode] = ts.mode;
       d.info[kInfoFlags] = 0;
       d.info[kInfoSweeps] = 0;
       d.info[kInfoChiL] = chiL;
@@ -165,6 +195,24 @@ __global__ __launch_bounds__(256) void k_theta(const UpdDesc* __restrict__ descs
       d.info[kInfoChiR] = chiR;
     }
     for (int s = threadIdx.x; s < kMaxSweeps; s += blockDim.x) d.offs[s * nt] = 0.0;
+  }
+  if (ts.mode != 0) {
+    const size_t tot = (size_t)ts.R * chiM;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += stride) {
+      if (ts.mode == 1) {
+        const int u = (int)(e / ts.n), r = (int)(e % ts.n);
+        double f = d.lamM[u];
+        if (d.lamR) f *= d.lamR[r % chiR];
+        d.X[(size_t)u * d.ldx + r] = cconj(cscale(d.gk1[e], f));
+      } else {
+        const int r = (int)(e / chiM), u = (int)(e % chiM);
+        double f = d.lamM[u];
+        if (d.lamL) f *= d.lamL[r >> 1];
+        d.X[(size_t)u * d.ldx + r] = cscale(d.gk[e], f);
+      }
+    }
+    return;
   }
   const int nvt = (chiL + 3) / 4, nwt = (chiR + 63) / 64;
   const int tv = threadIdx.x >> 6, tw = threadIdx.x & 63;
@@ -192,8 +240,7 @@ __global__ __launch_bounds__(256) void k_theta(const UpdDesc* __restrict__ descs
 // ---------------------------------------------------------------- W = I
 __global__ __launch_bounds__(256) void k_winit(const UpdDesc* __restrict__ descs) {
   const UpdDesc& d = descs[blockIdx.y];
-  const int chiL = d.dims[0], chiR = d.dims[2];
-  const int C = min(2 * chiL, 2 * chiR);
+  const int C = task_shape(d).C;
   if (blockIdx.x == 0)
     for (int i = threadIdx.x; i < d.pcnt_cap; i += blockDim.x) d.pcnt[i] = 0u;
   const size_t tot = (size_t)C * C;
@@ -789,14 +836,92 @@ __global__ __launch_bounds__(1024) void k_finish(const UpdDesc* __restrict__ des
   }
 }
 
+// ------------------------------------------------------ factor product
+// Factored modes, after k_finish: the kept singular vectors times the
+// isometric factor, for the kept columns j < rank.
+//   mode 1: T[j][row] = sum_u Gk[row, u] W[u, perm_j]          (row < m)
+//           = Gamma[k] of lamL Gamma[k] U_M (already divided by lamL)
+//   mode 2: T[j][c]   = sum_u conj(W[u, perm_j]) Gk1[u, c]     (c < n)
+//           = rows of V_A^H Gamma[k+1] (already divided by lamR)
+// Tile: 64 values of the free index x 32 values of j per 256-thread block,
+// u streamed through LDS 16 at a time.  Grid (x tiles, j tiles, task).
+__global__ __launch_bounds__(256) void k_fmul(const UpdDesc* __restrict__ descs) {
+  const UpdDesc& d = descs[blockIdx.z];
+  const int mode = d.info[kInfoMode];
+  if (mode == 0) return;
+  const int K = d.info[kInfoChiM];
+  const int rank = d.info[kInfoRank];
+  const int nx = (mode == 1) ? d.info[kInfoM] : d.info[kInfoN];
+  const int x0 = blockIdx.x * 64, j0 = blockIdx.y * 32;
+  if (x0 >= nx || j0 >= rank) return;
+  __shared__ double2 Ps[16][65];  // [u][x]
+  __shared__ double2 Ws[16][33];  // [u][j]
+  __shared__ int pj[32];
+  const int tid = threadIdx.x;
+  if (tid < 32) pj[tid] = (j0 + tid < rank) ? d.perm[j0 + tid] : -1;
+  __syncthreads();
+  const int tx = tid & 63, ty = tid >> 6;
+  double2 acc[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) acc[k] = make_double2(0.0, 0.0);
+  const double2* __restrict__ G = (mode == 1) ? d.gk : d.gk1;
+  for (int u0 = 0; u0 < K; u0 += 16) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int e = tid + 256 * q;
+      int uu, xx;
+      if (mode == 1) {  // Gk[x * K + u]: u contiguous
+        uu = e & 15;
+        xx = e >> 4;
+      } else {          // Gk1[u * n + x]: x contiguous
+        xx = e & 63;
+        uu = e >> 6;
+      }
+      const int u = u0 + uu, x = x0 + xx;
+      double2 v = make_double2(0.0, 0.0);
+      if (u < K && x < nx) v = (mode == 1) ? G[(size_t)x * K + u] : G[(size_t)u * nx + x];
+      Ps[uu][xx] = v;
+    }
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int e = tid + 256 * q;
+      const int uu = e & 15, jj = e >> 4;
+      const int u = u0 + uu, c = pj[jj];
+      double2 w = make_double2(0.0, 0.0);
+      if (c >= 0 && u < K) w = d.W[(size_t)c * d.ldw + u];
+      Ws[uu][jj] = (mode == 2) ? cconj(w) : w;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int uu = 0; uu < 16; ++uu) {
+      const double2 pv = Ps[uu][tx];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) acc[k] = cadd(acc[k], cmul(pv, Ws[uu][ty * 8 + k]));
+    }
+    __syncthreads();
+  }
+  const int x = x0 + tx;
+  if (x < nx) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int j = j0 + ty * 8 + k;
+      if (j < rank) d.T[(size_t)j * d.ldx + x] = acc[k];
+    }
+  }
+}
+
 // -------------------------------------------------------------- split
 // Gamma[k][v,s,j]   = U[(v,s), j] / lamL_v
 // Gamma[k+1][j,t,w] = (fac_j V^H[j, (t,w)]) / s_j / lamR_w
-// plus the partial sums of local_norm for both sites.
+// plus the partial sums of local_norm for both sites.  U and V^H come from
+//   full mode, orient 0: U = X / sraw, V = W;  orient 1: U = W, V = X / sraw
+//   mode 1: Gamma[k] = T (lamL already divided out), V = X / sraw
+//   mode 2: U = X / sraw, V^H / lamR = T
 __global__ __launch_bounds__(256) void k_split(const UpdDesc* __restrict__ descs) {
   const UpdDesc& d = descs[blockIdx.y];
   const int chiL = d.info[kInfoChiL], chiR = d.info[kInfoChiR];
   const int orient = d.info[kInfoOrient];
+  const int mode = d.info[kInfoMode];
   const int rank = d.info[kInfoRank];
   __shared__ double red[4];
   const size_t nk = (size_t)chiL * 2 * rank, nk1 = (size_t)rank * 2 * chiR;
@@ -807,16 +932,17 @@ __global__ __launch_bounds__(256) void k_split(const UpdDesc* __restrict__ descs
     const int row = (int)(e / rank);  // v*2 + s
     const int v = row >> 1;
     const int pj = d.perm[j];
+    const double wl = d.lamL ? d.lamL[v] : 1.0;
     double2 u;
-    if (orient == 0) {
-      u = cdivr(d.X[(size_t)pj * d.ldx + row], d.sraw[pj]);
+    if (mode == 1) {
+      u = d.T[(size_t)j * d.ldx + row];
     } else {
-      u = d.W[(size_t)pj * d.ldw + row];
-    }
-    double wl = 1.0;
-    if (d.lamL) {
-      wl = d.lamL[v];
-      u = cdivr(u, wl);
+      if (orient == 0) {
+        u = cdivr(d.X[(size_t)pj * d.ldx + row], d.sraw[pj]);
+      } else {
+        u = d.W[(size_t)pj * d.ldw + row];
+      }
+      if (d.lamL) u = cdivr(u, wl);
     }
     d.gk[e] = u;
     const double sj = d.sval[j];
@@ -827,19 +953,20 @@ __global__ __launch_bounds__(256) void k_split(const UpdDesc* __restrict__ descs
     const int cidx = (int)(e % (2 * (size_t)chiR));  // t*chiR + w
     const int w = cidx % chiR;
     const int pj = d.perm[j];
-    double2 vh;
-    if (orient == 0) {
-      vh = cconj(d.W[(size_t)pj * d.ldw + cidx]);
-    } else {
-      vh = cconj(cdivr(d.X[(size_t)pj * d.ldx + cidx], d.sraw[pj]));
-    }
-    double2 g = cscale(vh, d.fac[j]);
     const double sj = d.sval[j];
-    g = cdivr(g, sj);
-    double wr = 1.0;
-    if (d.lamR) {
-      wr = d.lamR[w];
-      g = cdivr(g, wr);
+    const double wr = d.lamR ? d.lamR[w] : 1.0;
+    double2 g;
+    if (mode == 2) {
+      g = cdivr(cscale(d.T[(size_t)j * d.ldx + cidx], d.fac[j]), sj);
+    } else {
+      double2 vh;
+      if (orient == 0) {
+        vh = cconj(d.W[(size_t)pj * d.ldw + cidx]);
+      } else {
+        vh = cconj(cdivr(d.X[(size_t)pj * d.ldx + cidx], d.sraw[pj]));
+      }
+      g = cdivr(cscale(vh, d.fac[j]), sj);
+      if (d.lamR) g = cdivr(g, wr);
     }
     d.gk1[e] = g;
     accB += cabs2(g) * (sj * sj) * (wr * wr);
@@ -1036,6 +1163,10 @@ int jacobi_coop_max_blocks() {
 void launch_finish(const UpdDesc* descs, int nt, const int* sweeps_dev, int sweeps_run,
                    hipStream_t s) {
   hipLaunchKernelGGL(k_finish, dim3(nt), dim3(1024), 0, s, descs, nt, sweeps_dev, sweeps_run);
+}
+void launch_fmul(const UpdDesc* descs, int nt, int maxX, int maxRank, hipStream_t s) {
+  hipLaunchKernelGGL(k_fmul, dim3((maxX + 63) / 64, (maxRank + 31) / 32, nt), dim3(256), 0, s,
+                     descs);
 }
 void launch_split(const UpdDesc* descs, int nt, hipStream_t s) {
   hipLaunchKernelGGL(k_split, dim3(kNPart, nt), dim3(256), 0, s, descs);

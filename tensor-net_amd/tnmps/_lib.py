@@ -40,7 +40,8 @@ class tn_perf(C.Structure):
                 ("ms_post", C.c_double), ("ms_measure", C.c_double),
                 ("alg_flops", C.c_double), ("svd_flops", C.c_double),
                 ("updates", C.c_size_t), ("sweeps", C.c_size_t),
-                ("jacobi_launches", C.c_size_t), ("ms_host_launch", C.c_double)]
+                ("jacobi_launches", C.c_size_t), ("ms_host_launch", C.c_double),
+                ("factored", C.c_size_t)]
 
 
 _sz = C.c_size_t
