@@ -60,21 +60,20 @@ double env_double(const char* name, double dflt) {
 // is of order conv_tol^2 = 1e-16.
 // Inner sweeps: cyclic Jacobi sweeps over a panel's Gram matrix per visit;
 // convergence is judged on the fresh Gram matrix of the next outer sweep.
+// Jacobi tolerances and inner-solve options, read from the environment when
+// an engine is made.
 struct Tolerances {
   double tol_rot = 1e-14;
   double conv_tol = 1e-8;
   int inner_sweeps = 1;
-  Tolerances() {
+  int cross_only = 0;
+  void read_env() {
     tol_rot = env_double("TNMPS_TOL_ROT", tol_rot);
     conv_tol = env_double("TNMPS_CONV_TOL", conv_tol);
     inner_sweeps = std::max(1, (int)env_double("TNMPS_INNER_SWEEPS", inner_sweeps));
+    cross_only = env_double("TNMPS_CROSS_ONLY", cross_only) != 0 ? 1 : 0;
   }
 };
-
-const Tolerances& tolerances() {
-  static Tolerances t;
-  return t;
-}
 
 void hfree(void* p) {
   if (p) (void)hipFree(p);
@@ -173,16 +172,14 @@ struct tn_engine {
   double* part = nullptr;
   double2* gpart = nullptr;
   size_t gpart_per = 0;  // elements of gpart per slot
-  unsigned* pcnt = nullptr;
-  int pcnt_per = 0;      // pair counters per slot
-  int* d_abort = nullptr;
-  int fused_blocks = 0;  // >0: fused row-split steps with grids up to this size
   double* offs = nullptr;
   int* info = nullptr;
   int cus = 0;           // compute units of the device
   int split_max = kMaxSplit;  // cap on workgroups per block pair (TNMPS_SPLIT_MAX)
+  int split_rows = 128;       // least X rows per row-split workgroup (TNMPS_SPLIT_ROWS)
   int bw_env = 0;             // forced Jacobi block width (TNMPS_BW), 0 = automatic
   bool factored = true;       // factored updates where a factor is isometric (TNMPS_FACTORED)
+  Tolerances tol;             // Jacobi tolerances and inner-solve options
   DescRing<UpdDesc> upd;
   DescRing<MeasDesc> meas;
   int mres = 0;  // measurement result slots
@@ -192,9 +189,6 @@ struct tn_engine {
   int* d_flag = nullptr;
   double* d_scalar = nullptr;  // scalar results (entropies)
   int nscalar = 0;
-  int coop_blocks = 0;         // >0: persistent cooperative Jacobi with this grid cap
-  int* d_sweeps = nullptr;     // sweeps used by the last cooperative SVD batch
-  int* h_sweeps = nullptr;
   bool timing = false;
   double perf_host_launch_ms = 0.0;  // host time spent enqueueing Jacobi steps
   hipEvent_t tev[6] = {};
@@ -208,9 +202,8 @@ struct tn_engine {
 
   void release_ws() {
     hfree(X); hfree(W); hfree(sraw); hfree(sval); hfree(fac); hfree(perm);
-    hfree(part); hfree(offs); hfree(info); hfree(gpart); hfree(pcnt);
+    hfree(part); hfree(offs); hfree(info); hfree(gpart);
     gpart = nullptr;
-    pcnt = nullptr;
     X = W = nullptr;
     sraw = sval = fac = part = offs = nullptr;
     perm = info = nullptr;
@@ -228,9 +221,6 @@ struct tn_engine {
     upd.release();
     meas.release();
     hfree(d_scalar); d_scalar = nullptr; nscalar = 0;
-    hfree(d_sweeps); d_sweeps = nullptr;
-    hhfree(h_sweeps); h_sweeps = nullptr;
-    hfree(d_abort); d_abort = nullptr;
     for (hipEvent_t& ev : tev)
       if (ev) (void)hipEventDestroy(ev), ev = nullptr;
     if (own_stream && stream) (void)hipStreamDestroy(stream);
@@ -249,30 +239,20 @@ struct tn_engine {
     chi_max = chi;
     ldx = ldw = (int)(2 * chi);
     for (hipEvent_t& ev : tev) TN_HIP(hipEventCreate(&ev));
-    TN_HIP(hipMalloc(&d_sweeps, sizeof(int)));
-    TN_HIP(hipHostMalloc(&h_sweeps, sizeof(int), hipHostMallocDefault));
-    // Jacobi launch mode: per-step launches (default) or, with
-    // TNMPS_JACOBI=coop, one persistent cooperative launch per SVD batch
-    const char* mode = std::getenv("TNMPS_JACOBI");
-    coop_blocks = (mode && std::strcmp(mode, "coop") == 0) ? jacobi_coop_max_blocks() : 0;
     hipDeviceProp_t prop;
     TN_HIP(hipGetDeviceProperties(&prop, dev));
     cus = prop.multiProcessorCount;
     split_max = std::max(1, std::min(kMaxSplit, (int)env_double("TNMPS_SPLIT_MAX", kMaxSplit)));
+    split_rows = std::max(64, (int)env_double("TNMPS_SPLIT_ROWS", 128));
     size_t gp = 0;
     for (int bw : {kBW, kBWWide}) {
       const size_t nbmax = (size_t)(ldw + bw - 1) / bw;
       gp = std::max(gp, (nbmax / 2 + 1) * kMaxSplit * (size_t)(4 * bw * bw));
     }
     gpart_per = gp;
-    pcnt_per = (ldw + kBW - 1) / kBW / 2 + 2;
-    TN_HIP(hipMalloc(&d_abort, sizeof(int)));
-    TN_HIP(hipMemset(d_abort, 0, sizeof(int)));
-    // fused row-split steps (one cooperative launch per step) with TNMPS_FUSED=1;
-    // measured slower than the two-launch step, so off by default
-    fused_blocks = (env_double("TNMPS_FUSED", 0) != 0) ? jacobi_fused_max_blocks() : 0;
     bw_env = (int)env_double("TNMPS_BW", 0);
     factored = env_double("TNMPS_FACTORED", 1) != 0;
+    tol.read_env();
     // workspace budget: TNMPS_WS_BYTES (default 24 GiB) bounds the tasks per launch
     const double budget = env_double("TNMPS_WS_BYTES", 24.0 * (1ull << 30));
     const double per = 16.0 * ((double)ldx * ldw + (double)ldw * ldw) + 64.0 * ldw + 4096.0;
@@ -302,7 +282,6 @@ struct tn_engine {
     TN_HIP(hipMalloc(&offs, sizeof(double) * kMaxSweeps * ns));
     TN_HIP(hipMalloc(&info, sizeof(int) * kInfoLen * ns));
     TN_HIP(hipMalloc(&gpart, sizeof(double2) * gpart_per * ns));
-    TN_HIP(hipMalloc(&pcnt, sizeof(unsigned) * pcnt_per * ns));
     slots = ns;
     return upd.ensure(ns, stream);
   }
@@ -440,10 +419,6 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
     d.fac = e->fac + (size_t)t * e->ldw;
     d.perm = e->perm + (size_t)t * e->ldw;
     d.part = e->part + (size_t)t * 2 * kNPart;
-    d.gpart = e->gpart + (size_t)t * e->gpart_per;
-    d.pcnt = e->pcnt + (size_t)t * e->pcnt_per;
-    d.pcnt_cap = e->pcnt_per;
-    d.abort_flag = e->d_abort;
     d.offs = e->offs + t;
     d.info = e->info + (size_t)t * kInfoLen;
     // the factored product goes to the second half of X's columns; the SVD
@@ -452,7 +427,7 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
     d.ldx = e->ldx;
     d.ldw = e->ldw;
     d.has_gate = q.has_gate ? 1 : 0;
-    d.inner_sweeps = tolerances().inner_sweeps;
+    d.inner_sweeps = e->tol.inner_sweeps;
     d.prec = h->prec;
     d.tkind = h->trunc.kind;
     d.tchi = (int)std::min<size_t>(h->trunc.chi, 1u << 30);
@@ -482,75 +457,72 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
   // Block width 8 (measured faster than 16 even for a lone 512-column SVD:
   // the 32x32 inner solve costs more than the halved step count saves);
   // TNMPS_BW=16 selects the wide panels.
-  int bw = (e->bw_env == kBWWide && e->coop_blocks <= 0) ? kBWWide : kBW;
+  const int bw = (e->bw_env == kBWWide) ? kBWWide : kBW;
   const int nb = (int)((maxC + bw - 1) / bw);
   const int nbe = nb + (nb & 1);
   const int npairs = nbe / 2, steps = nbe - 1;
-  const Tolerances& tol = tolerances();
+  const Tolerances& tol = e->tol;
   int sweeps_run = 0;
-  if (e->coop_blocks > 0) {
-    // all sweeps in one persistent launch; the sweep count stays on the device
-    const int grid = std::min(nt * npairs, e->coop_blocks);
+  // One launch per round-robin step (two when the rows are split); the host
+  // checks convergence per sweep.  Few pairs (a lone large SVD) leave most
+  // CUs idle: split each pair's rows over S workgroups (gram kernel, then
+  // solve + apply kernel).
+  int S = 1;
+  while (S < e->split_max && (size_t)nt * npairs * S * 2 <= (size_t)2 * e->cus &&
+         maxR >= (size_t)e->split_rows * S)
+    S *= 2;
+  JacobiArgs ja{};
+  ja.X = e->X;
+  ja.W = e->W;
+  ja.gpart = e->gpart;
+  ja.offs = e->offs;
+  ja.info = e->info;
+  ja.xslot = (size_t)e->ldx * e->ldw;
+  ja.wslot = (size_t)e->ldw * e->ldw;
+  ja.gslot = e->gpart_per;
+  ja.ldx = e->ldx;
+  ja.ldw = e->ldw;
+  ja.nt = nt;
+  ja.nbe = nbe;
+  ja.S = S;
+  ja.Rh = (int)maxR;
+  ja.Ch = (int)maxC;
+  ja.inner_sweeps = tol.inner_sweeps;
+  ja.cross_only = tol.cross_only;
+  ja.tol_rot = tol.tol_rot;
+  ja.conv_tol = tol.conv_tol;
+  std::vector<double> offs(nt);
+  for (int sw = 0; sw < kMaxSweeps; ++sw) {
     if (e->timing) TN_HIP(hipEventRecord(e->tev[2], e->stream));
-    TN_HIP(launch_jacobi_coop(dd, nt, npairs, steps, kMaxSweeps, tol.tol_rot, tol.conv_tol,
-                              e->d_sweeps, grid, e->stream));
+    const auto h0 = std::chrono::steady_clock::now();
+    for (int s = 0; s < steps; ++s) {
+      if (S == 1) {
+        launch_jacobi_step(ja, npairs, bw, sw, s, e->stream);
+      } else {
+        launch_jacobi_gram(ja, npairs, bw, sw, s, e->stream);
+        launch_jacobi_apply(ja, npairs, bw, sw, s, e->stream);
+      }
+    }
+    TN_HIP(hipGetLastError());
+    e->perf_host_launch_ms +=
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count();
+    if (e->timing) TN_HIP(hipEventRecord(e->tev[3], e->stream));
+    sweeps_run = sw + 1;
+    TN_HIP(hipMemcpyAsync(offs.data(), e->offs + (size_t)sw * nt, sizeof(double) * nt,
+                          hipMemcpyDeviceToHost, e->stream));
+    TN_HIP(hipStreamSynchronize(e->stream));
     if (e->timing) {
-      TN_HIP(hipEventRecord(e->tev[3], e->stream));
-      TN_HIP(hipMemcpyAsync(e->h_sweeps, e->d_sweeps, sizeof(int), hipMemcpyDeviceToHost,
-                            e->stream));
-      TN_HIP(hipStreamSynchronize(e->stream));
-      e->perf.ms_theta += e->span(0, 1);
+      if (sw == 0) e->perf.ms_theta += e->span(0, 1);
       e->perf.ms_jacobi += e->span(2, 3);
-      e->perf.jacobi_launches += 1;
-      sweeps_run = *e->h_sweeps;
+      e->perf.jacobi_launches += (size_t)steps * (S == 1 ? 1 : 2);
     }
-  } else {
-    // one launch per round-robin step; the host checks convergence per sweep.
-    // Few pairs (a lone large SVD) leave most CUs idle: split each pair's
-    // rows over S workgroups (gram kernel, then solve + apply kernel).
-    int S = 1;
-    while (S < e->split_max && (size_t)nt * npairs * S * 2 <= (size_t)2 * e->cus &&
-           maxR >= (size_t)128 * S)
-      S *= 2;
-    const bool fused = S > 1 && bw == kBW && e->fused_blocks > 0 &&
-                       (size_t)nt * npairs * S <= (size_t)e->fused_blocks;
-    std::vector<double> offs(nt);
-    for (int sw = 0; sw < kMaxSweeps; ++sw) {
-      if (e->timing) TN_HIP(hipEventRecord(e->tev[2], e->stream));
-      const auto h0 = std::chrono::steady_clock::now();
-      for (int s = 0; s < steps; ++s) {
-        if (S == 1) {
-          launch_jacobi_step(dd, nt, npairs, bw, sw, s, tol.tol_rot, tol.conv_tol, e->stream);
-        } else if (fused) {
-          TN_HIP(launch_jacobi_fused(dd, nt, npairs, S, sw, s, tol.tol_rot, tol.conv_tol,
-                                     e->stream));
-        } else {
-          launch_jacobi_gram(dd, nt, npairs, bw, S, sw, s, tol.conv_tol, e->stream);
-          launch_jacobi_apply(dd, nt, npairs, bw, S, sw, s, tol.tol_rot, tol.conv_tol,
-                              e->stream);
-        }
-      }
-      TN_HIP(hipGetLastError());
-      e->perf_host_launch_ms +=
-          std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count();
-      if (e->timing) TN_HIP(hipEventRecord(e->tev[3], e->stream));
-      sweeps_run = sw + 1;
-      TN_HIP(hipMemcpyAsync(offs.data(), e->offs + (size_t)sw * nt, sizeof(double) * nt,
-                            hipMemcpyDeviceToHost, e->stream));
-      TN_HIP(hipStreamSynchronize(e->stream));
-      if (e->timing) {
-        if (sw == 0) e->perf.ms_theta += e->span(0, 1);
-        e->perf.ms_jacobi += e->span(2, 3);
-        e->perf.jacobi_launches += (size_t)steps * ((S == 1 || fused) ? 1 : 2);
-      }
-      bool done = true;
-      for (int t = 0; t < nt; ++t)
-        if (!(offs[t] <= tol.conv_tol)) done = false;
-      if (done) break;
-    }
+    bool done = true;
+    for (int t = 0; t < nt; ++t)
+      if (!(offs[t] <= tol.conv_tol)) done = false;
+    if (done) break;
   }
   if (e->timing) TN_HIP(hipEventRecord(e->tev[4], e->stream));
-  launch_finish(dd, nt, e->coop_blocks > 0 ? e->d_sweeps : nullptr, sweeps_run, e->stream);
+  launch_finish(dd, nt, sweeps_run, e->stream);
   if (any_factored) launch_fmul(dd, nt, (int)maxR, (int)maxC, e->stream);
   launch_split(dd, nt, e->stream);
   launch_renorm(dd, nt, e->stream);

@@ -61,15 +61,11 @@ struct UpdDesc {
   double* fac;           // scale of the right factor rows (C)
   int* perm;             // sorted column order (C)
   double* part;          // per-block partial norms (2 * kNPart)
-  double2* gpart;        // row-split partial Gram matrices [pair][slice][PW*PW]
-  unsigned* pcnt;        // per-pair arrival counters of the fused row-split step
-  int* abort_flag;       // set when a pair barrier exceeds its spin limit
   double* offs;          // per-sweep max off-diagonal ratio, stride nt
   int* info;             // kInfoLen ints
   double2* T;            // factored modes: product of the kept singular vectors
                          // with the isometric factor (ldx rows)
   int ldx, ldw;
-  int pcnt_cap;          // entries of pcnt
   int has_gate;
   int mode;              // requested: 0 full, 1 left factor isometric, 2 right
   int inner_sweeps;      // inner Jacobi sweeps per panel visit
@@ -79,6 +75,29 @@ struct UpdDesc {
   int cap_rank;          // allocation bound on the new bond dimension
   double teps;
   double2 gate[16];      // 4x4 row-major, qubit k the most significant bit
+};
+
+// The Jacobi step kernels of one launch (all tasks of an update batch).  Task
+// t's workspace sits at fixed offsets from these bases, so a step kernel
+// computes every address from its arguments and issues its first loads at
+// entry.
+struct JacobiArgs {
+  double2* X;            // X of task t at X + t * xslot (ldx rows)
+  double2* W;            // W of task t at W + t * wslot (ldw rows)
+  double2* gpart;        // row-split partial Gram matrices, gslot per task:
+                         // [pair][slice][PW * PW]
+  double* offs;          // [sweep][task] max off-diagonal ratio
+  const int* info;       // kInfoLen ints per task
+  size_t xslot, wslot, gslot;
+  int ldx, ldw;
+  int nt;                // tasks in the launch
+  int nbe;               // round-robin blocks (even) of the widest task
+  int S;                 // workgroups per block pair (row split; 1 otherwise)
+  int Rh, Ch;            // bounds on every task's R and C
+  int inner_sweeps;      // inner Jacobi sweeps per panel visit
+  int cross_only;        // after a sweep's first step, rotate only cross-block pairs
+  double tol_rot;        // rotate a pair when |g_ij| > tol_rot sqrt(g_ii g_jj)
+  double conv_tol;       // converged after a sweep whose max ratio <= conv_tol
 };
 
 // One site measurement or probability query (mps.rs:1027-1134, 1427-1465).

@@ -1,0 +1,561 @@
+// Block one-sided Jacobi SVD steps for gfx950 (stands in for LAPACK zgesvd,
+// tensor-net/lib/mps.rs:299).
+//
+// Columns of X are grouped in blocks of BW (8 or 16).  A sweep visits every
+// pair of blocks once, in the round-robin ("circle") ordering of nbe blocks
+// (nbe = the widest task's block count rounded up to even; blocks past a
+// task's C hold no columns); one step holds nbe/2 disjoint pairs.  For its
+// pair (a, b) a workgroup forms the panel Gram matrix G = P^H P (PW x PW,
+// PW = 2 BW), diagonalises it with cyclic two-sided Jacobi rotations
+// (G <- Q^H G Q, Wl <- Wl Q), and applies Wl to the panel of X and of W.  The
+// largest relative off-diagonal |G_ij| / sqrt(G_ii G_jj) seen in a sweep goes
+// to offs[sweep]; a task whose previous sweep stayed at or below conv_tol has
+// converged and its workgroups return at once.
+//
+// Launch shapes:
+//   k_jacobi_step             one workgroup per block pair (many pairs)
+//   k_jacobi_gram + _apply    S workgroups per pair, each on a slice of the
+//                             rows: partial Gram matrices, then every one of
+//                             the S sums them in the same order, solves the
+//                             same G into bitwise the same Wl and applies it
+//                             to its own slice (few pairs: a lone SVD)
+//
+// The kernels take JacobiArgs by value and compute every address from it
+// and the task index, so their first global loads (the panel rows, the
+// partial Gram matrices, the task's R, C and convergence state) issue
+// together at entry.  Panel rows are loaded for the host's bounds on R and C
+// (inside the task's workspace slot) and masked once the task's own R and C
+// have arrived.
+#include "tn_device.h"
+#include "tn_devmath.h"
+#include "tn_kernels.h"
+
+#include <cfloat>
+
+namespace tnmps {
+
+__device__ __forceinline__ int rr_player(int i, int step, int nbe) {
+  return 1 + (i - 1 + step) % (nbe - 1);
+}
+
+// Columns whose squared norm is below kTinyNorm2 (norm 1e-17, under
+// f64::EPSILON times a unit-norm theta, so below every truncation threshold of
+// the rank rule) take part in no rotation and in no convergence measure.
+constexpr double kTinyNorm2 = 1e-34;
+
+template <int BW>
+struct PairSmem {
+  static constexpr int PW = 2 * BW, LD = PW + 1;
+  double2 Gs[2][PW][LD];            // panel Gram matrix (Gs[0]), double-buffered in the solve
+  double2 buf[64 * LD + 2 * PW * LD];
+  double red[4];
+  // buf holds the 64-row panel chunk Ps and the accumulated rotation Ws
+  // (double-buffered); the per-wave partial Gram matrices of panel_gram
+  // overlay both once its row loop is done.
+  static_assert(4 * PW * LD <= 64 * LD + 2 * PW * LD, "partial Gram overlay");
+  __device__ double2& Ps(int r, int c) { return buf[r * LD + c]; }
+  __device__ double2& Ws(int b, int i, int j) { return buf[64 * LD + (b * PW + i) * LD + j]; }
+  __device__ double2& part(int w, int i, int j) { return buf[(w * PW + i) * LD + j]; }
+};
+
+// Column of X (and W) held by panel column cc (0..PW-1) of pair p at `step`.
+template <int BW>
+__device__ __forceinline__ int pair_col(int nbe, int step, int p, int cc) {
+  const int blk = (cc < BW) ? ((p == 0) ? 0 : rr_player(p, step, nbe))
+                            : rr_player(nbe - 1 - p, step, nbe);
+  return blk * BW + (cc & (BW - 1));
+}
+
+// One 64-row chunk of a panel in registers: thread tid holds row tid & 63 of
+// panel columns (tid >> 6) + 4 q, q < BW / 2.  Chunks are loaded one ahead
+// of the LDS compute, so global-memory latency overlaps the work on the
+// previous chunk (and, in the apply kernel, the panel solve).
+template <int BW>
+struct Chunk {
+  double2 v[BW / 2];
+};
+
+// Columns of the thread's chunk elements (`lcol`) and of its outputs in
+// panel_apply (`ocol`: for BW = 8 the matrix-core output column lane & 15 in
+// ocol[0]; otherwise panel columns (tid >> 6) * BW / 2 + k).
+template <int BW>
+__device__ __forceinline__ void thread_cols(int nbe, int step, int p, int (&lcol)[BW / 2],
+                                            int (&ocol)[BW / 2]) {
+  const int wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < BW / 2; ++q) {
+    lcol[q] = pair_col<BW>(nbe, step, p, wid + 4 * q);
+    ocol[q] = (BW == 8) ? pair_col<BW>(nbe, step, p, threadIdx.x & 15)
+                        : pair_col<BW>(nbe, step, p, wid * (BW / 2) + q);
+  }
+}
+
+// Rows [r0, min(r0 + 64, rend)) of the panel of M, columns below cend.
+template <int BW>
+__device__ __forceinline__ void load_chunk(const double2* __restrict__ M, size_t ld, int r0,
+                                           int rend, int cend, const int (&lcol)[BW / 2],
+                                           Chunk<BW>& ck) {
+  const int r = r0 + (threadIdx.x & 63);
+#pragma unroll
+  for (int q = 0; q < BW / 2; ++q)
+    ck.v[q] = (lcol[q] < cend && r < rend) ? M[(size_t)lcol[q] * ld + r] : make_double2(0.0, 0.0);
+}
+
+// Zero the elements of a speculatively loaded chunk outside rows < rend and
+// columns < cend.
+template <int BW>
+__device__ __forceinline__ void mask_chunk(int r0, int rend, int cend, const int (&lcol)[BW / 2],
+                                           Chunk<BW>& ck) {
+  const int r = r0 + (threadIdx.x & 63);
+#pragma unroll
+  for (int q = 0; q < BW / 2; ++q)
+    if (!(lcol[q] < cend && r < rend)) ck.v[q] = make_double2(0.0, 0.0);
+}
+
+template <int BW>
+__device__ __forceinline__ void put_chunk(const Chunk<BW>& ck, PairSmem<BW>& sm) {
+  const int rr = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < BW / 2; ++q) sm.Ps(rr, wid + 4 * q) = ck.v[q];
+}
+
+// FP64 matrix-core tile D(16x16) += A(16x4) B(4x16): lane l holds
+// A[l % 16][l / 16] and B[l / 16][l % 16]; D[i][j] is element i / 4 of lane
+// j + 16 (i % 4) (lane l, element v: row l / 16 + 4 v, column l % 16).
+typedef double v4d __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ v4d mfma16(double a, double b, v4d c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// sm.Gs[0] = P^H P over rows [a, b) of the panel of X; `ck` holds rows
+// [a, a + 64).  Thread = (row slice rs, 4x4 tile of G); a row slice takes
+// every RS-th row of each 64-row chunk.  Slices are reduced by shuffles
+// (inside a wave) and through LDS (across the four waves).
+template <int BW>
+__device__ void panel_gram(const double2* __restrict__ X, size_t ldx, int a, int b, int cend,
+                           const int (&lcol)[BW / 2], Chunk<BW>& ck, PairSmem<BW>& sm) {
+  if constexpr (BW == 8) {
+    // 16-column panels on the matrix cores: wave w takes rows 16 w .. 16 w + 15
+    // of each chunk, four at a time, as the K dimension of
+    //   Re G += Pr^T Pr + Pi^T Pi,   Im G += Pr^T Pi - Pi^T Pr.
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    v4d gre = {0.0, 0.0, 0.0, 0.0}, gim = {0.0, 0.0, 0.0, 0.0};
+    for (int r0 = a; r0 < b; r0 += 64) {
+      put_chunk<BW>(ck, sm);
+      __syncthreads();
+      if (r0 + 64 < b) load_chunk<BW>(X, ldx, r0 + 64, b, cend, lcol, ck);
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        const double2 p = sm.Ps(16 * wid + 4 * g + (lane >> 4), lane & 15);
+        gre = mfma16(p.x, p.x, gre);
+        gre = mfma16(p.y, p.y, gre);
+        gim = mfma16(p.x, p.y, gim);
+        gim = mfma16(-p.y, p.x, gim);
+      }
+      __syncthreads();
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+      sm.part(wid, (lane >> 4) + 4 * v, lane & 15) = make_double2(gre[v], gim[v]);
+    __syncthreads();
+    {
+      const int gi = tid >> 4, gj = tid & 15;
+      double2 g = sm.part(0, gi, gj);
+      g = cadd(g, sm.part(1, gi, gj));
+      g = cadd(g, sm.part(2, gi, gj));
+      g = cadd(g, sm.part(3, gi, gj));
+      sm.Gs[0][gi][gj] = g;
+    }
+    __syncthreads();
+    return;
+  }
+  constexpr int PW = 2 * BW, TPR = PW / 4, NTILE = TPR * TPR, RS = 256 / NTILE;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int tile = tid % NTILE, rs = tid / NTILE;
+  const int ti = tile / TPR, tj = tile % TPR;
+  double2 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = make_double2(0.0, 0.0);
+  for (int r0 = a; r0 < b; r0 += 64) {
+    put_chunk<BW>(ck, sm);
+    __syncthreads();
+    if (r0 + 64 < b) load_chunk<BW>(X, ldx, r0 + 64, b, cend, lcol, ck);
+#pragma unroll 2
+    for (int k = 0; k < 64 / RS; ++k) {
+      const int rr = rs + RS * k;
+      double2 xi[4], xj[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) xi[i] = sm.Ps(rr, ti * 4 + i);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) xj[j] = sm.Ps(rr, tj * 4 + j);
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = cadd(acc[i][j], cmulc(xi[i], xj[j]));
+    }
+    __syncthreads();
+  }
+  // lanes tile, tile + NTILE, ... of one wave hold the same tile
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      double2 v = acc[i][j];
+#pragma unroll
+      for (int off = NTILE; off < 64; off <<= 1) {
+        v.x += __shfl_xor(v.x, off, 64);
+        v.y += __shfl_xor(v.y, off, 64);
+      }
+      acc[i][j] = v;
+    }
+  if (lane < NTILE) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) sm.part(wid, ti * 4 + i, tj * 4 + j) = acc[i][j];
+  }
+  __syncthreads();
+  for (int e = tid; e < PW * PW; e += 256) {
+    const int gi = e / PW, gj = e % PW;
+    double2 g = sm.part(0, gi, gj);
+    g = cadd(g, sm.part(1, gi, gj));
+    g = cadd(g, sm.part(2, gi, gj));
+    g = cadd(g, sm.part(3, gi, gj));
+    sm.Gs[0][gi][gj] = g;
+  }
+  __syncthreads();
+}
+
+// 1/sqrt(x) from the hardware approximation (relative error about 5e-8 on
+// gfx950) refined by two Newton steps, each three dependent operations
+// (r + (r/2)(1 - x r^2)): about 1 ulp, cheaper than the IEEE-exact square
+// root and division sequences on the rotation's critical path.
+__device__ __forceinline__ double rsq_nr(double x) {
+  double r = __builtin_amdgcn_rsq(x);
+  double h = 0.5 * r;
+  r = fma(h, fma(-x * r, r, 1.0), r);
+  h = 0.5 * r;
+  r = fma(h, fma(-x * r, r, 1.0), r);
+  return r;
+}
+
+// Jacobi rotation zeroing the off-diagonal of [[a, c], [conj(c), b]]:
+// Q = [[cs, sn], [-conj(sn), cs]], the rotation by the smaller angle
+// (|theta| <= pi/4) with t = tan(theta) = 2|c| / (|d| + r), d = b - a,
+// r = sqrt(d^2 + 4|c|^2), cs = 1 / sqrt(1 + t^2), and sn = t cs c / |c| times
+// the sign of d, written sn = (2 cs / (|d| + r)) c.  The angle may carry the
+// hardware approximations' error (about 1e-7 relative: the pair is then left
+// with 1e-7 of its off-diagonal, which the next visit removes); unitarity
+// does not, since cs^2 + |sn|^2 = cs^2 (1 + t^2) with cs computed to 1 ulp
+// from the same t^2 = 4 |c|^2 / (|d| + r)^2.  Identity (and false) when the
+// pair is already orthogonal to tol or a column is negligible.  Branch-free,
+// so that the two rotations a thread needs per round overlap.
+__device__ __forceinline__ bool rot2(double a, double b, double2 c, double tol, double& cs,
+                                     double2& sn) {
+  const double c2 = cabs2(c);
+  const bool go = a > kTinyNorm2 && b > kTinyNorm2 && c2 > tol * tol * a * b;
+  const double d = b - a;
+  const double r2 = go ? fma(d, d, 4.0 * c2) : 1.0;
+  const double s = fabs(d) + r2 * __builtin_amdgcn_rsq(r2);  // |d| + r
+  const double inv = __builtin_amdgcn_rcp(s);
+  const double t2 = 4.0 * c2 * inv * inv;
+  const double q = rsq_nr(go ? 1.0 + t2 : 1.0);
+  const double f = go ? (d >= 0.0 ? 2.0 : -2.0) * inv * q : 0.0;
+  cs = q;
+  sn = make_double2(f * c.x, f * c.y);
+  return go;
+}
+
+// From the panel Gram matrix in sm.Gs[0]: record the largest relative
+// off-diagonal into *rec (when not null), and unless the panel is already
+// orthogonal diagonalise G with cyclic two-sided Jacobi rotations,
+// accumulating the rotation in sm.Ws[*wsel].  Round r = r_first .. PW-1 pairs
+// index x with x ^ r: PW/2 disjoint rotations per round, and from r_first = 1
+// every pair once per inner sweep; rounds r >= BW pair a column of the first
+// block with one of the second (the cross pairs).  Every thread computes the rotations of its own row and column
+// pairs from the round's Gram buffer, so a round needs one barrier.  Returns
+// false (for the whole block) when there is nothing to apply.  A function of
+// sm.Gs[0] alone: blocks holding the same G compute bitwise the same rotation.
+template <int BW>
+__device__ bool panel_solve(double* rec, int inner_sweeps, double tol_rot, int r_first,
+                            PairSmem<BW>& sm, int& wsel) {
+  constexpr int PW = 2 * BW;
+  const int tid = threadIdx.x;
+  double ratio2 = 0.0;
+  for (int e = tid; e < PW * PW; e += 256) {
+    const int gi = e / PW, gj = e % PW;
+    sm.Ws(0, gi, gj) = make_double2(gi == gj ? 1.0 : 0.0, 0.0);
+    if (gi < gj) {
+      const double a = sm.Gs[0][gi][gi].x, b = sm.Gs[0][gj][gj].x;
+      if (a > kTinyNorm2 && b > kTinyNorm2) ratio2 = fmax(ratio2, cabs2(sm.Gs[0][gi][gj]) / (a * b));
+    }
+  }
+  const double off = sqrt(block_max(ratio2, sm.red));
+  if (rec && tid == 0 && off > 0.0)
+    atomicMax((unsigned long long*)rec, (unsigned long long)__double_as_longlong(off));
+  if (off <= tol_rot) return false;
+
+  int cur = 0;
+  for (int isw = 0; isw < inner_sweeps; ++isw) {
+    int rotated = 0;
+    for (int round = r_first; round < PW; ++round) {
+      for (int e = tid; e < PW * PW; e += 256) {
+        const int gi = e / PW, gj = e % PW;
+        const int qi = gi ^ round, qj = gj ^ round;
+        const int i1 = min(gi, qi), i2 = max(gi, qi);
+        const int j1 = min(gj, qj), j2 = max(gj, qj);
+        const double2 g11 = sm.Gs[cur][i1][i1], g22 = sm.Gs[cur][i2][i2], g12 = sm.Gs[cur][i1][i2];
+        const double2 h11 = sm.Gs[cur][j1][j1], h22 = sm.Gs[cur][j2][j2], h12 = sm.Gs[cur][j1][j2];
+        const double2 g_ij = sm.Gs[cur][gi][gj], g_ioj = sm.Gs[cur][gi][qj];
+        const double2 g_oij = sm.Gs[cur][qi][gj], g_oioj = sm.Gs[cur][qi][qj];
+        const double2 w_ij = sm.Ws(cur, gi, gj), w_ioj = sm.Ws(cur, gi, qj);
+        double csi, csj;
+        double2 sni, snj;
+        rotated |= rot2(g11.x, g22.x, g12, tol_rot, csi, sni);
+        rot2(h11.x, h22.x, h12, tol_rot, csj, snj);
+        // column j of Q: Q[j][j] = cs; off entry Q[qj][j] = -conj(sn) when j
+        // is the lower index of its pair, sn when it is the upper one
+        const double2 Qjj = make_double2(csj, 0.0);
+        const double2 Qoj = (gj < qj) ? make_double2(-snj.x, snj.y) : snj;
+        const double2 Qii = make_double2(csi, 0.0);
+        const double2 Qoi = (gi < qi) ? make_double2(-sni.x, sni.y) : sni;
+        const double2 t0 = cadd(cmul(g_ij, Qjj), cmul(g_ioj, Qoj));
+        const double2 t1 = cadd(cmul(g_oij, Qjj), cmul(g_oioj, Qoj));
+        sm.Gs[cur ^ 1][gi][gj] = cadd(cmulc(Qii, t0), cmulc(Qoi, t1));
+        sm.Ws(cur ^ 1, gi, gj) = cadd(cmul(w_ij, Qjj), cmul(w_ioj, Qoj));
+      }
+      cur ^= 1;
+      __syncthreads();
+    }
+    if (!__syncthreads_or(rotated)) break;
+  }
+  wsel = cur;
+  return true;
+}
+
+// Rows [a, b) of the panel of M (leading dimension ld, columns below cend)
+// <- panel * Ws[wsel].  Thread = (row lane 0..63, output group of PW/4 columns);
+// rows are staged through LDS 64 at a time.  `ck` holds rows [a, a + 64).
+template <int BW>
+__device__ void panel_apply(double2* M, size_t ld, int a, int b, int cend,
+                            const int (&lcol)[BW / 2], const int (&ocol)[BW / 2], Chunk<BW>& ck,
+                            PairSmem<BW>& sm, int wsel) {
+  if constexpr (BW == 8) {
+    // 16-column panels on the matrix cores: wave w computes rows 16 w ..
+    // 16 w + 15 of each chunk, O = P Wl with the panel columns as K:
+    //   Re O = Pr Wr - Pi Wi,   Im O = Pr Wi + Pi Wr.
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    double2 wk[4];
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) wk[kk] = sm.Ws(wsel, 4 * kk + (lane >> 4), lane & 15);
+    const int oc = ocol[0];
+    for (int r0 = a; r0 < b; r0 += 64) {
+      put_chunk<BW>(ck, sm);
+      __syncthreads();
+      if (r0 + 64 < b) load_chunk<BW>(M, ld, r0 + 64, b, cend, lcol, ck);
+      v4d ore = {0.0, 0.0, 0.0, 0.0}, oim = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int kk = 0; kk < 4; ++kk) {
+        const double2 p = sm.Ps(16 * wid + (lane & 15), 4 * kk + (lane >> 4));
+        ore = mfma16(p.x, wk[kk].x, ore);
+        ore = mfma16(-p.y, wk[kk].y, ore);
+        oim = mfma16(p.x, wk[kk].y, oim);
+        oim = mfma16(p.y, wk[kk].x, oim);
+      }
+      if (oc < cend) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int r = r0 + 16 * wid + (lane >> 4) + 4 * v;
+          if (r < b) M[(size_t)oc * ld + r] = make_double2(ore[v], oim[v]);
+        }
+      }
+      __syncthreads();
+    }
+    return;
+  }
+  constexpr int PW = 2 * BW, NJ = PW / 4;
+  const int tid = threadIdx.x;
+  const int rl = tid & 63, jg = tid >> 6;
+  for (int r0 = a; r0 < b; r0 += 64) {
+    put_chunk<BW>(ck, sm);
+    __syncthreads();
+    if (r0 + 64 < b) load_chunk<BW>(M, ld, r0 + 64, b, cend, lcol, ck);
+    const int r = r0 + rl;
+    double2 o[NJ];
+#pragma unroll
+    for (int k = 0; k < NJ; ++k) o[k] = make_double2(0.0, 0.0);
+#pragma unroll 4
+    for (int c = 0; c < PW; ++c) {
+      const double2 x = sm.Ps(rl, c);
+#pragma unroll
+      for (int k = 0; k < NJ; ++k) o[k] = cadd(o[k], cmul(x, sm.Ws(wsel, c, jg * NJ + k)));
+    }
+    if (r < b) {
+#pragma unroll
+      for (int k = 0; k < NJ; ++k)
+        if (ocol[k] < cend) M[(size_t)ocol[k] * ld + r] = o[k];
+    }
+    __syncthreads();
+  }
+}
+
+// Row range of slice q of S over n rows (slices are multiples of 64 rows).
+__device__ __forceinline__ void split_rows(int n, int S, int q, int& a, int& b) {
+  const int per = (((n + S - 1) / S) + 63) / 64 * 64;
+  a = min(n, q * per);
+  b = min(n, a + per);
+}
+
+// First inner round of a step: all pairs of the panel (1), or with
+// a.cross_only only the pairs across the two blocks (BW) after the sweep's
+// first step, which already rotated every block's pairs among themselves:
+// every column pair is still rotated at least once per sweep.
+template <int BW>
+__device__ __forceinline__ int first_round(const JacobiArgs& a, int step) {
+  return (a.cross_only && step > 0) ? BW : 1;
+}
+
+// Convergence state of task t before `sweep`: true when its previous sweep
+// stayed at or below conv_tol.
+__device__ __forceinline__ bool converged(const JacobiArgs& a, int t, int sweep) {
+  return sweep > 0 && a.offs[(size_t)(sweep - 1) * a.nt + t] <= a.conv_tol;
+}
+
+// One launch per step, one workgroup per block pair.
+template <int BW>
+__global__ __launch_bounds__(256, BW == 8 ? 2 : 1) void k_jacobi_step(JacobiArgs a, int sweep,
+                                                                      int step) {
+  __shared__ PairSmem<BW> sm;
+  const int t = blockIdx.y, p = blockIdx.x;
+  const int* inf = a.info + (size_t)t * kInfoLen;
+  const int R = inf[kInfoR], C = inf[kInfoC];
+  const bool done = converged(a, t, sweep);
+  int lcol[BW / 2], ocol[BW / 2];
+  thread_cols<BW>(a.nbe, step, p, lcol, ocol);
+  double2* X = a.X + (size_t)t * a.xslot;
+  double2* W = a.W + (size_t)t * a.wslot;
+  Chunk<BW> ck;
+  load_chunk<BW>(X, (size_t)a.ldx, 0, a.Rh, a.ldw, lcol, ck);
+  if (done) return;
+  mask_chunk<BW>(0, R, C, lcol, ck);
+  panel_gram<BW>(X, (size_t)a.ldx, 0, R, C, lcol, ck, sm);
+  Chunk<BW> cx, cw;
+  load_chunk<BW>(X, (size_t)a.ldx, 0, R, C, lcol, cx);
+  load_chunk<BW>(W, (size_t)a.ldw, 0, C, C, lcol, cw);
+  int wsel = 0;
+  if (!panel_solve<BW>(a.offs + (size_t)sweep * a.nt + t, a.inner_sweeps, a.tol_rot,
+                       first_round<BW>(a, step), sm, wsel))
+    return;
+  panel_apply<BW>(X, (size_t)a.ldx, 0, R, C, lcol, ocol, cx, sm, wsel);
+  panel_apply<BW>(W, (size_t)a.ldw, 0, C, C, lcol, ocol, cw, sm, wsel);
+}
+
+// Row-split step, part 1: workgroup (p, q) writes the Gram matrix of row
+// slice q of pair p's panel to gpart[p][q].
+template <int BW>
+__global__ __launch_bounds__(256, BW == 8 ? 2 : 1) void k_jacobi_gram(JacobiArgs a, int sweep,
+                                                                      int step) {
+  constexpr int PW = 2 * BW;
+  __shared__ PairSmem<BW> sm;
+  const int t = blockIdx.y, p = blockIdx.x / a.S, q = blockIdx.x - p * a.S;
+  const int* inf = a.info + (size_t)t * kInfoLen;
+  const int R = inf[kInfoR], C = inf[kInfoC];
+  const bool done = converged(a, t, sweep);
+  int lcol[BW / 2], ocol[BW / 2];
+  thread_cols<BW>(a.nbe, step, p, lcol, ocol);
+  int xa, xb;
+  split_rows(a.Rh, a.S, q, xa, xb);
+  const double2* X = a.X + (size_t)t * a.xslot;
+  Chunk<BW> ck;
+  load_chunk<BW>(X, (size_t)a.ldx, xa, xb, a.ldw, lcol, ck);
+  if (done) return;
+  const int xe = min(xb, R);
+  mask_chunk<BW>(xa, xe, C, lcol, ck);
+  panel_gram<BW>(X, (size_t)a.ldx, xa, xe, C, lcol, ck, sm);
+  double2* gp = a.gpart + (size_t)t * a.gslot + ((size_t)p * a.S + q) * (PW * PW);
+  for (int e = threadIdx.x; e < PW * PW; e += 256) gp[e] = sm.Gs[0][e / PW][e % PW];
+}
+
+// Row-split step, part 2: every workgroup of pair p sums the S partial Gram
+// matrices in the same order, so all of them solve the same G into the same
+// Wl, and applies it to its own slice of the rows of X and of W.
+template <int BW>
+__global__ __launch_bounds__(256, BW == 8 ? 2 : 1) void k_jacobi_apply(JacobiArgs a, int sweep,
+                                                                       int step) {
+  constexpr int PW = 2 * BW, NE = (PW * PW + 255) / 256;
+  __shared__ PairSmem<BW> sm;
+  const int t = blockIdx.y, p = blockIdx.x / a.S, q = blockIdx.x - p * a.S;
+  const int* inf = a.info + (size_t)t * kInfoLen;
+  const int R = inf[kInfoR], C = inf[kInfoC];
+  const bool done = converged(a, t, sweep);
+  int lcol[BW / 2], ocol[BW / 2];
+  thread_cols<BW>(a.nbe, step, p, lcol, ocol);
+  int xa, xb, wa, wb;
+  split_rows(a.Rh, a.S, q, xa, xb);
+  split_rows(a.Ch, a.S, q, wa, wb);
+  double2* X = a.X + (size_t)t * a.xslot;
+  double2* W = a.W + (size_t)t * a.wslot;
+  const double2* gp = a.gpart + (size_t)t * a.gslot + (size_t)p * a.S * (PW * PW);
+  // the slice's first chunks and the partial Gram matrices load together
+  Chunk<BW> cx, cw;
+  load_chunk<BW>(X, (size_t)a.ldx, xa, xb, a.ldw, lcol, cx);
+  load_chunk<BW>(W, (size_t)a.ldw, wa, wb, a.ldw, lcol, cw);
+  double2 g[NE][kMaxSplit];
+#pragma unroll
+  for (int k = 0; k < NE; ++k) {
+    const int e = threadIdx.x + 256 * k;
+#pragma unroll
+    for (int s = 0; s < kMaxSplit; ++s)
+      g[k][s] = (s < a.S && e < PW * PW) ? gp[(size_t)s * (PW * PW) + e] : make_double2(0.0, 0.0);
+  }
+  if (done) return;
+#pragma unroll
+  for (int k = 0; k < NE; ++k) {
+    const int e = threadIdx.x + 256 * k;
+    double2 sum = g[k][0];
+#pragma unroll
+    for (int s = 1; s < kMaxSplit; ++s) sum = cadd(sum, g[k][s]);
+    if (e < PW * PW) sm.Gs[0][e / PW][e % PW] = sum;
+  }
+  __syncthreads();
+  const int xe = min(xb, R), we = min(wb, C);
+  mask_chunk<BW>(xa, xe, C, lcol, cx);
+  mask_chunk<BW>(wa, we, C, lcol, cw);
+  int wsel = 0;
+  if (!panel_solve<BW>(q == 0 ? a.offs + (size_t)sweep * a.nt + t : nullptr, a.inner_sweeps,
+                       a.tol_rot, first_round<BW>(a, step), sm, wsel))
+    return;
+  panel_apply<BW>(X, (size_t)a.ldx, xa, xe, C, lcol, ocol, cx, sm, wsel);
+  panel_apply<BW>(W, (size_t)a.ldw, wa, we, C, lcol, ocol, cw, sm, wsel);
+}
+
+// ------------------------------------------------------------- launch
+void launch_jacobi_step(const JacobiArgs& a, int npairs, int bw, int sweep, int step,
+                        hipStream_t s) {
+  if (bw == 16)
+    hipLaunchKernelGGL(k_jacobi_step<16>, dim3(npairs, a.nt), dim3(256), 0, s, a, sweep, step);
+  else
+    hipLaunchKernelGGL(k_jacobi_step<8>, dim3(npairs, a.nt), dim3(256), 0, s, a, sweep, step);
+}
+void launch_jacobi_gram(const JacobiArgs& a, int npairs, int bw, int sweep, int step,
+                        hipStream_t s) {
+  if (bw == 16)
+    hipLaunchKernelGGL(k_jacobi_gram<16>, dim3(npairs * a.S, a.nt), dim3(256), 0, s, a, sweep,
+                       step);
+  else
+    hipLaunchKernelGGL(k_jacobi_gram<8>, dim3(npairs * a.S, a.nt), dim3(256), 0, s, a, sweep,
+                       step);
+}
+void launch_jacobi_apply(const JacobiArgs& a, int npairs, int bw, int sweep, int step,
+                         hipStream_t s) {
+  if (bw == 16)
+    hipLaunchKernelGGL(k_jacobi_apply<16>, dim3(npairs * a.S, a.nt), dim3(256), 0, s, a, sweep,
+                       step);
+  else
+    hipLaunchKernelGGL(k_jacobi_apply<8>, dim3(npairs * a.S, a.nt), dim3(256), 0, s, a, sweep,
+                       step);
+}
+
+}  // namespace tnmps

@@ -8,9 +8,8 @@
 // Kernels (reference lines each one replaces):
 //   k_theta        theta = lambda Gamma lambda Gamma lambda, gate applied
 //                  (mps.rs:628-651, 732-754, 1343-1352)
-//   k_winit        W = I for the Jacobi SVD
-//   k_jacobi_step  one round-robin step of a block one-sided Jacobi SVD
-//                  (stands in for zgesvd, mps.rs:299)
+//   k_winit        W = I for the Jacobi SVD (the SVD steps, standing in for
+//                  zgesvd at mps.rs:299, are in tn_jacobi.hip)
 //   k_finish       singular values, sort, rank rule, renormalisation
 //                  (process_svd, mps.rs:236-293)
 //   k_fmul         factored updates: kept singular vectors times the
@@ -23,64 +22,12 @@
 //   k_entropy      von Neumann / Renyi entropy of one bond (mps.rs:1251-1282)
 //   k_u1           single-site unitary (mps.rs:1298-1314)
 #include "tn_device.h"
+#include "tn_devmath.h"
 #include "tn_kernels.h"
-
-#include <hip/hip_cooperative_groups.h>
 
 #include <cfloat>
 
-namespace cg = cooperative_groups;
-
 namespace tnmps {
-
-__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
-  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
-// conj(a) * b
-__device__ __forceinline__ double2 cmulc(double2 a, double2 b) {
-  return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
-}
-__device__ __forceinline__ double2 cadd(double2 a, double2 b) {
-  return make_double2(a.x + b.x, a.y + b.y);
-}
-__device__ __forceinline__ double2 cscale(double2 a, double s) {
-  return make_double2(a.x * s, a.y * s);
-}
-__device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
-__device__ __forceinline__ double cabs2(double2 a) { return a.x * a.x + a.y * a.y; }
-// complex / real as num_complex divides by Complex{re: d, im: 0}
-__device__ __forceinline__ double2 cdivr(double2 a, double d) {
-  const double den = d * d;
-  return make_double2((a.x * d) / den, (a.y * d) / den);
-}
-
-// Block-wide sum of one double; deterministic (fixed tree).  `buf` holds at
-// least blockDim.x / 64 doubles.  Every thread gets the result.
-__device__ double block_sum(double v, double* buf) {
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  __syncthreads();
-  if (lane == 0) buf[wid] = v;
-  __syncthreads();
-  double s = 0.0;
-  const int nw = (blockDim.x + 63) >> 6;
-  for (int i = 0; i < nw; ++i) s += buf[i];
-  __syncthreads();
-  return s;
-}
-
-__device__ double block_max(double v, double* buf) {
-  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  __syncthreads();
-  if (lane == 0) buf[wid] = v;
-  __syncthreads();
-  double s = 0.0;
-  const int nw = (blockDim.x + 63) >> 6;
-  for (int i = 0; i < nw; ++i) s = fmax(s, buf[i]);
-  __syncthreads();
-  return s;
-}
 
 // ---------------------------------------------------------------- theta
 template <typename C2, typename R>
@@ -241,493 +188,11 @@ __global__ __launch_bounds__(256) void k_theta(const UpdDesc* __restrict__ descs
 __global__ __launch_bounds__(256) void k_winit(const UpdDesc* __restrict__ descs) {
   const UpdDesc& d = descs[blockIdx.y];
   const int C = task_shape(d).C;
-  if (blockIdx.x == 0)
-    for (int i = threadIdx.x; i < d.pcnt_cap; i += blockDim.x) d.pcnt[i] = 0u;
   const size_t tot = (size_t)C * C;
   for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < tot;
        e += (size_t)gridDim.x * blockDim.x) {
     const int i = (int)(e % C), j = (int)(e / C);
     d.W[(size_t)j * d.ldw + i] = make_double2(i == j ? 1.0 : 0.0, 0.0);
-  }
-}
-
-// ---------------------------------------------------------- Jacobi step
-// Block one-sided Jacobi (Hestenes) over column blocks of width BW (8 or 16).
-// A sweep visits every pair of blocks once, in the round-robin ("circle")
-// ordering of the nbe blocks (nbe = number of blocks rounded up to even; a
-// ghost block holds no columns); one step holds nbe/2 disjoint pairs.  For its
-// pair (a, b) a workgroup forms the panel Gram matrix G = P^H P (PW x PW,
-// PW = 2 BW), diagonalises it with cyclic two-sided Jacobi rotations
-// (G <- Q^H G Q, Wl <- Wl Q), and applies Wl to the panel of X and of W.  The
-// largest relative off-diagonal |G_ij| / sqrt(G_ii G_jj) seen in a sweep goes
-// to offs[sweep]; a task whose previous sweep stayed at or below conv_tol has
-// converged.  BW = 16 halves the steps per sweep; it serves lone large SVDs,
-// whose time is set by the chain of steps rather than by the flops.
-__device__ __forceinline__ int rr_player(int i, int step, int nbe) {
-  return 1 + (i - 1 + step) % (nbe - 1);
-}
-
-// Columns whose squared norm is below kTinyNorm2 (norm 1e-17, under
-// f64::EPSILON times a unit-norm theta, so below every truncation threshold of
-// the rank rule) take part in no rotation and in no convergence measure.
-constexpr double kTinyNorm2 = 1e-34;
-
-template <int BW>
-struct PairSmem {
-  static constexpr int PW = 2 * BW, LD = PW + 1;
-  double2 Ps[64][LD];           // 64-row chunk of the panel
-  double2 G[PW][LD];            // panel Gram matrix
-  double2 Wl[PW][LD];           // its accumulated rotation
-  double2 scratch[4 * PW * LD]; // per-wave partial Gram, then Gb[2] + Wb[2]
-  int rr_partner[PW - 1][PW];   // round-robin partner of each player
-  int rr_pair[PW - 1][PW];      // index of the player's pair in the round
-  double rcs[PW / 2];           // rotation of each pair of the round
-  double2 rsn[PW / 2];
-  int col[PW];
-  double red[4];
-  __device__ double2& part(int w, int i, int j) { return scratch[(w * PW + i) * LD + j]; }
-  __device__ double2& Gb(int b, int i, int j) { return scratch[(b * PW + i) * LD + j]; }
-  __device__ double2& Wb(int b, int i, int j) { return scratch[((2 + b) * PW + i) * LD + j]; }
-};
-
-// Columns of block pair p at round-robin step `step` into sm.col (-1 for
-// columns beyond C).  False (for the whole block) when the pair does not
-// exist for this task's C.
-template <int BW>
-__device__ bool pair_cols(int C, int step, int p, PairSmem<BW>& sm) {
-  constexpr int PW = 2 * BW;
-  const int nb = (C + BW - 1) / BW;
-  const int nbe = nb + (nb & 1);
-  if (step >= nbe - 1 || p >= nbe / 2) return false;
-  const int ba = (p == 0) ? 0 : rr_player(p, step, nbe);
-  const int bb = rr_player(nbe - 1 - p, step, nbe);
-  const int tid = threadIdx.x;
-  if (tid < PW) {
-    const int c = (tid < BW) ? ba * BW + tid : bb * BW + (tid - BW);
-    sm.col[tid] = (c < C) ? c : -1;
-  }
-  __syncthreads();
-  return true;
-}
-
-// Stage rows [r0, min(r0 + 64, r_end)) of the panel of M into sm.Ps (zeros
-// past r_end or for absent columns).
-template <int BW>
-__device__ __forceinline__ void stage_rows(const double2* __restrict__ M, size_t ld, int r0,
-                                           int r_end, PairSmem<BW>& sm) {
-  constexpr int PW = 2 * BW;
-  const int tid = threadIdx.x;
-#pragma unroll
-  for (int q = 0; q < PW / 4; ++q) {
-    const int e = tid + 256 * q;
-    const int rr = e & 63, cc = e >> 6;
-    const int c = sm.col[cc];
-    sm.Ps[rr][cc] = (c >= 0 && r0 + rr < r_end) ? M[(size_t)c * ld + r0 + rr]
-                                                : make_double2(0.0, 0.0);
-  }
-}
-
-// sm.G = P^H P over rows [r_begin, r_end) of the panel of X.  Thread = (row
-// slice rs, 4x4 tile of G); a row slice takes every RS-th row of each
-// 64-row chunk.  Slices are reduced by shuffles (inside a wave) and through
-// LDS (across the four waves).
-template <int BW>
-__device__ void panel_gram(const UpdDesc& d, int r_begin, int r_end, PairSmem<BW>& sm) {
-  constexpr int PW = 2 * BW, TPR = PW / 4, NTILE = TPR * TPR, RS = 256 / NTILE;
-  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-  const int tile = tid % NTILE, rs = tid / NTILE;
-  const int ti = tile / TPR, tj = tile % TPR;
-  double2 acc[4][4];
-#pragma unroll
-  for (int a = 0; a < 4; ++a)
-#pragma unroll
-    for (int b = 0; b < 4; ++b) acc[a][b] = make_double2(0.0, 0.0);
-  for (int r0 = r_begin; r0 < r_end; r0 += 64) {
-    stage_rows<BW>(d.X, (size_t)d.ldx, r0, r_end, sm);
-    __syncthreads();
-#pragma unroll 2
-    for (int k = 0; k < 64 / RS; ++k) {
-      const int rr = rs + RS * k;
-      double2 xi[4], xj[4];
-#pragma unroll
-      for (int a = 0; a < 4; ++a) xi[a] = sm.Ps[rr][ti * 4 + a];
-#pragma unroll
-      for (int b = 0; b < 4; ++b) xj[b] = sm.Ps[rr][tj * 4 + b];
-#pragma unroll
-      for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b) acc[a][b] = cadd(acc[a][b], cmulc(xi[a], xj[b]));
-    }
-    __syncthreads();
-  }
-  // lanes tile, tile + NTILE, ... of one wave hold the same tile
-#pragma unroll
-  for (int a = 0; a < 4; ++a)
-#pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      double2 v = acc[a][b];
-#pragma unroll
-      for (int off = NTILE; off < 64; off <<= 1) {
-        v.x += __shfl_xor(v.x, off, 64);
-        v.y += __shfl_xor(v.y, off, 64);
-      }
-      acc[a][b] = v;
-    }
-  if (lane < NTILE) {
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-      for (int b = 0; b < 4; ++b) sm.part(wid, ti * 4 + a, tj * 4 + b) = acc[a][b];
-  }
-  __syncthreads();
-  for (int e = tid; e < PW * PW; e += 256) {
-    const int gi = e / PW, gj = e % PW;
-    double2 g = sm.part(0, gi, gj);
-    g = cadd(g, sm.part(1, gi, gj));
-    g = cadd(g, sm.part(2, gi, gj));
-    g = cadd(g, sm.part(3, gi, gj));
-    sm.G[gi][gj] = g;
-  }
-  __syncthreads();
-}
-
-// 1/sqrt(x) and 1/x from the hardware approximations refined by two Newton
-// steps each (quadratic convergence to double precision): cheaper than the
-// IEEE-exact division and square-root sequences on the rotation critical path.
-__device__ __forceinline__ double rsq_nr(double x) {
-  double r = __builtin_amdgcn_rsq(x);
-  r = r * (1.5 - 0.5 * x * r * r);
-  r = r * (1.5 - 0.5 * x * r * r);
-  return r;
-}
-__device__ __forceinline__ double rcp_nr(double x) {
-  double r = __builtin_amdgcn_rcp(x);
-  r = r * (2.0 - x * r);
-  r = r * (2.0 - x * r);
-  return r;
-}
-
-// Jacobi rotation zeroing the off-diagonal of [[a, c], [conj(c), b]]:
-// Q = [[cs, sn], [-conj(sn), cs]] with sn = s e^{i arg c} (Golub-Van Loan
-// symmetric Schur 2x2 with the phase of c factored out).  Returns false
-// (identity) when the pair is already orthogonal to tol or a column is
-// negligible.
-__device__ __forceinline__ bool rot2(double a, double b, double2 c, double tol, double& cs,
-                                     double2& sn) {
-  const double c2 = cabs2(c);
-  if (!(a > kTinyNorm2 && b > kTinyNorm2 && c2 > tol * tol * a * b)) {
-    cs = 1.0;
-    sn = make_double2(0.0, 0.0);
-    return false;
-  }
-  const double rc = rsq_nr(c2);  // 1/|c|
-  const double tau = (b - a) * 0.5 * rc;
-  const double x = 1.0 + tau * tau;
-  const double t = (tau >= 0.0 ? 1.0 : -1.0) * rcp_nr(fabs(tau) + x * rsq_nr(x));
-  cs = rsq_nr(1.0 + t * t);
-  const double f = t * cs * rc;
-  sn = make_double2(f * c.x, f * c.y);
-  return true;
-}
-
-// From the panel Gram matrix in sm.G: record the largest relative
-// off-diagonal (when `record`), and unless the panel is already orthogonal
-// diagonalise G with cyclic two-sided Jacobi rotations (round-robin order,
-// PW/2 disjoint rotations per round; in each pair the lower index plays the
-// first role), accumulating the rotation in sm.Wl.  Returns false (for the
-// whole block) when there is nothing to apply.  A function of sm.G alone:
-// blocks holding the same G compute bitwise the same Wl.
-template <int BW>
-__device__ bool panel_solve(const UpdDesc& d, int nt, int sweep, bool record, double tol_rot,
-                            PairSmem<BW>& sm) {
-  constexpr int PW = 2 * BW;
-  const int tid = threadIdx.x;
-  double ratio = 0.0;
-  for (int e = tid; e < PW * PW; e += 256) {
-    const int gi = e / PW, gj = e % PW;
-    if (gi < gj) {
-      const double a = sm.G[gi][gi].x, b = sm.G[gj][gj].x;
-      if (a > kTinyNorm2 && b > kTinyNorm2)
-        ratio = fmax(ratio, sqrt(cabs2(sm.G[gi][gj])) / sqrt(a * b));
-    }
-  }
-  const double off = block_max(ratio, sm.red);
-  if (record && tid == 0 && off > 0.0)
-    atomicMax((unsigned long long*)&d.offs[sweep * nt],
-              (unsigned long long)__double_as_longlong(off));
-  if (off <= tol_rot) return false;
-
-  for (int e = tid; e < (PW - 1) * PW; e += 256) {
-    const int r = e / PW, x = e % PW;
-    const int pos = (x == 0) ? 0 : 1 + ((x - 1 - r) % (PW - 1) + (PW - 1)) % (PW - 1);
-    const int other = PW - 1 - pos;
-    sm.rr_partner[r][x] = (other == 0) ? 0 : rr_player(other, r, PW);
-    sm.rr_pair[r][x] = pos < PW / 2 ? pos : other;
-  }
-  for (int e = tid; e < PW * PW; e += 256) {
-    const int gi = e / PW, gj = e % PW;
-    sm.Gb(0, gi, gj) = sm.G[gi][gj];
-    sm.Wb(0, gi, gj) = make_double2(gi == gj ? 1.0 : 0.0, 0.0);
-  }
-  __syncthreads();
-  int cur = 0;
-  for (int isw = 0; isw < d.inner_sweeps; ++isw) {
-    int rotated = 0;
-    for (int round = 0; round < PW - 1; ++round) {
-      if (tid < PW / 2) {
-        const int p1 = (tid == 0) ? 0 : rr_player(tid, round, PW);
-        const int p2 = rr_player(PW - 1 - tid, round, PW);
-        const int i1 = min(p1, p2), i2 = max(p1, p2);
-        double cs;
-        double2 sn;
-        rotated |= rot2(sm.Gb(cur, i1, i1).x, sm.Gb(cur, i2, i2).x, sm.Gb(cur, i1, i2), tol_rot,
-                        cs, sn);
-        sm.rcs[tid] = cs;
-        sm.rsn[tid] = sn;
-      }
-      __syncthreads();
-      for (int e = tid; e < PW * PW; e += 256) {
-        const int gi = e / PW, gj = e % PW;
-        const int qi = sm.rr_partner[round][gi], qj = sm.rr_partner[round][gj];
-        const int pi = sm.rr_pair[round][gi], pj = sm.rr_pair[round][gj];
-        // column j of Q: Q[j][j] = cs; off entry Q[qj][j] = -conj(sn) when j
-        // is the lower index of its pair, sn when it is the upper one
-        const double2 snj = sm.rsn[pj], sni = sm.rsn[pi];
-        const double2 Qjj = make_double2(sm.rcs[pj], 0.0);
-        const double2 Qoj = (gj < qj) ? make_double2(-snj.x, snj.y) : snj;
-        const double2 Qii = make_double2(sm.rcs[pi], 0.0);
-        const double2 Qoi = (gi < qi) ? make_double2(-sni.x, sni.y) : sni;
-        const double2 g_ij = sm.Gb(cur, gi, gj), g_ioj = sm.Gb(cur, gi, qj);
-        const double2 g_oij = sm.Gb(cur, qi, gj), g_oioj = sm.Gb(cur, qi, qj);
-        const double2 t0 = cadd(cmul(g_ij, Qjj), cmul(g_ioj, Qoj));
-        const double2 t1 = cadd(cmul(g_oij, Qjj), cmul(g_oioj, Qoj));
-        sm.Gb(cur ^ 1, gi, gj) = cadd(cmulc(Qii, t0), cmulc(Qoi, t1));
-        sm.Wb(cur ^ 1, gi, gj) =
-            cadd(cmul(sm.Wb(cur, gi, gj), Qjj), cmul(sm.Wb(cur, gi, qj), Qoj));
-      }
-      cur ^= 1;
-      __syncthreads();
-    }
-    if (!__syncthreads_or(rotated)) break;
-  }
-  for (int e = tid; e < PW * PW; e += 256) {
-    const int gi = e / PW, gj = e % PW;
-    sm.Wl[gi][gj] = sm.Wb(cur, gi, gj);
-  }
-  __syncthreads();
-  return true;
-}
-
-// Rows [a, b) of the panel of M (leading dimension ld) <- panel * Wl.
-// Thread = (row lane 0..63, output group jg 0..3 of PW/4 columns); rows are
-// staged through LDS 64 at a time.
-template <int BW>
-__device__ void panel_apply(double2* M, size_t ld, int a, int b, PairSmem<BW>& sm) {
-  constexpr int PW = 2 * BW, NJ = PW / 4;
-  const int tid = threadIdx.x;
-  const int rl = tid & 63, jg = tid >> 6;
-  for (int r0 = a; r0 < b; r0 += 64) {
-    stage_rows<BW>(M, ld, r0, b, sm);
-    __syncthreads();
-    const int r = r0 + rl;
-    double2 o[NJ];
-#pragma unroll
-    for (int k = 0; k < NJ; ++k) o[k] = make_double2(0.0, 0.0);
-#pragma unroll 4
-    for (int c = 0; c < PW; ++c) {
-      const double2 x = sm.Ps[rl][c];
-#pragma unroll
-      for (int k = 0; k < NJ; ++k) o[k] = cadd(o[k], cmul(x, sm.Wl[c][jg * NJ + k]));
-    }
-    if (r < b) {
-#pragma unroll
-      for (int k = 0; k < NJ; ++k) {
-        const int c = sm.col[jg * NJ + k];
-        if (c >= 0) M[(size_t)c * ld + r] = o[k];
-      }
-    }
-    __syncthreads();
-  }
-}
-
-// Process block pair p of step `step` of sweep `sweep` for task d with one
-// workgroup.  Returns normally in every case (no barrier is skipped by a
-// subset of the block).
-template <int BW>
-__device__ void jacobi_pair(const UpdDesc& d, int nt, int sweep, int step, int p,
-                            double tol_rot, PairSmem<BW>& sm) {
-  const int R = d.info[kInfoR], C = d.info[kInfoC];
-  if (!pair_cols<BW>(C, step, p, sm)) return;
-  panel_gram<BW>(d, 0, R, sm);
-  if (!panel_solve<BW>(d, nt, sweep, true, tol_rot, sm)) return;
-  panel_apply<BW>(d.X, (size_t)d.ldx, 0, R, sm);
-  panel_apply<BW>(d.W, (size_t)d.ldw, 0, C, sm);
-}
-
-// Row range of slice q of S over n rows (slices are multiples of 64 rows).
-__device__ __forceinline__ void split_rows(int n, int S, int q, int& a, int& b) {
-  const int per = (((n + S - 1) / S) + 63) / 64 * 64;
-  a = min(n, q * per);
-  b = min(n, a + per);
-}
-
-// Row-split step, part 1: workgroup (p, q) writes the Gram matrix of rows
-// slice q of pair p's panel to gpart[p][q].
-template <int BW>
-__global__ __launch_bounds__(256, BW == 8 ? 2 : 1) void k_jacobi_gram(const UpdDesc* __restrict__ descs, int nt,
-                                                     int sweep, int step, int S,
-                                                     double conv_tol) {
-  constexpr int PW = 2 * BW;
-  __shared__ PairSmem<BW> sm;
-  const UpdDesc& d = descs[blockIdx.y];
-  if (sweep > 0 && d.offs[(sweep - 1) * nt] <= conv_tol) return;
-  const int R = d.info[kInfoR], C = d.info[kInfoC];
-  const int p = blockIdx.x / S, q = blockIdx.x - (blockIdx.x / S) * S;
-  if (!pair_cols<BW>(C, step, p, sm)) return;
-  int a, b;
-  split_rows(R, S, q, a, b);
-  panel_gram<BW>(d, a, b, sm);
-  for (int e = threadIdx.x; e < PW * PW; e += 256)
-    d.gpart[((size_t)p * S + q) * (PW * PW) + e] = sm.G[e / PW][e % PW];
-}
-
-// Row-split step, part 2: every workgroup of pair p sums the S partial Gram
-// matrices in the same order, so all of them solve the same G into the same
-// Wl, and applies it to its own slice of the rows of X and of W.
-template <int BW>
-__global__ __launch_bounds__(256, BW == 8 ? 2 : 1) void k_jacobi_apply(const UpdDesc* __restrict__ descs, int nt,
-                                                      int sweep, int step, int S,
-                                                      double tol_rot, double conv_tol) {
-  constexpr int PW = 2 * BW;
-  __shared__ PairSmem<BW> sm;
-  const UpdDesc& d = descs[blockIdx.y];
-  if (sweep > 0 && d.offs[(sweep - 1) * nt] <= conv_tol) return;
-  const int R = d.info[kInfoR], C = d.info[kInfoC];
-  const int p = blockIdx.x / S, q = blockIdx.x - (blockIdx.x / S) * S;
-  if (!pair_cols<BW>(C, step, p, sm)) return;
-  for (int e = threadIdx.x; e < PW * PW; e += 256) {
-    double2 g = make_double2(0.0, 0.0);
-    for (int s = 0; s < S; ++s) g = cadd(g, d.gpart[((size_t)p * S + s) * (PW * PW) + e]);
-    sm.G[e / PW][e % PW] = g;
-  }
-  __syncthreads();
-  if (!panel_solve<BW>(d, nt, sweep, q == 0, tol_rot, sm)) return;
-  int a, b;
-  split_rows(R, S, q, a, b);
-  panel_apply<BW>(d.X, (size_t)d.ldx, a, b, sm);
-  split_rows(C, S, q, a, b);
-  panel_apply<BW>(d.W, (size_t)d.ldw, a, b, sm);
-}
-
-// Fused row-split step (cooperative launch: every workgroup of the grid is
-// resident): workgroup (p, q) writes the Gram matrix of its row slice, meets
-// the other S-1 workgroups of pair p at a pair barrier (an arrival counter in
-// device memory; epoch = the task's step count so far), then sums the S
-// partials in the same order as its partners, solves and applies to its
-// rows.  The spin is bounded: past kSpinLimit polls the workgroup raises
-// abort_flag and every later wait returns at once, so a failed barrier ends
-// the kernel with wrong results instead of hanging it.
-constexpr unsigned kSpinLimit = 1u << 22;
-
-__device__ __forceinline__ void pair_barrier(unsigned* cnt, unsigned target, int* abort_flag) {
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    atomicAdd(cnt, 1u);
-    unsigned spins = 0;
-    while (__hip_atomic_load(cnt, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
-      if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
-      __builtin_amdgcn_s_sleep(1);
-      if (++spins > kSpinLimit) {
-        atomicExch(abort_flag, 1);
-        break;
-      }
-    }
-  }
-  __syncthreads();
-  __threadfence();
-}
-
-template <int BW>
-__global__ __launch_bounds__(256, BW == 8 ? 2 : 1) void k_jacobi_fused(
-    const UpdDesc* __restrict__ descs, int nt, int sweep, int step, int S, double tol_rot,
-    double conv_tol) {
-  constexpr int PW = 2 * BW;
-  __shared__ PairSmem<BW> sm;
-  const UpdDesc& d = descs[blockIdx.y];
-  if (sweep > 0 && d.offs[(sweep - 1) * nt] <= conv_tol) return;
-  const int R = d.info[kInfoR], C = d.info[kInfoC];
-  const int p = blockIdx.x / S, q = blockIdx.x - (blockIdx.x / S) * S;
-  if (!pair_cols<BW>(C, step, p, sm)) return;
-  const int nb = (C + BW - 1) / BW;
-  const int nbe = nb + (nb & 1);
-  const unsigned epoch = (unsigned)sweep * (unsigned)(nbe - 1) + (unsigned)step;
-  int a, b;
-  split_rows(R, S, q, a, b);
-  panel_gram<BW>(d, a, b, sm);
-  for (int e = threadIdx.x; e < PW * PW; e += 256)
-    d.gpart[((size_t)p * S + q) * (PW * PW) + e] = sm.G[e / PW][e % PW];
-  pair_barrier(&d.pcnt[p], (unsigned)S * (epoch + 1u), d.abort_flag);
-  for (int e = threadIdx.x; e < PW * PW; e += 256) {
-    double2 g = make_double2(0.0, 0.0);
-    for (int s = 0; s < S; ++s) g = cadd(g, d.gpart[((size_t)p * S + s) * (PW * PW) + e]);
-    sm.G[e / PW][e % PW] = g;
-  }
-  __syncthreads();
-  if (!panel_solve<BW>(d, nt, sweep, q == 0, tol_rot, sm)) return;
-  panel_apply<BW>(d.X, (size_t)d.ldx, a, b, sm);
-  split_rows(C, S, q, a, b);
-  panel_apply<BW>(d.W, (size_t)d.ldw, a, b, sm);
-}
-
-// One launch per step, one workgroup per block pair.
-template <int BW>
-__global__ __launch_bounds__(256, BW == 8 ? 2 : 1) void k_jacobi_step(const UpdDesc* __restrict__ descs, int nt,
-                                                     int sweep, int step, double tol_rot,
-                                                     double conv_tol) {
-  __shared__ PairSmem<BW> sm;
-  const UpdDesc& d = descs[blockIdx.y];
-  if (sweep > 0 && d.offs[(sweep - 1) * nt] <= conv_tol) return;
-  jacobi_pair<BW>(d, nt, sweep, step, blockIdx.x, tol_rot, sm);
-}
-
-// Persistent cooperative kernel: all sweeps and steps of a batch of SVDs in
-// one launch.  Work item = (task, pair); a grid barrier separates steps.
-// Every block runs the same sequence of grid barriers: the loops have
-// grid-uniform bounds and the convergence decision after each sweep is
-// computed by every block from the same (final) per-sweep maxima.
-__global__ __launch_bounds__(256) void k_jacobi_coop(const UpdDesc* __restrict__ descs, int nt,
-                                                     int npairs, int steps, int max_sweeps,
-                                                     double tol_rot, double conv_tol,
-                                                     int* sweeps_out) {
-  __shared__ PairSmem<8> sm;
-  __shared__ int s_done;
-  cg::grid_group grid = cg::this_grid();
-  const int items = nt * npairs;
-  for (int sw = 0; sw < max_sweeps; ++sw) {
-    for (int st = 0; st < steps; ++st) {
-      for (int it = blockIdx.x; it < items; it += gridDim.x) {
-        const int t = it / npairs, p = it - t * npairs;
-        const UpdDesc& d = descs[t];
-        if (sw > 0 && d.offs[(sw - 1) * nt] <= conv_tol) continue;
-        jacobi_pair<8>(d, nt, sw, st, p, tol_rot, sm);
-      }
-      grid.sync();
-    }
-    if (threadIdx.x == 0) {
-      int done = 1;
-      for (int t = 0; t < nt; ++t)
-        if (!(descs[t].offs[sw * nt] <= conv_tol)) {
-          done = 0;
-          break;
-        }
-      s_done = done;
-      if (blockIdx.x == 0) *sweeps_out = sw + 1;
-    }
-    __syncthreads();
-    const int done = s_done;
-    __syncthreads();
-    if (done) break;
   }
 }
 
@@ -738,7 +203,7 @@ constexpr double kF32NoiseFloor = 9.5367431640625e-07;  // 2^-20
 // descending, apply process_svd (mps.rs:236-293): normalise, pick the rank,
 // renormalise, and record the right-factor scale fac_j = s_j * norm2.
 __global__ __launch_bounds__(1024) void k_finish(const UpdDesc* __restrict__ descs, int nt,
-                                                 const int* sweeps_dev, int sweeps_run) {
+                                                 int sweeps_run) {
   const UpdDesc& d = descs[blockIdx.x];
   const int R = d.info[kInfoR], C = d.info[kInfoC];
   __shared__ double key[kMaxCols];
@@ -747,7 +212,7 @@ __global__ __launch_bounds__(1024) void k_finish(const UpdDesc* __restrict__ des
   __shared__ int first_bad;
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
 
-  if (tid == 0) d.info[kInfoSweeps] = sweeps_dev ? *sweeps_dev : sweeps_run;
+  if (tid == 0) d.info[kInfoSweeps] = sweeps_run;
 
   for (int j = wid; j < C; j += 16) {
     double s = 0.0;
@@ -1096,73 +561,8 @@ void launch_theta(const UpdDesc* descs, int nt, int gx, hipStream_t s) {
 void launch_winit(const UpdDesc* descs, int nt, int gx, hipStream_t s) {
   hipLaunchKernelGGL(k_winit, dim3(gx, nt), dim3(256), 0, s, descs);
 }
-void launch_jacobi_step(const UpdDesc* descs, int nt, int npairs, int bw, int sweep, int step,
-                        double tol_rot, double conv_tol, hipStream_t s) {
-  if (bw == 16)
-    hipLaunchKernelGGL(k_jacobi_step<16>, dim3(npairs, nt), dim3(256), 0, s, descs, nt, sweep,
-                       step, tol_rot, conv_tol);
-  else
-    hipLaunchKernelGGL(k_jacobi_step<8>, dim3(npairs, nt), dim3(256), 0, s, descs, nt, sweep,
-                       step, tol_rot, conv_tol);
-}
-void launch_jacobi_gram(const UpdDesc* descs, int nt, int npairs, int bw, int S, int sweep,
-                        int step, double conv_tol, hipStream_t s) {
-  if (bw == 16)
-    hipLaunchKernelGGL(k_jacobi_gram<16>, dim3(npairs * S, nt), dim3(256), 0, s, descs, nt,
-                       sweep, step, S, conv_tol);
-  else
-    hipLaunchKernelGGL(k_jacobi_gram<8>, dim3(npairs * S, nt), dim3(256), 0, s, descs, nt,
-                       sweep, step, S, conv_tol);
-}
-void launch_jacobi_apply(const UpdDesc* descs, int nt, int npairs, int bw, int S, int sweep,
-                         int step, double tol_rot, double conv_tol, hipStream_t s) {
-  if (bw == 16)
-    hipLaunchKernelGGL(k_jacobi_apply<16>, dim3(npairs * S, nt), dim3(256), 0, s, descs, nt,
-                       sweep, step, S, tol_rot, conv_tol);
-  else
-    hipLaunchKernelGGL(k_jacobi_apply<8>, dim3(npairs * S, nt), dim3(256), 0, s, descs, nt,
-                       sweep, step, S, tol_rot, conv_tol);
-}
-hipError_t launch_jacobi_fused(const UpdDesc* descs, int nt, int npairs, int S, int sweep,
-                               int step, double tol_rot, double conv_tol, hipStream_t s) {
-  void* args[] = {(void*)&descs, (void*)&nt, (void*)&sweep, (void*)&step, (void*)&S,
-                  (void*)&tol_rot, (void*)&conv_tol};
-  return hipLaunchCooperativeKernel((const void*)k_jacobi_fused<8>, dim3(npairs * S, nt),
-                                    dim3(256), args, 0, s);
-}
-int jacobi_fused_max_blocks() {
-  int dev = 0, per_cu = 0;
-  hipDeviceProp_t prop;
-  if (hipGetDevice(&dev) != hipSuccess) return 0;
-  if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return 0;
-  if (!prop.cooperativeLaunch) return 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_jacobi_fused<8>, 256,
-                                                   0) != hipSuccess)
-    return 0;
-  return per_cu * prop.multiProcessorCount;
-}
-hipError_t launch_jacobi_coop(const UpdDesc* descs, int nt, int npairs, int steps,
-                              int max_sweeps, double tol_rot, double conv_tol, int* sweeps_out,
-                              int grid, hipStream_t s) {
-  void* args[] = {(void*)&descs, (void*)&nt, (void*)&npairs, (void*)&steps, (void*)&max_sweeps,
-                  (void*)&tol_rot, (void*)&conv_tol, (void*)&sweeps_out};
-  return hipLaunchCooperativeKernel((const void*)k_jacobi_coop, dim3(grid), dim3(256), args, 0,
-                                    s);
-}
-int jacobi_coop_max_blocks() {
-  int dev = 0, per_cu = 0;
-  hipDeviceProp_t prop;
-  if (hipGetDevice(&dev) != hipSuccess) return 0;
-  if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return 0;
-  if (!prop.cooperativeLaunch) return 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_jacobi_coop, 256, 0) !=
-      hipSuccess)
-    return 0;
-  return per_cu * prop.multiProcessorCount;
-}
-void launch_finish(const UpdDesc* descs, int nt, const int* sweeps_dev, int sweeps_run,
-                   hipStream_t s) {
-  hipLaunchKernelGGL(k_finish, dim3(nt), dim3(1024), 0, s, descs, nt, sweeps_dev, sweeps_run);
+void launch_finish(const UpdDesc* descs, int nt, int sweeps_run, hipStream_t s) {
+  hipLaunchKernelGGL(k_finish, dim3(nt), dim3(1024), 0, s, descs, nt, sweeps_run);
 }
 void launch_fmul(const UpdDesc* descs, int nt, int maxX, int maxRank, hipStream_t s) {
   hipLaunchKernelGGL(k_fmul, dim3((maxX + 63) / 64, (maxRank + 31) / 32, nt), dim3(256), 0, s,

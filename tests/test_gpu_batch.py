@@ -1,5 +1,5 @@
 """Lock-step batching of many trajectories on one engine, the FP32
-contraction mode, and the two Jacobi launch modes.
+contraction mode, and the Jacobi launch shapes.
 
 Each trajectory of a batch must give the oracle's results (same tolerances
 as tests/test_gpu_parity.py) and the same results as when run alone.
@@ -102,17 +102,26 @@ def test_fp32_contraction_error(gpu):
     assert s0 < 1e-4
 
 
-def test_step_and_cooperative_jacobi_agree(gpu, monkeypatch):
-    """The per-step launch mode (default; row-split when few pairs) and the
-    persistent cooperative kernel (TNMPS_JACOBI=coop) give the same spectra."""
+def test_step_and_row_split_jacobi_agree(gpu, monkeypatch):
+    """The one-workgroup-per-pair step (TNMPS_SPLIT_MAX=1) and the row-split
+    step (the default for lone updates) give the oracle's spectra, and so do
+    the wide 16-column panels (TNMPS_BW=16) and the cross-block-only inner
+    rounds after a sweep's first step (TNMPS_CROSS_ONLY=1)."""
     n, depth, p, chi = 12, 10, 0.1, 32
     c = haar_circuit(n, depth, p, 31)
     ref = run_trajectory(n, c.layers, ("const", chi), rs=c.draws)
-    a = MPS.new_qubits(n, ("const", chi))
-    a.run_circuit(c.layers, forced=ref["record"])
-    monkeypatch.setenv("TNMPS_JACOBI", "coop")
-    b = MPS.new_qubits(n, ("const", chi))
-    b.run_circuit(c.layers, forced=ref["record"])
-    for bd in range(n - 1):
-        assert_spectrum_close(a.schmidt(bd), ref["state"].L[bd])
-        assert_spectrum_close(b.schmidt(bd), ref["state"].L[bd])
+    runs = []
+    for env in ({}, {"TNMPS_SPLIT_MAX": "1"}, {"TNMPS_BW": "16"},
+                {"TNMPS_BW": "16", "TNMPS_SPLIT_MAX": "1"}, {"TNMPS_CROSS_ONLY": "1"},
+                {"TNMPS_CROSS_ONLY": "1", "TNMPS_SPLIT_MAX": "1"}):
+        for k in ("TNMPS_SPLIT_MAX", "TNMPS_BW", "TNMPS_CROSS_ONLY"):
+            monkeypatch.delenv(k, raising=False)
+        for k, v in env.items():
+            monkeypatch.setenv(k, v)
+        st = MPS.new_qubits(n, ("const", chi))
+        st.run_circuit(c.layers, forced=ref["record"])
+        runs.append(st)
+    for st in runs:
+        assert st.bond_dims() == ref["state"].bond_dims()
+        for bd in range(n - 1):
+            assert_spectrum_close(st.schmidt(bd), ref["state"].L[bd])
