@@ -180,6 +180,7 @@ struct tn_engine {
   int bw_env = 0;             // forced Jacobi block width (TNMPS_BW), 0 = automatic
   bool factored = true;       // factored updates where a factor is isometric (TNMPS_FACTORED)
   Tolerances tol;             // Jacobi tolerances and inner-solve options
+  bool trace_sweeps = false;  // print each sweep's largest off-diagonal (TNMPS_TRACE_SWEEPS)
   DescRing<UpdDesc> upd;
   DescRing<MeasDesc> meas;
   int mres = 0;  // measurement result slots
@@ -253,6 +254,7 @@ struct tn_engine {
     bw_env = (int)env_double("TNMPS_BW", 0);
     factored = env_double("TNMPS_FACTORED", 1) != 0;
     tol.read_env();
+    trace_sweeps = env_double("TNMPS_TRACE_SWEEPS", 0) != 0;
     // workspace budget: TNMPS_WS_BYTES (default 24 GiB) bounds the tasks per launch
     const double budget = env_double("TNMPS_WS_BYTES", 24.0 * (1ull << 30));
     const double per = 16.0 * ((double)ldx * ldw + (double)ldw * ldw) + 64.0 * ldw + 4096.0;
@@ -371,11 +373,28 @@ int factor_mode(const tn_engine* e, const tn_mps* h, size_t k) {
   return fl ? 1 : (fr ? 2 : 0);
 }
 
+// Gauge prepass of a refactor of bond k with neither site fresh but
+// K < min(m, n): theta = A M with A = lambda Gamma[k] lambda[k] (m x K) and
+// M = Gamma[k+1] lambda[k+1] (K x n).  Mode 2 on A (or mode 1 on M^H, the
+// one with fewer rows) is a valid gauge change even when the other factor is
+// not an isometry: it writes the same theta up to a positive scale, with
+// site k left-fresh (site k+1 right-fresh).  The refactor that follows is
+// then factored too: two SVDs of K columns instead of one of a
+// rank-deficient min(m, n)-column theta.  0: no prepass.
+int prepass_mode(const tn_engine* e, const tn_mps* h, size_t k) {
+  if (!e->factored) return 0;
+  if ((h->fresh[k] & kFreshLeft) || (h->fresh[k + 1] & kFreshRight)) return 0;
+  const size_t m = 2 * h->uL(k), n = 2 * h->uR(k + 1);
+  if (!(h->ub[k] < std::min(m, n))) return 0;
+  return m <= n ? 2 : 1;
+}
+
 // One two-site update of one handle.
 struct Task {
   tn_mps* h;
   size_t k;
   bool has_gate;
+  int pre = 0;  // gauge prepass in this mode (not an update of the reference)
   double gate[32];
 };
 
@@ -404,7 +423,7 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
     const size_t k = q.k;
     UpdDesc& d = hd[t];
     std::memset(&d, 0, sizeof(d));
-    d.mode = q.has_gate ? 0 : factor_mode(e, h, k);
+    d.mode = q.has_gate ? 0 : (q.pre ? q.pre : factor_mode(e, h, k));
     any_factored = any_factored || d.mode != 0;
     d.gk = h->gam + h->goff[k];
     d.gk1 = h->gam + h->goff[k + 1];
@@ -517,8 +536,14 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
       e->perf.jacobi_launches += (size_t)steps * (S == 1 ? 1 : 2);
     }
     bool done = true;
-    for (int t = 0; t < nt; ++t)
+    double worst = 0.0;
+    for (int t = 0; t < nt; ++t) {
       if (!(offs[t] <= tol.conv_tol)) done = false;
+      worst = std::max(worst, offs[t]);
+    }
+    if (e->trace_sweeps)
+      std::fprintf(stderr, "[tnmps] batch of %d (C <= %zu): sweep %d max off-diagonal %.3e\n", nt,
+                   maxC, sw, worst);
     if (done) break;
   }
   if (e->timing) TN_HIP(hipEventRecord(e->tev[4], e->stream));
@@ -527,7 +552,9 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
   launch_split(dd, nt, e->stream);
   launch_renorm(dd, nt, e->stream);
   TN_HIP(hipGetLastError());
-  e->updates += nt;
+  int n_updates = 0;
+  for (int t = 0; t < nt; ++t) n_updates += tasks[t].pre ? 0 : 1;
+  e->updates += n_updates;
   e->sweeps += (unsigned long long)sweeps_run;
   if (e->timing) {
     TN_HIP(hipEventRecord(e->tev[5], e->stream));
@@ -536,7 +563,7 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
                           hipMemcpyDeviceToHost, e->stream));
     TN_HIP(hipStreamSynchronize(e->stream));
     e->perf.ms_post += e->span(4, 5);
-    e->perf.updates += nt;
+    e->perf.updates += n_updates;
     e->perf.sweeps += sweeps_run;
     for (int t = 0; t < nt; ++t) {
       // flops of the SVD posed (R x C; LAPACK-style count of a complex SVD
@@ -554,21 +581,42 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
     }
   }
   for (int t = 0; t < nt; ++t) {
-    tn_mps* h = tasks[t].h;
-    const size_t k = tasks[t].k;
+    const Task& q = tasks[t];
+    tn_mps* h = q.h;
+    const size_t k = q.k;
     const size_t chiL = h->uL(k), chiR = h->uR(k + 1);
-    h->ub[k] = std::min({2 * chiL, 2 * chiR, rank_bound(h), h->bmax[k]});
-    h->fresh[k] = kFreshLeft;
-    h->fresh[k + 1] = kFreshRight;
+    // a refactor's rank is at most min(K, m, n) <= ub[k]; a gate's min(m, n)
+    const size_t b = std::min({2 * chiL, 2 * chiR, rank_bound(h), h->bmax[k]});
+    h->ub[k] = q.has_gate ? b : std::min(b, h->ub[k]);
+    // the prepass makes only its decomposed factor's site fresh
+    h->fresh[k] = (q.pre == 1) ? 0 : kFreshLeft;
+    h->fresh[k + 1] = (q.pre == 2) ? 0 : kFreshRight;
   }
   return TN_OK;
 }
 
-tn_status run_updates(tn_engine* e, const std::vector<Task>& tasks) {
+tn_status run_tasks(tn_engine* e, const std::vector<Task>& tasks) {
   const int nt = (int)tasks.size();
   for (int a = 0; a < nt; a += e->max_slots)
     TN_TRY(run_chunk(e, tasks.data() + a, std::min(e->max_slots, nt - a)));
   return TN_OK;
+}
+
+// Updates of one batch (bonds of one handle at least 2 apart), each refactor
+// that needs one preceded by its gauge prepass.
+tn_status run_updates(tn_engine* e, const std::vector<Task>& tasks) {
+  std::vector<Task> pre;
+  for (const Task& t : tasks) {
+    if (t.has_gate) continue;
+    const int m = prepass_mode(e, t.h, t.k);
+    if (m) {
+      Task p = t;
+      p.pre = m;
+      pre.push_back(p);
+    }
+  }
+  if (!pre.empty()) TN_TRY(run_tasks(e, pre));
+  return run_tasks(e, tasks);
 }
 
 tn_status refactor(tn_mps* h, size_t k) {

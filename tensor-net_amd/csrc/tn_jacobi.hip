@@ -39,8 +39,9 @@ __device__ __forceinline__ int rr_player(int i, int step, int nbe) {
 }
 
 // Columns whose squared norm is below kTinyNorm2 (norm 1e-17, under
-// f64::EPSILON times a unit-norm theta, so below every truncation threshold of
-// the rank rule) take part in no rotation and in no convergence measure.
+// f64::EPSILON times the unit-norm SVD input, so below every truncation
+// threshold of the rank rule) take part in no rotation and in no convergence
+// measure.
 constexpr double kTinyNorm2 = 1e-34;
 
 template <int BW>
@@ -252,10 +253,10 @@ __device__ __forceinline__ double rsq_nr(double x) {
 // from the same t^2 = 4 |c|^2 / (|d| + r)^2.  Identity (and false) when the
 // pair is already orthogonal to tol or a column is negligible.  Branch-free,
 // so that the two rotations a thread needs per round overlap.
-__device__ __forceinline__ bool rot2(double a, double b, double2 c, double tol, double& cs,
-                                     double2& sn) {
+__device__ __forceinline__ bool rot2(double a, double b, double2 c, double tol, double tiny2,
+                                     double& cs, double2& sn) {
   const double c2 = cabs2(c);
-  const bool go = a > kTinyNorm2 && b > kTinyNorm2 && c2 > tol * tol * a * b;
+  const bool go = a > tiny2 && b > tiny2 && c2 > tol * tol * a * b;
   const double d = b - a;
   const double r2 = go ? fma(d, d, 4.0 * c2) : 1.0;
   const double s = fabs(d) + r2 * __builtin_amdgcn_rsq(r2);  // |d| + r
@@ -274,13 +275,14 @@ __device__ __forceinline__ bool rot2(double a, double b, double2 c, double tol, 
 // accumulating the rotation in sm.Ws[*wsel].  Round r = r_first .. PW-1 pairs
 // index x with x ^ r: PW/2 disjoint rotations per round, and from r_first = 1
 // every pair once per inner sweep; rounds r >= BW pair a column of the first
-// block with one of the second (the cross pairs).  Every thread computes the rotations of its own row and column
-// pairs from the round's Gram buffer, so a round needs one barrier.  Returns
-// false (for the whole block) when there is nothing to apply.  A function of
-// sm.Gs[0] alone: blocks holding the same G compute bitwise the same rotation.
+// block with one of the second (the cross pairs).  Each work item computes
+// the rotations it needs from the round's Gram buffer, so a round needs one
+// barrier.  Returns false (for the whole block) when there is nothing to
+// apply.  A function of sm.Gs[0] alone: blocks holding the same G compute
+// bitwise the same rotation.
 template <int BW>
-__device__ bool panel_solve(double* rec, int inner_sweeps, double tol_rot, int r_first,
-                            PairSmem<BW>& sm, int& wsel) {
+__device__ bool panel_solve(double* rec, int inner_sweeps, double tol_rot, double tiny2,
+                            int r_first, PairSmem<BW>& sm, int& wsel) {
   constexpr int PW = 2 * BW;
   const int tid = threadIdx.x;
   double ratio2 = 0.0;
@@ -289,7 +291,7 @@ __device__ bool panel_solve(double* rec, int inner_sweeps, double tol_rot, int r
     sm.Ws(0, gi, gj) = make_double2(gi == gj ? 1.0 : 0.0, 0.0);
     if (gi < gj) {
       const double a = sm.Gs[0][gi][gi].x, b = sm.Gs[0][gj][gj].x;
-      if (a > kTinyNorm2 && b > kTinyNorm2) ratio2 = fmax(ratio2, cabs2(sm.Gs[0][gi][gj]) / (a * b));
+      if (a > tiny2 && b > tiny2) ratio2 = fmax(ratio2, cabs2(sm.Gs[0][gi][gj]) / (a * b));
     }
   }
   const double off = sqrt(block_max(ratio2, sm.red));
@@ -297,34 +299,58 @@ __device__ bool panel_solve(double* rec, int inner_sweeps, double tol_rot, int r
     atomicMax((unsigned long long*)rec, (unsigned long long)__double_as_longlong(off));
   if (off <= tol_rot) return false;
 
+  // Work items of a round: the (PW/2)^2 2x2 blocks of G on the round's row
+  // pair x column pair (G' = Qa^H G Qb touches only its own block), then the
+  // PW x PW/2 (row, column pair) items of W' = W Qb.  Pair a of round r is
+  // (x, x ^ r) with x = a with a zero bit inserted at r's highest set bit h.
+  constexpr int HP = PW / 2, NG = HP * HP, NW = PW * HP;
   int cur = 0;
   for (int isw = 0; isw < inner_sweeps; ++isw) {
     int rotated = 0;
     for (int round = r_first; round < PW; ++round) {
-      for (int e = tid; e < PW * PW; e += 256) {
-        const int gi = e / PW, gj = e % PW;
-        const int qi = gi ^ round, qj = gj ^ round;
-        const int i1 = min(gi, qi), i2 = max(gi, qi);
-        const int j1 = min(gj, qj), j2 = max(gj, qj);
-        const double2 g11 = sm.Gs[cur][i1][i1], g22 = sm.Gs[cur][i2][i2], g12 = sm.Gs[cur][i1][i2];
-        const double2 h11 = sm.Gs[cur][j1][j1], h22 = sm.Gs[cur][j2][j2], h12 = sm.Gs[cur][j1][j2];
-        const double2 g_ij = sm.Gs[cur][gi][gj], g_ioj = sm.Gs[cur][gi][qj];
-        const double2 g_oij = sm.Gs[cur][qi][gj], g_oioj = sm.Gs[cur][qi][qj];
-        const double2 w_ij = sm.Ws(cur, gi, gj), w_ioj = sm.Ws(cur, gi, qj);
-        double csi, csj;
-        double2 sni, snj;
-        rotated |= rot2(g11.x, g22.x, g12, tol_rot, csi, sni);
-        rot2(h11.x, h22.x, h12, tol_rot, csj, snj);
-        // column j of Q: Q[j][j] = cs; off entry Q[qj][j] = -conj(sn) when j
-        // is the lower index of its pair, sn when it is the upper one
-        const double2 Qjj = make_double2(csj, 0.0);
-        const double2 Qoj = (gj < qj) ? make_double2(-snj.x, snj.y) : snj;
-        const double2 Qii = make_double2(csi, 0.0);
-        const double2 Qoi = (gi < qi) ? make_double2(-sni.x, sni.y) : sni;
-        const double2 t0 = cadd(cmul(g_ij, Qjj), cmul(g_ioj, Qoj));
-        const double2 t1 = cadd(cmul(g_oij, Qjj), cmul(g_oioj, Qoj));
-        sm.Gs[cur ^ 1][gi][gj] = cadd(cmulc(Qii, t0), cmulc(Qoi, t1));
-        sm.Ws(cur ^ 1, gi, gj) = cadd(cmul(w_ij, Qjj), cmul(w_ioj, Qoj));
+      const int h = 31 - __builtin_clz(round);
+      const int lowm = (1 << h) - 1;
+      for (int e = tid; e < NG + NW; e += 256) {
+        if (e < NG) {
+          const int pa = e / HP, pb = e % HP;
+          const int x1 = ((pa >> h) << (h + 1)) | (pa & lowm), x2 = x1 ^ round;
+          const int y1 = ((pb >> h) << (h + 1)) | (pb & lowm), y2 = y1 ^ round;
+          const double ax = sm.Gs[cur][x1][x1].x, bx = sm.Gs[cur][x2][x2].x;
+          const double2 cx = sm.Gs[cur][x1][x2];
+          const double ay = sm.Gs[cur][y1][y1].x, by = sm.Gs[cur][y2][y2].x;
+          const double2 cy = sm.Gs[cur][y1][y2];
+          const double2 g11 = sm.Gs[cur][x1][y1], g12 = sm.Gs[cur][x1][y2];
+          const double2 g21 = sm.Gs[cur][x2][y1], g22 = sm.Gs[cur][x2][y2];
+          double csa, csb;
+          double2 sna, snb;
+          rotated |= rot2(ax, bx, cx, tol_rot, tiny2, csa, sna);
+          rot2(ay, by, cy, tol_rot, tiny2, csb, snb);
+          // T = G[block] Qb, then G' = Qa^H T, with Q = [[cs, sn], [-conj(sn), cs]]
+          const double2 msb = make_double2(-snb.x, snb.y);  // -conj(sn_b)
+          const double2 t11 = cadd(cscale(g11, csb), cmul(g12, msb));
+          const double2 t12 = cadd(cmul(g11, snb), cscale(g12, csb));
+          const double2 t21 = cadd(cscale(g21, csb), cmul(g22, msb));
+          const double2 t22 = cadd(cmul(g21, snb), cscale(g22, csb));
+          const double2 nsa = make_double2(-sna.x, -sna.y);  // -sn_a
+          const double2 csa_ = cconj(sna);                    // conj(sn_a)
+          sm.Gs[cur ^ 1][x1][y1] = cadd(cscale(t11, csa), cmul(nsa, t21));
+          sm.Gs[cur ^ 1][x1][y2] = cadd(cscale(t12, csa), cmul(nsa, t22));
+          sm.Gs[cur ^ 1][x2][y1] = cadd(cmul(csa_, t11), cscale(t21, csa));
+          sm.Gs[cur ^ 1][x2][y2] = cadd(cmul(csa_, t12), cscale(t22, csa));
+        } else {
+          const int w = e - NG;
+          const int i = w / HP, pb = w % HP;
+          const int y1 = ((pb >> h) << (h + 1)) | (pb & lowm), y2 = y1 ^ round;
+          const double ay = sm.Gs[cur][y1][y1].x, by = sm.Gs[cur][y2][y2].x;
+          const double2 cy = sm.Gs[cur][y1][y2];
+          const double2 w1 = sm.Ws(cur, i, y1), w2 = sm.Ws(cur, i, y2);
+          double csb;
+          double2 snb;
+          rot2(ay, by, cy, tol_rot, tiny2, csb, snb);
+          const double2 msb = make_double2(-snb.x, snb.y);
+          sm.Ws(cur ^ 1, i, y1) = cadd(cscale(w1, csb), cmul(w2, msb));
+          sm.Ws(cur ^ 1, i, y2) = cadd(cmul(w1, snb), cscale(w2, csb));
+        }
       }
       cur ^= 1;
       __syncthreads();
@@ -446,7 +472,7 @@ __global__ __launch_bounds__(256, BW == 8 ? 2 : 1) void k_jacobi_step(JacobiArgs
   load_chunk<BW>(W, (size_t)a.ldw, 0, C, C, lcol, cw);
   int wsel = 0;
   if (!panel_solve<BW>(a.offs + (size_t)sweep * a.nt + t, a.inner_sweeps, a.tol_rot,
-                       first_round<BW>(a, step), sm, wsel))
+                       kTinyNorm2, first_round<BW>(a, step), sm, wsel))
     return;
   panel_apply<BW>(X, (size_t)a.ldx, 0, R, C, lcol, ocol, cx, sm, wsel);
   panel_apply<BW>(W, (size_t)a.ldw, 0, C, C, lcol, ocol, cw, sm, wsel);
@@ -525,7 +551,7 @@ __global__ __launch_bounds__(256, BW == 8 ? 2 : 1) void k_jacobi_apply(JacobiArg
   mask_chunk<BW>(wa, we, C, lcol, cw);
   int wsel = 0;
   if (!panel_solve<BW>(q == 0 ? a.offs + (size_t)sweep * a.nt + t : nullptr, a.inner_sweeps,
-                       a.tol_rot, first_round<BW>(a, step), sm, wsel))
+                       a.tol_rot, kTinyNorm2, first_round<BW>(a, step), sm, wsel))
     return;
   panel_apply<BW>(X, (size_t)a.ldx, xa, xe, C, lcol, ocol, cx, sm, wsel);
   panel_apply<BW>(W, (size_t)a.ldw, wa, we, C, lcol, ocol, cw, sm, wsel);
