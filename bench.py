@@ -36,10 +36,13 @@ for _p in (REPO, REPO / "tensor-net_amd"):
         sys.path.insert(0, str(_p))
 
 METRIC = "gate+SVD updates/s @chi=256 FP64 vs mixed; trajectories/s + p99 latency"
-# HBM bytes per dispatch of the dominant Jacobi kernels on this workload, from
-# rocprofv3 FETCH_SIZE / WRITE_SIZE runs scaled by a 1 GiB copy calibration
-# (profiles/jacobi_single_traj_chi256_hbm_traffic.md).
-TRAFFIC_BYTES = {"k_jacobi_apply<8>": 251.3 * 1024, "k_jacobi_gram<8>": 82.7 * 1024}
+# Memory traffic per dispatch of the dominant Jacobi kernels on this workload:
+# rocprofv3 FETCH_SIZE and WRITE_SIZE (KiB per dispatch, one counter per run)
+# scaled by what the same counters read for a plain 1 GiB device copy
+# (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0); profiles/jacobi_single_traj_chi256_hbm_traffic.md.
+PMC_KIB = {"k_jacobi_apply<8>": (3222.654, 375.280), "k_jacobi_gram<8>": (943.203, 455.115)}
+PMC_SCALE = (2.0, 1.0)
+TRAFFIC_BYTES = {k: 1024.0 * (f * PMC_SCALE[0] + w * PMC_SCALE[1]) for k, (f, w) in PMC_KIB.items()}
 FP64_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
 

@@ -152,6 +152,22 @@ struct EntReq {
 
 }  // namespace
 
+// Device-time accounting of one update batch or measurement launch.  Its
+// events and a pinned copy of the batch's info are read back lazily (when
+// they have completed, or when the counters are asked for), so timing adds
+// no host synchronisation to the stream.
+struct PerfRec {
+  hipEvent_t ev[6] = {};  // theta begin/end, sweeps begin/end, post begin/end
+  int* h_info = nullptr;  // pinned copy of the batch's info
+  int cap = 0;            // tasks h_info holds
+  std::vector<char> gate; // has_gate per task
+  int kind = 0;           // 0 update batch, 1 measurement launch (ev[0], ev[1])
+  int nt = 0, n_updates = 0, sweeps = 0;
+  size_t launches = 0;
+  bool pending = false;
+};
+constexpr int kPerfRing = 16;
+
 // --------------------------------------------------------------------------
 // Engine: stream, update workspace, descriptor staging, result buffers.
 // --------------------------------------------------------------------------
@@ -192,9 +208,11 @@ struct tn_engine {
   int nscalar = 0;
   bool timing = false;
   double perf_host_launch_ms = 0.0;  // host time spent enqueueing Jacobi steps
-  hipEvent_t tev[6] = {};
+  hipEvent_t sev[kMaxSweeps] = {};  // per-sweep completion (convergence readback)
+  double* h_offs = nullptr;         // pinned copy of offs
+  PerfRec prec[kPerfRing];          // timing records, oldest at prec_next
+  int prec_next = 0;
   tn_perf perf{};
-  std::vector<int> h_info;
   unsigned long long updates = 0, sweeps = 0;
   int users = 0;               // handles bound to this engine
   bool auto_free = false;      // freed with its last handle
@@ -204,6 +222,8 @@ struct tn_engine {
   void release_ws() {
     hfree(X); hfree(W); hfree(sraw); hfree(sval); hfree(fac); hfree(perm);
     hfree(part); hfree(offs); hfree(info); hfree(gpart);
+    hhfree(h_offs);
+    h_offs = nullptr;
     gpart = nullptr;
     X = W = nullptr;
     sraw = sval = fac = part = offs = nullptr;
@@ -222,7 +242,15 @@ struct tn_engine {
     upd.release();
     meas.release();
     hfree(d_scalar); d_scalar = nullptr; nscalar = 0;
-    for (hipEvent_t& ev : tev)
+    for (PerfRec& r : prec) {
+      for (hipEvent_t& ev : r.ev)
+        if (ev) (void)hipEventDestroy(ev), ev = nullptr;
+      hhfree(r.h_info);
+      r.h_info = nullptr;
+      r.cap = 0;
+      r.pending = false;
+    }
+    for (hipEvent_t& ev : sev)
       if (ev) (void)hipEventDestroy(ev), ev = nullptr;
     if (own_stream && stream) (void)hipStreamDestroy(stream);
     stream = nullptr;
@@ -239,7 +267,9 @@ struct tn_engine {
     }
     chi_max = chi;
     ldx = ldw = (int)(2 * chi);
-    for (hipEvent_t& ev : tev) TN_HIP(hipEventCreate(&ev));
+    for (PerfRec& r : prec)
+      for (hipEvent_t& ev : r.ev) TN_HIP(hipEventCreate(&ev));
+    for (hipEvent_t& ev : sev) TN_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
     hipDeviceProp_t prop;
     TN_HIP(hipGetDeviceProperties(&prop, dev));
     cus = prop.multiProcessorCount;
@@ -262,10 +292,77 @@ struct tn_engine {
     return TN_OK;
   }
 
-  float span(int a, int b) {
+  static float span(const PerfRec& r, int a, int b) {
     float ms = 0.f;
-    if (hipEventElapsedTime(&ms, tev[a], tev[b]) != hipSuccess) return 0.f;
+    if (hipEventElapsedTime(&ms, r.ev[a], r.ev[b]) != hipSuccess) return 0.f;
     return ms;
+  }
+
+  // Add a completed record to the counters.
+  void perf_process(PerfRec& r) {
+    if (r.kind == 1) {
+      perf.ms_measure += span(r, 0, 1);
+    } else {
+      perf.ms_theta += span(r, 0, 1);
+      perf.ms_jacobi += span(r, 2, 3);
+      perf.ms_post += span(r, 4, 5);
+      perf.jacobi_launches += r.launches;
+      perf.updates += r.n_updates;
+      perf.sweeps += r.sweeps;
+      for (int t = 0; t < r.nt; ++t) {
+        // flops of the SVD posed (R x C; LAPACK-style count of a complex SVD
+        // with both factors) plus the contraction or the factor product
+        const int* inf = r.h_info + (size_t)t * kInfoLen;
+        const double m = inf[kInfoM], n = inf[kInfoN], K = inf[kInfoChiM];
+        const double M = inf[kInfoR], N = inf[kInfoC], rank = inf[kInfoRank];
+        const double fsvd = 4.0 * (4.0 * M * M * N + 8.0 * M * N * N + 9.0 * N * N * N);
+        perf.svd_flops += fsvd;
+        const int mode = inf[kInfoMode];
+        if (mode != 0) perf.factored++;
+        const double fpre = mode == 0 ? 8.0 * m * n * K + (r.gate[t] ? 32.0 * m * n : 0.0)
+                                      : 8.0 * (mode == 1 ? m : n) * K * rank;
+        perf.alg_flops += fpre + fsvd;
+      }
+    }
+    r.pending = false;
+  }
+
+  // Process pending records oldest first; without `block`, stop at the first
+  // one whose events have not completed.
+  tn_status perf_drain(bool block) {
+    for (int i = 0; i < kPerfRing; ++i) {
+      PerfRec& r = prec[(prec_next + i) % kPerfRing];
+      if (!r.pending) continue;
+      const hipEvent_t last = r.ev[r.kind == 1 ? 1 : 5];
+      if (block) {
+        TN_HIP(hipEventSynchronize(last));
+      } else if (hipEventQuery(last) != hipSuccess) {
+        break;
+      }
+      perf_process(r);
+    }
+    return TN_OK;
+  }
+
+  // The next record, processed first if still pending, with room for nt tasks.
+  tn_status perf_begin(int kind, int nt, PerfRec** out) {
+    PerfRec& r = prec[prec_next];
+    if (r.pending) {
+      TN_HIP(hipEventSynchronize(r.ev[r.kind == 1 ? 1 : 5]));
+      perf_process(r);
+    }
+    if (kind == 0 && nt > r.cap) {
+      hhfree(r.h_info);
+      r.h_info = nullptr;
+      r.cap = 0;
+      TN_HIP(hipHostMalloc(&r.h_info, sizeof(int) * kInfoLen * nt, hipHostMallocDefault));
+      r.cap = nt;
+    }
+    r.kind = kind;
+    r.nt = nt;
+    prec_next = (prec_next + 1) % kPerfRing;
+    *out = &r;
+    return TN_OK;
   }
 
   tn_status ensure_slots(int need) {
@@ -282,6 +379,7 @@ struct tn_engine {
     TN_HIP(hipMalloc(&perm, sizeof(int) * ldw * ns));
     TN_HIP(hipMalloc(&part, sizeof(double) * 2 * kNPart * ns));
     TN_HIP(hipMalloc(&offs, sizeof(double) * kMaxSweeps * ns));
+    TN_HIP(hipHostMalloc(&h_offs, sizeof(double) * kMaxSweeps * ns, hipHostMallocDefault));
     TN_HIP(hipMalloc(&info, sizeof(int) * kInfoLen * ns));
     TN_HIP(hipMalloc(&gpart, sizeof(double2) * gpart_per * ns));
     slots = ns;
@@ -466,13 +564,17 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
   }
   TN_TRY(e->upd.upload(rs, nt, e->stream));
   const UpdDesc* dd = e->upd.d[rs];
-  if (e->timing) TN_HIP(hipEventRecord(e->tev[0], e->stream));
+  PerfRec* pr = nullptr;
+  if (e->timing) {
+    TN_TRY(e->perf_begin(0, nt, &pr));
+    TN_HIP(hipEventRecord(pr->ev[0], e->stream));
+  }
   const int gx_theta = (int)std::min<size_t>(maxTiles, 1024);
   const int gx_w = (int)std::min<size_t>((maxCC + 255) / 256, 1024);
   launch_theta(dd, nt, gx_theta, e->stream);
   launch_winit(dd, nt, gx_w, e->stream);
   TN_HIP(hipGetLastError());
-  if (e->timing) TN_HIP(hipEventRecord(e->tev[1], e->stream));
+  if (pr) TN_HIP(hipEventRecord(pr->ev[1], e->stream));
   // Block width 8 (measured faster than 16 even for a lone 512-column SVD:
   // the 32x32 inner solve costs more than the halved step count saves);
   // TNMPS_BW=16 selects the wide panels.
@@ -510,43 +612,59 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
   ja.cross_only = tol.cross_only;
   ja.tol_rot = tol.tol_rot;
   ja.conv_tol = tol.conv_tol;
-  std::vector<double> offs(nt);
-  for (int sw = 0; sw < kMaxSweeps; ++sw) {
-    if (e->timing) TN_HIP(hipEventRecord(e->tev[2], e->stream));
+  // Sweeps are enqueued one ahead while the last sweep read back is still far
+  // from convergence (max off-diagonal above kAheadOff: with quadratic
+  // convergence the next sweep cannot reach conv_tol, so the one after it is
+  // needed too); each sweep's maxima come back through pinned memory behind
+  // an event.  Near convergence the host waits for every sweep, so no sweep
+  // is enqueued that is not needed; a sweep enqueued after convergence
+  // returns at once on the device.
+  constexpr double kAheadOff = 1e-3;
+  int launched = 0, checked = 0;
+  double last_worst = 1.0;
+  bool done = false;
+  if (pr) TN_HIP(hipEventRecord(pr->ev[2], e->stream));
+  while (!done) {
+    const int depth = last_worst > kAheadOff ? 2 : 1;
     const auto h0 = std::chrono::steady_clock::now();
-    for (int s = 0; s < steps; ++s) {
-      if (S == 1) {
-        launch_jacobi_step(ja, npairs, bw, sw, s, e->stream);
-      } else {
-        launch_jacobi_gram(ja, npairs, bw, sw, s, e->stream);
-        launch_jacobi_apply(ja, npairs, bw, sw, s, e->stream);
+    while (launched < kMaxSweeps && launched - checked < depth) {
+      for (int s = 0; s < steps; ++s) {
+        if (S == 1) {
+          launch_jacobi_step(ja, npairs, bw, launched, s, e->stream);
+        } else {
+          launch_jacobi_gram(ja, npairs, bw, launched, s, e->stream);
+          launch_jacobi_apply(ja, npairs, bw, launched, s, e->stream);
+        }
       }
+      TN_HIP(hipGetLastError());
+      TN_HIP(hipMemcpyAsync(e->h_offs + (size_t)launched * nt, e->offs + (size_t)launched * nt,
+                            sizeof(double) * nt, hipMemcpyDeviceToHost, e->stream));
+      TN_HIP(hipEventRecord(e->sev[launched], e->stream));
+      ++launched;
     }
-    TN_HIP(hipGetLastError());
     e->perf_host_launch_ms +=
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count();
-    if (e->timing) TN_HIP(hipEventRecord(e->tev[3], e->stream));
-    sweeps_run = sw + 1;
-    TN_HIP(hipMemcpyAsync(offs.data(), e->offs + (size_t)sw * nt, sizeof(double) * nt,
-                          hipMemcpyDeviceToHost, e->stream));
-    TN_HIP(hipStreamSynchronize(e->stream));
-    if (e->timing) {
-      if (sw == 0) e->perf.ms_theta += e->span(0, 1);
-      e->perf.ms_jacobi += e->span(2, 3);
-      e->perf.jacobi_launches += (size_t)steps * (S == 1 ? 1 : 2);
-    }
-    bool done = true;
+    if (checked >= launched) break;  // sweep limit
+    TN_HIP(hipEventSynchronize(e->sev[checked]));
+    const double* o = e->h_offs + (size_t)checked * nt;
+    done = true;
     double worst = 0.0;
     for (int t = 0; t < nt; ++t) {
-      if (!(offs[t] <= tol.conv_tol)) done = false;
-      worst = std::max(worst, offs[t]);
+      if (!(o[t] <= tol.conv_tol)) done = false;
+      worst = std::max(worst, o[t]);
     }
     if (e->trace_sweeps)
       std::fprintf(stderr, "[tnmps] batch of %d (C <= %zu): sweep %d max off-diagonal %.3e\n", nt,
-                   maxC, sw, worst);
-    if (done) break;
+                   maxC, checked, worst);
+    last_worst = worst;
+    ++checked;
   }
-  if (e->timing) TN_HIP(hipEventRecord(e->tev[4], e->stream));
+  sweeps_run = checked;
+  if (pr) {
+    TN_HIP(hipEventRecord(pr->ev[3], e->stream));
+    pr->launches = (size_t)launched * steps * (S == 1 ? 1 : 2);
+    TN_HIP(hipEventRecord(pr->ev[4], e->stream));
+  }
   launch_finish(dd, nt, sweeps_run, e->stream);
   if (any_factored) launch_fmul(dd, nt, (int)maxR, (int)maxC, e->stream);
   launch_split(dd, nt, e->stream);
@@ -556,29 +674,16 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
   for (int t = 0; t < nt; ++t) n_updates += tasks[t].pre ? 0 : 1;
   e->updates += n_updates;
   e->sweeps += (unsigned long long)sweeps_run;
-  if (e->timing) {
-    TN_HIP(hipEventRecord(e->tev[5], e->stream));
-    e->h_info.resize((size_t)nt * kInfoLen);
-    TN_HIP(hipMemcpyAsync(e->h_info.data(), e->info, sizeof(int) * kInfoLen * nt,
-                          hipMemcpyDeviceToHost, e->stream));
-    TN_HIP(hipStreamSynchronize(e->stream));
-    e->perf.ms_post += e->span(4, 5);
-    e->perf.updates += n_updates;
-    e->perf.sweeps += sweeps_run;
-    for (int t = 0; t < nt; ++t) {
-      // flops of the SVD posed (R x C; LAPACK-style count of a complex SVD
-      // with both factors) plus the contraction or the factor product
-      const int* inf = e->h_info.data() + (size_t)t * kInfoLen;
-      const double m = inf[kInfoM], n = inf[kInfoN], K = inf[kInfoChiM];
-      const double M = inf[kInfoR], N = inf[kInfoC], rank = inf[kInfoRank];
-      const double fsvd = 4.0 * (4.0 * M * M * N + 8.0 * M * N * N + 9.0 * N * N * N);
-      e->perf.svd_flops += fsvd;
-      const int mode = inf[kInfoMode];
-      if (mode != 0) e->perf.factored++;
-      const double fpre = mode == 0 ? 8.0 * m * n * K + (tasks[t].has_gate ? 32.0 * m * n : 0.0)
-                                    : 8.0 * (mode == 1 ? m : n) * K * rank;
-      e->perf.alg_flops += fpre + fsvd;
-    }
+  if (pr) {
+    TN_HIP(hipEventRecord(pr->ev[5], e->stream));
+    TN_HIP(hipMemcpyAsync(pr->h_info, e->info, sizeof(int) * kInfoLen * nt, hipMemcpyDeviceToHost,
+                          e->stream));
+    pr->gate.resize(nt);
+    for (int t = 0; t < nt; ++t) pr->gate[t] = tasks[t].has_gate ? 1 : 0;
+    pr->n_updates = n_updates;
+    pr->sweeps = sweeps_run;
+    pr->pending = true;
+    TN_TRY(e->perf_drain(false));
   }
   for (int t = 0; t < nt; ++t) {
     const Task& q = tasks[t];
@@ -653,13 +758,17 @@ tn_status run_measures(tn_engine* e, const std::vector<MTask>& ms) {
     d.out_flag = e->d_flag;
   }
   TN_TRY(e->meas.upload(rs, nm, e->stream));
-  if (e->timing) TN_HIP(hipEventRecord(e->tev[0], e->stream));
+  PerfRec* pr = nullptr;
+  if (e->timing) {
+    TN_TRY(e->perf_begin(1, nm, &pr));
+    TN_HIP(hipEventRecord(pr->ev[0], e->stream));
+  }
   launch_measure(e->meas.d[rs], nm, e->stream);
   TN_HIP(hipGetLastError());
-  if (e->timing) {
-    TN_HIP(hipEventRecord(e->tev[1], e->stream));
-    TN_HIP(hipStreamSynchronize(e->stream));
-    e->perf.ms_measure += e->span(0, 1);
+  if (pr) {
+    TN_HIP(hipEventRecord(pr->ev[1], e->stream));
+    pr->pending = true;
+    TN_TRY(e->perf_drain(false));
   }
   return TN_OK;
 }
@@ -1343,12 +1452,14 @@ tn_status tn_engine_stats(tn_engine* e, size_t* updates, size_t* sweeps) {
 tn_status tn_engine_set_timing(tn_engine* e, int on) {
   if (!e) return TN_INVALID_ARG;
   TN_HIP(hipStreamSynchronize(e->stream));
+  TN_TRY(e->perf_drain(true));
   e->timing = on != 0;
   return TN_OK;
 }
 
 tn_status tn_engine_get_perf(tn_engine* e, tn_perf* out) {
   if (!e || !out) return TN_INVALID_ARG;
+  TN_TRY(e->perf_drain(true));
   *out = e->perf;
   out->ms_host_launch = e->perf_host_launch_ms;
   return TN_OK;
@@ -1356,6 +1467,7 @@ tn_status tn_engine_get_perf(tn_engine* e, tn_perf* out) {
 
 tn_status tn_engine_reset_perf(tn_engine* e) {
   if (!e) return TN_INVALID_ARG;
+  TN_TRY(e->perf_drain(true));
   e->perf = tn_perf{};
   e->perf_host_launch_ms = 0.0;
   return TN_OK;
@@ -1377,12 +1489,14 @@ tn_status tn_stats(tn_mps* h, size_t* updates, size_t* sweeps) {
 tn_status tn_set_timing(tn_mps* h, int on) {
   if (!h) return TN_INVALID_ARG;
   TN_HIP(hipStreamSynchronize(h->eng->stream));
+  TN_TRY(h->eng->perf_drain(true));
   h->eng->timing = on != 0;
   return TN_OK;
 }
 
 tn_status tn_get_perf(tn_mps* h, tn_perf* out) {
   if (!h || !out) return TN_INVALID_ARG;
+  TN_TRY(h->eng->perf_drain(true));
   *out = h->eng->perf;
   out->ms_host_launch = h->eng->perf_host_launch_ms;
   return TN_OK;
@@ -1390,6 +1504,7 @@ tn_status tn_get_perf(tn_mps* h, tn_perf* out) {
 
 tn_status tn_reset_perf(tn_mps* h) {
   if (!h) return TN_INVALID_ARG;
+  TN_TRY(h->eng->perf_drain(true));
   h->eng->perf = tn_perf{};
   h->eng->perf_host_launch_ms = 0.0;
   return TN_OK;
