@@ -58,20 +58,13 @@ double env_double(const char* name, double dflt) {
 // largest such ratio is at most conv_tol.  That sweep still rotates every pair
 // above tol_rot, and Jacobi converges quadratically, so the residual after it
 // is of order conv_tol^2 = 1e-16.
-// Inner sweeps: cyclic Jacobi sweeps over a panel's Gram matrix per visit;
-// convergence is judged on the fresh Gram matrix of the next outer sweep.
-// Jacobi tolerances and inner-solve options, read from the environment when
-// an engine is made.
+// Both are read from the environment when an engine is made.
 struct Tolerances {
   double tol_rot = 1e-14;
   double conv_tol = 1e-8;
-  int inner_sweeps = 1;
-  int cross_only = 0;
   void read_env() {
     tol_rot = env_double("TNMPS_TOL_ROT", tol_rot);
     conv_tol = env_double("TNMPS_CONV_TOL", conv_tol);
-    inner_sweeps = std::max(1, (int)env_double("TNMPS_INNER_SWEEPS", inner_sweeps));
-    cross_only = env_double("TNMPS_CROSS_ONLY", cross_only) != 0 ? 1 : 0;
   }
 };
 
@@ -544,7 +537,6 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
     d.ldx = e->ldx;
     d.ldw = e->ldw;
     d.has_gate = q.has_gate ? 1 : 0;
-    d.inner_sweeps = e->tol.inner_sweeps;
     d.prec = h->prec;
     d.tkind = h->trunc.kind;
     d.tchi = (int)std::min<size_t>(h->trunc.chi, 1u << 30);
@@ -608,8 +600,6 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
   ja.S = S;
   ja.Rh = (int)maxR;
   ja.Ch = (int)maxC;
-  ja.inner_sweeps = tol.inner_sweeps;
-  ja.cross_only = tol.cross_only;
   ja.tol_rot = tol.tol_rot;
   ja.conv_tol = tol.conv_tol;
   // Sweeps are enqueued one ahead while the last sweep read back is still far

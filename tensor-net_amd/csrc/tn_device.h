@@ -68,7 +68,6 @@ struct UpdDesc {
   int ldx, ldw;
   int has_gate;
   int mode;              // requested: 0 full, 1 left factor isometric, 2 right
-  int inner_sweeps;      // inner Jacobi sweeps per panel visit
   int prec;              // 0: FP64 contraction, 1: FP32 contraction
   int tkind;             // 0 None, 1 Const, 2 Cutoff (BondDim, mps.rs:157-166)
   int tchi;
@@ -94,8 +93,6 @@ struct JacobiArgs {
   int nbe;               // round-robin blocks (even) of the widest task
   int S;                 // workgroups per block pair (row split; 1 otherwise)
   int Rh, Ch;            // bounds on every task's R and C
-  int inner_sweeps;      // inner Jacobi sweeps per panel visit
-  int cross_only;        // after a sweep's first step, rotate only cross-block pairs
   double tol_rot;        // rotate a pair when |g_ij| > tol_rot sqrt(g_ii g_jj)
   double conv_tol;       // converged after a sweep whose max ratio <= conv_tol
 };

@@ -230,61 +230,115 @@ __device__ void panel_gram(const double2* __restrict__ X, size_t ldx, int a, int
 }
 
 // 1/sqrt(x) from the hardware approximation (relative error about 5e-8 on
-// gfx950) refined by two Newton steps, each three dependent operations
-// (r + (r/2)(1 - x r^2)): about 1 ulp, cheaper than the IEEE-exact square
-// root and division sequences on the rotation's critical path.
-__device__ __forceinline__ double rsq_nr(double x) {
-  double r = __builtin_amdgcn_rsq(x);
-  double h = 0.5 * r;
-  r = fma(h, fma(-x * r, r, 1.0), r);
-  h = 0.5 * r;
-  r = fma(h, fma(-x * r, r, 1.0), r);
-  return r;
+// gfx950) refined by one third-order Newton step,
+// r (1 + e/2 + 3e^2/8) with e = 1 - x r^2: within 1.3 ulp, and four
+// dependent operations on the rotation's critical path.
+__device__ __forceinline__ double rsq_ref(double x) {
+  const double r = __builtin_amdgcn_rsq(x);
+  const double e = fma(-x * r, r, 1.0);
+  return fma(r * e, fma(e, 0.375, 0.5), r);
 }
 
 // Jacobi rotation zeroing the off-diagonal of [[a, c], [conj(c), b]]:
 // Q = [[cs, sn], [-conj(sn), cs]], the rotation by the smaller angle
-// (|theta| <= pi/4) with t = tan(theta) = 2|c| / (|d| + r), d = b - a,
-// r = sqrt(d^2 + 4|c|^2), cs = 1 / sqrt(1 + t^2), and sn = t cs c / |c| times
-// the sign of d, written sn = (2 cs / (|d| + r)) c.  The angle may carry the
-// hardware approximations' error (about 1e-7 relative: the pair is then left
-// with 1e-7 of its off-diagonal, which the next visit removes); unitarity
-// does not, since cs^2 + |sn|^2 = cs^2 (1 + t^2) with cs computed to 1 ulp
-// from the same t^2 = 4 |c|^2 / (|d| + r)^2.  Identity (and false) when the
-// pair is already orthogonal to tol or a column is negligible.  Branch-free,
-// so that the two rotations a thread needs per round overlap.
-__device__ __forceinline__ bool rot2(double a, double b, double2 c, double tol, double tiny2,
+// (|theta| <= pi/4) with t = tan(theta) = 2|c| / s, s = |d| + r, d = b - a,
+// r = sqrt(d^2 + 4|c|^2) (from the hardware reciprocal square root), written
+// cs = s / sqrt(s^2 + 4|c|^2) and sn = 2 c / sqrt(s^2 + 4|c|^2) times the sign
+// of d.  The angle carries the approximation's error (about 1e-7 relative:
+// the pair keeps 1e-7 of its off-diagonal, which the next visit removes);
+// unitarity does not, since cs^2 + |sn|^2 = (s^2 + 4|c|^2) / u with
+// 1/sqrt(u) refined to about 1 ulp from the same u.  Identity (and false)
+// when the pair is already orthogonal to sqrt(tol2) or a column is
+// negligible; branch-free, the selects come last.
+__device__ __forceinline__ bool rot2(double a, double b, double2 c, double tol2, double tiny2,
                                      double& cs, double2& sn) {
   const double c2 = cabs2(c);
-  const bool go = a > tiny2 && b > tiny2 && c2 > tol * tol * a * b;
+  const bool go = a > tiny2 && b > tiny2 && c2 > tol2 * a * b;
   const double d = b - a;
-  const double r2 = go ? fma(d, d, 4.0 * c2) : 1.0;
-  const double s = fabs(d) + r2 * __builtin_amdgcn_rsq(r2);  // |d| + r
-  const double inv = __builtin_amdgcn_rcp(s);
-  const double t2 = 4.0 * c2 * inv * inv;
-  const double q = rsq_nr(go ? 1.0 + t2 : 1.0);
-  const double f = go ? (d >= 0.0 ? 2.0 : -2.0) * inv * q : 0.0;
-  cs = q;
-  sn = make_double2(f * c.x, f * c.y);
+  const double c4 = 4.0 * c2;
+  const double r2 = fma(d, d, c4);
+  const double s = fma(r2, __builtin_amdgcn_rsq(r2), fabs(d));
+  const double ru = rsq_ref(fma(s, s, c4));
+  const double f = (d >= 0.0 ? 2.0 : -2.0) * ru;
+  cs = go ? s * ru : 1.0;
+  sn = go ? make_double2(f * c.x, f * c.y) : make_double2(0.0, 0.0);
   return go;
+}
+
+// One round of the 16 x 16 panel solve (BW = 8), unrolled: ROUND pairs index
+// x with x ^ ROUND (x with a zero bit inserted at ROUND's highest set bit),
+// 8 disjoint rotations.  Threads 0..127 each compute one column of a 2 x 2
+// block of G' = Q^H G Q (the block's row pair x column pair, with both
+// rotations); threads 128..255 the two entries of one row of W' = W Q in a
+// column pair.  Reads buffer CUR, writes CUR ^ 1.
+template <int ROUND, int CUR>
+__device__ __forceinline__ void solve_round16(PairSmem<8>& sm, double tol2, double tiny2) {
+  constexpr int NXT = CUR ^ 1;
+  constexpr int h = ROUND >= 8 ? 3 : ROUND >= 4 ? 2 : ROUND >= 2 ? 1 : 0;
+  constexpr int lowm = (1 << h) - 1;
+  auto first = [](int p) { return ((p >> h) << (h + 1)) | (p & lowm); };
+  const int tid = threadIdx.x;
+  if (tid < 128) {
+    const int blk = tid >> 1, half = tid & 1;
+    const int x1 = first(blk >> 3), x2 = x1 ^ ROUND;
+    const int y1 = first(blk & 7), y2 = y1 ^ ROUND;
+    const double ax = sm.Gs[CUR][x1][x1].x, bx = sm.Gs[CUR][x2][x2].x;
+    const double2 cx = sm.Gs[CUR][x1][x2];
+    const double ay = sm.Gs[CUR][y1][y1].x, by = sm.Gs[CUR][y2][y2].x;
+    const double2 cy = sm.Gs[CUR][y1][y2];
+    const double2 g11 = sm.Gs[CUR][x1][y1], g12 = sm.Gs[CUR][x1][y2];
+    const double2 g21 = sm.Gs[CUR][x2][y1], g22 = sm.Gs[CUR][x2][y2];
+    double csa, csb;
+    double2 sna, snb;
+    rot2(ax, bx, cx, tol2, tiny2, csa, sna);
+    rot2(ay, by, cy, tol2, tiny2, csb, snb);
+    // column yo of G Qb = g[., y1] qa + g[., y2] qb
+    const int yo = half ? y2 : y1;
+    const double2 qa = half ? snb : make_double2(csb, 0.0);
+    const double2 qb = half ? make_double2(csb, 0.0) : make_double2(-snb.x, snb.y);
+    const double2 t1 = cadd(cmul(g11, qa), cmul(g12, qb));
+    const double2 t2 = cadd(cmul(g21, qa), cmul(g22, qb));
+    sm.Gs[NXT][x1][yo] = cadd(cscale(t1, csa), cmul(make_double2(-sna.x, -sna.y), t2));
+    sm.Gs[NXT][x2][yo] = cadd(cmul(cconj(sna), t1), cscale(t2, csa));
+  } else {
+    const int w = tid - 128;
+    const int i = w >> 3;
+    const int y1 = first(w & 7), y2 = y1 ^ ROUND;
+    const double ay = sm.Gs[CUR][y1][y1].x, by = sm.Gs[CUR][y2][y2].x;
+    const double2 cy = sm.Gs[CUR][y1][y2];
+    const double2 w1 = sm.Ws(CUR, i, y1), w2 = sm.Ws(CUR, i, y2);
+    double csb;
+    double2 snb;
+    rot2(ay, by, cy, tol2, tiny2, csb, snb);
+    sm.Ws(NXT, i, y1) = cadd(cscale(w1, csb), cmul(w2, make_double2(-snb.x, snb.y)));
+    sm.Ws(NXT, i, y2) = cadd(cmul(w1, snb), cscale(w2, csb));
+  }
+}
+
+template <int ROUND, int CUR>
+__device__ __forceinline__ void solve_rounds16(PairSmem<8>& sm, double tol2, double tiny2) {
+  if constexpr (ROUND < 16) {
+    solve_round16<ROUND, CUR>(sm, tol2, tiny2);
+    __syncthreads();
+    solve_rounds16<ROUND + 1, CUR ^ 1>(sm, tol2, tiny2);
+  }
 }
 
 // From the panel Gram matrix in sm.Gs[0]: record the largest relative
 // off-diagonal into *rec (when not null), and unless the panel is already
-// orthogonal diagonalise G with cyclic two-sided Jacobi rotations,
-// accumulating the rotation in sm.Ws[*wsel].  Round r = r_first .. PW-1 pairs
-// index x with x ^ r: PW/2 disjoint rotations per round, and from r_first = 1
-// every pair once per inner sweep; rounds r >= BW pair a column of the first
-// block with one of the second (the cross pairs).  Each work item computes
-// the rotations it needs from the round's Gram buffer, so a round needs one
-// barrier.  Returns false (for the whole block) when there is nothing to
-// apply.  A function of sm.Gs[0] alone: blocks holding the same G compute
-// bitwise the same rotation.
+// orthogonal apply one cyclic two-sided Jacobi sweep to G (G <- Q^H G Q),
+// accumulating the rotation in sm.Ws[wsel].  Round r = 1 .. PW-1 pairs index
+// x with x ^ r: PW/2 disjoint rotations per round, every pair once per sweep.
+// Each work item computes the rotations it needs from the round's Gram
+// buffer, so a round needs one barrier.  Returns false (for the whole block)
+// when there is nothing to apply.  A function of sm.Gs[0] alone: blocks
+// holding the same G compute bitwise the same rotation.
 template <int BW>
-__device__ bool panel_solve(double* rec, int inner_sweeps, double tol_rot, double tiny2,
-                            int r_first, PairSmem<BW>& sm, int& wsel) {
+__device__ bool panel_solve(double* rec, double tol_rot, double tiny2, PairSmem<BW>& sm,
+                            int& wsel) {
   constexpr int PW = 2 * BW;
   const int tid = threadIdx.x;
+  const double tol2 = tol_rot * tol_rot;
   double ratio2 = 0.0;
   for (int e = tid; e < PW * PW; e += 256) {
     const int gi = e / PW, gj = e % PW;
@@ -299,63 +353,66 @@ __device__ bool panel_solve(double* rec, int inner_sweeps, double tol_rot, doubl
     atomicMax((unsigned long long*)rec, (unsigned long long)__double_as_longlong(off));
   if (off <= tol_rot) return false;
 
+  if constexpr (BW == 8) {
+    // 15 rounds, unrolled: every index is a compile-time function of the
+    // thread id, and the last round writes buffer 1
+    solve_rounds16<1, 0>(sm, tol2, tiny2);
+    wsel = 1;
+    return true;
+  }
   // Work items of a round: the (PW/2)^2 2x2 blocks of G on the round's row
   // pair x column pair (G' = Qa^H G Qb touches only its own block), then the
   // PW x PW/2 (row, column pair) items of W' = W Qb.  Pair a of round r is
   // (x, x ^ r) with x = a with a zero bit inserted at r's highest set bit h.
   constexpr int HP = PW / 2, NG = HP * HP, NW = PW * HP;
   int cur = 0;
-  for (int isw = 0; isw < inner_sweeps; ++isw) {
-    int rotated = 0;
-    for (int round = r_first; round < PW; ++round) {
-      const int h = 31 - __builtin_clz(round);
-      const int lowm = (1 << h) - 1;
-      for (int e = tid; e < NG + NW; e += 256) {
-        if (e < NG) {
-          const int pa = e / HP, pb = e % HP;
-          const int x1 = ((pa >> h) << (h + 1)) | (pa & lowm), x2 = x1 ^ round;
-          const int y1 = ((pb >> h) << (h + 1)) | (pb & lowm), y2 = y1 ^ round;
-          const double ax = sm.Gs[cur][x1][x1].x, bx = sm.Gs[cur][x2][x2].x;
-          const double2 cx = sm.Gs[cur][x1][x2];
-          const double ay = sm.Gs[cur][y1][y1].x, by = sm.Gs[cur][y2][y2].x;
-          const double2 cy = sm.Gs[cur][y1][y2];
-          const double2 g11 = sm.Gs[cur][x1][y1], g12 = sm.Gs[cur][x1][y2];
-          const double2 g21 = sm.Gs[cur][x2][y1], g22 = sm.Gs[cur][x2][y2];
-          double csa, csb;
-          double2 sna, snb;
-          rotated |= rot2(ax, bx, cx, tol_rot, tiny2, csa, sna);
-          rot2(ay, by, cy, tol_rot, tiny2, csb, snb);
-          // T = G[block] Qb, then G' = Qa^H T, with Q = [[cs, sn], [-conj(sn), cs]]
-          const double2 msb = make_double2(-snb.x, snb.y);  // -conj(sn_b)
-          const double2 t11 = cadd(cscale(g11, csb), cmul(g12, msb));
-          const double2 t12 = cadd(cmul(g11, snb), cscale(g12, csb));
-          const double2 t21 = cadd(cscale(g21, csb), cmul(g22, msb));
-          const double2 t22 = cadd(cmul(g21, snb), cscale(g22, csb));
-          const double2 nsa = make_double2(-sna.x, -sna.y);  // -sn_a
-          const double2 csa_ = cconj(sna);                    // conj(sn_a)
-          sm.Gs[cur ^ 1][x1][y1] = cadd(cscale(t11, csa), cmul(nsa, t21));
-          sm.Gs[cur ^ 1][x1][y2] = cadd(cscale(t12, csa), cmul(nsa, t22));
-          sm.Gs[cur ^ 1][x2][y1] = cadd(cmul(csa_, t11), cscale(t21, csa));
-          sm.Gs[cur ^ 1][x2][y2] = cadd(cmul(csa_, t12), cscale(t22, csa));
-        } else {
-          const int w = e - NG;
-          const int i = w / HP, pb = w % HP;
-          const int y1 = ((pb >> h) << (h + 1)) | (pb & lowm), y2 = y1 ^ round;
-          const double ay = sm.Gs[cur][y1][y1].x, by = sm.Gs[cur][y2][y2].x;
-          const double2 cy = sm.Gs[cur][y1][y2];
-          const double2 w1 = sm.Ws(cur, i, y1), w2 = sm.Ws(cur, i, y2);
-          double csb;
-          double2 snb;
-          rot2(ay, by, cy, tol_rot, tiny2, csb, snb);
-          const double2 msb = make_double2(-snb.x, snb.y);
-          sm.Ws(cur ^ 1, i, y1) = cadd(cscale(w1, csb), cmul(w2, msb));
-          sm.Ws(cur ^ 1, i, y2) = cadd(cmul(w1, snb), cscale(w2, csb));
-        }
+  for (int round = 1; round < PW; ++round) {
+    const int h = 31 - __builtin_clz(round);
+    const int lowm = (1 << h) - 1;
+    for (int e = tid; e < NG + NW; e += 256) {
+      if (e < NG) {
+        const int pa = e / HP, pb = e % HP;
+        const int x1 = ((pa >> h) << (h + 1)) | (pa & lowm), x2 = x1 ^ round;
+        const int y1 = ((pb >> h) << (h + 1)) | (pb & lowm), y2 = y1 ^ round;
+        const double ax = sm.Gs[cur][x1][x1].x, bx = sm.Gs[cur][x2][x2].x;
+        const double2 cx = sm.Gs[cur][x1][x2];
+        const double ay = sm.Gs[cur][y1][y1].x, by = sm.Gs[cur][y2][y2].x;
+        const double2 cy = sm.Gs[cur][y1][y2];
+        const double2 g11 = sm.Gs[cur][x1][y1], g12 = sm.Gs[cur][x1][y2];
+        const double2 g21 = sm.Gs[cur][x2][y1], g22 = sm.Gs[cur][x2][y2];
+        double csa, csb;
+        double2 sna, snb;
+        rot2(ax, bx, cx, tol2, tiny2, csa, sna);
+        rot2(ay, by, cy, tol2, tiny2, csb, snb);
+        // T = G[block] Qb, then G' = Qa^H T, with Q = [[cs, sn], [-conj(sn), cs]]
+        const double2 msb = make_double2(-snb.x, snb.y);  // -conj(sn_b)
+        const double2 t11 = cadd(cscale(g11, csb), cmul(g12, msb));
+        const double2 t12 = cadd(cmul(g11, snb), cscale(g12, csb));
+        const double2 t21 = cadd(cscale(g21, csb), cmul(g22, msb));
+        const double2 t22 = cadd(cmul(g21, snb), cscale(g22, csb));
+        const double2 nsa = make_double2(-sna.x, -sna.y);  // -sn_a
+        const double2 csa_ = cconj(sna);                    // conj(sn_a)
+        sm.Gs[cur ^ 1][x1][y1] = cadd(cscale(t11, csa), cmul(nsa, t21));
+        sm.Gs[cur ^ 1][x1][y2] = cadd(cscale(t12, csa), cmul(nsa, t22));
+        sm.Gs[cur ^ 1][x2][y1] = cadd(cmul(csa_, t11), cscale(t21, csa));
+        sm.Gs[cur ^ 1][x2][y2] = cadd(cmul(csa_, t12), cscale(t22, csa));
+      } else {
+        const int w = e - NG;
+        const int i = w / HP, pb = w % HP;
+        const int y1 = ((pb >> h) << (h + 1)) | (pb & lowm), y2 = y1 ^ round;
+        const double ay = sm.Gs[cur][y1][y1].x, by = sm.Gs[cur][y2][y2].x;
+        const double2 cy = sm.Gs[cur][y1][y2];
+        const double2 w1 = sm.Ws(cur, i, y1), w2 = sm.Ws(cur, i, y2);
+        double csb;
+        double2 snb;
+        rot2(ay, by, cy, tol2, tiny2, csb, snb);
+        const double2 msb = make_double2(-snb.x, snb.y);
+        sm.Ws(cur ^ 1, i, y1) = cadd(cscale(w1, csb), cmul(w2, msb));
+        sm.Ws(cur ^ 1, i, y2) = cadd(cmul(w1, snb), cscale(w2, csb));
       }
-      cur ^= 1;
-      __syncthreads();
     }
-    if (!__syncthreads_or(rotated)) break;
+    cur ^= 1;
+    __syncthreads();
   }
   wsel = cur;
   return true;
@@ -434,15 +491,6 @@ __device__ __forceinline__ void split_rows(int n, int S, int q, int& a, int& b) 
   b = min(n, a + per);
 }
 
-// First inner round of a step: all pairs of the panel (1), or with
-// a.cross_only only the pairs across the two blocks (BW) after the sweep's
-// first step, which already rotated every block's pairs among themselves:
-// every column pair is still rotated at least once per sweep.
-template <int BW>
-__device__ __forceinline__ int first_round(const JacobiArgs& a, int step) {
-  return (a.cross_only && step > 0) ? BW : 1;
-}
-
 // Convergence state of task t before `sweep`: true when its previous sweep
 // stayed at or below conv_tol.
 __device__ __forceinline__ bool converged(const JacobiArgs& a, int t, int sweep) {
@@ -471,8 +519,7 @@ __global__ __launch_bounds__(256, BW == 8 ? 2 : 1) void k_jacobi_step(JacobiArgs
   load_chunk<BW>(X, (size_t)a.ldx, 0, R, C, lcol, cx);
   load_chunk<BW>(W, (size_t)a.ldw, 0, C, C, lcol, cw);
   int wsel = 0;
-  if (!panel_solve<BW>(a.offs + (size_t)sweep * a.nt + t, a.inner_sweeps, a.tol_rot,
-                       kTinyNorm2, first_round<BW>(a, step), sm, wsel))
+  if (!panel_solve<BW>(a.offs + (size_t)sweep * a.nt + t, a.tol_rot, kTinyNorm2, sm, wsel))
     return;
   panel_apply<BW>(X, (size_t)a.ldx, 0, R, C, lcol, ocol, cx, sm, wsel);
   panel_apply<BW>(W, (size_t)a.ldw, 0, C, C, lcol, ocol, cw, sm, wsel);
@@ -550,8 +597,8 @@ __global__ __launch_bounds__(256, BW == 8 ? 2 : 1) void k_jacobi_apply(JacobiArg
   mask_chunk<BW>(xa, xe, C, lcol, cx);
   mask_chunk<BW>(wa, we, C, lcol, cw);
   int wsel = 0;
-  if (!panel_solve<BW>(q == 0 ? a.offs + (size_t)sweep * a.nt + t : nullptr, a.inner_sweeps,
-                       a.tol_rot, kTinyNorm2, first_round<BW>(a, step), sm, wsel))
+  if (!panel_solve<BW>(q == 0 ? a.offs + (size_t)sweep * a.nt + t : nullptr, a.tol_rot,
+                       kTinyNorm2, sm, wsel))
     return;
   panel_apply<BW>(X, (size_t)a.ldx, xa, xe, C, lcol, ocol, cx, sm, wsel);
   panel_apply<BW>(W, (size_t)a.ldw, wa, we, C, lcol, ocol, cw, sm, wsel);

@@ -105,16 +105,14 @@ def test_fp32_contraction_error(gpu):
 def test_step_and_row_split_jacobi_agree(gpu, monkeypatch):
     """The one-workgroup-per-pair step (TNMPS_SPLIT_MAX=1) and the row-split
     step (the default for lone updates) give the oracle's spectra, and so do
-    the wide 16-column panels (TNMPS_BW=16) and the cross-block-only inner
-    rounds after a sweep's first step (TNMPS_CROSS_ONLY=1)."""
+    the wide 16-column panels (TNMPS_BW=16)."""
     n, depth, p, chi = 12, 10, 0.1, 32
     c = haar_circuit(n, depth, p, 31)
     ref = run_trajectory(n, c.layers, ("const", chi), rs=c.draws)
     runs = []
     for env in ({}, {"TNMPS_SPLIT_MAX": "1"}, {"TNMPS_BW": "16"},
-                {"TNMPS_BW": "16", "TNMPS_SPLIT_MAX": "1"}, {"TNMPS_CROSS_ONLY": "1"},
-                {"TNMPS_CROSS_ONLY": "1", "TNMPS_SPLIT_MAX": "1"}):
-        for k in ("TNMPS_SPLIT_MAX", "TNMPS_BW", "TNMPS_CROSS_ONLY"):
+                {"TNMPS_BW": "16", "TNMPS_SPLIT_MAX": "1"}):
+        for k in ("TNMPS_SPLIT_MAX", "TNMPS_BW"):
             monkeypatch.delenv(k, raising=False)
         for k, v in env.items():
             monkeypatch.setenv(k, v)
