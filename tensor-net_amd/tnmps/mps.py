@@ -92,20 +92,36 @@ def _iptr(a: np.ndarray):
     return a.ctypes.data_as(C.POINTER(C.c_int))
 
 
+def _meas_entries(n, msites, rit):
+    """(site, forced outcome or -1, draw) per listed measurement.  An entry of
+    ``msites`` is a site (``Meas::Rand``) or a (site, outcome) pair
+    (``Meas::Proj``).  Draws from ``rit`` go to the ``Meas::Rand`` entries
+    that ``apply_meas_prob_layer`` processes (mps.rs:2212-2219: in range,
+    stably sorted by site, the first entry per site kept), in that order."""
+    ops = []
+    for m in msites:
+        if isinstance(m, (tuple, list)):
+            ops.append((int(m[0]), int(m[1])))
+        else:
+            ops.append((int(m), None))
+    kept = {}
+    for k, o in sorted((x for x in ops if x[0] < n), key=lambda x: x[0]):
+        kept.setdefault(k, o)
+    rmap = {k: float(next(rit)) for k in sorted(kept) if kept[k] is None}
+    return [(k, -1, rmap.get(k, 0.0)) if o is None else (k, o, 0.0) for k, o in ops]
+
+
 def _pack_layer(n, unis, msites, forced_layer=None, rit=None):
     """Flatten one bilayer: gate sites/ops and measurement sites/outcomes/draws.
     With ``forced_layer`` ([(k, o)]) the measurements are ``Meas::Proj``;
-    otherwise ``Meas::Rand`` consuming ``rit`` in processing order (sorted,
-    deduplicated, in-range sites)."""
+    otherwise as ``_meas_entries``."""
     uks = [int(k) for k, _ in unis]
     ops = [np.ascontiguousarray(np.asarray(m, dtype=np.complex128)).view(np.float64).reshape(-1)
            for _, m in unis]
     if forced_layer is not None:
         ml = [(int(k), int(o), 0.0) for (k, o) in forced_layer]
     else:
-        srt = sorted(set(int(k) for k in msites if k < n))
-        rmap = {k: next(rit) for k in srt}
-        ml = [(int(k), -1, float(rmap.get(int(k), 0.0))) for k in msites]
+        ml = _meas_entries(n, msites, rit)
     return uks, ops, ml
 
 
