@@ -23,6 +23,7 @@
 #define TNMPS_H
 
 #include <stddef.h>
+#include <stdint.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -206,6 +207,45 @@ tn_status tn_batch_bilayer(tn_engine* e, size_t count, tn_mps* const* hs, const 
                            const size_t* mks, const int* forced, const double* r,
                            const size_t* ent_bonds, double* ent_out, size_t* out_counts,
                            int* out_outcomes, double* out_probs);
+
+/* ---- device-side workload generation ------------------------------------
+ * Replaces the host generators of the reference's workloads: gate::haar
+ * (gate.rs:888-909), haar_layer and uniform_meas (circuit.rs:558-584), with
+ * a counter-based generator, Philox4x64-10 (Salmon et al., SC 2011), keyed by
+ * (seed, 0).  A trajectory is fixed by (seed, trajectory id); its streams use
+ * counter (block, c1, layer, trajectory id) with c1 = site for the Haar gate
+ * on (site, site+1), c1 = 2^32 for the measurement mask, c1 = 2^33 for the
+ * Born-sampling draws.  A 64-bit word w gives the uniform (w >> 11) 2^-53.
+ * The random streams differ from the reference's rand/statrs streams; the
+ * distributions are the same. */
+
+/* Philox4x64-10 words of counters (b, c1, c2, c3), b = 0..nblocks-1, key
+ * (seed, 0): 4 words per block into out. Host code, no device needed. */
+tn_status tn_rng_words(uint64_t seed, uint64_t c1, uint64_t c2, uint64_t c3,
+                       size_t nblocks, uint64_t* out);
+
+/* `count` Haar-random 4x4 unitaries generated on the device, unitary u from
+ * counter words ids[3u..3u+2] (blocks 0..7: 32 uniforms; Box-Muller pairs
+ * give Re, Im of Z's entries row-major; Q of Z = QR with R's diagonal real
+ * positive), copied to out (host, 32 doubles each, row-major interleaved). */
+tn_status tn_gen_haar4(tn_engine* e, uint64_t seed, size_t count,
+                       const uint64_t* ids, double* out);
+
+/* One generated bilayer on each of `count` handles of engine e, in lock step
+ * (as tn_batch_bilayer): layer `layer` of trajectory traj_ids[i] applies
+ * Haar gates generated on the device on the brickwork sites k = layer % 2,
+ * layer % 2 + 2, ... < n - 1 (TileQ2, circuit.rs:523-546), then measures
+ * each qubit q whose mask uniform is below p_meas[i] (uniform_meas,
+ * circuit.rs:580-584) with apply_meas_prob_layer (mps.rs:2206-2252), the
+ * j-th measured qubit sampled with the j-th draw.  out_sites receives the
+ * measured qubits, handle after handle (out_counts[i] of them), alongside
+ * out_outcomes / out_probs; ent_bonds / ent_out as in tn_batch_bilayer. No
+ * gate or mask crosses the host-device boundary. */
+tn_status tn_batch_bilayer_gen(tn_engine* e, size_t count, tn_mps* const* hs,
+                               uint64_t seed, const uint64_t* traj_ids, uint64_t layer,
+                               const double* p_meas, const size_t* ent_bonds,
+                               double* ent_out, size_t* out_counts, size_t* out_sites,
+                               int* out_outcomes, double* out_probs);
 
 /* Counters of the device work done by this handle (for throughput
  * accounting): number of two-site updates (contract -> SVD -> split) and

@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "tn_device.h"
+#include "tn_rng.h"
 #include "tn_kernels.h"
 #include "tnmps.h"
 
@@ -199,6 +200,9 @@ struct tn_engine {
   int* d_flag = nullptr;
   double* d_scalar = nullptr;  // scalar results (entropies)
   int nscalar = 0;
+  double2* gen_u = nullptr;    // generated gates, 16 entries each
+  uint64_t* gen_ids = nullptr; // their counter words, 3 each
+  size_t gen_cap = 0;
   bool timing = false;
   double perf_host_launch_ms = 0.0;  // host time spent enqueueing Jacobi steps
   hipEvent_t sev[kMaxSweeps] = {};  // per-sweep completion (convergence readback)
@@ -235,6 +239,7 @@ struct tn_engine {
     upd.release();
     meas.release();
     hfree(d_scalar); d_scalar = nullptr; nscalar = 0;
+    hfree(gen_u); hfree(gen_ids); gen_u = nullptr; gen_ids = nullptr; gen_cap = 0;
     for (PerfRec& r : prec) {
       for (hipEvent_t& ev : r.ev)
         if (ev) (void)hipEventDestroy(ev), ev = nullptr;
@@ -392,6 +397,23 @@ struct tn_engine {
     return TN_OK;
   }
 
+  // Room for `need` generated gates (the stream is drained before a resize,
+  // so no pending launch reads a freed buffer).
+  tn_status ensure_gen(size_t need) {
+    if (need <= gen_cap) return TN_OK;
+    TN_HIP(hipStreamSynchronize(stream));
+    hfree(gen_u);
+    hfree(gen_ids);
+    gen_u = nullptr;
+    gen_ids = nullptr;
+    gen_cap = 0;
+    const size_t cap = std::max<size_t>(need, 1024);
+    TN_HIP(hipMalloc(&gen_u, sizeof(double2) * 16 * cap));
+    TN_HIP(hipMalloc(&gen_ids, sizeof(uint64_t) * 3 * cap));
+    gen_cap = cap;
+    return TN_OK;
+  }
+
   tn_status ensure_scalars(int need) {
     if (need <= nscalar) return TN_OK;
     TN_HIP(hipStreamSynchronize(stream));
@@ -487,6 +509,7 @@ struct Task {
   bool has_gate;
   int pre = 0;  // gauge prepass in this mode (not an update of the reference)
   double gate[32];
+  const double2* gdev = nullptr;  // the gate in device memory (generated), else gate[]
 };
 
 // One site measurement (or probability query with forced == -2).
@@ -542,8 +565,9 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
     d.tchi = (int)std::min<size_t>(h->trunc.chi, 1u << 30);
     d.cap_rank = (int)h->bmax[k];
     d.teps = h->trunc.eps;
-    if (q.has_gate)
+    if (q.has_gate && !q.gdev)
       for (int i = 0; i < 16; ++i) d.gate[i] = make_double2(q.gate[2 * i], q.gate[2 * i + 1]);
+    d.gsrc = q.has_gate ? q.gdev : nullptr;
     const size_t chiL = h->uL(k), chiR = h->uR(k + 1);
     // a factored request has C = K when K < min(m, n), else the full
     // C = min(m, n) <= K: either way C <= min(ub[k], m, n)
@@ -1013,6 +1037,37 @@ tn_status new_handle(tn_engine* e, size_t n, tn_trunc trunc, size_t chi_cap, tn_
   return TN_OK;
 }
 
+// Run the measurement schedules of a batch in lock step, then the
+// entropies, and read back outcomes, probabilities and counts.
+tn_status finish_layers(tn_engine* e, const std::vector<tn_mps*>& hv,
+                        const std::vector<std::vector<Op>>& mops,
+                        const std::vector<size_t>& counts, int slot, const size_t* ent_bonds,
+                        double* ent_out, size_t* out_counts, int* out_outcomes,
+                        double* out_probs) {
+  const size_t count = hv.size();
+  TN_TRY(exec_ops(e, hv, mops));
+  std::vector<EntReq> es;
+  if (ent_out && ent_bonds) {
+    for (size_t i = 0; i < count; ++i) {
+      tn_mps* h = hv[i];
+      if (h->n >= 2 && ent_bonds[i] < h->n - 1)
+        es.push_back({h->lam + h->loff[ent_bonds[i]], h->dims + ent_bonds[i] + 1, 1.0, (int)i});
+    }
+    TN_TRY(run_entropies(e, es));
+  }
+  std::vector<int> flags(slot);
+  TN_TRY(read_results(e, 0, slot, out_outcomes, out_probs, nullptr, flags.data()));
+  if (out_counts)
+    for (size_t i = 0; i < count; ++i) out_counts[i] = counts[i];
+  if (ent_out && ent_bonds) {
+    std::vector<double> tmp(count, 0.0);
+    TN_HIP(hipMemcpy(tmp.data(), e->d_scalar, sizeof(double) * count, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < count; ++i)
+      ent_out[i] = (hv[i]->n >= 2 && ent_bonds[i] < hv[i]->n - 1) ? tmp[i] : NAN;
+  }
+  return check_flags(flags);
+}
+
 tn_status check_trunc(tn_trunc t) {
   return (t.kind < 0 || t.kind > 2) ? TN_INVALID_ARG : TN_OK;
 }
@@ -1409,27 +1464,103 @@ tn_status tn_batch_bilayer(tn_engine* e, size_t count, tn_mps* const* hs, const 
     counts[i] = done.size();
     mo += nm[i];
   }
-  TN_TRY(exec_ops(e, hv, mops));
-  std::vector<EntReq> es;
-  if (ent_out && ent_bonds) {
-    for (size_t i = 0; i < count; ++i) {
-      tn_mps* h = hv[i];
-      if (h->n >= 2 && ent_bonds[i] < h->n - 1)
-        es.push_back({h->lam + h->loff[ent_bonds[i]], h->dims + ent_bonds[i] + 1, 1.0, (int)i});
+  return finish_layers(e, hv, mops, counts, slot, ent_bonds, ent_out, out_counts, out_outcomes,
+                       out_probs);
+}
+
+tn_status tn_rng_words(uint64_t seed, uint64_t c1, uint64_t c2, uint64_t c3, size_t nblocks,
+                       uint64_t* out) {
+  if (nblocks && !out) return TN_INVALID_ARG;
+  for (size_t b = 0; b < nblocks; ++b) {
+    const Philox4x64 w = philox4x64_10(b, c1, c2, c3, seed, 0);
+    for (int j = 0; j < 4; ++j) out[4 * b + j] = w.v[j];
+  }
+  return TN_OK;
+}
+
+tn_status tn_gen_haar4(tn_engine* e, uint64_t seed, size_t count, const uint64_t* ids,
+                       double* out) {
+  if (!e || (count && (!ids || !out))) return TN_INVALID_ARG;
+  if (count == 0) return TN_OK;
+  if (count > (size_t)INT32_MAX) return TN_INVALID_ARG;
+  TN_TRY(e->ensure_gen(count));
+  TN_HIP(hipMemcpyAsync(e->gen_ids, ids, sizeof(uint64_t) * 3 * count, hipMemcpyHostToDevice,
+                        e->stream));
+  launch_haar4(seed, e->gen_ids, (int)count, e->gen_u, e->stream);
+  TN_HIP(hipGetLastError());
+  TN_HIP(hipMemcpyAsync(out, e->gen_u, sizeof(double2) * 16 * count, hipMemcpyDeviceToHost,
+                        e->stream));
+  TN_HIP(hipStreamSynchronize(e->stream));
+  return TN_OK;
+}
+
+tn_status tn_batch_bilayer_gen(tn_engine* e, size_t count, tn_mps* const* hs, uint64_t seed,
+                               const uint64_t* traj_ids, uint64_t layer, const double* p_meas,
+                               const size_t* ent_bonds, double* ent_out, size_t* out_counts,
+                               size_t* out_sites, int* out_outcomes, double* out_probs) {
+  if (!e || (count && (!hs || !traj_ids || !p_meas))) return TN_INVALID_ARG;
+  std::vector<tn_mps*> hv(hs, hs + count);
+  for (tn_mps* h : hv)
+    if (!h || h->eng != e) return TN_INVALID_ARG;
+  // gates on the brickwork sites of the layer (TileQ2, circuit.rs:523-546):
+  // generated on the device, applied from there
+  std::vector<uint64_t> ids;
+  std::vector<std::pair<size_t, size_t>> at;  // (handle, site)
+  for (size_t i = 0; i < count; ++i)
+    for (size_t k = layer & 1; k + 1 < hv[i]->n; k += 2) {
+      ids.insert(ids.end(), {(uint64_t)k, layer, traj_ids[i]});
+      at.push_back({i, k});
     }
-    TN_TRY(run_entropies(e, es));
+  if (at.size() > (size_t)INT32_MAX) return TN_INVALID_ARG;
+  if (!at.empty()) {
+    TN_TRY(e->ensure_gen(at.size()));
+    TN_HIP(hipMemcpyAsync(e->gen_ids, ids.data(), sizeof(uint64_t) * ids.size(),
+                          hipMemcpyHostToDevice, e->stream));
+    launch_haar4(seed, e->gen_ids, (int)at.size(), e->gen_u, e->stream);
+    TN_HIP(hipGetLastError());
+    std::vector<Task> tasks(at.size());
+    for (size_t j = 0; j < at.size(); ++j) {
+      tasks[j].h = hv[at[j].first];
+      tasks[j].k = at[j].second;
+      tasks[j].has_gate = true;
+      tasks[j].gdev = e->gen_u + 16 * j;
+    }
+    TN_TRY(run_updates(e, tasks));
   }
-  std::vector<int> flags(slot);
-  TN_TRY(read_results(e, 0, slot, out_outcomes, out_probs, nullptr, flags.data()));
-  if (out_counts)
-    for (size_t i = 0; i < count; ++i) out_counts[i] = counts[i];
-  if (ent_out && ent_bonds) {
-    std::vector<double> tmp(count, 0.0);
-    TN_HIP(hipMemcpy(tmp.data(), e->d_scalar, sizeof(double) * count, hipMemcpyDeviceToHost));
-    for (size_t i = 0; i < count; ++i)
-      ent_out[i] = (hv[i]->n >= 2 && ent_bonds[i] < hv[i]->n - 1) ? tmp[i] : NAN;
+  // measurements: qubit q measured when its mask uniform is below p
+  // (uniform_meas, circuit.rs:580-584); one draw per measured qubit in
+  // increasing site order, the order apply_meas_prob_layer consumes draws
+  std::vector<std::vector<Op>> mops(count);
+  std::vector<size_t> counts(count);
+  int slot = 0;
+  size_t total = 0;
+  std::vector<std::vector<size_t>> sites(count);
+  std::vector<std::vector<double>> draws(count);
+  for (size_t i = 0; i < count; ++i) {
+    const size_t n = hv[i]->n;
+    for (size_t q = 0; q < n; ++q) {
+      const Philox4x64 w = philox4x64_10(q / 4, kMaskStream, layer, traj_ids[i], seed, 0);
+      if (u01(w.v[q % 4]) < p_meas[i]) sites[i].push_back(q);
+    }
+    for (size_t j = 0; j < sites[i].size(); ++j) {
+      const Philox4x64 w = philox4x64_10(j / 4, kDrawStream, layer, traj_ids[i], seed, 0);
+      draws[i].push_back(u01(w.v[j % 4]));
+    }
+    total += sites[i].size();
   }
-  return check_flags(flags);
+  TN_TRY(e->ensure_results((int)std::max<size_t>(total, 1)));
+  TN_TRY(e->ensure_scalars((int)std::max<size_t>(count, 1)));
+  size_t so = 0;
+  for (size_t i = 0; i < count; ++i) {
+    std::vector<size_t> done;
+    meas_schedule(hv[i], sites[i].size(), sites[i].data(), nullptr, draws[i].data(), &slot,
+                  &mops[i], &done);
+    counts[i] = done.size();
+    if (out_sites)
+      for (size_t q : done) out_sites[so++] = q;
+  }
+  return finish_layers(e, hv, mops, counts, slot, ent_bonds, ent_out, out_counts, out_outcomes,
+                       out_probs);
 }
 
 tn_status tn_engine_stats(tn_engine* e, size_t* updates, size_t* sweeps) {

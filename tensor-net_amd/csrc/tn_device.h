@@ -74,6 +74,7 @@ struct UpdDesc {
   int cap_rank;          // allocation bound on the new bond dimension
   double teps;
   double2 gate[16];      // 4x4 row-major, qubit k the most significant bit
+  const double2* gsrc;   // the gate in device memory instead (generated), or null
 };
 
 // The Jacobi step kernels of one launch (all tasks of an update batch).  Task

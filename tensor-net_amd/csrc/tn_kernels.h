@@ -25,5 +25,8 @@ void launch_renorm(const UpdDesc* descs, int nt, hipStream_t s);
 void launch_measure(const MeasDesc* descs, int nm, hipStream_t s);
 void launch_entropy(const double* lam, const int* dimp, double a, double* out, hipStream_t s);
 void launch_u1(double2* g, const int* dims, const double* op8, int gx, hipStream_t s);
+// Haar 4x4 unitaries from the counter-based generator (tn_rng.h): unitary u
+// from counter words ids[3u .. 3u+2], written row-major to out + 16 u.
+void launch_haar4(uint64_t seed, const uint64_t* ids, int count, double2* out, hipStream_t s);
 
 }  // namespace tnmps

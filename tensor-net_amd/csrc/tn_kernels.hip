@@ -21,9 +21,12 @@
 //                  (mps.rs:1027-1134, 1427-1465, 2229-2242)
 //   k_entropy      von Neumann / Renyi entropy of one bond (mps.rs:1251-1282)
 //   k_u1           single-site unitary (mps.rs:1298-1314)
+//   k_haar4        Haar 4x4 unitaries from the counter-based generator
+//                  (tn_rng.h; the distribution of gate.rs:888-909)
 #include "tn_device.h"
 #include "tn_devmath.h"
 #include "tn_kernels.h"
+#include "tn_rng.h"
 
 #include <cfloat>
 
@@ -77,12 +80,13 @@ __device__ __forceinline__ void theta_point(const UpdDesc& d, int v, int w, int 
     for (int i = 0; i < 4; ++i) { acc[i].x *= lr; acc[i].y *= lr; }
   }
   if (d.has_gate) {
+    const double2* g = d.gsrc ? d.gsrc : d.gate;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       C2 o;
       o.x = o.y = (R)0;
 #pragma unroll
-      for (int j = 0; j < 4; ++j) o = tmad<C2, R>(o, tc<C2, R>(d.gate[i * 4 + j]), acc[j]);
+      for (int j = 0; j < 4; ++j) o = tmad<C2, R>(o, tc<C2, R>(g[i * 4 + j]), acc[j]);
       out[i] = make_double2((double)o.x, (double)o.y);
     }
   } else {
@@ -584,6 +588,63 @@ void launch_u1(double2* g, const int* dims, const double* op8, int gx, hipStream
   Gate2 op;
   for (int i = 0; i < 4; ++i) op.m[i] = make_double2(op8[2 * i], op8[2 * i + 1]);
   hipLaunchKernelGGL(k_u1, dim3(gx), dim3(256), 0, s, g, dims, op);
+}
+
+// ---------------------------------------------------------------- haar
+// One thread per unitary u: counter (block, ids[3u], ids[3u+1], ids[3u+2]),
+// key (seed, 0), blocks 0..7 give 32 uniforms, Box-Muller pairs give the
+// normals Re z_e, Im z_e of entry e = 4 i + j of Z (row-major), and
+// Gram-Schmidt on Z's columns (each projection applied twice) gives the Q of
+// Z = Q R with R's diagonal real and positive.
+__global__ __launch_bounds__(64) void k_haar4(uint64_t seed, const uint64_t* __restrict__ ids,
+                                              int count, double2* __restrict__ out) {
+  const int u = blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= count) return;
+  const uint64_t c1 = ids[3 * (size_t)u], c2 = ids[3 * (size_t)u + 1],
+                 c3 = ids[3 * (size_t)u + 2];
+  double2 q[4][4];  // q[column][row]
+#pragma unroll
+  for (int b = 0; b < 8; ++b) {
+    const Philox4x64 w = philox4x64_10((uint64_t)b, c1, c2, c3, seed, 0);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const double rad = sqrt(-2.0 * log(1.0 - u01(w.v[2 * h])));
+      double sn, cs;
+      sincospi(2.0 * u01(w.v[2 * h + 1]), &sn, &cs);
+      const int e = 2 * b + h;  // entry (e / 4, e % 4)
+      q[e % 4][e / 4] = make_double2(rad * cs, rad * sn);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+#pragma unroll
+    for (int pass = 0; pass < 2; ++pass) {
+#pragma unroll
+      for (int i = 0; i < j; ++i) {
+        double2 c = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) c = cadd(c, cmulc(q[i][r], q[j][r]));
+#pragma unroll
+        for (int r = 0; r < 4; ++r) q[j][r] = cadd(q[j][r], cmul(make_double2(-c.x, -c.y), q[i][r]));
+      }
+    }
+    double nrm2 = 0.0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) nrm2 += cabs2(q[j][r]);
+    const double inv = 1.0 / sqrt(nrm2);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) q[j][r] = cscale(q[j][r], inv);
+  }
+  double2* o = out + 16 * (size_t)u;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) o[4 * i + j] = q[j][i];
+}
+
+void launch_haar4(uint64_t seed, const uint64_t* ids, int count, double2* out, hipStream_t s) {
+  if (count <= 0) return;
+  hipLaunchKernelGGL(k_haar4, dim3((count + 63) / 64), dim3(64), 0, s, seed, ids, count, out);
 }
 
 }  // namespace tnmps

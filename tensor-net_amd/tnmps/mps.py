@@ -433,6 +433,20 @@ class MPS:
         return {"record": record, "probs": probs, "entropy": list(ent)}
 
 
+def rng_words(seed: int, c1: int, c2: int, c3: int, nblocks: int) -> np.ndarray:
+    """Philox4x64-10 words of counters (b, c1, c2, c3), b < nblocks, key
+    (seed, 0), from the library's host generator (``tn_rng_words``)."""
+    lib = L.load()
+    out = np.zeros(4 * int(nblocks), dtype=np.uint64)
+    _check(lib.tn_rng_words(int(seed), int(c1), int(c2), int(c3), int(nblocks),
+                            out.ctypes.data_as(C.POINTER(C.c_uint64))), "rng_words")
+    return out
+
+
+MASK_STREAM = 1 << 32
+DRAW_STREAM = 2 << 32
+
+
 class Engine:
     """A HIP stream plus the update workspace shared by many trajectories.
 
@@ -481,6 +495,47 @@ class Engine:
         _check(self._lib.tn_mps_new_qubits_in(self._h, int(n), to_trunc(trunc), int(chi_cap),
                                               C.byref(h)), "new_qubits_in")
         return MPS(h.value, self._lib, engine=self)
+
+    def gen_haar4(self, seed: int, ids) -> np.ndarray:
+        """Haar 4x4 unitaries generated on the device (``tn_gen_haar4``):
+        unitary u from counter words ``ids[u] = (c1, c2, c3)``."""
+        ids = np.ascontiguousarray(np.asarray(ids, dtype=np.uint64).reshape(-1, 3))
+        out = np.zeros((ids.shape[0], 4, 4), dtype=np.complex128)
+        _check(self._lib.tn_gen_haar4(self._h, int(seed), ids.shape[0],
+                                      ids.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                      _dptr(out.view(np.float64).reshape(-1))), "gen_haar4")
+        return out
+
+    def batch_bilayer_gen(self, states: Sequence[MPS], seed: int, traj_ids, layer: int,
+                          p_meas, ent_bonds=None):
+        """One generated bilayer on every state (``tn_batch_bilayer_gen``):
+        layer ``layer`` of trajectory ``traj_ids[i]`` with Haar gates made on
+        the device and measurement probability ``p_meas[i]``.  Returns per
+        state (record [(k, outcome)], probs [(k, p)], entropy)."""
+        cnt = len(states)
+        hs = (C.c_void_p * max(cnt, 1))(*[st._h.value for st in states])
+        tid = np.ascontiguousarray(np.asarray(traj_ids, dtype=np.uint64).reshape(-1))
+        pm = np.ascontiguousarray(np.broadcast_to(np.asarray(p_meas, dtype=np.float64), (cnt,)))
+        eb = _szarr(ent_bonds if ent_bonds is not None else [st.n // 2 - 1 for st in states])
+        tot = max(sum(st.n for st in states), 1)
+        ent = np.zeros(max(cnt, 1))
+        counts = np.zeros(max(cnt, 1), dtype=np.uintp)
+        sites = np.zeros(tot, dtype=np.uintp)
+        oo = np.zeros(tot, dtype=np.int32)
+        op = np.zeros(tot)
+        _check(self._lib.tn_batch_bilayer_gen(self._h, cnt, C.cast(hs, C.POINTER(C.c_void_p)),
+                                              int(seed), tid.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                              int(layer), _dptr(pm), _szptr(eb), _dptr(ent),
+                                              _szptr(counts), _szptr(sites), _iptr(oo), _dptr(op)),
+               "batch_bilayer_gen")
+        out, pos = [], 0
+        for i in range(cnt):
+            c = int(counts[i])
+            rec = [(int(sites[pos + j]), int(oo[pos + j])) for j in range(c)]
+            prs = [(int(sites[pos + j]), float(op[pos + j])) for j in range(c)]
+            pos += c
+            out.append((rec, prs, float(ent[i])))
+        return out
 
     def batch_bilayer(self, states: Sequence[MPS], layers, forced=None, draws=None,
                       ent_bonds=None):

@@ -42,3 +42,21 @@ def assert_spectrum_close(got, want, rtol=1e-10, atol=1e-12, msg=""):
 def state_overlap_err(a: np.ndarray, b: np.ndarray) -> float:
     """1 - |<a|b>| / (|a||b|): zero iff equal up to a global phase."""
     return float(1.0 - abs(np.vdot(a, b)) / (np.linalg.norm(a) * np.linalg.norm(b)))
+
+
+def assert_spectra_equiv(got_spectra, want_spectra, rtol=1e-10, atol=1e-12, tail=1e-13):
+    """Schmidt spectra per bond, equal up to round-off, where one side may
+    keep extra values at the level of round-off: a singular value of an
+    exactly rank-deficient theta comes out as noise near f64::EPSILON, and
+    whether it passes the rank rule's EPSILON threshold (mps.rs:255-262)
+    depends on the SVD's rounding, in the reference as here."""
+    assert len(got_spectra) == len(want_spectra)
+    for b, (g, w) in enumerate(zip(got_spectra, want_spectra)):
+        g, w = np.asarray(g), np.asarray(w)
+        s0 = float(max(w[0] if w.size else 0.0, g[0] if g.size else 0.0, 1e-300))
+        m = min(g.size, w.size)
+        assert np.all(g[m:] <= tail * s0) and np.all(w[m:] <= tail * s0), \
+            f"bond {b}: lengths {g.size} vs {w.size} beyond round-off"
+        if m:
+            err = float(np.max(np.abs(g[:m] - w[:m])))
+            assert err <= atol + rtol * s0, f"bond {b}: spectrum error {err:.3e}"
