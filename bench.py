@@ -227,9 +227,14 @@ def run_serving(args) -> dict:
         n = int(rng.choice(ns))
         p = float(rng.uniform(0.0, 0.4))
         chi = int(rng.choice(chis))
-        c = haar_circuit_fast(n, n, p, int(rng.integers(1 << 30)))
-        reqs.append(Request(rid=len(reqs), n=n, p=p, chi=chi, layers=c.layers, draws=c.draws,
-                            arrival=t, prec="f32"))
+        if args.serve_host_circuits:
+            c = haar_circuit_fast(n, n, p, int(rng.integers(1 << 30)))
+            reqs.append(Request(rid=len(reqs), n=n, p=p, chi=chi, layers=c.layers,
+                                draws=c.draws, arrival=t, prec="f32"))
+        else:
+            # gates, measurement sites and draws generated on the device
+            reqs.append(Request(rid=len(reqs), n=n, p=p, chi=chi, arrival=t, prec="f32",
+                                gen_seed=args.seed, traj_id=len(reqs), gen_depth=n))
     log(f"{len(reqs)} requests over {dur}s (rate {rate}/s)")
     srv = Server(chi_max=max(chis), hbm_budget=args.hbm_budget * 1e9, log=log)
     u0, _ = srv.engine.stats()
@@ -246,6 +251,8 @@ def run_serving(args) -> dict:
         "dtype": "f32 contraction, f64 SVD", "data": "synthetic",
         "config": {"workload": f"Poisson arrivals {rate}/s for {dur}s, N in {ns}, p~U[0,0.4], "
                                f"chi_max in {chis}, depth=N, Haar brickwork + Z measurements",
+                   "circuits": "host-generated" if args.serve_host_circuits
+                               else "generated on the device (Philox4x64-10 streams)",
                    "requests": len(reqs), "completed": len(lat),
                    "p50_latency_s": round(percentile(lat, 50), 3),
                    "p99_latency_s": round(percentile(lat, 99), 3),
@@ -277,6 +284,8 @@ def main() -> int:
     ap.add_argument("--serve-n", default="16,32,64,128")
     ap.add_argument("--serve-chi", default="64,128,256")
     ap.add_argument("--hbm-budget", type=float, default=64.0, help="serving: GB of MPS state")
+    ap.add_argument("--serve-host-circuits", action="store_true",
+                    help="serving: requests carry host-generated circuits")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (calibrates "
                          "the FETCH_SIZE / WRITE_SIZE counters) and exit")

@@ -232,9 +232,9 @@ tn_status tn_gen_haar4(tn_engine* e, uint64_t seed, size_t count,
                        const uint64_t* ids, double* out);
 
 /* One generated bilayer on each of `count` handles of engine e, in lock step
- * (as tn_batch_bilayer): layer `layer` of trajectory traj_ids[i] applies
- * Haar gates generated on the device on the brickwork sites k = layer % 2,
- * layer % 2 + 2, ... < n - 1 (TileQ2, circuit.rs:523-546), then measures
+ * (as tn_batch_bilayer): layer t = layers[i] of trajectory traj_ids[i]
+ * applies Haar gates generated on the device on the brickwork sites
+ * k = t % 2, t % 2 + 2, ... < n - 1 (TileQ2, circuit.rs:523-546), then measures
  * each qubit q whose mask uniform is below p_meas[i] (uniform_meas,
  * circuit.rs:580-584) with apply_meas_prob_layer (mps.rs:2206-2252), the
  * j-th measured qubit sampled with the j-th draw.  out_sites receives the
@@ -242,8 +242,9 @@ tn_status tn_gen_haar4(tn_engine* e, uint64_t seed, size_t count,
  * out_outcomes / out_probs; ent_bonds / ent_out as in tn_batch_bilayer. No
  * gate or mask crosses the host-device boundary. */
 tn_status tn_batch_bilayer_gen(tn_engine* e, size_t count, tn_mps* const* hs,
-                               uint64_t seed, const uint64_t* traj_ids, uint64_t layer,
-                               const double* p_meas, const size_t* ent_bonds,
+                               uint64_t seed, const uint64_t* traj_ids,
+                               const uint64_t* layers, const double* p_meas,
+                               const size_t* ent_bonds,
                                double* ent_out, size_t* out_counts, size_t* out_sites,
                                int* out_outcomes, double* out_probs);
 

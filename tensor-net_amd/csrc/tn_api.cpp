@@ -1495,10 +1495,11 @@ tn_status tn_gen_haar4(tn_engine* e, uint64_t seed, size_t count, const uint64_t
 }
 
 tn_status tn_batch_bilayer_gen(tn_engine* e, size_t count, tn_mps* const* hs, uint64_t seed,
-                               const uint64_t* traj_ids, uint64_t layer, const double* p_meas,
+                               const uint64_t* traj_ids, const uint64_t* layers,
+                               const double* p_meas,
                                const size_t* ent_bonds, double* ent_out, size_t* out_counts,
                                size_t* out_sites, int* out_outcomes, double* out_probs) {
-  if (!e || (count && (!hs || !traj_ids || !p_meas))) return TN_INVALID_ARG;
+  if (!e || (count && (!hs || !traj_ids || !layers || !p_meas))) return TN_INVALID_ARG;
   std::vector<tn_mps*> hv(hs, hs + count);
   for (tn_mps* h : hv)
     if (!h || h->eng != e) return TN_INVALID_ARG;
@@ -1507,8 +1508,8 @@ tn_status tn_batch_bilayer_gen(tn_engine* e, size_t count, tn_mps* const* hs, ui
   std::vector<uint64_t> ids;
   std::vector<std::pair<size_t, size_t>> at;  // (handle, site)
   for (size_t i = 0; i < count; ++i)
-    for (size_t k = layer & 1; k + 1 < hv[i]->n; k += 2) {
-      ids.insert(ids.end(), {(uint64_t)k, layer, traj_ids[i]});
+    for (size_t k = layers[i] & 1; k + 1 < hv[i]->n; k += 2) {
+      ids.insert(ids.end(), {(uint64_t)k, layers[i], traj_ids[i]});
       at.push_back({i, k});
     }
   if (at.size() > (size_t)INT32_MAX) return TN_INVALID_ARG;
@@ -1539,11 +1540,11 @@ tn_status tn_batch_bilayer_gen(tn_engine* e, size_t count, tn_mps* const* hs, ui
   for (size_t i = 0; i < count; ++i) {
     const size_t n = hv[i]->n;
     for (size_t q = 0; q < n; ++q) {
-      const Philox4x64 w = philox4x64_10(q / 4, kMaskStream, layer, traj_ids[i], seed, 0);
+      const Philox4x64 w = philox4x64_10(q / 4, kMaskStream, layers[i], traj_ids[i], seed, 0);
       if (u01(w.v[q % 4]) < p_meas[i]) sites[i].push_back(q);
     }
     for (size_t j = 0; j < sites[i].size(); ++j) {
-      const Philox4x64 w = philox4x64_10(j / 4, kDrawStream, layer, traj_ids[i], seed, 0);
+      const Philox4x64 w = philox4x64_10(j / 4, kDrawStream, layers[i], traj_ids[i], seed, 0);
       draws[i].push_back(u01(w.v[j % 4]));
     }
     total += sites[i].size();

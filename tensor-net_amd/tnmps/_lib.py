@@ -100,7 +100,8 @@ SIGNATURES = {
                                C.POINTER(C.c_uint64)]),
     "tn_gen_haar4": (C.c_int, [_h, C.c_uint64, _sz, C.POINTER(C.c_uint64), _dp]),
     "tn_batch_bilayer_gen": (C.c_int, [_h, _sz, _hp, C.c_uint64, C.POINTER(C.c_uint64),
-                                       C.c_uint64, _dp, _szp, _dp, _szp, _szp, _ip, _dp]),
+                                       C.POINTER(C.c_uint64), _dp, _szp, _dp, _szp, _szp, _ip,
+                                       _dp]),
 }
 
 _LIB = None

@@ -506,15 +506,17 @@ class Engine:
                                       _dptr(out.view(np.float64).reshape(-1))), "gen_haar4")
         return out
 
-    def batch_bilayer_gen(self, states: Sequence[MPS], seed: int, traj_ids, layer: int,
+    def batch_bilayer_gen(self, states: Sequence[MPS], seed: int, traj_ids, layers,
                           p_meas, ent_bonds=None):
         """One generated bilayer on every state (``tn_batch_bilayer_gen``):
-        layer ``layer`` of trajectory ``traj_ids[i]`` with Haar gates made on
-        the device and measurement probability ``p_meas[i]``.  Returns per
-        state (record [(k, outcome)], probs [(k, p)], entropy)."""
+        layer ``layers[i]`` (or one layer for all) of trajectory
+        ``traj_ids[i]``, with Haar gates made on the device and measurement
+        probability ``p_meas[i]``.  Returns per state (record [(k, outcome)],
+        probs [(k, p)], entropy)."""
         cnt = len(states)
         hs = (C.c_void_p * max(cnt, 1))(*[st._h.value for st in states])
         tid = np.ascontiguousarray(np.asarray(traj_ids, dtype=np.uint64).reshape(-1))
+        lay = np.ascontiguousarray(np.broadcast_to(np.asarray(layers, dtype=np.uint64), (cnt,)))
         pm = np.ascontiguousarray(np.broadcast_to(np.asarray(p_meas, dtype=np.float64), (cnt,)))
         eb = _szarr(ent_bonds if ent_bonds is not None else [st.n // 2 - 1 for st in states])
         tot = max(sum(st.n for st in states), 1)
@@ -525,7 +527,8 @@ class Engine:
         op = np.zeros(tot)
         _check(self._lib.tn_batch_bilayer_gen(self._h, cnt, C.cast(hs, C.POINTER(C.c_void_p)),
                                               int(seed), tid.ctypes.data_as(C.POINTER(C.c_uint64)),
-                                              int(layer), _dptr(pm), _szptr(eb), _dptr(ent),
+                                              lay.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                              _dptr(pm), _szptr(eb), _dptr(ent),
                                               _szptr(counts), _szptr(sites), _iptr(oo), _dptr(op)),
                "batch_bilayer_gen")
         out, pos = [], 0

@@ -7,10 +7,17 @@ Each request is one trajectory: an ``MPS`` of ``n`` qubits with
 reference's trajectory drivers do one MPS per thread
 (workspace/src/haar_coev_probs.rs:33-83, 162-203).
 
+A request either carries its circuit (``layers`` and ``draws``) or is
+generated: ``gen_seed`` and ``traj_id`` fix its Haar gates, measurement
+sites and draws, which the library makes on the device
+(``tn_batch_bilayer_gen``), so no gate crosses the host-device boundary.
+
 The server keeps every admitted trajectory resident on the device and, per
-tick, advances all of them by one layer in one lock-step ``batch_bilayer``
-call, so their gates and sweep steps share launches.  Requests that have
-arrived are admitted in arrival order whenever the HBM held by resident
+tick, advances all of them by one layer in lock step (one
+``batch_bilayer`` call for carried circuits, one ``batch_bilayer_gen``
+call per generator seed), so their gates and sweep steps share launches.
+Requests that have arrived are admitted in arrival order whenever the HBM
+held by resident
 trajectories stays within the budget (slots are counted in bytes of Gamma
 and lambda); finished trajectories leave at once and free their memory.
 """
@@ -39,15 +46,19 @@ def mps_bytes(n: int, chi: int) -> int:
 
 @dataclass
 class Request:
-    """One trajectory request: its circuit, draws and arrival time (s)."""
+    """One trajectory request: its circuit and draws, or its generator seed,
+    trajectory id and depth; and its arrival time (s)."""
     rid: int
     n: int
     p: float
     chi: int
-    layers: list                 # [(unis, meas_sites)] of length depth
-    draws: Sequence[float]
+    layers: Optional[list] = None   # [(unis, meas_sites)] of length depth
+    draws: Sequence[float] = ()
     arrival: float = 0.0
     prec: str = "f64"
+    gen_seed: Optional[int] = None  # generated on the device when set
+    traj_id: int = 0
+    gen_depth: int = 0
     # filled by the server
     admitted: Optional[float] = None
     done: Optional[float] = None
@@ -57,7 +68,7 @@ class Request:
 
     @property
     def depth(self) -> int:
-        return len(self.layers)
+        return self.gen_depth if self.gen_seed is not None else len(self.layers)
 
     @property
     def latency(self) -> Optional[float]:
@@ -117,11 +128,25 @@ class Server:
             if not active:
                 continue
             # one lock-step layer for every resident trajectory
-            res = self.engine.batch_bilayer(
-                [a[1] for a in active],
-                [a[0].layers[a[3]] for a in active],
-                draws=[a[2] for a in active],
-                ent_bonds=[a[0].n // 2 - 1 for a in active])
+            res = [None] * len(active)
+            carried = [i for i, a in enumerate(active) if a[0].gen_seed is None]
+            if carried:
+                got = self.engine.batch_bilayer(
+                    [active[i][1] for i in carried],
+                    [active[i][0].layers[active[i][3]] for i in carried],
+                    draws=[active[i][2] for i in carried],
+                    ent_bonds=[active[i][0].n // 2 - 1 for i in carried])
+                for i, g in zip(carried, got):
+                    res[i] = g
+            for seed in sorted({a[0].gen_seed for a in active if a[0].gen_seed is not None}):
+                idx = [i for i, a in enumerate(active) if a[0].gen_seed == seed]
+                got = self.engine.batch_bilayer_gen(
+                    [active[i][1] for i in idx], seed,
+                    [active[i][0].traj_id for i in idx], [active[i][3] for i in idx],
+                    [active[i][0].p for i in idx],
+                    ent_bonds=[active[i][0].n // 2 - 1 for i in idx])
+                for i, g in zip(idx, got):
+                    res[i] = g
             self.ticks += 1
             self.layers_done += len(active)
             if self.log and self.ticks % 10 == 0:

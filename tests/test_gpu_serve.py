@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 from oracle.mps_oracle import run_trajectory
-from tnmps import MPS
+from tnmps import MPS, Engine
 from tnmps.serve import Request, Server, mps_bytes
 from tnmps.workload import haar_circuit
 
@@ -59,3 +59,35 @@ def test_serving_mix_equals_alone(gpu):
             assert close(a, b), (r.rid, a, b)
     # admission respected the HBM budget
     assert srv.peak_held <= budget
+
+
+def test_serving_generated_equals_alone(gpu):
+    """Requests generated on the device (seed, trajectory id) served in a mix
+    with carried circuits give the same results as the same generated
+    trajectory run alone."""
+    reqs = _requests()[:3]
+    seed = 4242
+    for i, (n, p, chi, t) in enumerate([(10, 0.2, 16, 0.0), (12, 0.3, 8, 0.5), (8, 0.1, 32, 2.0)]):
+        reqs.append(Request(rid=10 + i, n=n, p=p, chi=chi, arrival=t, gen_seed=seed,
+                            traj_id=100 + i, gen_depth=n))
+    srv = Server(chi_max=32)
+    ticks = {"t": 0.0}
+
+    def clock():
+        ticks["t"] += 0.5
+        return ticks["t"]
+
+    srv.run(reqs, clock=clock, realtime=False)
+    for r in reqs[3:]:
+        assert r.done is not None and len(r.record) == r.depth
+        eng = Engine(r.chi)
+        st = eng.new_qubits(r.n, ("const", r.chi))
+        ent = [0.0]
+        for t in range(r.depth):
+            (rec, prs, e), = eng.batch_bilayer_gen([st], seed, [r.traj_id], [t], [r.p])
+            assert rec == r.record[t]
+            for (ka, pa), (kb, pb) in zip(prs, r.probs[t]):
+                assert ka == kb and close(pa, pb, rtol=1e-12)
+            ent.append(e)
+        for a, b in zip(ent, r.entropy):
+            assert close(a, b, rtol=1e-12)
