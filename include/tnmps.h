@@ -248,6 +248,19 @@ tn_status tn_batch_bilayer_gen(tn_engine* e, size_t count, tn_mps* const* hs,
                                double* ent_out, size_t* out_counts, size_t* out_sites,
                                int* out_outcomes, double* out_probs);
 
+/* ---- dense cross-checks and batched read-outs --------------------------- */
+
+/* entropy_ry_schmidt (mps.rs:1268-1282; alphas[i] == 1 is entropy_vn,
+ * :1251-1261) of bond bonds[i] of handle hs[i], for `count` read-outs in one
+ * launch; NaN where the reference returns None (bond out of range). */
+tn_status tn_entropies(tn_engine* e, size_t count, tn_mps* const* hs,
+                       const size_t* bonds, const double* alphas, double* out);
+
+/* MPS::contract (mps.rs:794-798): the flat state vector, qubit 0 slowest,
+ * contracted on the device and copied to out (2^(n+1) doubles, interleaved;
+ * cap is out's length in doubles).  n <= 30. */
+tn_status tn_contract(tn_mps* h, double* out, size_t cap);
+
 /* Counters of the device work done by this handle (for throughput
  * accounting): number of two-site updates (contract -> SVD -> split) and
  * the sum over them of Jacobi sweeps. */

@@ -136,13 +136,7 @@ struct DescRing {
   }
 };
 
-// One entropy read-out: lambda of a bond, its length, Renyi order, result slot.
-struct EntReq {
-  const double* lam;
-  const int* dim;
-  double a;
-  int slot;
-};
+using EntReq = EntDesc;
 
 }  // namespace
 
@@ -193,6 +187,7 @@ struct tn_engine {
   bool trace_sweeps = false;  // print each sweep's largest off-diagonal (TNMPS_TRACE_SWEEPS)
   DescRing<UpdDesc> upd;
   DescRing<MeasDesc> meas;
+  DescRing<EntDesc> ents;
   int mres = 0;  // measurement result slots
   int* d_out = nullptr;
   double* d_prob = nullptr;
@@ -238,6 +233,7 @@ struct tn_engine {
     release_res();
     upd.release();
     meas.release();
+    ents.release();
     hfree(d_scalar); d_scalar = nullptr; nscalar = 0;
     hfree(gen_u); hfree(gen_ids); gen_u = nullptr; gen_ids = nullptr; gen_cap = 0;
     for (PerfRec& r : prec) {
@@ -787,8 +783,17 @@ tn_status run_measures(tn_engine* e, const std::vector<MTask>& ms) {
   return TN_OK;
 }
 
+// All read-outs in one launch, results to d_scalar[slot].
 tn_status run_entropies(tn_engine* e, const std::vector<EntReq>& es) {
-  for (const EntReq& q : es) launch_entropy(q.lam, q.dim, q.a, e->d_scalar + q.slot, e->stream);
+  if (es.empty()) return TN_OK;
+  const int ne = (int)es.size();
+  TN_TRY(e->ents.ensure(ne, e->stream));
+  EntDesc* hd = nullptr;
+  int rs = 0;
+  TN_TRY(e->ents.next(&hd, &rs));
+  std::copy(es.begin(), es.end(), hd);
+  TN_TRY(e->ents.upload(rs, ne, e->stream));
+  launch_entropy(e->ents.d[rs], ne, e->d_scalar, e->stream);
   TN_HIP(hipGetLastError());
   return TN_OK;
 }
@@ -1330,11 +1335,70 @@ tn_status tn_entropy_ry_schmidt(tn_mps* h, double a, size_t b, double* out) {
   if (h->n < 2 || b >= h->n - 1) return TN_NONE;  // mps.rs:1252
   tn_engine* e = h->eng;
   TN_TRY(e->ensure_scalars(1));
-  launch_entropy(h->lam + h->loff[b], h->dims + b + 1, a, e->d_scalar, e->stream);
-  TN_HIP(hipGetLastError());
+  TN_TRY(run_entropies(e, {{h->lam + h->loff[b], h->dims + b + 1, a, 0}}));
   TN_HIP(hipMemcpyAsync(out, e->d_scalar, sizeof(double), hipMemcpyDeviceToHost, e->stream));
   TN_HIP(hipStreamSynchronize(e->stream));
   return TN_OK;
+}
+
+tn_status tn_entropies(tn_engine* e, size_t count, tn_mps* const* hs, const size_t* bonds,
+                       const double* alphas, double* out) {
+  if (!e || (count && (!hs || !bonds || !alphas || !out))) return TN_INVALID_ARG;
+  if (count == 0) return TN_OK;
+  if (count > (size_t)INT32_MAX) return TN_INVALID_ARG;
+  std::vector<EntReq> es;
+  for (size_t i = 0; i < count; ++i) {
+    tn_mps* h = hs[i];
+    if (!h || h->eng != e) return TN_INVALID_ARG;
+    if (h->n >= 2 && bonds[i] < h->n - 1)
+      es.push_back({h->lam + h->loff[bonds[i]], h->dims + bonds[i] + 1, alphas[i], (int)i});
+  }
+  TN_TRY(e->ensure_scalars((int)count));
+  TN_TRY(run_entropies(e, es));
+  std::vector<double> tmp(count);
+  TN_HIP(hipMemcpyAsync(tmp.data(), e->d_scalar, sizeof(double) * count, hipMemcpyDeviceToHost,
+                        e->stream));
+  TN_HIP(hipStreamSynchronize(e->stream));
+  for (size_t i = 0; i < count; ++i) {
+    tn_mps* h = hs[i];
+    out[i] = (h->n >= 2 && bonds[i] < h->n - 1) ? tmp[i] : NAN;
+  }
+  return TN_OK;
+}
+
+tn_status tn_contract(tn_mps* h, double* out, size_t cap) {
+  if (!h || !out) return TN_INVALID_ARG;
+  const size_t n = h->n;
+  if (n > 30) return TN_INVALID_ARG;
+  const size_t len = (size_t)1 << n;
+  if (cap < 2 * len) return TN_INVALID_ARG;
+  tn_engine* e = h->eng;
+  double2 *a = nullptr, *b = nullptr;
+  auto done = [&](tn_status st) {
+    (void)hipStreamSynchronize(e->stream);
+    hfree(a);
+    hfree(b);
+    return st;
+  };
+  if (hipMalloc(&a, sizeof(double2) * len) != hipSuccess ||
+      hipMalloc(&b, sizeof(double2) * len) != hipSuccess)
+    return done(TN_DEVICE);
+  const double2 one = make_double2(1.0, 0.0);
+  if (hipMemcpyAsync(a, &one, sizeof(double2), hipMemcpyHostToDevice, e->stream) != hipSuccess)
+    return done(TN_DEVICE);
+  for (size_t k = 0; k < n; ++k) {
+    const size_t P = (size_t)1 << k;
+    const size_t ubR = k + 1 < n ? h->bmax[k] : 1;
+    launch_contract(a, P, h->dims + k, h->gam + h->goff[k],
+                    k == 0 ? nullptr : h->lam + h->loff[k - 1], 2 * P * ubR, b, e->stream);
+    std::swap(a, b);
+  }
+  if (hipGetLastError() != hipSuccess ||
+      hipMemcpyAsync(out, a, sizeof(double2) * len, hipMemcpyDeviceToHost, e->stream) !=
+          hipSuccess ||
+      hipStreamSynchronize(e->stream) != hipSuccess)
+    return done(TN_DEVICE);
+  return done(TN_OK);
 }
 
 tn_status tn_entropy_vn(tn_mps* h, size_t b, double* out) {

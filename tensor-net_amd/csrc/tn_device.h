@@ -98,6 +98,15 @@ struct JacobiArgs {
   double conv_tol;       // converged after a sweep whose max ratio <= conv_tol
 };
 
+// One entropy read-out: lambda of a bond, where its length is, Renyi order a
+// (1: von Neumann), result slot.
+struct EntDesc {
+  const double* lam;
+  const int* dim;
+  double a;
+  int slot;
+};
+
 // One site measurement or probability query (mps.rs:1027-1134, 1427-1465).
 struct MeasDesc {
   double2* g;            // Gamma[k]

@@ -23,7 +23,12 @@ void launch_fmul(const UpdDesc* descs, int nt, int maxX, int maxRank, hipStream_
 void launch_split(const UpdDesc* descs, int nt, hipStream_t s);
 void launch_renorm(const UpdDesc* descs, int nt, hipStream_t s);
 void launch_measure(const MeasDesc* descs, int nm, hipStream_t s);
-void launch_entropy(const double* lam, const int* dimp, double a, double* out, hipStream_t s);
+// One workgroup per entropy read-out descs[i], result to out[descs[i].slot].
+void launch_entropy(const EntDesc* descs, int count, double* out, hipStream_t s);
+// One site of the state-vector contraction: out (2P x chiR) from acc
+// (P x chiL); max_out bounds 2P x chiR for the grid.
+void launch_contract(const double2* acc, size_t P, const int* dims, const double2* g,
+                     const double* lamL, size_t max_out, double2* out, hipStream_t s);
 void launch_u1(double2* g, const int* dims, const double* op8, int gx, hipStream_t s);
 // Haar 4x4 unitaries from the counter-based generator (tn_rng.h): unitary u
 // from counter words ids[3u .. 3u+2], written row-major to out + 16 u.

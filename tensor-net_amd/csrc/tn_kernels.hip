@@ -28,6 +28,7 @@
 #include "tn_kernels.h"
 #include "tn_rng.h"
 
+#include <algorithm>
 #include <cfloat>
 
 namespace tnmps {
@@ -527,19 +528,48 @@ __global__ __launch_bounds__(1024) void k_measure(const MeasDesc* __restrict__ d
 }
 
 // ------------------------------------------------------------ entropy
-// a == 1: -sum s^2 ln s^2; otherwise ln(sum (s^2)^a) / (1 - a).
-__global__ __launch_bounds__(256) void k_entropy(const double* lam, const int* dimp, double a,
+// One workgroup per read-out: a == 1: -sum s^2 ln s^2; otherwise
+// ln(sum (s^2)^a) / (1 - a).
+__global__ __launch_bounds__(256) void k_entropy(const EntDesc* __restrict__ descs,
                                                  double* out) {
   __shared__ double red[4];
-  const int len = *dimp;
+  const EntDesc q = descs[blockIdx.x];
+  const int len = *q.dim;
   double acc = 0.0;
   for (int i = threadIdx.x; i < len; i += blockDim.x) {
-    const double s2 = lam[i] * lam[i];
-    if (a == 1.0) acc += -s2 * log(s2);
-    else acc += pow(s2, a);
+    const double s2 = q.lam[i] * q.lam[i];
+    if (q.a == 1.0) acc += -s2 * log(s2);
+    else acc += pow(s2, q.a);
   }
   const double s = block_sum(acc, red);
-  if (threadIdx.x == 0) *out = (a == 1.0) ? s : log(s) * (1.0 / (1.0 - a));
+  if (threadIdx.x == 0) out[q.slot] = (q.a == 1.0) ? s : log(s) * (1.0 / (1.0 - q.a));
+}
+
+// ------------------------------------------------------------ contract
+// One site of MPS::contract (mps.rs:794-798): out[(p, s), r] =
+// sum_l acc[p, l] lamL[l] Gamma[l, s, r] for p < P, with chiL = dims[0] and
+// chiR = dims[1]; acc holds P x chiL, out 2P x chiR (qubit 0 slowest).
+__global__ __launch_bounds__(256) void k_contract(const double2* __restrict__ acc, size_t P,
+                                                  const int* __restrict__ dims,
+                                                  const double2* __restrict__ g,
+                                                  const double* __restrict__ lamL,
+                                                  double2* __restrict__ out) {
+  const int chiL = dims[0], chiR = dims[1];
+  const size_t tot = 2 * P * (size_t)chiR;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < tot;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const size_t ps = e / chiR;
+    const int r = (int)(e % chiR);
+    const size_t p = ps >> 1;
+    const int s = (int)(ps & 1);
+    double2 o = make_double2(0.0, 0.0);
+    for (int l = 0; l < chiL; ++l) {
+      double2 a = acc[p * chiL + l];
+      if (lamL) a = cscale(a, lamL[l]);
+      o = cadd(o, cmul(a, g[((size_t)l * 2 + s) * chiR + r]));
+    }
+    out[e] = o;
+  }
 }
 
 // ----------------------------------------------------------------- u1
@@ -581,8 +611,15 @@ void launch_renorm(const UpdDesc* descs, int nt, hipStream_t s) {
 void launch_measure(const MeasDesc* descs, int nm, hipStream_t s) {
   hipLaunchKernelGGL(k_measure, dim3(nm), dim3(1024), 0, s, descs);
 }
-void launch_entropy(const double* lam, const int* dimp, double a, double* out, hipStream_t s) {
-  hipLaunchKernelGGL(k_entropy, dim3(1), dim3(256), 0, s, lam, dimp, a, out);
+void launch_entropy(const EntDesc* descs, int count, double* out, hipStream_t s) {
+  if (count <= 0) return;
+  hipLaunchKernelGGL(k_entropy, dim3(count), dim3(256), 0, s, descs, out);
+}
+void launch_contract(const double2* acc, size_t P, const int* dims, const double2* g,
+                     const double* lamL, size_t max_out, double2* out, hipStream_t s) {
+  const size_t blocks = std::min<size_t>((max_out + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_contract, dim3((unsigned)std::max<size_t>(blocks, 1)), dim3(256), 0, s,
+                     acc, P, dims, g, lamL, out);
 }
 void launch_u1(double2* g, const int* dims, const double* op8, int gx, hipStream_t s) {
   Gate2 op;

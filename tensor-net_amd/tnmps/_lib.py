@@ -96,6 +96,8 @@ SIGNATURES = {
     "tn_engine_get_perf": (C.c_int, [_h, C.POINTER(tn_perf)]),
     "tn_engine_reset_perf": (C.c_int, [_h]),
     "tn_engine_sync": (C.c_int, [_h]),
+    "tn_entropies": (C.c_int, [_h, _sz, _hp, _szp, _dp, _dp]),
+    "tn_contract": (C.c_int, [_h, _dp, _sz]),
     "tn_rng_words": (C.c_int, [C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, _sz,
                                C.POINTER(C.c_uint64)]),
     "tn_gen_haar4": (C.c_int, [_h, C.c_uint64, _sz, C.POINTER(C.c_uint64), _dp]),

@@ -340,15 +340,12 @@ class MPS:
         return buf.view(np.complex128).reshape(int(shp[0]), int(shp[1]), int(shp[2]))
 
     def contract(self) -> np.ndarray:
-        """Flat state vector, qubit 0 slowest (``MPS::contract``, mps.rs:794-798).
-        Host-side read-out for small n (tests)."""
-        n = self.n
-        acc = self.gamma(0).reshape(2, -1)
-        for k in range(1, n):
-            lam = self.schmidt(k - 1)
-            g = self.gamma(k) * lam[:, None, None]
-            acc = (acc @ g.reshape(g.shape[0], -1)).reshape(-1, g.shape[2])
-        return acc.reshape(-1)
+        """Flat state vector, qubit 0 slowest (``MPS::contract``, mps.rs:794-798),
+        contracted on the device (``tn_contract``, n <= 30)."""
+        out = np.zeros(1 << self.n, dtype=np.complex128)
+        _check(self._lib.tn_contract(self._h, _dptr(out.view(np.float64)), 2 * out.size),
+               "contract")
+        return out
 
     def stats(self) -> Tuple[int, int]:
         u = C.c_size_t()
@@ -495,6 +492,19 @@ class Engine:
         _check(self._lib.tn_mps_new_qubits_in(self._h, int(n), to_trunc(trunc), int(chi_cap),
                                               C.byref(h)), "new_qubits_in")
         return MPS(h.value, self._lib, engine=self)
+
+    def entropies(self, states: Sequence[MPS], bonds, alphas) -> np.ndarray:
+        """``entropy_ry_schmidt`` (mps.rs:1268-1282; alpha 1 is
+        ``entropy_vn``) of bonds[i] of states[i], all in one launch
+        (``tn_entropies``); NaN where the reference returns None."""
+        cnt = len(states)
+        hs = (C.c_void_p * max(cnt, 1))(*[st._h.value for st in states])
+        b = _szarr(np.broadcast_to(np.asarray(bonds), (cnt,)).tolist())
+        a = np.ascontiguousarray(np.broadcast_to(np.asarray(alphas, dtype=np.float64), (cnt,)))
+        out = np.zeros(max(cnt, 1))
+        _check(self._lib.tn_entropies(self._h, cnt, C.cast(hs, C.POINTER(C.c_void_p)),
+                                      _szptr(b), _dptr(a), _dptr(out)), "entropies")
+        return out[:cnt]
 
     def gen_haar4(self, seed: int, ids) -> np.ndarray:
         """Haar 4x4 unitaries generated on the device (``tn_gen_haar4``):
