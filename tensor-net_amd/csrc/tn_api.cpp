@@ -60,12 +60,19 @@ double env_double(const char* name, double dflt) {
 // above tol_rot, and Jacobi converges quadratically, so the residual after it
 // is of order conv_tol^2 = 1e-16.
 // Both are read from the environment when an engine is made.
+// A column whose norm is below tiny_norm (relative to the unit-norm SVD
+// input) is numerically null: it takes part in no rotation and in no
+// convergence measure.  Below f64::EPSILON, such a column's value is dropped
+// by the rank rule (mps.rs:255-262) whatever further rotations would make of
+// it, and a rotation only shrinks the smaller column of its pair.
 struct Tolerances {
   double tol_rot = 1e-14;
   double conv_tol = 1e-8;
+  double tiny_norm = 1.5e-16;
   void read_env() {
     tol_rot = env_double("TNMPS_TOL_ROT", tol_rot);
     conv_tol = env_double("TNMPS_CONV_TOL", conv_tol);
+    tiny_norm = env_double("TNMPS_TINY_NORM", tiny_norm);
   }
 };
 
@@ -527,6 +534,7 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
   TN_TRY(e->upd.next(&hd, &rs));
   size_t maxC = 1, maxR = 1, maxTiles = 1, maxCC = 1;
   bool any_factored = false;
+  int mode0 = 0;  // first task's mode, for the sweep trace
   for (int t = 0; t < nt; ++t) {
     const Task& q = tasks[t];
     tn_mps* h = q.h;
@@ -534,6 +542,7 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
     UpdDesc& d = hd[t];
     std::memset(&d, 0, sizeof(d));
     d.mode = q.has_gate ? 0 : (q.pre ? q.pre : factor_mode(e, h, k));
+    if (t == 0) mode0 = d.mode;
     any_factored = any_factored || d.mode != 0;
     d.gk = h->gam + h->goff[k];
     d.gk1 = h->gam + h->goff[k + 1];
@@ -622,6 +631,7 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
   ja.Ch = (int)maxC;
   ja.tol_rot = tol.tol_rot;
   ja.conv_tol = tol.conv_tol;
+  ja.tiny2 = tol.tiny_norm * tol.tiny_norm;
   // Sweeps are enqueued one ahead while the last sweep read back is still far
   // from convergence (max off-diagonal above kAheadOff: with quadratic
   // convergence the next sweep cannot reach conv_tol, so the one after it is
@@ -664,8 +674,10 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
       worst = std::max(worst, o[t]);
     }
     if (e->trace_sweeps)
-      std::fprintf(stderr, "[tnmps] batch of %d (C <= %zu): sweep %d max off-diagonal %.3e\n", nt,
-                   maxC, checked, worst);
+      std::fprintf(stderr,
+                   "[tnmps] batch of %d (C <= %zu, gate %d, prepass %d, mode %d): sweep %d max "
+                   "off-diagonal %.3e\n",
+                   nt, maxC, tasks[0].has_gate ? 1 : 0, tasks[0].pre, mode0, checked, worst);
     last_worst = worst;
     ++checked;
   }

@@ -96,6 +96,7 @@ struct JacobiArgs {
   int Rh, Ch;            // bounds on every task's R and C
   double tol_rot;        // rotate a pair when |g_ij| > tol_rot sqrt(g_ii g_jj)
   double conv_tol;       // converged after a sweep whose max ratio <= conv_tol
+  double tiny2;          // columns with squared norm below take no part
 };
 
 // One entropy read-out: lambda of a bond, where its length is, Renyi order a

@@ -38,11 +38,10 @@ __device__ __forceinline__ int rr_player(int i, int step, int nbe) {
   return 1 + (i - 1 + step) % (nbe - 1);
 }
 
-// Columns whose squared norm is below kTinyNorm2 (norm 1e-17, under
-// f64::EPSILON times the unit-norm SVD input, so below every truncation
-// threshold of the rank rule) take part in no rotation and in no convergence
-// measure.
-constexpr double kTinyNorm2 = 1e-34;
+// Columns whose squared norm is below a.tiny2 (by default norm 1.5e-16,
+// under f64::EPSILON times the unit-norm SVD input, so below every
+// truncation threshold of the rank rule) take part in no rotation and in no
+// convergence measure.
 
 template <int BW>
 struct PairSmem {
@@ -519,7 +518,7 @@ __global__ __launch_bounds__(256, BW == 8 ? 2 : 1) void k_jacobi_step(JacobiArgs
   load_chunk<BW>(X, (size_t)a.ldx, 0, R, C, lcol, cx);
   load_chunk<BW>(W, (size_t)a.ldw, 0, C, C, lcol, cw);
   int wsel = 0;
-  if (!panel_solve<BW>(a.offs + (size_t)sweep * a.nt + t, a.tol_rot, kTinyNorm2, sm, wsel))
+  if (!panel_solve<BW>(a.offs + (size_t)sweep * a.nt + t, a.tol_rot, a.tiny2, sm, wsel))
     return;
   panel_apply<BW>(X, (size_t)a.ldx, 0, R, C, lcol, ocol, cx, sm, wsel);
   panel_apply<BW>(W, (size_t)a.ldw, 0, C, C, lcol, ocol, cw, sm, wsel);
@@ -598,7 +597,7 @@ __global__ __launch_bounds__(256, BW == 8 ? 2 : 1) void k_jacobi_apply(JacobiArg
   mask_chunk<BW>(wa, we, C, lcol, cw);
   int wsel = 0;
   if (!panel_solve<BW>(q == 0 ? a.offs + (size_t)sweep * a.nt + t : nullptr, a.tol_rot,
-                       kTinyNorm2, sm, wsel))
+                       a.tiny2, sm, wsel))
     return;
   panel_apply<BW>(X, (size_t)a.ldx, xa, xe, C, lcol, ocol, cx, sm, wsel);
   panel_apply<BW>(W, (size_t)a.ldw, wa, we, C, lcol, ocol, cw, sm, wsel);

@@ -105,14 +105,15 @@ def test_fp32_contraction_error(gpu):
 def test_step_and_row_split_jacobi_agree(gpu, monkeypatch):
     """The one-workgroup-per-pair step (TNMPS_SPLIT_MAX=1) and the row-split
     step (the default for lone updates) give the oracle's spectra, and so do
-    the wide 16-column panels (TNMPS_BW=16)."""
+    the wide 16-column panels (TNMPS_BW=16), and the row split forced down to
+    64-row slices (TNMPS_SPLIT_ROWS=64)."""
     n, depth, p, chi = 12, 10, 0.1, 32
     c = haar_circuit(n, depth, p, 31)
     ref = run_trajectory(n, c.layers, ("const", chi), rs=c.draws)
     runs = []
     for env in ({}, {"TNMPS_SPLIT_MAX": "1"}, {"TNMPS_BW": "16"},
-                {"TNMPS_BW": "16", "TNMPS_SPLIT_MAX": "1"}):
-        for k in ("TNMPS_SPLIT_MAX", "TNMPS_BW"):
+                {"TNMPS_BW": "16", "TNMPS_SPLIT_MAX": "1"}, {"TNMPS_SPLIT_ROWS": "64"}):
+        for k in ("TNMPS_SPLIT_MAX", "TNMPS_BW", "TNMPS_SPLIT_ROWS"):
             monkeypatch.delenv(k, raising=False)
         for k, v in env.items():
             monkeypatch.setenv(k, v)
@@ -123,3 +124,19 @@ def test_step_and_row_split_jacobi_agree(gpu, monkeypatch):
         assert st.bond_dims() == ref["state"].bond_dims()
         for bd in range(n - 1):
             assert_spectrum_close(st.schmidt(bd), ref["state"].L[bd])
+
+
+def test_row_split_matches_oracle(gpu):
+    """Bonds at chi = 64 (theta 128 x 128): lone refactors take the row-split
+    path at its default slice size; spectra, records and entropies match the
+    oracle."""
+    n, depth, p, chi = 16, 12, 0.15, 64
+    c = haar_circuit(n, depth, p, 77)
+    ref = run_trajectory(n, c.layers, ("const", chi), rs=c.draws)
+    st = MPS.new_qubits(n, ("const", chi))
+    got = st.run_circuit(c.layers, forced=ref["record"])
+    assert st.bond_dims() == ref["state"].bond_dims()
+    for bd in range(n - 1):
+        assert_spectrum_close(st.schmidt(bd), ref["state"].L[bd])
+    for a, b in zip(got["entropy"], ref["entropy"]):
+        assert abs(a - b) <= 1e-12 + 1e-10 * abs(b)
