@@ -40,7 +40,7 @@ METRIC = "gate+SVD updates/s @chi=256 FP64 vs mixed; trajectories/s + p99 latenc
 # rocprofv3 FETCH_SIZE and WRITE_SIZE (KiB per dispatch, one counter per run)
 # scaled by what the same counters read for a plain 1 GiB device copy
 # (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0); profiles/jacobi_single_traj_chi256_hbm_traffic.md.
-PMC_KIB = {"k_jacobi_apply<8>": (3222.654, 375.280), "k_jacobi_gram<8>": (943.203, 455.115)}
+PMC_KIB = {"k_jacobi_apply<8>": (3296.538, 353.617), "k_jacobi_gram<8>": (940.272, 429.440)}
 PMC_SCALE = (2.0, 1.0)
 TRAFFIC_BYTES = {k: 1024.0 * (f * PMC_SCALE[0] + w * PMC_SCALE[1]) for k, (f, w) in PMC_KIB.items()}
 FP64_PEAK_TFLOPS = 78.6
