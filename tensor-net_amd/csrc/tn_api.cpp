@@ -185,7 +185,10 @@ struct tn_engine {
   size_t gpart_per = 0;  // elements of gpart per slot
   double* offs = nullptr;
   int* info = nullptr;
+  int* act = nullptr;    // active-task list of the current sweep: count, then tasks
   int cus = 0;           // compute units of the device
+  int list_min = 64;     // tasks from which the step kernel runs over act (TNMPS_LIST_MIN)
+  int list_wgs = 0;      // workgroups of such a launch (TNMPS_LIST_WGS; default 8 per CU)
   int split_max = kMaxSplit;  // cap on workgroups per block pair (TNMPS_SPLIT_MAX)
   int split_rows = 128;       // least X rows per row-split workgroup (TNMPS_SPLIT_ROWS)
   int bw_env = 0;             // forced Jacobi block width (TNMPS_BW), 0 = automatic
@@ -220,13 +223,13 @@ struct tn_engine {
 
   void release_ws() {
     hfree(X); hfree(W); hfree(sraw); hfree(sval); hfree(fac); hfree(perm);
-    hfree(part); hfree(offs); hfree(info); hfree(gpart);
+    hfree(part); hfree(offs); hfree(info); hfree(gpart); hfree(act);
     hhfree(h_offs);
     h_offs = nullptr;
     gpart = nullptr;
     X = W = nullptr;
     sraw = sval = fac = part = offs = nullptr;
-    perm = info = nullptr;
+    perm = info = act = nullptr;
     slots = 0;
   }
   void release_res() {
@@ -276,6 +279,8 @@ struct tn_engine {
     cus = prop.multiProcessorCount;
     split_max = std::max(1, std::min(kMaxSplit, (int)env_double("TNMPS_SPLIT_MAX", kMaxSplit)));
     split_rows = std::max(64, (int)env_double("TNMPS_SPLIT_ROWS", 128));
+    list_min = std::max(1, (int)env_double("TNMPS_LIST_MIN", 64));
+    list_wgs = std::max(1, (int)env_double("TNMPS_LIST_WGS", 8.0 * cus));
     size_t gp = 0;
     for (int bw : {kBW, kBWWide}) {
       const size_t nbmax = (size_t)(ldw + bw - 1) / bw;
@@ -383,6 +388,7 @@ struct tn_engine {
     TN_HIP(hipHostMalloc(&h_offs, sizeof(double) * kMaxSweeps * ns, hipHostMallocDefault));
     TN_HIP(hipMalloc(&info, sizeof(int) * kInfoLen * ns));
     TN_HIP(hipMalloc(&gpart, sizeof(double2) * gpart_per * ns));
+    TN_HIP(hipMalloc(&act, sizeof(int) * (ns + 1)));
     slots = ns;
     return upd.ensure(ns, stream);
   }
@@ -619,6 +625,14 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
   ja.gpart = e->gpart;
   ja.offs = e->offs;
   ja.info = e->info;
+  // Many tasks: the step kernel takes the sweep's unconverged tasks from a
+  // list that a one-workgroup kernel builds before the sweep, with a fixed
+  // number of workgroups (list_wgs) looping over them, so tasks that have
+  // converged (a lock-step batch runs until its slowest task has) cost no
+  // workgroups.  The arithmetic per task is the same either way.
+  const bool use_list = S == 1 && nt >= e->list_min && nt <= 8192;
+  ja.act = use_list ? e->act : nullptr;
+  const int gy = use_list ? std::min(nt, std::max(1, (e->list_wgs + npairs - 1) / npairs)) : nt;
   ja.xslot = (size_t)e->ldx * e->ldw;
   ja.wslot = (size_t)e->ldw * e->ldw;
   ja.gslot = e->gpart_per;
@@ -648,9 +662,10 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
     const int depth = last_worst > kAheadOff ? 2 : 1;
     const auto h0 = std::chrono::steady_clock::now();
     while (launched < kMaxSweeps && launched - checked < depth) {
+      if (use_list) launch_active(ja, launched, e->act, e->stream);
       for (int s = 0; s < steps; ++s) {
         if (S == 1) {
-          launch_jacobi_step(ja, npairs, bw, launched, s, e->stream);
+          launch_jacobi_step(ja, npairs, bw, launched, s, gy, e->stream);
         } else {
           launch_jacobi_gram(ja, npairs, bw, launched, s, e->stream);
           launch_jacobi_apply(ja, npairs, bw, launched, s, e->stream);
@@ -669,15 +684,19 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
     const double* o = e->h_offs + (size_t)checked * nt;
     done = true;
     double worst = 0.0;
+    int open = 0;
     for (int t = 0; t < nt; ++t) {
-      if (!(o[t] <= tol.conv_tol)) done = false;
+      if (!(o[t] <= tol.conv_tol)) {
+        done = false;
+        ++open;
+      }
       worst = std::max(worst, o[t]);
     }
     if (e->trace_sweeps)
       std::fprintf(stderr,
                    "[tnmps] batch of %d (C <= %zu, gate %d, prepass %d, mode %d): sweep %d max "
-                   "off-diagonal %.3e\n",
-                   nt, maxC, tasks[0].has_gate ? 1 : 0, tasks[0].pre, mode0, checked, worst);
+                   "off-diagonal %.3e, %d not converged\n",
+                   nt, maxC, tasks[0].has_gate ? 1 : 0, tasks[0].pre, mode0, checked, worst, open);
     last_worst = worst;
     ++checked;
   }

@@ -88,6 +88,8 @@ struct JacobiArgs {
                          // [pair][slice][PW * PW]
   double* offs;          // [sweep][task] max off-diagonal ratio
   const int* info;       // kInfoLen ints per task
+  const int* act;        // or null: the sweep's unconverged tasks (act[0] of them,
+                         // indices from act + 1) for k_jacobi_step to loop over
   size_t xslot, wslot, gslot;
   int ldx, ldw;
   int nt;                // tasks in the launch

@@ -496,12 +496,10 @@ __device__ __forceinline__ bool converged(const JacobiArgs& a, int t, int sweep)
   return sweep > 0 && a.offs[(size_t)(sweep - 1) * a.nt + t] <= a.conv_tol;
 }
 
-// One launch per step, one workgroup per block pair.
+// Block pair p of task t at one step: Gram matrix, solve, apply to X and W.
 template <int BW>
-__global__ __launch_bounds__(256, BW == 8 ? 2 : 1) void k_jacobi_step(JacobiArgs a, int sweep,
-                                                                      int step) {
-  __shared__ PairSmem<BW> sm;
-  const int t = blockIdx.y, p = blockIdx.x;
+__device__ __forceinline__ void step_task(const JacobiArgs& a, int t, int p, int sweep, int step,
+                                          PairSmem<BW>& sm) {
   const int* inf = a.info + (size_t)t * kInfoLen;
   const int R = inf[kInfoR], C = inf[kInfoC];
   const bool done = converged(a, t, sweep);
@@ -522,6 +520,52 @@ __global__ __launch_bounds__(256, BW == 8 ? 2 : 1) void k_jacobi_step(JacobiArgs
     return;
   panel_apply<BW>(X, (size_t)a.ldx, 0, R, C, lcol, ocol, cx, sm, wsel);
   panel_apply<BW>(W, (size_t)a.ldw, 0, C, C, lcol, ocol, cw, sm, wsel);
+}
+
+// One launch per step, one workgroup per block pair: of task blockIdx.y, or,
+// with an active-task list, of the listed tasks blockIdx.y, blockIdx.y +
+// gridDim.y, ...  (converged tasks then cost no workgroup).
+template <int BW>
+__global__ __launch_bounds__(256, BW == 8 ? 2 : 1) void k_jacobi_step(JacobiArgs a, int sweep,
+                                                                      int step) {
+  __shared__ PairSmem<BW> sm;
+  const int p = blockIdx.x;
+  if (a.act == nullptr) {
+    step_task<BW>(a, blockIdx.y, p, sweep, step, sm);
+    return;
+  }
+  const int cnt = a.act[0];
+  for (int i = blockIdx.y; i < cnt; i += gridDim.y) {
+    step_task<BW>(a, a.act[1 + i], p, sweep, step, sm);
+    __syncthreads();  // the next task reuses the LDS
+  }
+}
+
+// Active-task list of a sweep: act[0] = number of tasks not converged before
+// `sweep` (all of them at sweep 0), act[1 ..] their indices in ascending
+// order.  One workgroup of 1024 threads; thread i counts a run of
+// ceil(nt / 1024) consecutive tasks, a scan gives each run its offset.
+__global__ __launch_bounds__(1024) void k_active(JacobiArgs a, int sweep, int* act) {
+  __shared__ int wsum[16];
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int per = (a.nt + 1023) / 1024;
+  const int t0 = min(a.nt, tid * per), t1 = min(a.nt, t0 + per);
+  int c = 0;
+  for (int t = t0; t < t1; ++t) c += converged(a, t, sweep) ? 0 : 1;
+  int x = c;  // inclusive scan over the wave
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int y = __shfl_up(x, off, 64);
+    if (lane >= off) x += y;
+  }
+  if (lane == 63) wsum[wid] = x;
+  __syncthreads();
+  int base = 0;
+  for (int w = 0; w < wid; ++w) base += wsum[w];
+  int pos = base + x - c;
+  for (int t = t0; t < t1; ++t)
+    if (!converged(a, t, sweep)) act[1 + pos++] = t;
+  if (tid == 1023) act[0] = base + x;
 }
 
 // Row-split step, part 1: workgroup (p, q) writes the Gram matrix of row
@@ -604,12 +648,15 @@ __global__ __launch_bounds__(256, BW == 8 ? 2 : 1) void k_jacobi_apply(JacobiArg
 }
 
 // ------------------------------------------------------------- launch
-void launch_jacobi_step(const JacobiArgs& a, int npairs, int bw, int sweep, int step,
+void launch_jacobi_step(const JacobiArgs& a, int npairs, int bw, int sweep, int step, int gy,
                         hipStream_t s) {
   if (bw == 16)
-    hipLaunchKernelGGL(k_jacobi_step<16>, dim3(npairs, a.nt), dim3(256), 0, s, a, sweep, step);
+    hipLaunchKernelGGL(k_jacobi_step<16>, dim3(npairs, gy), dim3(256), 0, s, a, sweep, step);
   else
-    hipLaunchKernelGGL(k_jacobi_step<8>, dim3(npairs, a.nt), dim3(256), 0, s, a, sweep, step);
+    hipLaunchKernelGGL(k_jacobi_step<8>, dim3(npairs, gy), dim3(256), 0, s, a, sweep, step);
+}
+void launch_active(const JacobiArgs& a, int sweep, int* act, hipStream_t s) {
+  hipLaunchKernelGGL(k_active, dim3(1), dim3(1024), 0, s, a, sweep, act);
 }
 void launch_jacobi_gram(const JacobiArgs& a, int npairs, int bw, int sweep, int step,
                         hipStream_t s) {

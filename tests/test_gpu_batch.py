@@ -140,3 +140,39 @@ def test_row_split_matches_oracle(gpu):
         assert_spectrum_close(st.schmidt(bd), ref["state"].L[bd])
     for a, b in zip(got["entropy"], ref["entropy"]):
         assert abs(a - b) <= 1e-12 + 1e-10 * abs(b)
+
+
+def test_active_task_list_bitwise(gpu, monkeypatch):
+    """Large lock-step batches run their Jacobi steps over a per-sweep list of
+    unconverged tasks (TNMPS_LIST_MIN tasks and up, TNMPS_LIST_WGS workgroups
+    looping over it).  Off, default, and forced on every batch with a tiny
+    grid, the states are bitwise the same, and match the oracle."""
+    T, n, depth, chi = 40, 10, 8, 16
+    ps = [0.05 + 0.3 * (i % 7) / 6 for i in range(T)]
+    circuits = [haar_circuit(n, depth, ps[i], 500 + i) for i in range(T)]
+    refs = [run_trajectory(n, c.layers, ("const", chi), rs=c.draws) for c in circuits[:4]]
+    recs = [r["record"] for r in refs]
+    for c in circuits[4:]:
+        st = MPS.new_qubits(n, ("const", chi))
+        recs.append(st.run_circuit(c.layers, rs=c.draws)["record"])
+    runs = []
+    for env in ({"TNMPS_LIST_MIN": "100000"}, {},
+                {"TNMPS_LIST_MIN": "1", "TNMPS_LIST_WGS": "8"}):
+        for k in ("TNMPS_LIST_MIN", "TNMPS_LIST_WGS"):
+            monkeypatch.delenv(k, raising=False)
+        for k, v in env.items():
+            monkeypatch.setenv(k, v)
+        eng = Engine(chi)
+        states = [eng.new_qubits(n, ("const", chi)) for _ in range(T)]
+        for t in range(depth):
+            eng.batch_bilayer(states, [c.layers[t] for c in circuits],
+                              forced=[r[t] for r in recs])
+        runs.append([([st.gamma(k) for k in range(n)], [st.schmidt(b) for b in range(n - 1)])
+                     for st in states])
+    for other in runs[1:]:
+        for (g0, l0), (g1, l1) in zip(runs[0], other):
+            assert all(np.array_equal(a, b) for a, b in zip(g0, g1))
+            assert all(np.array_equal(a, b) for a, b in zip(l0, l1))
+    for i, ref in enumerate(refs):
+        for b in range(n - 1):
+            assert_spectrum_close(runs[1][i][1][b], ref["state"].L[b], msg=f"traj {i} bond {b}")
