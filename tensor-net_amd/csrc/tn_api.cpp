@@ -188,7 +188,6 @@ struct tn_engine {
   int* act = nullptr;    // active-task list of the current sweep: count, then tasks
   int cus = 0;           // compute units of the device
   int list_min = 64;     // tasks from which the step kernel runs over act (TNMPS_LIST_MIN)
-  int list_wgs = 0;      // workgroups of such a launch (TNMPS_LIST_WGS; default 8 per CU)
   int split_max = kMaxSplit;  // cap on workgroups per block pair (TNMPS_SPLIT_MAX)
   int split_rows = 128;       // least X rows per row-split workgroup (TNMPS_SPLIT_ROWS)
   int bw_env = 0;             // forced Jacobi block width (TNMPS_BW), 0 = automatic
@@ -280,7 +279,6 @@ struct tn_engine {
     split_max = std::max(1, std::min(kMaxSplit, (int)env_double("TNMPS_SPLIT_MAX", kMaxSplit)));
     split_rows = std::max(64, (int)env_double("TNMPS_SPLIT_ROWS", 128));
     list_min = std::max(1, (int)env_double("TNMPS_LIST_MIN", 64));
-    list_wgs = std::max(1, (int)env_double("TNMPS_LIST_WGS", 8.0 * cus));
     size_t gp = 0;
     for (int bw : {kBW, kBWWide}) {
       const size_t nbmax = (size_t)(ldw + bw - 1) / bw;
@@ -626,13 +624,14 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
   ja.offs = e->offs;
   ja.info = e->info;
   // Many tasks: the step kernel takes the sweep's unconverged tasks from a
-  // list that a one-workgroup kernel builds before the sweep, with a fixed
-  // number of workgroups (list_wgs) looping over them, so tasks that have
-  // converged (a lock-step batch runs until its slowest task has) cost no
-  // workgroups.  The arithmetic per task is the same either way.
+  // list that a one-workgroup kernel builds before the sweep, so tasks that
+  // have converged (a lock-step batch runs until its slowest task has) cost
+  // no workgroups.  The grid's rows are the unconverged count of the last
+  // sweep read back (or nt), a bound on the list's length since a converged
+  // task stays converged.  The arithmetic per task is the same either way.
   const bool use_list = S == 1 && nt >= e->list_min && nt <= 8192;
   ja.act = use_list ? e->act : nullptr;
-  const int gy = use_list ? std::min(nt, std::max(1, (e->list_wgs + npairs - 1) / npairs)) : nt;
+  int open_known = nt;
   ja.xslot = (size_t)e->ldx * e->ldw;
   ja.wslot = (size_t)e->ldw * e->ldw;
   ja.gslot = e->gpart_per;
@@ -665,7 +664,8 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
       if (use_list) launch_active(ja, launched, e->act, e->stream);
       for (int s = 0; s < steps; ++s) {
         if (S == 1) {
-          launch_jacobi_step(ja, npairs, bw, launched, s, gy, e->stream);
+          launch_jacobi_step(ja, npairs, bw, launched, s, use_list ? open_known : nt,
+                             e->stream);
         } else {
           launch_jacobi_gram(ja, npairs, bw, launched, s, e->stream);
           launch_jacobi_apply(ja, npairs, bw, launched, s, e->stream);
@@ -698,6 +698,7 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
                    "off-diagonal %.3e, %d not converged\n",
                    nt, maxC, tasks[0].has_gate ? 1 : 0, tasks[0].pre, mode0, checked, worst, open);
     last_worst = worst;
+    open_known = std::max(open, 1);
     ++checked;
   }
   sweeps_run = checked;

@@ -89,7 +89,7 @@ struct JacobiArgs {
   double* offs;          // [sweep][task] max off-diagonal ratio
   const int* info;       // kInfoLen ints per task
   const int* act;        // or null: the sweep's unconverged tasks (act[0] of them,
-                         // indices from act + 1) for k_jacobi_step to loop over
+                         // indices from act + 1), k_jacobi_step's grid rows
   size_t xslot, wslot, gslot;
   int ldx, ldw;
   int nt;                // tasks in the launch

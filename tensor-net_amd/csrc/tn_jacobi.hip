@@ -523,22 +523,18 @@ __device__ __forceinline__ void step_task(const JacobiArgs& a, int t, int p, int
 }
 
 // One launch per step, one workgroup per block pair: of task blockIdx.y, or,
-// with an active-task list, of the listed tasks blockIdx.y, blockIdx.y +
-// gridDim.y, ...  (converged tasks then cost no workgroup).
+// with an active-task list, of listed task blockIdx.y (the grid has at least
+// as many rows as the list has tasks; converged tasks then cost no
+// workgroup).  Both words of the list load together.
 template <int BW>
 __global__ __launch_bounds__(256, BW == 8 ? 2 : 1) void k_jacobi_step(JacobiArgs a, int sweep,
                                                                       int step) {
   __shared__ PairSmem<BW> sm;
-  const int p = blockIdx.x;
-  if (a.act == nullptr) {
-    step_task<BW>(a, blockIdx.y, p, sweep, step, sm);
-    return;
-  }
-  const int cnt = a.act[0];
-  for (int i = blockIdx.y; i < cnt; i += gridDim.y) {
-    step_task<BW>(a, a.act[1 + i], p, sweep, step, sm);
-    __syncthreads();  // the next task reuses the LDS
-  }
+  const int i = blockIdx.y;
+  const int cnt = a.act ? a.act[0] : a.nt;
+  const int t = a.act ? a.act[1 + i] : i;
+  if (i >= cnt) return;
+  step_task<BW>(a, t, blockIdx.x, sweep, step, sm);
 }
 
 // Active-task list of a sweep: act[0] = number of tasks not converged before

@@ -9,8 +9,8 @@ namespace tnmps {
 void launch_theta(const UpdDesc* descs, int nt, int gx, hipStream_t s);
 void launch_winit(const UpdDesc* descs, int nt, int gx, hipStream_t s);
 // Jacobi step with block width bw (8 or 16): one workgroup per block pair
-// and task (grid rows gy = a.nt), or, with a.act set, gy rows of workgroups
-// that take the listed tasks in turn.
+// and task (grid rows gy = a.nt), or, with a.act set, per block pair and
+// listed task (gy at least the list's length).
 void launch_jacobi_step(const JacobiArgs& a, int npairs, int bw, int sweep, int step, int gy,
                         hipStream_t s);
 // act[0] = tasks not converged before `sweep`, act[1 ..] their indices in

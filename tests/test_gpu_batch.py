@@ -144,9 +144,9 @@ def test_row_split_matches_oracle(gpu):
 
 def test_active_task_list_bitwise(gpu, monkeypatch):
     """Large lock-step batches run their Jacobi steps over a per-sweep list of
-    unconverged tasks (TNMPS_LIST_MIN tasks and up, TNMPS_LIST_WGS workgroups
-    looping over it).  Off, default, and forced on every batch with a tiny
-    grid, the states are bitwise the same, and match the oracle."""
+    unconverged tasks (batches of TNMPS_LIST_MIN tasks and up).  Off, default,
+    and forced on every batch, the states are bitwise the same, and match the
+    oracle."""
     T, n, depth, chi = 40, 10, 8, 16
     ps = [0.05 + 0.3 * (i % 7) / 6 for i in range(T)]
     circuits = [haar_circuit(n, depth, ps[i], 500 + i) for i in range(T)]
@@ -157,9 +157,8 @@ def test_active_task_list_bitwise(gpu, monkeypatch):
         recs.append(st.run_circuit(c.layers, rs=c.draws)["record"])
     runs = []
     for env in ({"TNMPS_LIST_MIN": "100000"}, {},
-                {"TNMPS_LIST_MIN": "1", "TNMPS_LIST_WGS": "8"}):
-        for k in ("TNMPS_LIST_MIN", "TNMPS_LIST_WGS"):
-            monkeypatch.delenv(k, raising=False)
+                {"TNMPS_LIST_MIN": "1"}):
+        monkeypatch.delenv("TNMPS_LIST_MIN", raising=False)
         for k, v in env.items():
             monkeypatch.setenv(k, v)
         eng = Engine(chi)
