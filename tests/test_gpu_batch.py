@@ -175,3 +175,37 @@ def test_active_task_list_bitwise(gpu, monkeypatch):
     for i, ref in enumerate(refs):
         for b in range(n - 1):
             assert_spectrum_close(runs[1][i][1][b], ref["state"].L[b], msg=f"traj {i} bond {b}")
+
+
+def test_engines_in_threads_bitwise(gpu):
+    """Lock-step groups on separate engines (streams), driven from separate
+    host threads at once, give bitwise the states that the same groups give
+    one after the other."""
+    import threading
+
+    T, n, depth, chi = 12, 10, 6, 16
+    circuits = [haar_circuit(n, depth, 0.1 + 0.05 * (i % 4), 900 + i) for i in range(T)]
+    groups = [list(range(g, T, 3)) for g in range(3)]
+
+    def run(g, out):
+        eng = Engine(chi)
+        sts = [eng.new_qubits(n, ("const", chi)) for _ in groups[g]]
+        its = [iter(circuits[i].draws) for i in groups[g]]
+        for t in range(depth):
+            eng.batch_bilayer(sts, [circuits[i].layers[t] for i in groups[g]], draws=its)
+        eng.sync()
+        out[g] = [[st.schmidt(b) for b in range(n - 1)] + [st.gamma(k) for k in range(n)]
+                  for st in sts]
+
+    seq = {}
+    for g in range(3):
+        run(g, seq)
+    par = {}
+    th = [threading.Thread(target=run, args=(g, par)) for g in range(3)]
+    for x in th:
+        x.start()
+    for x in th:
+        x.join()
+    for g in range(3):
+        for a, b in zip(seq[g], par[g]):
+            assert all(np.array_equal(u, v) for u, v in zip(a, b))
