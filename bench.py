@@ -272,12 +272,13 @@ def run_serving(args) -> dict:
             reqs.append(Request(rid=len(reqs), n=n, p=p, chi=chi, arrival=t, prec="f32",
                                 gen_seed=args.seed, traj_id=len(reqs), gen_depth=n))
     log(f"{len(reqs)} requests over {dur}s (rate {rate}/s)")
-    srv = Server(chi_max=max(chis), hbm_budget=args.hbm_budget * 1e9, log=log)
-    u0, _ = srv.engine.stats()
+    srv = Server(chi_max=max(chis), hbm_budget=args.hbm_budget * 1e9, log=log,
+                 workers=args.serve_workers)
+    u0, _ = srv.stats()
     t0 = time.perf_counter()
     srv.run(reqs)
     wall = time.perf_counter() - t0
-    u1, _ = srv.engine.stats()
+    u1, _ = srv.stats()
     lat = [r.latency for r in reqs if r.latency is not None]
     return {
         "metric": "trajectories/s + p50/p99 latency (continuous batching)",
@@ -286,7 +287,8 @@ def run_serving(args) -> dict:
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32 contraction, f64 SVD", "data": "synthetic",
         "config": {"workload": f"Poisson arrivals {rate}/s for {dur}s, N in {ns}, p~U[0,0.4], "
-                               f"chi_max in {chis}, depth=N, Haar brickwork + Z measurements",
+                               f"chi_max in {chis}, depth=N, Haar brickwork + Z measurements, "
+                               f"{args.serve_workers} lock-step worker(s)",
                    "circuits": "host-generated" if args.serve_host_circuits
                                else "generated on the device (Philox4x64-10 streams)",
                    "requests": len(reqs), "completed": len(lat),
@@ -323,6 +325,8 @@ def main() -> int:
     ap.add_argument("--serve-n", default="16,32,64,128")
     ap.add_argument("--serve-chi", default="64,128,256")
     ap.add_argument("--hbm-budget", type=float, default=64.0, help="serving: GB of MPS state")
+    ap.add_argument("--serve-workers", type=int, default=4,
+                    help="serving: lock-step loops, one engine, stream and host thread each")
     ap.add_argument("--serve-host-circuits", action="store_true",
                     help="serving: requests carry host-generated circuits")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",

@@ -20,9 +20,16 @@ Requests that have arrived are admitted in arrival order whenever the HBM
 held by resident
 trajectories stays within the budget (slots are counted in bytes of Gamma
 and lambda); finished trajectories leave at once and free their memory.
+
+With ``workers=W > 1`` the server runs W such lock-step loops at once, each
+on its own engine (stream) and host thread; request i of the arrival order
+goes to worker i mod W.  A lock-step tick lasts as long as its slowest
+SVD, so independent loops keep one worker's slow tail from holding back
+the others' trajectories.  The HBM budget is shared.
 """
 from __future__ import annotations
 
+import threading
 import time
 from dataclasses import dataclass, field
 from typing import Callable, List, Optional, Sequence
@@ -77,11 +84,13 @@ class Request:
 
 class Server:
     """Admit, advance in lock step, retire.  ``hbm_budget`` bounds the bytes of
-    the resident trajectories' state; ``max_active`` bounds their number."""
+    the resident trajectories' state; ``max_active`` bounds their number (per
+    worker); ``workers`` lock-step loops run at once, one engine each."""
 
     def __init__(self, chi_max: int = 256, hbm_budget: float = 64e9, max_active: int = 1024,
-                 log: Optional[Callable[[str], None]] = None):
-        self.engine = Engine(chi_max)
+                 log: Optional[Callable[[str], None]] = None, workers: int = 1):
+        self.engines = [Engine(chi_max) for _ in range(max(1, workers))]
+        self.engine = self.engines[0]
         self.chi_max = chi_max
         self.hbm_budget = hbm_budget
         self.max_active = max_active
@@ -89,17 +98,126 @@ class Server:
         self.layers_done = 0
         self.peak_held = 0.0   # largest bytes resident while more than one was active
         self.log = log
+        self._lock = threading.Lock()
+        self._held = 0.0
+        self._nactive = 0
 
+    def stats(self):
+        """(updates, sweeps) summed over the server's engines."""
+        u = s = 0
+        for e in self.engines:
+            a, b = e.stats()
+            u += a
+            s += b
+        return u, s
+
+    # ------------------------------------------------------------ admission
+    def _try_admit(self, r: Request, eng: Engine, active: list, t0: float, clock) -> bool:
+        """Admit r onto eng if the budget allows (always when nothing is
+        resident); under the lock."""
+        need = mps_bytes(r.n, r.chi)
+        if self._nactive and self._held + need > self.hbm_budget:
+            return False
+        st = eng.new_qubits(r.n, ("const", r.chi))
+        if r.prec != "f64":
+            st.set_precision(r.prec)
+        r.admitted = clock() - t0
+        r.entropy = [0.0]  # product state: entropy before layer 1
+        active.append([r, st, iter(r.draws), 0])
+        self._held += need
+        self._nactive += 1
+        if self._nactive > 1:
+            self.peak_held = max(self.peak_held, self._held)
+        return True
+
+    # ----------------------------------------------------------------- tick
+    def _tick(self, eng: Engine, active: list, t0: float, clock) -> list:
+        """Advance every trajectory of ``active`` by one layer on ``eng``;
+        retire finished ones and return the rest."""
+        res = [None] * len(active)
+        carried = [i for i, a in enumerate(active) if a[0].gen_seed is None]
+        if carried:
+            got = eng.batch_bilayer(
+                [active[i][1] for i in carried],
+                [active[i][0].layers[active[i][3]] for i in carried],
+                draws=[active[i][2] for i in carried],
+                ent_bonds=[active[i][0].n // 2 - 1 for i in carried])
+            for i, g in zip(carried, got):
+                res[i] = g
+        for seed in sorted({a[0].gen_seed for a in active if a[0].gen_seed is not None}):
+            idx = [i for i, a in enumerate(active) if a[0].gen_seed == seed]
+            got = eng.batch_bilayer_gen(
+                [active[i][1] for i in idx], seed,
+                [active[i][0].traj_id for i in idx], [active[i][3] for i in idx],
+                [active[i][0].p for i in idx],
+                ent_bonds=[active[i][0].n // 2 - 1 for i in idx])
+            for i, g in zip(idx, got):
+                res[i] = g
+        t_layer = clock() - t0
+        keep = []
+        done = []
+        for a, (rec, prs, ent) in zip(active, res):
+            r = a[0]
+            r.record.append(rec)
+            r.probs.append(prs)
+            r.entropy.append(ent)
+            a[3] += 1
+            if a[3] >= r.depth:
+                r.done = t_layer
+                a[1].free()
+                done.append(r)
+            else:
+                keep.append(a)
+        with self._lock:
+            self.ticks += 1
+            self.layers_done += len(active)
+            for r in done:
+                self._held -= mps_bytes(r.n, r.chi)
+                self._nactive -= 1
+            if self.log and self.ticks % 10 == 0:
+                self.log(f"tick {self.ticks}: {self._nactive} active")
+        if self.log:
+            for r in done:
+                self.log(f"t={r.done:.1f}s request {r.rid} (n={r.n}, chi={r.chi}) done, "
+                         f"latency {r.latency:.2f}s")
+        return keep
+
+    # ------------------------------------------------------------------ run
     def run(self, requests: List[Request], clock: Callable[[], float] = time.perf_counter,
             realtime: bool = True) -> List[Request]:
         """Serve ``requests`` (arrival times relative to the start).  With
         ``realtime`` the server waits for arrivals on the wall clock;
         otherwise every request counts as arrived at its time stamp as soon
-        as the server is idle or at a tick boundary past it."""
+        as the server is idle or at a tick boundary past it (one worker
+        only)."""
         pending = sorted(requests, key=lambda r: (r.arrival, r.rid))
-        active: list = []           # [(req, state, draws iterator, next layer)]
-        held = 0.0
         t0 = clock()
+        if len(self.engines) == 1:
+            self._loop(self.engine, pending, t0, clock, realtime)
+            return requests
+        if not realtime:
+            raise ValueError("more than one worker needs realtime=True")
+        W = len(self.engines)
+        errors: list = []
+
+        def work(w):
+            try:
+                self._loop(self.engines[w], pending[w::W], t0, clock, True)
+            except Exception as exc:  # surfaced after the join
+                errors.append(exc)
+
+        th = [threading.Thread(target=work, args=(w,)) for w in range(W)]
+        for x in th:
+            x.start()
+        for x in th:
+            x.join()
+        if errors:
+            raise errors[0]
+        return requests
+
+    def _loop(self, eng: Engine, pending: list, t0: float, clock, realtime: bool) -> None:
+        """One lock-step loop over ``pending`` (in arrival order) on ``eng``."""
+        active: list = []           # [(req, state, draws iterator, next layer)]
         qi = 0
         while qi < len(pending) or active:
             now = clock() - t0
@@ -109,67 +227,17 @@ class Server:
                     continue
                 now = pending[qi].arrival
             # admit in arrival order while memory and slots allow
-            while (qi < len(pending) and pending[qi].arrival <= now + 1e-12
-                   and len(active) < self.max_active):
-                r = pending[qi]
-                need = mps_bytes(r.n, r.chi)
-                if active and held + need > self.hbm_budget:
-                    break
-                st = self.engine.new_qubits(r.n, ("const", r.chi))
-                if r.prec != "f64":
-                    st.set_precision(r.prec)
-                r.admitted = clock() - t0
-                r.entropy = [0.0]  # product state: entropy before layer 1
-                active.append([r, st, iter(r.draws), 0])
-                held += need
-                if len(active) > 1:
-                    self.peak_held = max(self.peak_held, held)
-                qi += 1
+            with self._lock:
+                while (qi < len(pending) and pending[qi].arrival <= now + 1e-12
+                       and len(active) < self.max_active):
+                    if not self._try_admit(pending[qi], eng, active, t0, clock):
+                        break
+                    qi += 1
             if not active:
+                if realtime and qi < len(pending):
+                    time.sleep(0.005)   # budget held by other workers
                 continue
-            # one lock-step layer for every resident trajectory
-            res = [None] * len(active)
-            carried = [i for i, a in enumerate(active) if a[0].gen_seed is None]
-            if carried:
-                got = self.engine.batch_bilayer(
-                    [active[i][1] for i in carried],
-                    [active[i][0].layers[active[i][3]] for i in carried],
-                    draws=[active[i][2] for i in carried],
-                    ent_bonds=[active[i][0].n // 2 - 1 for i in carried])
-                for i, g in zip(carried, got):
-                    res[i] = g
-            for seed in sorted({a[0].gen_seed for a in active if a[0].gen_seed is not None}):
-                idx = [i for i, a in enumerate(active) if a[0].gen_seed == seed]
-                got = self.engine.batch_bilayer_gen(
-                    [active[i][1] for i in idx], seed,
-                    [active[i][0].traj_id for i in idx], [active[i][3] for i in idx],
-                    [active[i][0].p for i in idx],
-                    ent_bonds=[active[i][0].n // 2 - 1 for i in idx])
-                for i, g in zip(idx, got):
-                    res[i] = g
-            self.ticks += 1
-            self.layers_done += len(active)
-            if self.log and self.ticks % 10 == 0:
-                self.log(f"tick {self.ticks}: {len(active)} active, {len(pending) - qi} queued")
-            t_layer = clock() - t0
-            keep = []
-            for a, (rec, prs, ent) in zip(active, res):
-                r = a[0]
-                r.record.append(rec)
-                r.probs.append(prs)
-                r.entropy.append(ent)
-                a[3] += 1
-                if a[3] >= r.depth:
-                    r.done = t_layer
-                    held -= mps_bytes(r.n, r.chi)
-                    a[1].free()
-                    if self.log:
-                        self.log(f"t={t_layer:.1f}s request {r.rid} (n={r.n}, chi={r.chi}) done, "
-                                 f"latency {r.latency:.2f}s")
-                else:
-                    keep.append(a)
-            active = keep
-        return requests
+            active = self._tick(eng, active, t0, clock)
 
 
 def percentile(xs: Sequence[float], q: float) -> float:

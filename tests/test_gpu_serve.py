@@ -91,3 +91,36 @@ def test_serving_generated_equals_alone(gpu):
             ent.append(e)
         for a, b in zip(ent, r.entropy):
             assert close(a, b, rtol=1e-12)
+
+
+def test_serving_workers_equal_alone(gpu):
+    """Two lock-step workers (engines and host threads) sharing one budget:
+    every request, carried or generated on the device, gives the results it
+    gives alone."""
+    reqs = _requests()
+    for r in reqs:
+        r.arrival *= 0.05   # real time: arrivals within 0.4 s
+    seed = 777
+    for i, (n, p, chi, t) in enumerate([(10, 0.2, 16, 0.0), (12, 0.3, 8, 0.1)]):
+        reqs.append(Request(rid=20 + i, n=n, p=p, chi=chi, arrival=t, gen_seed=seed,
+                            traj_id=200 + i, gen_depth=n))
+    budget = sum(mps_bytes(r.n, r.chi) for r in reqs[:4])
+    srv = Server(chi_max=32, hbm_budget=budget, workers=2)
+    srv.run(reqs)
+    assert srv.peak_held <= budget
+    assert srv.stats()[0] > 0
+    for r in reqs:
+        assert r.done is not None and len(r.record) == r.depth
+        if r.gen_seed is None:
+            solo = MPS.new_qubits(r.n, ("const", r.chi))
+            got = solo.run_circuit(r.layers, rs=iter(r.draws))
+            assert got["record"] == r.record
+            for a, b in zip(got["entropy"], r.entropy):
+                assert close(a, b, rtol=1e-12), (r.rid, a, b)
+        else:
+            eng = Engine(r.chi)
+            st = eng.new_qubits(r.n, ("const", r.chi))
+            for t in range(r.depth):
+                (rec, prs, e), = eng.batch_bilayer_gen([st], seed, [r.traj_id], [t], [r.p])
+                assert rec == r.record[t]
+                assert close(e, r.entropy[t + 1], rtol=1e-12)
