@@ -96,13 +96,19 @@ def run_batched(args) -> dict:
 
     n, depth, chi, T = args.n or 32, args.depth or 32, args.chi or 128, args.traj or 1024
     W, K, G = args.warmup, args.steps, max(1, args.groups)
-    pgrid = np.linspace(0.05, 0.30, 11)
+    lo, hi, npts = args.pgrid.split(",")
+    pgrid = np.linspace(float(lo), float(hi), int(npts))
     t_gen = time.perf_counter()
-    circs = [haar_circuit_fast(n, depth, float(pgrid[i % 11]), args.seed + 7 * i) for i in range(T)]
+    circs = [haar_circuit_fast(n, depth, float(pgrid[i % len(pgrid)]), args.seed + 7 * i)
+             for i in range(T)]
     log(f"generated {T} circuits in {time.perf_counter() - t_gen:.1f}s")
     groups = [list(range(g, T, G)) for g in range(G)]
     engs = [Engine(chi) for _ in range(G)]
     states = [[engs[g].new_qubits(n, ("const", chi)) for _ in idx] for g, idx in enumerate(groups)]
+    if args.prec != "f64":
+        for grp in states:
+            for st in grp:
+                st.set_precision(args.prec)
     draws = [[iter(circs[i].draws) for i in idx] for idx in groups]
     ents = [[0.0] * len(idx) for idx in groups]
     errors = []
@@ -154,9 +160,10 @@ def run_batched(args) -> dict:
         "metric": "two-site updates/s (batched lock-step trajectories)",
         "value": round(updates / wall, 2), "unit": "two-site updates/s", "n_gpus": 1,
         "steps": K, "warmup": W, "ms_per_step": round(wall / K * 1e3, 2),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if args.prec == "f64" else f"{args.prec} contraction, f64 SVD",
         "data": "synthetic",
-        "config": {"workload": f"{T} trajectories N={n} depth={depth} p in linspace(0.05,0.30,11) "
+        "config": {"workload": f"{T} trajectories N={n} depth={depth} p in linspace({lo},{hi},{npts}) "
                                f"Const({chi}), {G} lock-step group(s) on {G} stream(s); "
                                f"step = one layer of every trajectory",
                    "trajectories": T, "groups": G, "n_qubits": n, "chi_max": chi,
@@ -317,6 +324,10 @@ def main() -> int:
                     choices=["single", "batched", "mixed", "serving"])
     ap.add_argument("--traj", type=int, default=0, help="trajectories (batched, mixed)")
     ap.add_argument("--depth", type=int, default=0, help="circuit depth (batched, mixed)")
+    ap.add_argument("--pgrid", default="0.05,0.30,11",
+                    help="batched: measurement rates linspace(lo,hi,count), dealt round-robin")
+    ap.add_argument("--prec", default="f64", choices=["f64", "f32"],
+                    help="batched: contraction precision (the SVD is FP64 either way)")
     ap.add_argument("--groups", type=int, default=4,
                     help="batched: lock-step groups, one engine, stream and host thread each "
                          "(4 = the hardware queues HIP gives a process)")
