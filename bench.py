@@ -179,7 +179,26 @@ def run_batched(args) -> dict:
                      "busiest_stream_jacobi_s": round(jac_s, 3)},
         "breakdown": {k: round(perf[k], 2) for k in ("ms_theta", "ms_jacobi", "ms_post",
                                                      "ms_measure", "ms_host_launch")},
+        "memory_kernels": memory_kernels(perf, G),
     }
+
+
+def memory_kernels(perf: dict, streams: int) -> dict:
+    """Achieved HBM rate of the memory-bound kernels: algorithmic bytes (each
+    Gamma element read once and written once, counted by the library from
+    every update's m, n and rank and every measured site's chiL, chiR) over
+    the kernels' summed HIP-event spans.  With several streams the spans
+    overlap other streams' work, so the rates are lower bounds."""
+    out = {"peak_GBps": HBM_PEAK_GBS, "streams": streams}
+    for name, ms_key, b_key in (("k_split", "ms_split", "bytes_split"),
+                                ("k_renorm", "ms_renorm", "bytes_renorm"),
+                                ("k_measure", "ms_measure", "bytes_measure")):
+        ms, b = perf[ms_key], perf[b_key]
+        gbs = b / (ms / 1e3) / 1e9 if ms > 0 else 0.0
+        out[name] = {"bytes": b, "ms": round(ms, 3), "GBps": round(gbs, 1),
+                     "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    out["k_measure"]["sites"] = perf["measure_sites"]
+    return out
 
 
 def run_mixed(args) -> dict:
@@ -455,6 +474,7 @@ def main() -> int:
         "roofline": roof,
         "cpu_baseline": cpu,
         "breakdown": breakdown,
+        "memory_kernels": memory_kernels(perf, 1),
     }
     print(json.dumps(out), flush=True)
     return 0

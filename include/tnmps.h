@@ -284,6 +284,17 @@ typedef struct {
     size_t jacobi_launches; /* Jacobi step kernels launched */
     double ms_host_launch;  /* host time spent enqueueing Jacobi step kernels */
     size_t factored;        /* updates that decomposed one factor of theta only */
+    /* The memory-bound kernels: event spans and algorithmic bytes (each
+     * Gamma element read once and written once; theta of m x n, new rank r):
+     * split reads the factors and writes Gamma[k], Gamma[k+1] (32 r (m + n)
+     * B), renormalise reads and writes both (32 r (m + n) B), a measurement
+     * reads Gamma[k] (32 chiL chiR B) and, when it projects, writes it. */
+    double ms_split;
+    double ms_renorm;
+    double bytes_split;
+    double bytes_renorm;
+    double bytes_measure;
+    size_t measure_sites;
 } tn_perf;
 
 /* Turn device-time accounting on (1) or off (0).  On adds one stream

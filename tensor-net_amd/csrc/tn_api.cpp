@@ -152,11 +152,15 @@ using EntReq = EntDesc;
 // they have completed, or when the counters are asked for), so timing adds
 // no host synchronisation to the stream.
 struct PerfRec {
-  hipEvent_t ev[6] = {};  // theta begin/end, sweeps begin/end, post begin/end
-  int* h_info = nullptr;  // pinned copy of the batch's info
-  int cap = 0;            // tasks h_info holds
+  // theta begin/end, sweeps begin/end, post begin/end, split begin, renorm begin
+  hipEvent_t ev[8] = {};
+  int* h_info = nullptr;  // pinned copy of the batch's info (update batch) or the
+                          // measured sites' chiL, chiR (measurement launch)
+  int cap = 0;            // tasks (or sites) h_info holds
+  std::vector<char> proj; // measurement launch: the site is projected
   std::vector<char> gate; // has_gate per task
-  int kind = 0;           // 0 update batch, 1 measurement launch (ev[0], ev[1])
+  int kind = 0;           // 0 update batch, 1 measurement launch (ev[0], ev[1];
+                          // ev[2] once the sites' dimensions are on the host)
   int nt = 0, n_updates = 0, sweeps = 0;
   size_t launches = 0;
   bool pending = false;
@@ -202,6 +206,8 @@ struct tn_engine {
   double* d_prob = nullptr;
   double* d_probs = nullptr;
   int* d_flag = nullptr;
+  int* d_mdims = nullptr;      // measured sites' chiL, chiR (timing only)
+  int n_mdims = 0;
   double* d_scalar = nullptr;  // scalar results (entropies)
   int nscalar = 0;
   double2* gen_u = nullptr;    // generated gates, 16 entries each
@@ -232,8 +238,9 @@ struct tn_engine {
     slots = 0;
   }
   void release_res() {
-    hfree(d_out); hfree(d_prob); hfree(d_probs); hfree(d_flag);
-    d_out = nullptr; d_prob = d_probs = nullptr; d_flag = nullptr;
+    hfree(d_out); hfree(d_prob); hfree(d_probs); hfree(d_flag); hfree(d_mdims);
+    d_out = nullptr; d_prob = d_probs = nullptr; d_flag = nullptr; d_mdims = nullptr;
+    n_mdims = 0;
     mres = 0;
   }
   void release() {
@@ -306,10 +313,17 @@ struct tn_engine {
   void perf_process(PerfRec& r) {
     if (r.kind == 1) {
       perf.ms_measure += span(r, 0, 1);
+      for (int i = 0; i < r.nt; ++i) {
+        const double cells = (double)r.h_info[2 * i] * (double)r.h_info[2 * i + 1];
+        perf.bytes_measure += 32.0 * cells * (r.proj[i] ? 2.0 : 1.0);
+      }
+      perf.measure_sites += (size_t)r.nt;
     } else {
       perf.ms_theta += span(r, 0, 1);
       perf.ms_jacobi += span(r, 2, 3);
       perf.ms_post += span(r, 4, 5);
+      perf.ms_split += span(r, 6, 7);
+      perf.ms_renorm += span(r, 7, 5);
       perf.jacobi_launches += r.launches;
       perf.updates += r.n_updates;
       perf.sweeps += r.sweeps;
@@ -326,6 +340,8 @@ struct tn_engine {
         const double fpre = mode == 0 ? 8.0 * m * n * K + (r.gate[t] ? 32.0 * m * n : 0.0)
                                       : 8.0 * (mode == 1 ? m : n) * K * rank;
         perf.alg_flops += fpre + fsvd;
+        perf.bytes_split += 32.0 * rank * (m + n);
+        perf.bytes_renorm += 32.0 * rank * (m + n);
       }
     }
     r.pending = false;
@@ -337,7 +353,7 @@ struct tn_engine {
     for (int i = 0; i < kPerfRing; ++i) {
       PerfRec& r = prec[(prec_next + i) % kPerfRing];
       if (!r.pending) continue;
-      const hipEvent_t last = r.ev[r.kind == 1 ? 1 : 5];
+      const hipEvent_t last = r.ev[r.kind == 1 ? 2 : 5];
       if (block) {
         TN_HIP(hipEventSynchronize(last));
       } else if (hipEventQuery(last) != hipSuccess) {
@@ -352,10 +368,10 @@ struct tn_engine {
   tn_status perf_begin(int kind, int nt, PerfRec** out) {
     PerfRec& r = prec[prec_next];
     if (r.pending) {
-      TN_HIP(hipEventSynchronize(r.ev[r.kind == 1 ? 1 : 5]));
+      TN_HIP(hipEventSynchronize(r.ev[r.kind == 1 ? 2 : 5]));
       perf_process(r);
     }
-    if (kind == 0 && nt > r.cap) {
+    if (nt > r.cap) {
       hhfree(r.h_info);
       r.h_info = nullptr;
       r.cap = 0;
@@ -389,6 +405,18 @@ struct tn_engine {
     TN_HIP(hipMalloc(&act, sizeof(int) * (ns + 1)));
     slots = ns;
     return upd.ensure(ns, stream);
+  }
+
+  tn_status ensure_mdims(int need) {
+    if (need <= n_mdims) return TN_OK;
+    TN_HIP(hipStreamSynchronize(stream));
+    hfree(d_mdims);
+    d_mdims = nullptr;
+    n_mdims = 0;
+    const int ns = std::max(need, 256);
+    TN_HIP(hipMalloc(&d_mdims, sizeof(int) * 2 * ns));
+    n_mdims = ns;
+    return TN_OK;
   }
 
   tn_status ensure_results(int need) {
@@ -709,7 +737,9 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
   }
   launch_finish(dd, nt, sweeps_run, e->stream);
   if (any_factored) launch_fmul(dd, nt, (int)maxR, (int)maxC, e->stream);
+  if (pr) TN_HIP(hipEventRecord(pr->ev[6], e->stream));
   launch_split(dd, nt, e->stream);
+  if (pr) TN_HIP(hipEventRecord(pr->ev[7], e->stream));
   launch_renorm(dd, nt, e->stream);
   TN_HIP(hipGetLastError());
   int n_updates = 0;
@@ -799,16 +829,23 @@ tn_status run_measures(tn_engine* e, const std::vector<MTask>& ms) {
     d.out_probs = e->d_probs;
     d.out_flag = e->d_flag;
   }
+  if (e->timing) TN_TRY(e->ensure_mdims(nm));
+  for (int i = 0; i < nm && e->timing; ++i) hd[i].out_dims = e->d_mdims;
   TN_TRY(e->meas.upload(rs, nm, e->stream));
   PerfRec* pr = nullptr;
   if (e->timing) {
     TN_TRY(e->perf_begin(1, nm, &pr));
+    pr->proj.resize(nm);
+    for (int i = 0; i < nm; ++i) pr->proj[i] = ms[i].forced != -2 ? 1 : 0;
     TN_HIP(hipEventRecord(pr->ev[0], e->stream));
   }
   launch_measure(e->meas.d[rs], nm, e->stream);
   TN_HIP(hipGetLastError());
   if (pr) {
     TN_HIP(hipEventRecord(pr->ev[1], e->stream));
+    TN_HIP(hipMemcpyAsync(pr->h_info, e->d_mdims, sizeof(int) * 2 * nm, hipMemcpyDeviceToHost,
+                          e->stream));
+    TN_HIP(hipEventRecord(pr->ev[2], e->stream));  // the dims have arrived
     pr->pending = true;
     TN_TRY(e->perf_drain(false));
   }

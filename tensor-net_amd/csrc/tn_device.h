@@ -123,6 +123,7 @@ struct MeasDesc {
   double* out_prob;      // outcome probability
   double* out_probs;     // both probabilities, 2 per slot
   int* out_flag;         // 1 zero probability, 2 draw out of range
+  int* out_dims;         // or null: chiL, chiR of descriptor i at out_dims[2 i]
 };
 
 }  // namespace tnmps

@@ -454,10 +454,23 @@ __global__ __launch_bounds__(256) void k_renorm(const UpdDesc* __restrict__ desc
   const int chiL = d.info[kInfoChiL], chiR = d.info[kInfoChiR];
   const int rank = d.info[kInfoRank];
   __shared__ double nrm[2];
-  if (threadIdx.x < 2) {
-    double s = 0.0;
-    for (int b = 0; b < (int)gridDim.x; ++b) s += d.part[2 * b + threadIdx.x];
-    nrm[threadIdx.x] = sqrt(s);
+  if (threadIdx.x < 64) {
+    // the split kernel's per-block partial norms, one block per lane, summed
+    // across the wave in a fixed order
+    double a = 0.0, b = 0.0;
+    for (int q = threadIdx.x; q < (int)gridDim.x; q += 64) {
+      a += d.part[2 * q + 0];
+      b += d.part[2 * q + 1];
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      a += __shfl_xor(a, off, 64);
+      b += __shfl_xor(b, off, 64);
+    }
+    if (threadIdx.x == 0) {
+      nrm[0] = sqrt(a);
+      nrm[1] = sqrt(b);
+    }
   }
   __syncthreads();
   const double nA = nrm[0], nB = nrm[1];
@@ -513,6 +526,10 @@ __global__ __launch_bounds__(1024) void k_measure(const MeasDesc* __restrict__ d
     d.out_probs[2 * d.slot + 0] = p0;
     d.out_probs[2 * d.slot + 1] = p1;
     d.out_flag[d.slot] = flag;
+    if (d.out_dims) {
+      d.out_dims[2 * blockIdx.x + 0] = chiL;
+      d.out_dims[2 * blockIdx.x + 1] = chiR;
+    }
   }
   __syncthreads();
   if (d.forced == -2) return;

@@ -41,7 +41,9 @@ class tn_perf(C.Structure):
                 ("alg_flops", C.c_double), ("svd_flops", C.c_double),
                 ("updates", C.c_size_t), ("sweeps", C.c_size_t),
                 ("jacobi_launches", C.c_size_t), ("ms_host_launch", C.c_double),
-                ("factored", C.c_size_t)]
+                ("factored", C.c_size_t), ("ms_split", C.c_double), ("ms_renorm", C.c_double),
+                ("bytes_split", C.c_double), ("bytes_renorm", C.c_double),
+                ("bytes_measure", C.c_double), ("measure_sites", C.c_size_t)]
 
 
 _sz = C.c_size_t
