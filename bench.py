@@ -214,7 +214,7 @@ def run_mixed(args) -> dict:
     circs = [haar_circuit_fast(n, depth, 0.15, args.seed + 11 * i) for i in range(T)]
     out = {}
     records = None
-    for prec in ("f64", "f32"):
+    for prec in ("f64", "f32", "bf16x3"):
         eng = Engine(chi)
         states = [eng.new_qubits(n, ("const", chi)) for _ in range(T)]
         for s in states:
@@ -245,15 +245,19 @@ def run_mixed(args) -> dict:
         if records is None:
             records = recs
         del states, eng
-    e64, e32 = out["f64"]["ents"], out["f32"]["ents"]
-    ent_err = np.max(np.abs(e32 - e64), axis=1)
-    lam_err = []
-    for t in range(depth):
-        m = 0.0
-        for a, b in zip(out["f32"]["lams"][t], out["f64"]["lams"][t]):
-            k = min(len(a), len(b))
-            m = max(m, float(np.max(np.abs(a[:k] - b[:k]))) if k else 0.0)
-        lam_err.append(m)
+    def errors(prec):
+        ent_err = np.max(np.abs(out[prec]["ents"] - out["f64"]["ents"]), axis=1)
+        lam_err = []
+        for t in range(depth):
+            m = 0.0
+            for a, b in zip(out[prec]["lams"][t], out["f64"]["lams"][t]):
+                k = min(len(a), len(b))
+                m = max(m, float(np.max(np.abs(a[:k] - b[:k]))) if k else 0.0)
+            lam_err.append(m)
+        return ent_err, lam_err
+
+    ent_err, lam_err = errors("f32")
+    bf_ent, bf_lam = errors("bf16x3")
     return {
         "metric": "two-site updates/s, FP32 contraction vs FP64 (mixed precision)",
         "value": round(out["f32"]["rate"], 3), "unit": "two-site updates/s", "n_gpus": 1,
@@ -265,7 +269,12 @@ def run_mixed(args) -> dict:
                    "fp64_updates_per_s": round(out["f64"]["rate"], 3),
                    "entropy_abs_err_per_layer": [float(f"{x:.3e}") for x in ent_err],
                    "schmidt_abs_err_per_layer": [float(f"{x:.3e}") for x in lam_err],
-                   "max_entropy_err": float(f"{float(np.max(ent_err)):.3e}")},
+                   "max_entropy_err": float(f"{float(np.max(ent_err)):.3e}"),
+                   "bf16x3_updates_per_s": round(out["bf16x3"]["rate"], 3),
+                   "bf16x3_entropy_abs_err_per_layer": [float(f"{x:.3e}") for x in bf_ent],
+                   "bf16x3_schmidt_abs_err_per_layer": [float(f"{x:.3e}") for x in bf_lam],
+                   "bf16x3_max_entropy_err": float(f"{float(np.max(bf_ent)):.3e}"),
+                   "ms_theta": {p: round(out[p]["perf"]["ms_theta"], 2) for p in out}},
         "roofline": None,
     }
 
@@ -345,7 +354,7 @@ def main() -> int:
     ap.add_argument("--depth", type=int, default=0, help="circuit depth (batched, mixed)")
     ap.add_argument("--pgrid", default="0.05,0.30,11",
                     help="batched: measurement rates linspace(lo,hi,count), dealt round-robin")
-    ap.add_argument("--prec", default="f64", choices=["f64", "f32"],
+    ap.add_argument("--prec", default="f64", choices=["f64", "f32", "bf16x3"],
                     help="batched: contraction precision (the SVD is FP64 either way)")
     ap.add_argument("--groups", type=int, default=4,
                     help="batched: lock-step groups, one engine, stream and host thread each "

@@ -68,7 +68,21 @@ enum { TN_TRUNC_NONE = 0, TN_TRUNC_CONST = 1, TN_TRUNC_CUTOFF = 2 };
 /* Arithmetic of the two-site contraction theta = U (lambda Gamma lambda Gamma
  * lambda).  The SVD, truncation, lambda renormalisation and every stored
  * tensor stay FP64 in both modes. */
-enum { TN_PREC_F64 = 0, TN_PREC_F32 = 1 };
+enum { TN_PREC_F64 = 0, TN_PREC_F32 = 1, TN_PREC_BF16X3 = 2 };
+
+/* One row of an engine's launch-shape table.  An update batch whose largest
+ * SVD column count C (min(m, n) of theta, or the middle bond K of a factored
+ * update) is at most c_max uses the first such row; the last row also takes
+ * larger C.  No counterpart in mps.rs: these choose launch shapes only, and
+ * every choice computes the same SVD to rounding. */
+typedef struct {
+    int c_max;       /* bucket bound on C, rows in increasing order */
+    int theta_tile;  /* 32 or 64: theta tile edge on the matrix cores */
+    int bw;          /* 8 or 16: Jacobi block width (columns) */
+    int split_max;   /* 1..8: most workgroups per block pair (row-split steps) */
+    int split_rows;  /* >= 64: least X rows per row-split workgroup */
+    int ahead;       /* 1..3: sweeps enqueued ahead while far from convergence */
+} tn_tune;
 
 /* Human-readable name of a status code (static string). */
 const char* tn_status_str(tn_status s);
@@ -103,6 +117,15 @@ tn_status tn_mps_new_qubits_in(tn_engine* e, size_t n, tn_trunc trunc, size_t ch
                                tn_mps** out);
 /* Contraction arithmetic of this handle (TN_PREC_F64 default). */
 tn_status tn_set_precision(tn_mps* h, int prec);
+/* Replace the engine's launch-shape table (1..16 rows, c_max increasing,
+ * fields in their ranges; TN_INVALID_ARG otherwise, table unchanged).  The
+ * default table comes from the TNMPS_* environment knobs.  Takes effect from
+ * the next update batch; the engine's caller holds it (one caller at a time). */
+tn_status tn_engine_set_tuning(tn_engine* e, const tn_tune* rows, size_t count);
+/* Copy the engine's table into rows[0 .. cap); *count = its row count. */
+tn_status tn_engine_get_tuning(tn_engine* e, tn_tune* rows, size_t cap, size_t* count);
+/* The same for the engine of a handle. */
+tn_status tn_set_tuning(tn_mps* h, const tn_tune* rows, size_t count);
 /* MPS::clone (#[derive(Clone)], mps.rs:337): deep copy on the same stream. */
 tn_status tn_mps_clone(const tn_mps* h, tn_mps** out);
 size_t tn_mps_n(const tn_mps* h);

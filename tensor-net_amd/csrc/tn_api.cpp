@@ -195,6 +195,7 @@ struct tn_engine {
   int split_max = kMaxSplit;  // cap on workgroups per block pair (TNMPS_SPLIT_MAX)
   int split_rows = 128;       // least X rows per row-split workgroup (TNMPS_SPLIT_ROWS)
   int bw_env = 0;             // forced Jacobi block width (TNMPS_BW), 0 = automatic
+  std::vector<tn_tune> tune;  // launch shapes per C bucket (tn_engine_set_tuning)
   bool factored = true;       // factored updates where a factor is isometric (TNMPS_FACTORED)
   Tolerances tol;             // Jacobi tolerances and inner-solve options
   bool trace_sweeps = false;  // print each sweep's largest off-diagonal (TNMPS_TRACE_SWEEPS)
@@ -225,6 +226,13 @@ struct tn_engine {
   bool auto_free = false;      // freed with its last handle
 
   ~tn_engine() { release(); }
+
+  // the first row whose bucket holds C, else the last row
+  const tn_tune& tune_for(size_t C) const {
+    for (const tn_tune& r : tune)
+      if ((size_t)r.c_max >= C) return r;
+    return tune.back();
+  }
 
   void release_ws() {
     hfree(X); hfree(W); hfree(sraw); hfree(sval); hfree(fac); hfree(perm);
@@ -293,6 +301,19 @@ struct tn_engine {
     }
     gpart_per = gp;
     bw_env = (int)env_double("TNMPS_BW", 0);
+    // default table: one row per bucket; tiles of 32 keep small thetas on
+    // more workgroups, 64 halves the LDS traffic per flop of large ones
+    tune.clear();
+    for (int cm : {16, 32, 64, 128, 256, 512, 1024}) {
+      tn_tune r{};
+      r.c_max = cm;
+      r.theta_tile = (int)env_double("TNMPS_THETA_TILE", cm <= 128 ? 32 : 64) == 32 ? 32 : 64;
+      r.bw = bw_env == kBWWide ? kBWWide : kBW;
+      r.split_max = split_max;
+      r.split_rows = split_rows;
+      r.ahead = std::max(1, std::min(3, (int)env_double("TNMPS_AHEAD", 2)));
+      tune.push_back(r);
+    }
     factored = env_double("TNMPS_FACTORED", 1) != 0;
     tol.read_env();
     trace_sweeps = env_double("TNMPS_TRACE_SWEEPS", 0) != 0;
@@ -564,7 +585,7 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
   UpdDesc* hd = nullptr;
   int rs = 0;
   TN_TRY(e->upd.next(&hd, &rs));
-  size_t maxC = 1, maxR = 1, maxTiles = 1, maxCC = 1;
+  size_t maxC = 1, maxR = 1, maxTiles = 1, maxCC = 1, maxChiL = 1, maxChiR = 1;
   bool any_factored = false;
   int mode0 = 0;  // first task's mode, for the sweep trace
   for (int t = 0; t < nt; ++t) {
@@ -613,8 +634,12 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
     maxC = std::max(maxC, C);
     maxR = std::max(maxR, std::max(2 * chiL, 2 * chiR));
     maxCC = std::max(maxCC, C * C);
-    maxTiles = std::max(maxTiles, ((chiL + 3) / 4) * ((chiR + 63) / 64));
+    maxChiL = std::max(maxChiL, chiL);
+    maxChiR = std::max(maxChiR, chiR);
   }
+  const tn_tune& tu = e->tune_for(maxC);
+  const size_t tv = (size_t)tu.theta_tile / 2;
+  maxTiles = std::max<size_t>(1, ((maxChiL + tv - 1) / tv) * ((maxChiR + tv - 1) / tv));
   TN_TRY(e->upd.upload(rs, nt, e->stream));
   const UpdDesc* dd = e->upd.d[rs];
   PerfRec* pr = nullptr;
@@ -622,16 +647,18 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
     TN_TRY(e->perf_begin(0, nt, &pr));
     TN_HIP(hipEventRecord(pr->ev[0], e->stream));
   }
-  const int gx_theta = (int)std::min<size_t>(maxTiles, 1024);
+  // factored tasks fill X elementwise over the same grid
+  const int gx_theta = (int)std::min<size_t>(std::max<size_t>(maxTiles, any_factored ? 256 : 1),
+                                             1024);
   const int gx_w = (int)std::min<size_t>((maxCC + 255) / 256, 1024);
-  launch_theta(dd, nt, gx_theta, e->stream);
+  launch_theta(dd, nt, tu.theta_tile, gx_theta, e->stream);
   launch_winit(dd, nt, gx_w, e->stream);
   TN_HIP(hipGetLastError());
   if (pr) TN_HIP(hipEventRecord(pr->ev[1], e->stream));
-  // Block width 8 (measured faster than 16 even for a lone 512-column SVD:
-  // the 32x32 inner solve costs more than the halved step count saves);
-  // TNMPS_BW=16 selects the wide panels.
-  const int bw = (e->bw_env == kBWWide) ? kBWWide : kBW;
+  // Block width 8 by default (measured faster than 16 even for a lone
+  // 512-column SVD: the 32x32 inner solve costs more than the halved step
+  // count saves); the tuning table may select the wide panels per bucket.
+  const int bw = tu.bw;
   const int nb = (int)((maxC + bw - 1) / bw);
   const int nbe = nb + (nb & 1);
   const int npairs = nbe / 2, steps = nbe - 1;
@@ -642,8 +669,8 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
   // CUs idle: split each pair's rows over S workgroups (gram kernel, then
   // solve + apply kernel).
   int S = 1;
-  while (S < e->split_max && (size_t)nt * npairs * S * 2 <= (size_t)2 * e->cus &&
-         maxR >= (size_t)e->split_rows * S)
+  while (S < tu.split_max && (size_t)nt * npairs * S * 2 <= (size_t)2 * e->cus &&
+         maxR >= (size_t)tu.split_rows * S)
     S *= 2;
   JacobiArgs ja{};
   ja.X = e->X;
@@ -686,7 +713,7 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
   bool done = false;
   if (pr) TN_HIP(hipEventRecord(pr->ev[2], e->stream));
   while (!done) {
-    const int depth = last_worst > kAheadOff ? 2 : 1;
+    const int depth = last_worst > kAheadOff ? tu.ahead : 1;
     const auto h0 = std::chrono::steady_clock::now();
     while (launched < kMaxSweeps && launched - checked < depth) {
       if (use_list) launch_active(ja, launched, e->act, e->stream);
@@ -1290,8 +1317,36 @@ tn_status tn_set_trunc(tn_mps* h, tn_trunc trunc) {
   return TN_OK;
 }
 
+tn_status tn_engine_set_tuning(tn_engine* e, const tn_tune* rows, size_t count) {
+  if (!e || !rows || count == 0 || count > 16) return TN_INVALID_ARG;
+  for (size_t i = 0; i < count; ++i) {
+    const tn_tune& r = rows[i];
+    if (r.c_max < 1 || (i > 0 && r.c_max <= rows[i - 1].c_max)) return TN_INVALID_ARG;
+    if (r.theta_tile != 32 && r.theta_tile != 64) return TN_INVALID_ARG;
+    if (r.bw != kBW && r.bw != kBWWide) return TN_INVALID_ARG;
+    if (r.split_max < 1 || r.split_max > kMaxSplit) return TN_INVALID_ARG;
+    if (r.split_rows < 64) return TN_INVALID_ARG;
+    if (r.ahead < 1 || r.ahead > 3) return TN_INVALID_ARG;
+  }
+  e->tune.assign(rows, rows + count);
+  return TN_OK;
+}
+
+tn_status tn_engine_get_tuning(tn_engine* e, tn_tune* rows, size_t cap, size_t* count) {
+  if (!e || !count || (cap > 0 && !rows)) return TN_INVALID_ARG;
+  *count = e->tune.size();
+  for (size_t i = 0; i < cap && i < e->tune.size(); ++i) rows[i] = e->tune[i];
+  return TN_OK;
+}
+
+tn_status tn_set_tuning(tn_mps* h, const tn_tune* rows, size_t count) {
+  if (!h) return TN_INVALID_ARG;
+  return tn_engine_set_tuning(h->eng, rows, count);
+}
+
 tn_status tn_set_precision(tn_mps* h, int prec) {
-  if (!h || (prec != TN_PREC_F64 && prec != TN_PREC_F32)) return TN_INVALID_ARG;
+  if (!h || (prec != TN_PREC_F64 && prec != TN_PREC_F32 && prec != TN_PREC_BF16X3))
+    return TN_INVALID_ARG;
   h->prec = prec;
   return TN_OK;
 }

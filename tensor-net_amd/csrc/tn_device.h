@@ -68,7 +68,7 @@ struct UpdDesc {
   int ldx, ldw;
   int has_gate;
   int mode;              // requested: 0 full, 1 left factor isometric, 2 right
-  int prec;              // 0: FP64 contraction, 1: FP32 contraction
+  int prec;              // contraction: 0 FP64, 1 FP32, 2 BF16 split (FP32 accumulation)
   int tkind;             // 0 None, 1 Const, 2 Cutoff (BondDim, mps.rs:157-166)
   int tchi;
   int cap_rank;          // allocation bound on the new bond dimension

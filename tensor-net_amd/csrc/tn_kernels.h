@@ -6,7 +6,9 @@
 
 namespace tnmps {
 
-void launch_theta(const UpdDesc* descs, int nt, int gx, hipStream_t s);
+// theta on the matrix cores in tm x tm tiles (tm = 32 or 64); gx workgroups
+// per task loop over the task's ceil(chiL / (tm/2)) * ceil(chiR / (tm/2)) tiles.
+void launch_theta(const UpdDesc* descs, int nt, int tm, int gx, hipStream_t s);
 void launch_winit(const UpdDesc* descs, int nt, int gx, hipStream_t s);
 // Jacobi step with block width bw (8 or 16): one workgroup per block pair
 // and task (grid rows gy = a.nt), or, with a.act set, per block pair and

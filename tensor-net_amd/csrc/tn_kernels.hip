@@ -34,68 +34,6 @@
 namespace tnmps {
 
 // ---------------------------------------------------------------- theta
-template <typename C2, typename R>
-__device__ __forceinline__ C2 tc(double2 a) {
-  C2 r;
-  r.x = (R)a.x;
-  r.y = (R)a.y;
-  return r;
-}
-template <typename C2, typename R>
-__device__ __forceinline__ C2 tmad(C2 acc, C2 a, C2 b) {
-  acc.x += a.x * b.x - a.y * b.y;
-  acc.y += a.x * b.y + a.y * b.x;
-  return acc;
-}
-
-// All four (s,t) values of theta'[v,(s,t),w] in arithmetic C2/R (FP64 or
-// FP32 with FP32 accumulation); the result is widened to double.
-template <typename C2, typename R>
-__device__ __forceinline__ void theta_point(const UpdDesc& d, int v, int w, int chiM, int chiR,
-                                            double2 out[4]) {
-  C2 acc[4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i) acc[i].x = acc[i].y = (R)0;
-  const double2* __restrict__ a0 = d.gk + (size_t)(v * 2 + 0) * chiM;
-  const double2* __restrict__ a1 = d.gk + (size_t)(v * 2 + 1) * chiM;
-  for (int u = 0; u < chiM; ++u) {
-    const R lm = (R)d.lamM[u];
-    C2 b0 = tc<C2, R>(d.gk1[(size_t)(u * 2 + 0) * chiR + w]);
-    C2 b1 = tc<C2, R>(d.gk1[(size_t)(u * 2 + 1) * chiR + w]);
-    b0.x *= lm; b0.y *= lm;
-    b1.x *= lm; b1.y *= lm;
-    const C2 x0 = tc<C2, R>(a0[u]), x1 = tc<C2, R>(a1[u]);
-    acc[0] = tmad<C2, R>(acc[0], x0, b0);  // s'=0 t'=0
-    acc[1] = tmad<C2, R>(acc[1], x0, b1);  // s'=0 t'=1
-    acc[2] = tmad<C2, R>(acc[2], x1, b0);  // s'=1 t'=0
-    acc[3] = tmad<C2, R>(acc[3], x1, b1);  // s'=1 t'=1
-  }
-  if (d.lamL) {
-    const R ll = (R)d.lamL[v];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) { acc[i].x *= ll; acc[i].y *= ll; }
-  }
-  if (d.lamR) {
-    const R lr = (R)d.lamR[w];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) { acc[i].x *= lr; acc[i].y *= lr; }
-  }
-  if (d.has_gate) {
-    const double2* g = d.gsrc ? d.gsrc : d.gate;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      C2 o;
-      o.x = o.y = (R)0;
-#pragma unroll
-      for (int j = 0; j < 4; ++j) o = tmad<C2, R>(o, tc<C2, R>(g[i * 4 + j]), acc[j]);
-      out[i] = make_double2((double)o.x, (double)o.y);
-    }
-  } else {
-#pragma unroll
-    for (int i = 0; i < 4; ++i) out[i] = make_double2((double)acc[i].x, (double)acc[i].y);
-  }
-}
-
 // Shape of a task's SVD problem.  The factored modes apply when requested
 // (no gate) and K < min(m, n); otherwise the full theta is decomposed.  The
 // host bounds C by min(K, m, n) for a factored request: C never exceeds it.
@@ -120,13 +58,272 @@ __device__ __forceinline__ TaskShape task_shape(const UpdDesc& d) {
   return t;
 }
 
+// The contraction theta0 = A . lamM . B is a complex GEMM of
+//   A = Gamma[k] as (2 chiL) x chiM, row (v, s)      (its row-major layout)
+//   B = Gamma[k+1] as chiM x (2 chiR), column (t, w) (its row-major layout)
+// split into real GEMMs on the matrix cores:
+//   Re += Ar Br - Ai Bi,  Im += Ar Bi + Ai Br.
+// A block computes a TM x TM tile of theta0: rows (v, s) for TM/2 values of v
+// and both s, columns (t, w) for TM/2 values of w and both t, so every
+// (v, w) point has its four (s, t) values in the block and the gate, lamL and
+// lamR are applied in the epilogue.  K is staged through LDS 16 at a time;
+// each of the four waves owns a (TM/2) x (TM/2) quarter as 16 x 16 subtiles.
+//
+// Arithmetic per precision code P (UpdDesc::prec):
+//   0  FP64: v_mfma_f64_16x16x4f64, FP64 accumulation
+//   1  FP32: v_mfma_f32_16x16x4f32, operands rounded to FP32, FP32 accumulation
+//   2  BF16 split: each FP32 operand x = hi + lo with hi, lo in BF16;
+//      a b ~ ah bh + ah bl + al bh on v_mfma_f32_16x16x16bf16_1k, FP32
+//      accumulation (about 16 bits of each product are kept)
+// Lane layouts (16 x 16 output tiles; lane l, accumulator element e):
+//   f64 16x16x4:  A[i][k] lane i + 16k, B[k][j] lane j + 16k; D row l/16 + 4e
+//   f32 16x16x4:  A, B as for f64;                            D row 4(l/16) + e
+//   bf16 16x16x16: A[i][k] lane i + 16(k/4) element k%4, B[k][j] lane
+//                 j + 16(k/4) element k%4;                    D row 4(l/16) + e
+// and D column l % 16 throughout.
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef short v4s __attribute__((ext_vector_type(4)));
+
+template <int P> struct ThetaArith;
+template <> struct ThetaArith<0> {
+  using R = double; using Acc = v4d;
+  static __device__ __forceinline__ int drow(int lane, int e) { return (lane >> 4) + 4 * e; }
+};
+template <> struct ThetaArith<1> {
+  using R = float; using Acc = v4f;
+  static __device__ __forceinline__ int drow(int lane, int e) { return 4 * (lane >> 4) + e; }
+};
+template <> struct ThetaArith<2> {
+  using R = float; using Acc = v4f;
+  static __device__ __forceinline__ int drow(int lane, int e) { return 4 * (lane >> 4) + e; }
+};
+
+// BF16 of a float, rounded to nearest even (finite inputs), as raw bits.
+__device__ __forceinline__ short bf16_bits(float x) {
+  const unsigned u = __float_as_uint(x);
+  return (short)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+__device__ __forceinline__ float bf16_value(short h) {
+  return __uint_as_float(((unsigned)(unsigned short)h) << 16);
+}
+// x = hi + lo to about 2^-17 relative
+__device__ __forceinline__ void bf16_split(float x, short& hi, short& lo) {
+  hi = bf16_bits(x);
+  lo = bf16_bits(x - bf16_value(hi));
+}
+
+constexpr int kThetaKC = 16;  // K values per LDS stage
+
+template <int TM>
+struct ThetaSmemDims {
+  // staging: Ar, Ai [TM][KC+1]; Br, Bi [KC][TM+1]; epilogue: Or, Oi [TM][TM+1]
+  static constexpr int stage = 2 * TM * (kThetaKC + 1) + 2 * kThetaKC * (TM + 1);
+  static constexpr int out = 2 * TM * (TM + 1);
+  static constexpr int elems = stage > out ? stage : out;
+};
+
+template <int P, int TM>
+__device__ void theta_tiles(const UpdDesc& d, int chiL, int chiM, int chiR, int orient,
+                            double* smraw) {
+  using Ar = ThetaArith<P>;
+  using R = typename Ar::R;
+  using Acc = typename Ar::Acc;
+  constexpr int KC = kThetaKC;
+  constexpr int TV = TM / 2;   // v (and w) values per tile
+  constexpr int NS = TM / 32;  // 16 x 16 subtiles per wave edge
+  R* sar = reinterpret_cast<R*>(smraw);
+  R* sai = sar + TM * (KC + 1);
+  R* sbr = sai + TM * (KC + 1);
+  R* sbi = sbr + KC * (TM + 1);
+  R* sor = reinterpret_cast<R*>(smraw);
+  R* soi = sor + TM * (TM + 1);
+  auto AR = [&](int r, int k) -> R& { return sar[r * (KC + 1) + k]; };
+  auto AI = [&](int r, int k) -> R& { return sai[r * (KC + 1) + k]; };
+  auto BR = [&](int k, int c) -> R& { return sbr[k * (TM + 1) + c]; };
+  auto BI = [&](int k, int c) -> R& { return sbi[k * (TM + 1) + c]; };
+  auto OR = [&](int r, int c) -> R& { return sor[r * (TM + 1) + c]; };
+  auto OI = [&](int r, int c) -> R& { return soi[r * (TM + 1) + c]; };
+
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int wr = (wid >> 1) * (TM / 2), wc = (wid & 1) * (TM / 2);
+  const int nvt = (chiL + TV - 1) / TV, nwt = (chiR + TV - 1) / TV;
+  const double2* g = d.gsrc ? d.gsrc : d.gate;
+  for (int tile = blockIdx.x; tile < nvt * nwt; tile += gridDim.x) {
+    const int v0 = (tile / nwt) * TV, w0 = (tile % nwt) * TV;
+    Acc cre[NS][NS], cim[NS][NS];
+#pragma unroll
+    for (int i = 0; i < NS; ++i)
+#pragma unroll
+      for (int j = 0; j < NS; ++j)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) cre[i][j][e] = cim[i][j][e] = (R)0;
+    for (int k0 = 0; k0 < chiM; k0 += KC) {
+      __syncthreads();  // the previous stage (or tile epilogue) is read
+      for (int e = tid; e < TM * KC; e += 256) {
+        const int r = e / KC, kk = e % KC;
+        const int row = 2 * v0 + r, u = k0 + kk;
+        double2 x = make_double2(0.0, 0.0);
+        if (row < 2 * chiL && u < chiM) x = d.gk[(size_t)row * chiM + u];
+        AR(r, kk) = (R)x.x;
+        AI(r, kk) = (R)x.y;
+      }
+      for (int e = tid; e < KC * TM; e += 256) {
+        const int kk = e / TM, c = e % TM;
+        const int t = c / TV, w = w0 + c % TV, u = k0 + kk;
+        double2 x = make_double2(0.0, 0.0);
+        if (u < chiM && w < chiR) x = cscale(d.gk1[((size_t)u * 2 + t) * chiR + w], d.lamM[u]);
+        BR(kk, c) = (R)x.x;
+        BI(kk, c) = (R)x.y;
+      }
+      __syncthreads();
+      if constexpr (P == 2) {
+        v4s arh[NS], arl[NS], aih[NS], ail[NS], anh[NS], anl[NS];
+#pragma unroll
+        for (int i = 0; i < NS; ++i) {
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            short h, l;
+            bf16_split(AR(wr + 16 * i + li, 4 * lk + e), h, l);
+            arh[i][e] = h; arl[i][e] = l;
+            bf16_split(AI(wr + 16 * i + li, 4 * lk + e), h, l);
+            aih[i][e] = h; ail[i][e] = l;
+            anh[i][e] = (short)(h ^ (short)0x8000);  // -Ai: sign bit flipped
+            anl[i][e] = (short)(l ^ (short)0x8000);
+          }
+        }
+#pragma unroll
+        for (int j = 0; j < NS; ++j) {
+          v4s brh, brl, bih, bil;
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            short h, l;
+            bf16_split(BR(4 * lk + e, wc + 16 * j + li), h, l);
+            brh[e] = h; brl[e] = l;
+            bf16_split(BI(4 * lk + e, wc + 16 * j + li), h, l);
+            bih[e] = h; bil[e] = l;
+          }
+#pragma unroll
+          for (int i = 0; i < NS; ++i) {
+            // small terms first
+            cre[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(arl[i], brh, cre[i][j], 0, 0, 0);
+            cre[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(arh[i], brl, cre[i][j], 0, 0, 0);
+            cre[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(anl[i], bih, cre[i][j], 0, 0, 0);
+            cre[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(anh[i], bil, cre[i][j], 0, 0, 0);
+            cre[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(arh[i], brh, cre[i][j], 0, 0, 0);
+            cre[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(anh[i], bih, cre[i][j], 0, 0, 0);
+            cim[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(arl[i], bih, cim[i][j], 0, 0, 0);
+            cim[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(arh[i], bil, cim[i][j], 0, 0, 0);
+            cim[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ail[i], brh, cim[i][j], 0, 0, 0);
+            cim[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(aih[i], brl, cim[i][j], 0, 0, 0);
+            cim[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(arh[i], bih, cim[i][j], 0, 0, 0);
+            cim[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(aih[i], brh, cim[i][j], 0, 0, 0);
+          }
+        }
+      } else {
+#pragma unroll
+        for (int k4 = 0; k4 < KC / 4; ++k4) {
+          R a_r[NS], a_i[NS], b_r[NS], b_i[NS];
+#pragma unroll
+          for (int i = 0; i < NS; ++i) {
+            a_r[i] = AR(wr + 16 * i + li, 4 * k4 + lk);
+            a_i[i] = AI(wr + 16 * i + li, 4 * k4 + lk);
+          }
+#pragma unroll
+          for (int j = 0; j < NS; ++j) {
+            b_r[j] = BR(4 * k4 + lk, wc + 16 * j + li);
+            b_i[j] = BI(4 * k4 + lk, wc + 16 * j + li);
+          }
+#pragma unroll
+          for (int i = 0; i < NS; ++i)
+#pragma unroll
+            for (int j = 0; j < NS; ++j) {
+              if constexpr (P == 0) {
+                cre[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_r[i], b_r[j], cre[i][j], 0, 0, 0);
+                cre[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a_i[i], b_i[j], cre[i][j], 0, 0, 0);
+                cim[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_r[i], b_i[j], cim[i][j], 0, 0, 0);
+                cim[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_i[i], b_r[j], cim[i][j], 0, 0, 0);
+              } else {
+                cre[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a_r[i], b_r[j], cre[i][j], 0, 0, 0);
+                cre[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(-a_i[i], b_i[j], cre[i][j], 0, 0, 0);
+                cim[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a_r[i], b_i[j], cim[i][j], 0, 0, 0);
+                cim[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a_i[i], b_r[j], cim[i][j], 0, 0, 0);
+              }
+            }
+        }
+      }
+    }
+    __syncthreads();  // the last stage is read before the tile overwrites it
+#pragma unroll
+    for (int i = 0; i < NS; ++i)
+#pragma unroll
+      for (int j = 0; j < NS; ++j)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int r = wr + 16 * i + Ar::drow(lane, e), c = wc + 16 * j + li;
+          OR(r, c) = cre[i][j][e];
+          OI(r, c) = cim[i][j][e];
+        }
+    __syncthreads();
+    // epilogue per (v, w): lamL, lamR, the gate, the store into X.  Threads
+    // run along v for theta (X column-major: rows 2v, 2v+1 of a column are
+    // adjacent) and along w for theta^H.
+    for (int p = tid; p < TV * TV; p += 256) {
+      const int vv = orient == 0 ? p % TV : p / TV;
+      const int ww = orient == 0 ? p / TV : p % TV;
+      const int v = v0 + vv, w = w0 + ww;
+      if (v >= chiL || w >= chiR) continue;
+      R ax[4], ay[4];
+#pragma unroll
+      for (int st = 0; st < 4; ++st) {
+        const int s = st >> 1, t = st & 1;
+        ax[st] = OR(2 * vv + s, t * TV + ww);
+        ay[st] = OI(2 * vv + s, t * TV + ww);
+      }
+      R f = (R)1;
+      if (d.lamL) f *= (R)d.lamL[v];
+      if (d.lamR) f *= (R)d.lamR[w];
+#pragma unroll
+      for (int st = 0; st < 4; ++st) { ax[st] *= f; ay[st] *= f; }
+      double2 out[4];
+      if (d.has_gate) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          R ox = (R)0, oy = (R)0;
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const R gx = (R)g[i * 4 + j].x, gy = (R)g[i * 4 + j].y;
+            ox += gx * ax[j] - gy * ay[j];
+            oy += gx * ay[j] + gy * ax[j];
+          }
+          out[i] = make_double2((double)ox, (double)oy);
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) out[i] = make_double2((double)ax[i], (double)ay[i]);
+      }
+#pragma unroll
+      for (int st = 0; st < 4; ++st) {
+        const int s = st >> 1, t = st & 1;
+        const int row = v * 2 + s, col = t * chiR + w;
+        if (orient == 0)
+          d.X[(size_t)col * d.ldx + row] = out[st];
+        else
+          d.X[(size_t)row * d.ldx + col] = cconj(out[st]);
+      }
+    }
+  }
+}
+
 // theta'[v,(s,t),w] = sum_{s't'} U[(s,t),(s',t')] lamL_v lamR_w
 //                     sum_u Gk[v,s',u] lamM_u Gk1[u,t',w]
-// written into X as theta (orient 0, m >= n) or theta^H (orient 1).
-// Tile: 4 values of v x 64 values of w per 256-thread block.  In the
-// factored modes X is filled elementwise instead (no contraction):
+// written into X as theta (orient 0, m >= n) or theta^H (orient 1), on the
+// matrix cores in TM x TM tiles (TM = 32 or 64).  In the factored modes X is
+// filled elementwise instead (no contraction):
 //   mode 1: X[(t,w), u] = conj(lamM_u Gk1[u,t,w] lamR_w)
 //   mode 2: X[(v,s), u] = lamL_v Gk[v,s,u] lamM_u
+template <int TM>
 __global__ __launch_bounds__(256) void k_theta(const UpdDesc* __restrict__ descs, int nt) {
   const UpdDesc& d = descs[blockIdx.y];
   const int chiL = d.dims[0], chiM = d.dims[1], chiR = d.dims[2];
@@ -166,27 +363,13 @@ __global__ __launch_bounds__(256) void k_theta(const UpdDesc* __restrict__ descs
     }
     return;
   }
-  const int nvt = (chiL + 3) / 4, nwt = (chiR + 63) / 64;
-  const int tv = threadIdx.x >> 6, tw = threadIdx.x & 63;
-  for (int tile = blockIdx.x; tile < nvt * nwt; tile += gridDim.x) {
-    const int v = (tile / nwt) * 4 + tv;
-    const int w = (tile % nwt) * 64 + tw;
-    if (v >= chiL || w >= chiR) continue;
-    double2 out[4];
-    if (d.prec == 1)
-      theta_point<float2, float>(d, v, w, chiM, chiR, out);
-    else
-      theta_point<double2, double>(d, v, w, chiM, chiR, out);
-#pragma unroll
-    for (int st = 0; st < 4; ++st) {
-      const int s = st >> 1, t = st & 1;
-      const int row = v * 2 + s, col = t * chiR + w;
-      if (orient == 0)
-        d.X[(size_t)col * d.ldx + row] = out[st];
-      else
-        d.X[(size_t)row * d.ldx + col] = cconj(out[st]);
-    }
-  }
+  __shared__ double smraw[ThetaSmemDims<TM>::elems];
+  if (d.prec == 2)
+    theta_tiles<2, TM>(d, chiL, chiM, chiR, orient, smraw);
+  else if (d.prec == 1)
+    theta_tiles<1, TM>(d, chiL, chiM, chiR, orient, smraw);
+  else
+    theta_tiles<0, TM>(d, chiL, chiM, chiR, orient, smraw);
 }
 
 // ---------------------------------------------------------------- W = I
@@ -203,6 +386,7 @@ __global__ __launch_bounds__(256) void k_winit(const UpdDesc* __restrict__ descs
 
 // ------------------------------------------------------------- finish
 constexpr double kF32NoiseFloor = 9.5367431640625e-07;  // 2^-20
+constexpr double kBF16SplitNoiseFloor = 1.220703125e-04;  // 2^-13
 
 // Column norms of X are the singular values (theta = U S V^H).  Sort them
 // descending, apply process_svd (mps.rs:236-293): normalise, pick the rank,
@@ -267,8 +451,10 @@ __global__ __launch_bounds__(1024) void k_finish(const UpdDesc* __restrict__ des
   else thr = DBL_EPSILON;
   // With an FP32 contraction, normalised singular values below the FP32
   // round-off of theta (2^-20 relative) are noise, not Schmidt values:
-  // they are not counted as kept values of the rank rule.
+  // they are not counted as kept values of the rank rule.  The BF16-split
+  // contraction keeps about 16 bits per product: its floor is 2^-13.
   if (d.prec == 1) thr = fmax(thr, kF32NoiseFloor);
+  if (d.prec == 2) thr = fmax(thr, kBF16SplitNoiseFloor);
   for (int i = tid; i < C; i += blockDim.x) {
     const double s = key[i];
     const bool ok = (s >= DBL_MIN) && isfinite(s) && (s > thr);
@@ -606,8 +792,11 @@ __global__ __launch_bounds__(256) void k_u1(double2* g, const int* dims, Gate2 o
 }
 
 // ------------------------------------------------------------- launch
-void launch_theta(const UpdDesc* descs, int nt, int gx, hipStream_t s) {
-  hipLaunchKernelGGL(k_theta, dim3(gx, nt), dim3(256), 0, s, descs, nt);
+void launch_theta(const UpdDesc* descs, int nt, int tm, int gx, hipStream_t s) {
+  if (tm == 32)
+    hipLaunchKernelGGL(k_theta<32>, dim3(gx, nt), dim3(256), 0, s, descs, nt);
+  else
+    hipLaunchKernelGGL(k_theta<64>, dim3(gx, nt), dim3(256), 0, s, descs, nt);
 }
 void launch_winit(const UpdDesc* descs, int nt, int gx, hipStream_t s) {
   hipLaunchKernelGGL(k_winit, dim3(gx, nt), dim3(256), 0, s, descs);

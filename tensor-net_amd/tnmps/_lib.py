@@ -28,11 +28,19 @@ TN_INVALID_ARG = 11
 TN_NO_CONVERGENCE = 12
 
 TRUNC_NONE, TRUNC_CONST, TRUNC_CUTOFF = 0, 1, 2
-PREC_F64, PREC_F32 = 0, 1
+PREC_F64, PREC_F32, PREC_BF16X3 = 0, 1, 2
 
 
 class tn_trunc(C.Structure):
     _fields_ = [("kind", C.c_int), ("chi", C.c_size_t), ("eps", C.c_double)]
+
+
+class tn_tune(C.Structure):
+    _fields_ = [("c_max", C.c_int), ("theta_tile", C.c_int), ("bw", C.c_int),
+                ("split_max", C.c_int), ("split_rows", C.c_int), ("ahead", C.c_int)]
+
+
+TUNE_FIELDS = ("c_max", "theta_tile", "bw", "split_max", "split_rows", "ahead")
 
 
 class tn_perf(C.Structure):
@@ -64,6 +72,9 @@ SIGNATURES = {
     "tn_engine_free": (None, [_h]),
     "tn_mps_new_qubits_in": (C.c_int, [_h, _sz, tn_trunc, _sz, _hp]),
     "tn_set_precision": (C.c_int, [_h, C.c_int]),
+    "tn_engine_set_tuning": (C.c_int, [_h, C.POINTER(tn_tune), _sz]),
+    "tn_engine_get_tuning": (C.c_int, [_h, C.POINTER(tn_tune), _sz, _szp]),
+    "tn_set_tuning": (C.c_int, [_h, C.POINTER(tn_tune), _sz]),
     "tn_batch_bilayer": (C.c_int, [_h, _sz, _hp, _szp, _szp, _dp, _szp, _szp, _ip, _dp, _szp,
                                    _dp, _szp, _ip, _dp]),
     "tn_mps_clone": (C.c_int, [_h, _hp]),

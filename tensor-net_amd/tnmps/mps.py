@@ -50,6 +50,21 @@ def _check(code: int, where: str) -> None:
         raise MPSError(code, where)
 
 
+# contraction arithmetic names (tn_set_precision)
+PRECISIONS = {"f64": L.PREC_F64, "f32": L.PREC_F32, "bf16x3": L.PREC_BF16X3}
+
+
+def _tune_array(rows):
+    """ctypes array of tn_tune from dicts or field tuples."""
+    rows = list(rows)
+    arr = (L.tn_tune * max(1, len(rows)))()
+    for i, r in enumerate(rows):
+        vals = [r[f] for f in L.TUNE_FIELDS] if isinstance(r, dict) else list(r)
+        for f, v in zip(L.TUNE_FIELDS, vals):
+            setattr(arr[i], f, int(v))
+    return arr, len(rows)
+
+
 def to_trunc(trunc) -> L.tn_trunc:
     if trunc is None:
         return L.tn_trunc(L.TRUNC_NONE, 0, 0.0)
@@ -178,10 +193,18 @@ class MPS:
         _check(self._lib.tn_sync(self._h), "sync")
 
     def set_precision(self, prec: str) -> "MPS":
-        """Contraction arithmetic: "f64" (default) or "f32" (FP32 with FP32
-        accumulation); SVD, truncation and renormalisation stay FP64."""
-        code = {"f64": L.PREC_F64, "f32": L.PREC_F32}[prec]
+        """Contraction arithmetic: "f64" (default), "f32" (FP32 with FP32
+        accumulation) or "bf16x3" (each FP32 operand split into two BF16
+        halves, three BF16 products per real product, FP32 accumulation);
+        SVD, truncation and renormalisation stay FP64."""
+        code = PRECISIONS[prec]
         _check(self._lib.tn_set_precision(self._h, code), "set_precision")
+        return self
+
+    def set_tuning(self, rows) -> "MPS":
+        """Launch-shape table of this handle's engine (see :func:`tuning_rows`)."""
+        arr, n = _tune_array(rows)
+        _check(self._lib.tn_set_tuning(self._h, arr, n), "set_tuning")
         return self
 
     # ------------------------------------------------------------ unitaries
@@ -465,6 +488,21 @@ class Engine:
         if h is not None and h.value:
             self._lib.tn_engine_free(h)
             self._h = C.c_void_p()
+
+    def set_tuning(self, rows) -> None:
+        """Replace the engine's launch-shape table: a list of dicts (or
+        tuples) with the fields of ``tn_tune`` (include/tnmps.h), c_max
+        increasing.  The arithmetic is the same SVD to rounding whatever the
+        table says."""
+        arr, n = _tune_array(rows)
+        _check(self._lib.tn_engine_set_tuning(self._h, arr, n), "engine_set_tuning")
+
+    def get_tuning(self) -> List[dict]:
+        """The engine's launch-shape table as a list of dicts."""
+        arr = (L.tn_tune * 16)()
+        n = C.c_size_t()
+        _check(self._lib.tn_engine_get_tuning(self._h, arr, 16, C.byref(n)), "engine_get_tuning")
+        return [{f: getattr(arr[i], f) for f in L.TUNE_FIELDS} for i in range(n.value)]
 
     def stats(self) -> Tuple[int, int]:
         """(two-site updates, Jacobi sweeps) done by all handles of the engine."""

@@ -1,0 +1,119 @@
+"""The two-site contraction on the matrix cores: both tile sizes, the three
+contraction precisions and the launch-shape table.
+
+The tiles are checked against the oracle (same tolerances as
+tests/test_gpu_parity.py) at bond dimensions that leave partial tiles, and
+against each other bitwise: a tile size changes which workgroup computes an
+element of theta, not the order of its sums.
+"""
+import numpy as np
+import pytest
+
+from helpers import assert_spectrum_close
+from oracle.mps_oracle import run_trajectory
+from tnmps import MPS, Engine
+from tnmps.mps import MPSError
+from tnmps.workload import haar_circuit
+
+pytestmark = pytest.mark.gpu
+
+BUCKETS = (16, 32, 64, 128, 256, 512, 1024)
+
+
+def table(tile, bw=8, split_max=8, split_rows=128, ahead=2):
+    return [dict(c_max=c, theta_tile=tile, bw=bw, split_max=split_max, split_rows=split_rows,
+                 ahead=ahead) for c in BUCKETS]
+
+
+def test_theta_tiles_match_oracle_and_each_other(gpu):
+    """chi = 40: theta tiles of 32 (16 v values) and 64 (32 v values) both end
+    in partial tiles.  FP64 and FP32 contractions, each tile size."""
+    n, depth, p, chi = 14, 10, 0.1, 40
+    c = haar_circuit(n, depth, p, 404)
+    ref = run_trajectory(n, c.layers, ("const", chi), rs=c.draws)
+    runs = {}
+    for prec in ("f64", "f32"):
+        for tile in (32, 64):
+            st = MPS.new_qubits(n, ("const", chi)).set_precision(prec).set_tuning(table(tile))
+            got = st.run_circuit(c.layers, forced=ref["record"])
+            runs[prec, tile] = (st, got)
+    for tile in (32, 64):
+        st, got = runs["f64", tile]
+        assert got["record"] == ref["record"]
+        assert st.bond_dims() == ref["state"].bond_dims()
+        for b in range(n - 1):
+            assert_spectrum_close(st.schmidt(b), ref["state"].L[b], msg=f"tile {tile} bond {b}")
+        for a, w in zip(got["entropy"], ref["entropy"]):
+            assert abs(a - w) <= 1e-12 + 1e-10 * abs(w)
+    for prec in ("f64", "f32"):
+        a, b = runs[prec, 32][0], runs[prec, 64][0]
+        assert a.bond_dims() == b.bond_dims()
+        for bd in range(n - 1):
+            assert np.array_equal(a.schmidt(bd), b.schmidt(bd)), f"{prec} bond {bd}"
+
+
+def _prec_error(prec, seed):
+    n, depth, p, chi = 12, 12, 0.1, 16
+    c = haar_circuit(n, depth, p, seed)
+    ref = run_trajectory(n, c.layers, ("const", chi), rs=c.draws)
+    st = MPS.new_qubits(n, ("const", chi)).set_precision(prec)
+    got = st.run_circuit(c.layers, forced=ref["record"])
+    ent = max(abs(a - b) for a, b in zip(got["entropy"], ref["entropy"]))
+    lam = 0.0
+    for b in range(n - 1):
+        x, y = st.schmidt(b), ref["state"].L[b]
+        k = min(len(x), len(y))
+        lam = max(lam, float(np.max(np.abs(x[:k] - y[:k]))))
+    return ent, lam
+
+
+def test_bf16x3_contraction_error(gpu):
+    """TN_PREC_BF16X3: each FP32 operand split into two BF16 halves, three
+    BF16 products per real product, FP32 accumulation; about 16 bits of each
+    product are kept, so entropies and Schmidt values stay within 1e-3 of the
+    FP64 oracle, and the error is above the FP32 path's."""
+    ent, lam = _prec_error("bf16x3", 77)
+    ent32, lam32 = _prec_error("f32", 77)
+    assert ent < 1e-3 and lam < 1e-3, (ent, lam)
+    assert ent > 0.0 and lam > 0.0
+    assert max(ent, lam) > max(ent32, lam32)
+
+
+def test_tuning_table_roundtrip_and_validation(gpu):
+    eng = Engine(32)
+    rows = eng.get_tuning()
+    assert [r["c_max"] for r in rows] == list(BUCKETS)
+    new = table(64, bw=16, split_max=2, split_rows=256, ahead=3)[:3]
+    eng.set_tuning(new)
+    assert eng.get_tuning() == new
+    bad = [dict(new[0], theta_tile=48), dict(new[0], bw=12), dict(new[0], split_max=0),
+           dict(new[0], split_max=9), dict(new[0], split_rows=32), dict(new[0], ahead=0),
+           dict(new[0], ahead=4), dict(new[0], c_max=0)]
+    for r in bad:
+        with pytest.raises(MPSError):
+            eng.set_tuning([r])
+    with pytest.raises(MPSError):  # c_max not increasing
+        eng.set_tuning([new[1], new[0]])
+    with pytest.raises(MPSError):
+        eng.set_tuning([])
+    assert eng.get_tuning() == new  # a rejected table leaves the old one
+
+
+def test_tuned_engine_matches_oracle(gpu):
+    """Every knob of the table away from its default at once (wide Jacobi
+    panels, no row split, one sweep ahead, 64-tiles): same spectra as the
+    oracle, on a batch of uneven trajectories."""
+    specs = [(10, 8, 0.2, 16, 1), (12, 8, 0.1, 32, 2), (8, 8, 0.3, 8, 3)]
+    circuits = [haar_circuit(n, d, p, s) for (n, d, p, _, s) in specs]
+    refs = [run_trajectory(n, c.layers, ("const", chi), rs=c.draws)
+            for (n, _, _, chi, _), c in zip(specs, circuits)]
+    eng = Engine(32)
+    eng.set_tuning(table(64, bw=16, split_max=1, split_rows=64, ahead=1))
+    states = [eng.new_qubits(n, ("const", chi)) for (n, _, _, chi, _) in specs]
+    for t in range(8):
+        eng.batch_bilayer(states, [c.layers[t] for c in circuits],
+                          forced=[r["record"][t] for r in refs])
+    for st, ref, (n, _, _, _, _) in zip(states, refs, specs):
+        assert st.bond_dims() == ref["state"].bond_dims()
+        for b in range(n - 1):
+            assert_spectrum_close(st.schmidt(b), ref["state"].L[b])
