@@ -104,6 +104,10 @@ def run_batched(args) -> dict:
     log(f"generated {T} circuits in {time.perf_counter() - t_gen:.1f}s")
     groups = [list(range(g, T, G)) for g in range(G)]
     engs = [Engine(chi) for _ in range(G)]
+    tune_label, tune_rows = tuning_for(args, n, chi, args.prec, batched=True)
+    if tune_rows is not None:
+        for e in engs:
+            e.set_tuning(tune_rows)
     states = [[engs[g].new_qubits(n, ("const", chi)) for _ in idx] for g, idx in enumerate(groups)]
     if args.prec != "f64":
         for grp in states:
@@ -170,7 +174,8 @@ def run_batched(args) -> dict:
                    "updates_timed": updates,
                    "trajectory_layers_per_s": round(T * K / wall, 2),
                    "trajectories_per_s_equiv": round(T * K / wall / depth, 3),
-                   "mean_half_chain_entropy_last_layer": round(mean_ent, 6)},
+                   "mean_half_chain_entropy_last_layer": round(mean_ent, 6),
+                   "tuning": tune_label},
         "roofline": {"bound": "mfma", "kernel": "jacobi step kernels",
                      "achieved": round(achieved, 4), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / FP64_PEAK_TFLOPS, 5), "traffic": None,
@@ -180,7 +185,44 @@ def run_batched(args) -> dict:
         "breakdown": {k: round(perf[k], 2) for k in ("ms_theta", "ms_jacobi", "ms_post",
                                                      "ms_measure", "ms_host_launch")},
         "memory_kernels": memory_kernels(perf, G),
+        "contraction_kernel": contraction_kernel(perf, args.prec, G),
     }
+
+
+TUNED_TABLES = REPO / "tensor-net_amd" / "tnmps" / "tuning_mi355x.json"
+
+
+def tuning_for(args, n: int, chi: int, prec: str, batched: bool):
+    """(label, rows) of the launch-shape table to use, or ("default", None)."""
+    if args.tune == "none":
+        return "default", None
+    if args.tune != "auto":
+        path, key = args.tune.rsplit(":", 1)
+        from tnmps.autotune import load_table
+        return key, load_table(path, key)
+    if not TUNED_TABLES.exists():
+        return "default", None
+    data = json.loads(TUNED_TABLES.read_text())
+    for key, ent in data.items():
+        w = ent["workload"]
+        if (w["n"], w["chi"], w["prec"]) == (n, chi, prec) and (w["traj"] > 1) == batched:
+            return key, ent["table"]
+    return "default", None
+
+
+def contraction_kernel(perf: dict, prec: str, streams: int) -> dict:
+    """Achieved matrix-core rate of the contraction kernel k_theta:
+    algorithmic flops of the contracted updates (8 m n K, + 32 m n with a
+    gate) over the kernel's summed HIP-event spans.  The spans also hold the
+    factored updates' elementwise fills, so the rate is a lower bound.  The
+    peak is the dense MFMA peak of the arithmetic: FP64 78.6, FP32 157.3 and
+    BF16 2500 TFLOP/s (the BF16 split issues three products per real one)."""
+    peak = {"f64": FP64_PEAK_TFLOPS, "f32": 157.3, "bf16x3": 2500.0}[prec]
+    ms, fl = perf["ms_contract"], perf["contract_flops"]
+    tf = fl / (ms / 1e3) / 1e12 if ms > 0 else 0.0
+    return {"kernel": "k_theta", "arith": prec, "flops": fl, "ms": round(ms, 3),
+            "achieved_TFLOPs": round(tf, 3), "peak_TFLOPs": peak,
+            "frac": round(tf / peak, 4), "streams": streams}
 
 
 def memory_kernels(perf: dict, streams: int) -> dict:
@@ -274,7 +316,8 @@ def run_mixed(args) -> dict:
                    "bf16x3_entropy_abs_err_per_layer": [float(f"{x:.3e}") for x in bf_ent],
                    "bf16x3_schmidt_abs_err_per_layer": [float(f"{x:.3e}") for x in bf_lam],
                    "bf16x3_max_entropy_err": float(f"{float(np.max(bf_ent)):.3e}"),
-                   "ms_theta": {p: round(out[p]["perf"]["ms_theta"], 2) for p in out}},
+                   "contraction_kernel": {p: contraction_kernel(out[p]["perf"], p, 1)
+                                          for p in out}},
         "roofline": None,
     }
 
@@ -368,6 +411,10 @@ def main() -> int:
                     help="serving: lock-step loops, one engine, stream and host thread each")
     ap.add_argument("--serve-host-circuits", action="store_true",
                     help="serving: requests carry host-generated circuits")
+    ap.add_argument("--tune", default="auto",
+                    help="launch-shape table: 'auto' (the package's tuned table for this "
+                         "workload's N, chi and precision, if any), 'none', or PATH:KEY "
+                         "(a file written by python -m tnmps.autotune)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (calibrates "
                          "the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -402,6 +449,9 @@ def main() -> int:
     c = haar_circuit(n, W + K + 1, p, args.seed)
     draws = iter(c.draws)
     st = MPS.new_qubits(n, trunc)
+    tune_label, tune_rows = tuning_for(args, n, chi, "f64", batched=False)
+    if tune_rows is not None:
+        st.set_tuning(tune_rows)
     t_w = time.perf_counter()
     for t in range(W):
         st.run_circuit(c.layers[t:t + 1], rs=draws)
@@ -478,12 +528,13 @@ def main() -> int:
                         f"Const(chi_max={chi}), complex128, plain FP64 GPU path; "
                         f"step = one saturated circuit layer",
             "n_qubits": n, "p": p, "chi_max": chi, "trajectories": 1,
-            "updates_timed": updates,
+            "updates_timed": updates, "tuning": tune_label,
         },
         "roofline": roof,
         "cpu_baseline": cpu,
         "breakdown": breakdown,
         "memory_kernels": memory_kernels(perf, 1),
+        "contraction_kernel": contraction_kernel(perf, "f64", 1),
     }
     print(json.dumps(out), flush=True)
     return 0

@@ -318,6 +318,12 @@ typedef struct {
     double bytes_renorm;
     double bytes_measure;
     size_t measure_sites;
+    /* The contraction kernel alone (k_theta: theta, gate, lambda weights; or
+     * the factored updates' elementwise fill of X): its event span, and the
+     * algorithmic flops of the contracted updates (8 m n K, + 32 m n with a
+     * gate). */
+    double ms_contract;
+    double contract_flops;
 } tn_perf;
 
 /* Turn device-time accounting on (1) or off (0).  On adds one stream

@@ -152,8 +152,9 @@ using EntReq = EntDesc;
 // they have completed, or when the counters are asked for), so timing adds
 // no host synchronisation to the stream.
 struct PerfRec {
-  // theta begin/end, sweeps begin/end, post begin/end, split begin, renorm begin
-  hipEvent_t ev[8] = {};
+  // theta begin/end, sweeps begin/end, post begin/end, split begin, renorm
+  // begin, contraction end (winit begin)
+  hipEvent_t ev[9] = {};
   int* h_info = nullptr;  // pinned copy of the batch's info (update batch) or the
                           // measured sites' chiL, chiR (measurement launch)
   int cap = 0;            // tasks (or sites) h_info holds
@@ -341,6 +342,7 @@ struct tn_engine {
       perf.measure_sites += (size_t)r.nt;
     } else {
       perf.ms_theta += span(r, 0, 1);
+      perf.ms_contract += span(r, 0, 8);
       perf.ms_jacobi += span(r, 2, 3);
       perf.ms_post += span(r, 4, 5);
       perf.ms_split += span(r, 6, 7);
@@ -361,6 +363,7 @@ struct tn_engine {
         const double fpre = mode == 0 ? 8.0 * m * n * K + (r.gate[t] ? 32.0 * m * n : 0.0)
                                       : 8.0 * (mode == 1 ? m : n) * K * rank;
         perf.alg_flops += fpre + fsvd;
+        if (mode == 0) perf.contract_flops += fpre;
         perf.bytes_split += 32.0 * rank * (m + n);
         perf.bytes_renorm += 32.0 * rank * (m + n);
       }
@@ -652,6 +655,7 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
                                              1024);
   const int gx_w = (int)std::min<size_t>((maxCC + 255) / 256, 1024);
   launch_theta(dd, nt, tu.theta_tile, gx_theta, e->stream);
+  if (pr) TN_HIP(hipEventRecord(pr->ev[8], e->stream));
   launch_winit(dd, nt, gx_w, e->stream);
   TN_HIP(hipGetLastError());
   if (pr) TN_HIP(hipEventRecord(pr->ev[1], e->stream));

@@ -51,7 +51,8 @@ class tn_perf(C.Structure):
                 ("jacobi_launches", C.c_size_t), ("ms_host_launch", C.c_double),
                 ("factored", C.c_size_t), ("ms_split", C.c_double), ("ms_renorm", C.c_double),
                 ("bytes_split", C.c_double), ("bytes_renorm", C.c_double),
-                ("bytes_measure", C.c_double), ("measure_sites", C.c_size_t)]
+                ("bytes_measure", C.c_double), ("measure_sites", C.c_size_t),
+                ("ms_contract", C.c_double), ("contract_flops", C.c_double)]
 
 
 _sz = C.c_size_t

@@ -117,3 +117,22 @@ def test_tuned_engine_matches_oracle(gpu):
         assert st.bond_dims() == ref["state"].bond_dims()
         for b in range(n - 1):
             assert_spectrum_close(st.schmidt(b), ref["state"].L[b])
+
+
+def test_autotune_small_workload(gpu):
+    """The tuner times candidates on a small saturated workload and returns a
+    table the engine accepts; a trajectory under that table still gives the
+    oracle's spectra."""
+    from tnmps.autotune import tune
+
+    res = tune(n=10, chi=16, p=0.2, traj=2, layers=1, repeat=1, log=lambda m: None)
+    rows = res["table"]
+    assert [r["c_max"] for r in rows] == list(BUCKETS)
+    assert len(res["trials"]) > 10 and res["tuned_s"] <= res["default_s"]
+    n, depth, p, chi = 10, 8, 0.2, 16
+    c = haar_circuit(n, depth, p, 5)
+    ref = run_trajectory(n, c.layers, ("const", chi), rs=c.draws)
+    st = MPS.new_qubits(n, ("const", chi)).set_tuning(rows)
+    st.run_circuit(c.layers, forced=ref["record"])
+    for b in range(n - 1):
+        assert_spectrum_close(st.schmidt(b), ref["state"].L[b])
