@@ -322,6 +322,57 @@ def run_mixed(args) -> dict:
     }
 
 
+def run_gates(args) -> dict:
+    """Gate layers only, on saturated trajectories of one engine: every update
+    is a full two-site gate update (theta of 2chi x 2chi, C = 2chi), so the
+    contraction kernel's event span holds contractions only.  Reports the
+    update rate and the contraction kernel's matrix-core rate."""
+    from tnmps import Engine
+    from tnmps.workload import haar_circuit_fast
+
+    n, chi, T = args.n or 64, args.chi or 256, args.traj or 64
+    W, K = args.warmup, args.steps
+    circs = [haar_circuit_fast(n, W + K, args.p, args.seed + 7 * i) for i in range(T)]
+    eng = Engine(chi)
+    states = [eng.new_qubits(n, ("const", chi)) for _ in range(T)]
+    for st in states:
+        st.set_precision(args.prec)
+    draws = [iter(c.draws) for c in circs]
+    t_w = time.perf_counter()
+    for t in range(W):
+        eng.batch_bilayer(states, [c.layers[t] for c in circs], draws=draws)
+        log(f"warmup layer {t + 1}/{W} ({time.perf_counter() - t_w:.1f}s)")
+    eng.sync()
+    eng.set_timing(True)
+    eng.reset_perf()
+    u0, _ = eng.stats()
+    t0 = time.perf_counter()
+    for t in range(W, W + K):
+        eng.batch_bilayer(states, [(c.layers[t][0], []) for c in circs], forced=[[]] * T)
+    eng.sync()
+    wall = time.perf_counter() - t0
+    eng.set_timing(False)
+    u1, _ = eng.stats()
+    perf = eng.perf()
+    return {
+        "metric": "two-site gate updates/s (gate layers only)",
+        "value": round((u1 - u0) / wall, 2), "unit": "two-site updates/s", "n_gpus": 1,
+        "steps": K, "warmup": W, "ms_per_step": round(wall / K * 1e3, 2),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if args.prec == "f64" else f"{args.prec} contraction, f64 SVD",
+        "data": "synthetic",
+        "config": {"workload": f"{T} saturated trajectories N={n} Const({chi}), brickwork gate "
+                               f"layers without measurements, one engine",
+                   "trajectories": T, "n_qubits": n, "chi_max": chi,
+                   "updates_timed": u1 - u0,
+                   "max_bond": max(s.max_bond_dim() for s in states)},
+        "roofline": None,
+        "breakdown": {k: round(perf[k], 2) for k in ("ms_theta", "ms_jacobi", "ms_post",
+                                                     "ms_contract")},
+        "contraction_kernel": contraction_kernel(perf, args.prec, 1),
+    }
+
+
 def run_serving(args) -> dict:
     """Poisson stream of trajectory requests served by continuous batching."""
     import numpy as np
@@ -392,7 +443,7 @@ def main() -> int:
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="single",
-                    choices=["single", "batched", "mixed", "serving"])
+                    choices=["single", "batched", "mixed", "serving", "gates"])
     ap.add_argument("--traj", type=int, default=0, help="trajectories (batched, mixed)")
     ap.add_argument("--depth", type=int, default=0, help="circuit depth (batched, mixed)")
     ap.add_argument("--pgrid", default="0.05,0.30,11",
@@ -436,7 +487,8 @@ def main() -> int:
         return 2
 
     if args.workload != "single":
-        fn = {"batched": run_batched, "mixed": run_mixed, "serving": run_serving}[args.workload]
+        fn = {"batched": run_batched, "mixed": run_mixed, "serving": run_serving,
+              "gates": run_gates}[args.workload]
         print(json.dumps(fn(args)), flush=True)
         return 0
 
