@@ -65,14 +65,29 @@ double env_double(const char* name, double dflt) {
 // convergence measure.  Below f64::EPSILON, such a column's value is dropped
 // by the rank rule (mps.rs:255-262) whatever further rotations would make of
 // it, and a rotation only shrinks the smaller column of its pair.
+//
+// A theta contracted in FP32 (or BF16 split) carries relative round-off near
+// 6e-8 (1.5e-5) in every entry; orthogonalising its columns to 1e-16 buys
+// nothing.  Such a task's SVD has converged after a sweep whose largest ratio
+// is at most conv_tol_f32 (conv_tol_bf16), leaving a residual near its square.
+// The residual only tilts the new left factor's columns from orthogonal by
+// that much: X W = U~ exactly with W unitary, so the split still writes theta
+// exactly.  A batch uses the strictest tolerance of its tasks.
 struct Tolerances {
   double tol_rot = 1e-14;
   double conv_tol = 1e-8;
+  double conv_tol_f32 = 1e-5;
+  double conv_tol_bf16 = 1e-3;
   double tiny_norm = 1.5e-16;
   void read_env() {
     tol_rot = env_double("TNMPS_TOL_ROT", tol_rot);
     conv_tol = env_double("TNMPS_CONV_TOL", conv_tol);
+    conv_tol_f32 = env_double("TNMPS_CONV_TOL_F32", conv_tol_f32);
+    conv_tol_bf16 = env_double("TNMPS_CONV_TOL_BF16", conv_tol_bf16);
     tiny_norm = env_double("TNMPS_TINY_NORM", tiny_norm);
+  }
+  double for_prec(int prec) const {
+    return prec == TN_PREC_F32 ? conv_tol_f32 : prec == TN_PREC_BF16X3 ? conv_tol_bf16 : conv_tol;
   }
 };
 
@@ -702,7 +717,9 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
   ja.Rh = (int)maxR;
   ja.Ch = (int)maxC;
   ja.tol_rot = tol.tol_rot;
-  ja.conv_tol = tol.conv_tol;
+  double conv_tol = tol.for_prec(tasks[0].h->prec);
+  for (int t = 1; t < nt; ++t) conv_tol = std::min(conv_tol, tol.for_prec(tasks[t].h->prec));
+  ja.conv_tol = conv_tol;
   ja.tiny2 = tol.tiny_norm * tol.tiny_norm;
   // Sweeps are enqueued one ahead while the last sweep read back is still far
   // from convergence (max off-diagonal above kAheadOff: with quadratic
@@ -711,7 +728,8 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
   // an event.  Near convergence the host waits for every sweep, so no sweep
   // is enqueued that is not needed; a sweep enqueued after convergence
   // returns at once on the device.
-  constexpr double kAheadOff = 1e-3;
+  // (1e-3 for conv_tol = 1e-8; scaled by the square root for looser tolerances)
+  const double kAheadOff = std::min(0.5, 1e-3 * std::sqrt(conv_tol / 1e-8));
   int launched = 0, checked = 0;
   double last_worst = 1.0;
   bool done = false;
@@ -745,7 +763,7 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
     double worst = 0.0;
     int open = 0;
     for (int t = 0; t < nt; ++t) {
-      if (!(o[t] <= tol.conv_tol)) {
+      if (!(o[t] <= conv_tol)) {
         done = false;
         ++open;
       }
