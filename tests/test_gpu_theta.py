@@ -136,3 +136,24 @@ def test_autotune_small_workload(gpu):
     st.run_circuit(c.layers, forced=ref["record"])
     for b in range(n - 1):
         assert_spectrum_close(st.schmidt(b), ref["state"].L[b])
+
+
+def test_contraction_perf_counts(gpu):
+    """tn_perf's contraction fields: the algorithmic flops of the contracted
+    updates, 8 m n K + 32 m n per gate, from the task's own m, n, K."""
+    from helpers import CX, H, I2
+
+    bell = CX @ np.kron(H, I2)
+    st = MPS.new_qubits(4, ("const", 8))
+    st.set_timing(True)
+    st.reset_perf()
+    st.apply_unitary2(1, bell)  # m = n = 2, K = 1
+    st.sync()
+    p = st.perf()
+    assert p["contract_flops"] == 8 * 2 * 2 * 1 + 32 * 2 * 2
+    assert p["ms_contract"] > 0.0
+    assert st.bond_dims()[1] == 2
+    st.apply_unitary2(0, bell)  # m = 2, n = 4 (bond 1 has dimension 2), K = 1
+    st.sync()
+    p = st.perf()
+    assert p["contract_flops"] == 160 + 8 * 2 * 4 * 1 + 32 * 2 * 4
