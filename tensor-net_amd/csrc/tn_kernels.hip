@@ -346,16 +346,18 @@ __global__ __launch_bounds__(256) void k_theta(const UpdDesc* __restrict__ descs
     for (int s = threadIdx.x; s < kMaxSweeps; s += blockDim.x) d.offs[s * nt] = 0.0;
   }
   if (ts.mode != 0) {
-    const size_t tot = (size_t)ts.R * chiM;
-    const size_t stride = (size_t)gridDim.x * blockDim.x;
-    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += stride) {
+    // element counts stay below 2^32 (R <= 2048 rows, chiM <= 1024): 32-bit
+    // index arithmetic
+    const unsigned tot = (unsigned)ts.R * (unsigned)chiM;
+    const unsigned stride = gridDim.x * blockDim.x;
+    for (unsigned e = blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += stride) {
       if (ts.mode == 1) {
-        const int u = (int)(e / ts.n), r = (int)(e % ts.n);
+        const int u = (int)(e / (unsigned)ts.n), r = (int)(e % (unsigned)ts.n);
         double f = d.lamM[u];
         if (d.lamR) f *= d.lamR[r % chiR];
         d.X[(size_t)u * d.ldx + r] = cconj(cscale(d.gk1[e], f));
       } else {
-        const int r = (int)(e / chiM), u = (int)(e % chiM);
+        const int r = (int)(e / (unsigned)chiM), u = (int)(e % (unsigned)chiM);
         double f = d.lamM[u];
         if (d.lamL) f *= d.lamL[r >> 1];
         d.X[(size_t)u * d.ldx + r] = cscale(d.gk[e], f);
@@ -376,10 +378,9 @@ __global__ __launch_bounds__(256) void k_theta(const UpdDesc* __restrict__ descs
 __global__ __launch_bounds__(256) void k_winit(const UpdDesc* __restrict__ descs) {
   const UpdDesc& d = descs[blockIdx.y];
   const int C = task_shape(d).C;
-  const size_t tot = (size_t)C * C;
-  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < tot;
-       e += (size_t)gridDim.x * blockDim.x) {
-    const int i = (int)(e % C), j = (int)(e / C);
+  const unsigned tot = (unsigned)C * (unsigned)C;  // C <= 1024
+  for (unsigned e = blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += gridDim.x * blockDim.x) {
+    const int i = (int)(e % (unsigned)C), j = (int)(e / (unsigned)C);
     d.W[(size_t)j * d.ldw + i] = make_double2(i == j ? 1.0 : 0.0, 0.0);
   }
 }
@@ -580,12 +581,15 @@ __global__ __launch_bounds__(256) void k_split(const UpdDesc* __restrict__ descs
   const int mode = d.info[kInfoMode];
   const int rank = d.info[kInfoRank];
   __shared__ double red[4];
-  const size_t nk = (size_t)chiL * 2 * rank, nk1 = (size_t)rank * 2 * chiR;
-  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  // element counts stay below 2^32 (chi <= 1024, rank <= 1024): 32-bit
+  // index arithmetic
+  const unsigned nk = (unsigned)chiL * 2u * (unsigned)rank;
+  const unsigned nk1 = (unsigned)rank * 2u * (unsigned)chiR;
+  const unsigned stride = gridDim.x * blockDim.x;
   double accA = 0.0, accB = 0.0;
-  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < nk; e += stride) {
-    const int j = (int)(e % rank);
-    const int row = (int)(e / rank);  // v*2 + s
+  for (unsigned e = blockIdx.x * blockDim.x + threadIdx.x; e < nk; e += stride) {
+    const int j = (int)(e % (unsigned)rank);
+    const int row = (int)(e / (unsigned)rank);  // v*2 + s
     const int v = row >> 1;
     const int pj = d.perm[j];
     const double wl = d.lamL ? d.lamL[v] : 1.0;
@@ -604,10 +608,10 @@ __global__ __launch_bounds__(256) void k_split(const UpdDesc* __restrict__ descs
     const double sj = d.sval[j];
     accA += cabs2(u) * (wl * wl) * (sj * sj);
   }
-  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < nk1; e += stride) {
-    const int j = (int)(e / (2 * (size_t)chiR));
-    const int cidx = (int)(e % (2 * (size_t)chiR));  // t*chiR + w
-    const int w = cidx % chiR;
+  for (unsigned e = blockIdx.x * blockDim.x + threadIdx.x; e < nk1; e += stride) {
+    const int j = (int)(e / (2u * (unsigned)chiR));
+    const int cidx = (int)(e % (2u * (unsigned)chiR));  // t*chiR + w
+    const int w = cidx < chiR ? cidx : cidx - chiR;
     const int pj = d.perm[j];
     const double sj = d.sval[j];
     const double wr = d.lamR ? d.lamR[w] : 1.0;
@@ -660,11 +664,12 @@ __global__ __launch_bounds__(256) void k_renorm(const UpdDesc* __restrict__ desc
   }
   __syncthreads();
   const double nA = nrm[0], nB = nrm[1];
-  const size_t nk = (size_t)chiL * 2 * rank, nk1 = (size_t)rank * 2 * chiR;
-  const size_t stride = (size_t)gridDim.x * blockDim.x;
-  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < nk; e += stride)
+  const unsigned nk = (unsigned)chiL * 2u * (unsigned)rank;
+  const unsigned nk1 = (unsigned)rank * 2u * (unsigned)chiR;
+  const unsigned stride = gridDim.x * blockDim.x;
+  for (unsigned e = blockIdx.x * blockDim.x + threadIdx.x; e < nk; e += stride)
     d.gk[e] = cdivr(d.gk[e], nA);
-  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < nk1; e += stride)
+  for (unsigned e = blockIdx.x * blockDim.x + threadIdx.x; e < nk1; e += stride)
     d.gk1[e] = cdivr(d.gk1[e], nB);
 }
 
@@ -675,15 +680,22 @@ __global__ __launch_bounds__(1024) void k_measure(const MeasDesc* __restrict__ d
   __shared__ double red[16];
   __shared__ int s_out;
   __shared__ double s_prob;
+  // wave w takes rows v = w, w + 16, ...; its lanes run along u, so the two
+  // rows (v, s=0) and (v, s=1) of Gamma[k] are read contiguously
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
   double p0 = 0.0, p1 = 0.0;
-  const size_t tot = (size_t)chiL * chiR;
-  for (size_t e = threadIdx.x; e < tot; e += blockDim.x) {
-    const int v = (int)(e / chiR), u = (int)(e % chiR);
-    double wgt = 1.0;
-    if (d.lamL) wgt *= d.lamL[v] * d.lamL[v];
-    if (d.lamR) wgt *= d.lamR[u] * d.lamR[u];
-    p0 += cabs2(d.g[((size_t)v * 2 + 0) * chiR + u]) * wgt;
-    p1 += cabs2(d.g[((size_t)v * 2 + 1) * chiR + u]) * wgt;
+  for (int v = wid; v < chiL; v += nw) {
+    const double wl = d.lamL ? d.lamL[v] * d.lamL[v] : 1.0;
+    const double2* r0 = d.g + (size_t)v * 2 * chiR;
+    const double2* r1 = r0 + chiR;
+    double q0 = 0.0, q1 = 0.0;
+    for (int u = lane; u < chiR; u += 64) {
+      const double wr = d.lamR ? d.lamR[u] * d.lamR[u] : 1.0;
+      q0 += cabs2(r0[u]) * wr;
+      q1 += cabs2(r1[u]) * wr;
+    }
+    p0 += q0 * wl;
+    p1 += q1 * wl;
   }
   p0 = block_sum(p0, red);
   p1 = block_sum(p1, red);
@@ -721,12 +733,13 @@ __global__ __launch_bounds__(1024) void k_measure(const MeasDesc* __restrict__ d
   if (d.forced == -2) return;
   const int out = s_out;
   const double renorm = sqrt(s_prob);
-  for (size_t e = threadIdx.x; e < tot; e += blockDim.x) {
-    const int v = (int)(e / chiR), u = (int)(e % chiR);
-    double2* keep = d.g + ((size_t)v * 2 + out) * chiR + u;
-    double2* drop = d.g + ((size_t)v * 2 + (1 - out)) * chiR + u;
-    *keep = cdivr(*keep, renorm);
-    *drop = make_double2(0.0, 0.0);
+  for (int v = wid; v < chiL; v += nw) {
+    double2* keep = d.g + ((size_t)v * 2 + out) * chiR;
+    double2* drop = d.g + ((size_t)v * 2 + (1 - out)) * chiR;
+    for (int u = lane; u < chiR; u += 64) {
+      keep[u] = cdivr(keep[u], renorm);
+      drop[u] = make_double2(0.0, 0.0);
+    }
   }
 }
 
