@@ -118,7 +118,7 @@ tn_status tn_mps_new_qubits_in(tn_engine* e, size_t n, tn_trunc trunc, size_t ch
 /* Contraction arithmetic of this handle (TN_PREC_F64 default).  With
  * TN_PREC_F32 (TN_PREC_BF16X3) the rank rule drops normalised singular values
  * below 2^-20 (2^-13), and the Jacobi SVD stops after a sweep whose largest
- * relative off-diagonal is at most 1e-5 (1e-3) instead of 1e-8: theta's own
+ * relative off-diagonal is at most 1e-4 (1e-3) instead of 1e-8: theta's own
  * round-off is larger than what further sweeps would remove. */
 tn_status tn_set_precision(tn_mps* h, int prec);
 /* Replace the engine's launch-shape table (1..16 rows, c_max increasing,
