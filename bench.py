@@ -402,7 +402,7 @@ def run_serving(args) -> dict:
                                 gen_seed=args.seed, traj_id=len(reqs), gen_depth=n))
     log(f"{len(reqs)} requests over {dur}s (rate {rate}/s)")
     srv = Server(chi_max=max(chis), hbm_budget=args.hbm_budget * 1e9, log=log,
-                 workers=args.serve_workers)
+                 workers=args.serve_workers, dispatch=args.serve_dispatch)
     u0, _ = srv.stats()
     t0 = time.perf_counter()
     srv.run(reqs)
@@ -417,7 +417,8 @@ def run_serving(args) -> dict:
         "dtype": "f32 contraction, f64 SVD", "data": "synthetic",
         "config": {"workload": f"Poisson arrivals {rate}/s for {dur}s, N in {ns}, p~U[0,0.4], "
                                f"chi_max in {chis}, depth=N, Haar brickwork + Z measurements, "
-                               f"{args.serve_workers} lock-step worker(s)",
+                               f"{args.serve_workers} lock-step worker(s), "
+                               f"{args.serve_dispatch} dispatch",
                    "circuits": "host-generated" if args.serve_host_circuits
                                else "generated on the device (Philox4x64-10 streams)",
                    "requests": len(reqs), "completed": len(lat),
@@ -460,6 +461,9 @@ def main() -> int:
     ap.add_argument("--hbm-budget", type=float, default=64.0, help="serving: GB of MPS state")
     ap.add_argument("--serve-workers", type=int, default=4,
                     help="serving: lock-step loops, one engine, stream and host thread each")
+    ap.add_argument("--serve-dispatch", default="least_loaded",
+                    choices=["least_loaded", "round_robin"],
+                    help="serving: how requests are dealt to the workers")
     ap.add_argument("--serve-host-circuits", action="store_true",
                     help="serving: requests carry host-generated circuits")
     ap.add_argument("--tune", default="auto",

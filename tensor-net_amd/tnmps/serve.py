@@ -22,10 +22,16 @@ trajectories stays within the budget (slots are counted in bytes of Gamma
 and lambda); finished trajectories leave at once and free their memory.
 
 With ``workers=W > 1`` the server runs W such lock-step loops at once, each
-on its own engine (stream) and host thread; request i of the arrival order
-goes to worker i mod W.  A lock-step tick lasts as long as its slowest
-SVD, so independent loops keep one worker's slow tail from holding back
-the others' trajectories.  The HBM budget is shared.
+on its own engine (stream) and host thread.  A lock-step tick lasts as long
+as its slowest SVD, so independent loops keep one worker's slow tail from
+holding back the others' trajectories.  The HBM budget is shared.  Requests
+go to workers by one of two rules (``dispatch``):
+- ``"least_loaded"`` (default): one arrival-ordered queue; a worker admits
+  the next arrived request at its tick boundary only while no other worker
+  holds less outstanding work (an idle worker holds none).  A request's work
+  is estimated as its remaining layers x n x chi_eff^3, chi_eff the largest
+  bond it can reach: the cost of a layer's SVDs.
+- ``"round_robin"``: request i of the arrival order goes to worker i mod W.
 """
 from __future__ import annotations
 
@@ -37,6 +43,13 @@ from typing import Callable, List, Optional, Sequence
 import numpy as np
 
 from .mps import Engine, MPS
+
+
+def layer_cost(n: int, chi: int) -> float:
+    """Relative cost of one layer of an n-qubit Const(chi) trajectory: about
+    2n two-site SVDs of order chi_eff^3, chi_eff the largest reachable bond."""
+    ce = min(chi, 2 ** min(n // 2, 40))
+    return float(n) * float(ce) ** 3
 
 
 def mps_bytes(n: int, chi: int) -> int:
@@ -88,8 +101,13 @@ class Server:
     worker); ``workers`` lock-step loops run at once, one engine each."""
 
     def __init__(self, chi_max: int = 256, hbm_budget: float = 64e9, max_active: int = 1024,
-                 log: Optional[Callable[[str], None]] = None, workers: int = 1):
+                 log: Optional[Callable[[str], None]] = None, workers: int = 1,
+                 dispatch: str = "least_loaded"):
+        if dispatch not in ("least_loaded", "round_robin"):
+            raise ValueError(f"dispatch {dispatch!r}")
+        self.dispatch = dispatch
         self.engines = [Engine(chi_max) for _ in range(max(1, workers))]
+        self._load = [0.0] * len(self.engines)  # outstanding work per worker
         self.engine = self.engines[0]
         self.chi_max = chi_max
         self.hbm_budget = hbm_budget
@@ -199,10 +217,14 @@ class Server:
             raise ValueError("more than one worker needs realtime=True")
         W = len(self.engines)
         errors: list = []
+        shared = {"queue": pending, "next": 0}
 
         def work(w):
             try:
-                self._loop(self.engines[w], pending[w::W], t0, clock, True)
+                if self.dispatch == "round_robin":
+                    self._loop(self.engines[w], pending[w::W], t0, clock, True)
+                else:
+                    self._loop_shared(w, shared, t0, clock)
             except Exception as exc:  # surfaced after the join
                 errors.append(exc)
 
@@ -238,6 +260,43 @@ class Server:
                     time.sleep(0.005)   # budget held by other workers
                 continue
             active = self._tick(eng, active, t0, clock)
+
+
+    def _loop_shared(self, w: int, shared: dict, t0: float, clock) -> None:
+        """Worker w's lock-step loop over the shared arrival-ordered queue,
+        least-loaded rule (see the module docstring)."""
+        eng = self.engines[w]
+        queue = shared["queue"]
+        active: list = []
+        while True:
+            now = clock() - t0
+            with self._lock:
+                while (shared["next"] < len(queue)
+                       and queue[shared["next"]].arrival <= now + 1e-12
+                       and len(active) < self.max_active
+                       and may_admit(self._load, w)):
+                    r = queue[shared["next"]]
+                    if not self._try_admit(r, eng, active, t0, clock):
+                        break
+                    shared["next"] += 1
+                    self._load[w] += r.depth * layer_cost(r.n, r.chi)
+                drained = shared["next"] >= len(queue)
+            if not active:
+                if drained:
+                    return
+                nxt = queue[min(shared["next"], len(queue) - 1)].arrival
+                time.sleep(min(max(nxt - now, 0.0), 0.005) or 0.001)
+                continue
+            active = self._tick(eng, active, t0, clock)
+            with self._lock:
+                self._load[w] = sum((a[0].depth - a[3]) * layer_cost(a[0].n, a[0].chi)
+                                    for a in active)
+
+
+def may_admit(load: Sequence[float], w: int) -> bool:
+    """Least-loaded rule: worker w may take the next request when no other
+    worker holds less outstanding work."""
+    return load[w] <= min(load)
 
 
 def percentile(xs: Sequence[float], q: float) -> float:

@@ -93,10 +93,11 @@ def test_serving_generated_equals_alone(gpu):
             assert close(a, b, rtol=1e-12)
 
 
-def test_serving_workers_equal_alone(gpu):
+@pytest.mark.parametrize("dispatch", ["least_loaded", "round_robin"])
+def test_serving_workers_equal_alone(gpu, dispatch):
     """Two lock-step workers (engines and host threads) sharing one budget:
     every request, carried or generated on the device, gives the results it
-    gives alone."""
+    gives alone, whichever rule deals the requests to the workers."""
     reqs = _requests()
     for r in reqs:
         r.arrival *= 0.05   # real time: arrivals within 0.4 s
@@ -105,7 +106,7 @@ def test_serving_workers_equal_alone(gpu):
         reqs.append(Request(rid=20 + i, n=n, p=p, chi=chi, arrival=t, gen_seed=seed,
                             traj_id=200 + i, gen_depth=n))
     budget = sum(mps_bytes(r.n, r.chi) for r in reqs[:4])
-    srv = Server(chi_max=32, hbm_budget=budget, workers=2)
+    srv = Server(chi_max=32, hbm_budget=budget, workers=2, dispatch=dispatch)
     srv.run(reqs)
     assert srv.peak_held <= budget
     assert srv.stats()[0] > 0
