@@ -202,7 +202,8 @@ class MPS:
         return self
 
     def set_tuning(self, rows) -> "MPS":
-        """Launch-shape table of this handle's engine (see :func:`tuning_rows`)."""
+        """Launch-shape table of this handle's engine (rows as for
+        :meth:`Engine.set_tuning`)."""
         arr, n = _tune_array(rows)
         _check(self._lib.tn_set_tuning(self._h, arr, n), "set_tuning")
         return self
