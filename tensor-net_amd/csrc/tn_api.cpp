@@ -204,6 +204,8 @@ struct tn_engine {
   double* part = nullptr;
   double2* gpart = nullptr;
   size_t gpart_per = 0;  // elements of gpart per slot
+  int gpart_slots = 0;   // slots gpart covers: the row split (S > 1) only runs
+                         // for batches of at most `cus` tasks
   double* offs = nullptr;
   int* info = nullptr;
   int* act = nullptr;    // active-task list of the current sweep: count, then tasks
@@ -257,6 +259,7 @@ struct tn_engine {
     hhfree(h_offs);
     h_offs = nullptr;
     gpart = nullptr;
+    gpart_slots = 0;
     X = W = nullptr;
     sraw = sval = fac = part = offs = nullptr;
     perm = info = act = nullptr;
@@ -441,7 +444,8 @@ struct tn_engine {
     TN_HIP(hipMalloc(&offs, sizeof(double) * kMaxSweeps * ns));
     TN_HIP(hipHostMalloc(&h_offs, sizeof(double) * kMaxSweeps * ns, hipHostMallocDefault));
     TN_HIP(hipMalloc(&info, sizeof(int) * kInfoLen * ns));
-    TN_HIP(hipMalloc(&gpart, sizeof(double2) * gpart_per * ns));
+    gpart_slots = std::min(ns, std::max(cus, 1));
+    TN_HIP(hipMalloc(&gpart, sizeof(double2) * gpart_per * gpart_slots));
     TN_HIP(hipMalloc(&act, sizeof(int) * (ns + 1)));
     slots = ns;
     return upd.ensure(ns, stream);
@@ -689,8 +693,10 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
   // CUs idle: split each pair's rows over S workgroups (gram kernel, then
   // solve + apply kernel).
   int S = 1;
+  // (S > 1 needs nt * npairs <= cus, so nt <= cus: the partial Gram buffer
+  // covers min(slots, cus) tasks)
   while (S < tu.split_max && (size_t)nt * npairs * S * 2 <= (size_t)2 * e->cus &&
-         maxR >= (size_t)tu.split_rows * S)
+         maxR >= (size_t)tu.split_rows * S && nt <= e->gpart_slots)
     S *= 2;
   JacobiArgs ja{};
   ja.X = e->X;
