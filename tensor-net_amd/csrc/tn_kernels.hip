@@ -357,10 +357,12 @@ __global__ __launch_bounds__(256) void k_theta(const UpdDesc* __restrict__ descs
         if (d.lamR) f *= d.lamR[r % chiR];
         d.X[(size_t)u * d.ldx + r] = cconj(cscale(d.gk1[e], f));
       } else {
-        const int r = (int)(e / (unsigned)chiM), u = (int)(e % (unsigned)chiM);
+        // e runs down X's columns, so the writes are coalesced; Gamma[k]
+        // (row-major, 2 MiB at chi = 256) is read across its rows from L2
+        const int u = (int)(e / (unsigned)ts.m), r = (int)(e % (unsigned)ts.m);
         double f = d.lamM[u];
         if (d.lamL) f *= d.lamL[r >> 1];
-        d.X[(size_t)u * d.ldx + r] = cscale(d.gk[e], f);
+        d.X[(size_t)u * d.ldx + r] = cscale(d.gk[(size_t)r * chiM + u], f);
       }
     }
     return;
