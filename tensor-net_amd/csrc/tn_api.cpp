@@ -73,7 +73,7 @@ double env_double(const char* name, double dflt) {
 // (1e-8, 1e-6), below theta's own round-off.
 // The residual only tilts the new left factor's columns from orthogonal by
 // that much: X W = U~ exactly with W unitary, so the split still writes theta
-// exactly.  A batch uses the strictest tolerance of its tasks.
+// exactly.  A launch batch holds tasks of one precision (run_tasks).
 struct Tolerances {
   double tol_rot = 1e-14;
   double conv_tol = 1e-8;
@@ -724,8 +724,8 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
   ja.Rh = (int)maxR;
   ja.Ch = (int)maxC;
   ja.tol_rot = tol.tol_rot;
-  double conv_tol = tol.for_prec(tasks[0].h->prec);
-  for (int t = 1; t < nt; ++t) conv_tol = std::min(conv_tol, tol.for_prec(tasks[t].h->prec));
+  // one precision per launch batch (run_tasks)
+  const double conv_tol = tol.for_prec(tasks[0].h->prec);
   ja.conv_tol = conv_tol;
   ja.tiny2 = tol.tiny_norm * tol.tiny_norm;
   // Sweeps are enqueued one ahead while the last sweep read back is still far
@@ -828,7 +828,22 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
   return TN_OK;
 }
 
+// Tasks of different contraction precisions run as separate launch batches:
+// each precision's SVDs stop at its own tolerance, so a task's sweeps do not
+// depend on the batch it runs in.  (The tasks of one call are independent,
+// so any grouping gives the same results.)
 tn_status run_tasks(tn_engine* e, const std::vector<Task>& tasks) {
+  bool mixed = false;
+  for (const Task& t : tasks) mixed = mixed || t.h->prec != tasks[0].h->prec;
+  if (mixed) {
+    for (int prec : {TN_PREC_F64, TN_PREC_F32, TN_PREC_BF16X3}) {
+      std::vector<Task> sub;
+      for (const Task& t : tasks)
+        if (t.h->prec == prec) sub.push_back(t);
+      if (!sub.empty()) TN_TRY(run_tasks(e, sub));
+    }
+    return TN_OK;
+  }
   const int nt = (int)tasks.size();
   for (int a = 0; a < nt; a += e->max_slots)
     TN_TRY(run_chunk(e, tasks.data() + a, std::min(e->max_slots, nt - a)));

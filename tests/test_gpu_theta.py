@@ -157,3 +157,26 @@ def test_contraction_perf_counts(gpu):
     st.sync()
     p = st.perf()
     assert p["contract_flops"] == 160 + 8 * 2 * 4 * 1 + 32 * 2 * 4
+
+
+def test_mixed_precision_batch_equals_solo(gpu):
+    """One engine batch holding an FP64, an FP32 and a BF16-split trajectory:
+    every task converges by its own precision's tolerance, so each one gives
+    what it gives alone (1e-12, as for single-precision batches)."""
+    n, depth, p, chi = 12, 10, 0.15, 32
+    c = haar_circuit(n, depth, p, 91)
+    rec = run_trajectory(n, c.layers, ("const", chi), rs=c.draws)["record"]
+    precs = ("f64", "f32", "bf16x3")
+    solo = []
+    for prec in precs:
+        st = MPS.new_qubits(n, ("const", chi)).set_precision(prec)
+        st.run_circuit(c.layers, forced=rec)
+        solo.append(st)
+    eng = Engine(chi)
+    batch = [eng.new_qubits(n, ("const", chi)).set_precision(prec) for prec in precs]
+    for t in range(depth):
+        eng.batch_bilayer(batch, [c.layers[t]] * 3, forced=[rec[t]] * 3)
+    for prec, a, b in zip(precs, batch, solo):
+        assert a.bond_dims() == b.bond_dims(), prec
+        for bd in range(n - 1):
+            assert_spectrum_close(a.schmidt(bd), b.schmidt(bd), rtol=1e-12, msg=f"{prec} bond {bd}")
