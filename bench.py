@@ -116,6 +116,11 @@ def run_batched(args) -> dict:
     draws = [[iter(circs[i].draws) for i in idx] for idx in groups]
     ents = [[0.0] * len(idx) for idx in groups]
     errors = []
+    # outcome records and entropies of the first trajectories of group 0, for
+    # the FP64 replay that states a reduced-precision run's error
+    n_rep = min(2, len(groups[0])) if args.prec != "f64" else 0
+    rep_rec = [[] for _ in range(n_rep)]
+    rep_ent = [[] for _ in range(n_rep)]
 
     def layers(g, t0, t1, label):
         try:
@@ -123,6 +128,10 @@ def run_batched(args) -> dict:
                 res = engs[g].batch_bilayer(states[g], [circs[i].layers[t] for i in groups[g]],
                                             draws=draws[g])
                 ents[g] = [r[2] for r in res]
+                if g == 0:
+                    for j in range(n_rep):
+                        rep_rec[j].append(res[j][0])
+                        rep_ent[j].append(res[j][2])
                 if g == 0:
                     log(f"{label} layer {t - t0 + 1}/{t1 - t0} ({time.perf_counter() - t_w:.1f}s)")
         except Exception as exc:  # surfaced after the join
@@ -160,6 +169,20 @@ def run_batched(args) -> dict:
     jac_s = max(pf["ms_jacobi"] for pf in perfs) / 1e3
     achieved = perf["svd_flops"] / wall / 1e12
     mean_ent = float(np.mean([x for e in ents for x in e]))
+    replay = None
+    if n_rep:
+        # the same trajectories in FP64 on their recorded outcomes (untimed)
+        from tnmps import MPS
+        err = 0.0
+        for j in range(n_rep):
+            i = groups[0][j]
+            ref = MPS.new_qubits(n, ("const", chi))
+            got = ref.run_circuit(circs[i].layers[:W + K], forced=rep_rec[j])
+            err = max(err, max(abs(a - b) for a, b in zip(rep_ent[j], got["entropy"][1:])))
+            del ref
+        replay = {"trajectories": n_rep, "layers": W + K,
+                  "max_half_chain_entropy_abs_err_vs_fp64": float(f"{err:.3e}")}
+        log(f"FP64 replay of {n_rep} trajectories: entropy error {err:.3e}")
     return {
         "metric": "two-site updates/s (batched lock-step trajectories)",
         "value": round(updates / wall, 2), "unit": "two-site updates/s", "n_gpus": 1,
@@ -175,7 +198,7 @@ def run_batched(args) -> dict:
                    "trajectory_layers_per_s": round(T * K / wall, 2),
                    "trajectories_per_s_equiv": round(T * K / wall / depth, 3),
                    "mean_half_chain_entropy_last_layer": round(mean_ent, 6),
-                   "tuning": tune_label},
+                   "tuning": tune_label, "fp64_replay": replay},
         "roofline": {"bound": "mfma", "kernel": "jacobi step kernels",
                      "achieved": round(achieved, 4), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / FP64_PEAK_TFLOPS, 5), "traffic": None,
