@@ -66,8 +66,10 @@ typedef struct {
 enum { TN_TRUNC_NONE = 0, TN_TRUNC_CONST = 1, TN_TRUNC_CUTOFF = 2 };
 
 /* Arithmetic of the two-site contraction theta = U (lambda Gamma lambda Gamma
- * lambda).  The SVD, truncation, lambda renormalisation and every stored
- * tensor stay FP64 in both modes. */
+ * lambda): FP64; FP32 with FP32 accumulation; or each FP32 operand split into
+ * two BF16 halves, three BF16 products per real product, FP32 accumulation.
+ * The SVD, truncation, lambda renormalisation and every stored tensor stay
+ * FP64 in all three modes. */
 enum { TN_PREC_F64 = 0, TN_PREC_F32 = 1, TN_PREC_BF16X3 = 2 };
 
 /* One row of an engine's launch-shape table.  An update batch whose largest
