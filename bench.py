@@ -104,7 +104,7 @@ def run_batched(args) -> dict:
     log(f"generated {T} circuits in {time.perf_counter() - t_gen:.1f}s")
     groups = [list(range(g, T, G)) for g in range(G)]
     engs = [Engine(chi) for _ in range(G)]
-    tune_label, tune_rows = tuning_for(args, n, chi, args.prec, batched=True)
+    tune_label, tune_rows = tuning_for(args, n, chi, args.prec, batched=True, groups=G)
     if tune_rows is not None:
         for e in engs:
             e.set_tuning(tune_rows)
@@ -215,8 +215,10 @@ def run_batched(args) -> dict:
 TUNED_TABLES = REPO / "tensor-net_amd" / "tnmps" / "tuning_mi355x.json"
 
 
-def tuning_for(args, n: int, chi: int, prec: str, batched: bool):
-    """(label, rows) of the launch-shape table to use, or ("default", None)."""
+def tuning_for(args, n: int, chi: int, prec: str, batched: bool, groups: int = 1):
+    """(label, rows) of the launch-shape table to use, or ("default", None):
+    the tuned entry whose workload has this N, chi, precision, batching and
+    number of concurrent groups."""
     if args.tune == "none":
         return "default", None
     if args.tune != "auto":
@@ -228,7 +230,8 @@ def tuning_for(args, n: int, chi: int, prec: str, batched: bool):
     data = json.loads(TUNED_TABLES.read_text())
     for key, ent in data.items():
         w = ent["workload"]
-        if (w["n"], w["chi"], w["prec"]) == (n, chi, prec) and (w["traj"] > 1) == batched:
+        if ((w["n"], w["chi"], w["prec"]) == (n, chi, prec) and (w["traj"] > 1) == batched
+                and w.get("groups", 1) == groups):
             return key, ent["table"]
     return "default", None
 

@@ -26,6 +26,7 @@ from __future__ import annotations
 import argparse
 import json
 import sys
+import threading
 import time
 from typing import Dict, List, Sequence
 
@@ -52,46 +53,76 @@ def bucket_of(table: List[dict], C: int) -> int:
 
 
 class _Workload:
-    """Saturated trajectories on one engine plus a recorded stretch of layers."""
+    """Saturated trajectories plus a recorded stretch of layers, dealt
+    round-robin into ``groups`` lock-step groups, one engine each; the groups
+    run in their own host threads at once, as the batched driver runs them."""
 
-    def __init__(self, n, chi, p, traj, warm, layers, seed, prec, log):
-        self.eng = Engine(chi)
+    def __init__(self, n, chi, p, traj, warm, layers, seed, prec, log, groups=1):
+        G = max(1, min(groups, traj))
+        self.engs = [Engine(chi) for _ in range(G)]
         self.n, self.chi = n, chi
         depth = warm + layers
         circs = [haar_circuit_fast(n, depth, p, seed + 11 * i) for i in range(traj)]
-        self.states = [self.eng.new_qubits(n, ("const", chi)) for _ in range(traj)]
-        for st in self.states:
-            st.set_precision(prec)
+        self.idx = [list(range(g, traj, G)) for g in range(G)]
+        self.states = [[self.engs[g].new_qubits(n, ("const", chi)) for _ in ix]
+                       for g, ix in enumerate(self.idx)]
+        for grp in self.states:
+            for st in grp:
+                st.set_precision(prec)
         draws = [iter(c.draws) for c in circs]
         t0 = time.perf_counter()
-        for t in range(warm):
-            self.eng.batch_bilayer(self.states, [c.layers[t] for c in circs], draws=draws)
-        self.eng.sync()
-        log(f"autotune: {traj} x N={n} chi={chi} p={p} saturated in "
+        for g, ix in enumerate(self.idx):
+            for t in range(warm):
+                self.engs[g].batch_bilayer(self.states[g], [circs[i].layers[t] for i in ix],
+                                           draws=[draws[i] for i in ix])
+            self.engs[g].sync()
+        log(f"autotune: {traj} x N={n} chi={chi} p={p} in {G} group(s) saturated in "
             f"{time.perf_counter() - t0:.1f}s, max bond "
-            f"{max(s.max_bond_dim() for s in self.states)}")
-        self.layers = [[c.layers[t] for c in circs] for t in range(warm, depth)]
+            f"{max(s.max_bond_dim() for grp in self.states for s in grp)}")
+        self.layers = [[[circs[i].layers[t] for i in ix] for t in range(warm, depth)]
+                       for ix in self.idx]
         # the outcomes of the timed stretch, from one sampled pass on clones
-        clones = [s.clone() for s in self.states]
-        self.forced = [[] for _ in range(traj)]
-        for lay in self.layers:
-            res = self.eng.batch_bilayer(clones, lay, draws=draws)
-            for i, r in enumerate(res):
-                self.forced[i].append(r[0])
-        self.eng.sync()
-        del clones
+        self.forced = []
+        for g, ix in enumerate(self.idx):
+            clones = [s.clone() for s in self.states[g]]
+            rec = [[] for _ in ix]
+            for lay in self.layers[g]:
+                res = self.engs[g].batch_bilayer(clones, lay, draws=[draws[i] for i in ix])
+                for j, r in enumerate(res):
+                    rec[j].append(r[0])
+            self.engs[g].sync()
+            self.forced.append(rec)
+            del clones
 
     def time_table(self, table: List[dict], repeat: int) -> float:
-        """Best wall time of the recorded stretch under ``table``."""
-        self.eng.set_tuning(table)
+        """Best wall time of the recorded stretch under ``table`` (every group
+        at once)."""
+        for e in self.engs:
+            e.set_tuning(table)
         best = float("inf")
         for _ in range(repeat):
-            clones = [s.clone() for s in self.states]
-            self.eng.sync()
+            clones = [[s.clone() for s in grp] for grp in self.states]
+            for e in self.engs:
+                e.sync()
+            errors: list = []
+
+            def run(g):
+                try:
+                    for t, lay in enumerate(self.layers[g]):
+                        self.engs[g].batch_bilayer(clones[g], lay,
+                                                   forced=[f[t] for f in self.forced[g]])
+                    self.engs[g].sync()
+                except Exception as exc:  # surfaced after the join
+                    errors.append(exc)
+
             t0 = time.perf_counter()
-            for t, lay in enumerate(self.layers):
-                self.eng.batch_bilayer(clones, lay, forced=[f[t] for f in self.forced])
-            self.eng.sync()
+            th = [threading.Thread(target=run, args=(g,)) for g in range(len(self.engs))]
+            for x in th:
+                x.start()
+            for x in th:
+                x.join()
+            if errors:
+                raise errors[0]
             best = min(best, time.perf_counter() - t0)
             del clones
         return best
@@ -99,7 +130,7 @@ class _Workload:
 
 def tune(n: int = 64, chi: int = 256, p: float = 0.1, traj: int = 1, warm: int = 0,
          layers: int = 2, repeat: int = 2, seed: int = 10546, prec: str = "f64",
-         min_gain: float = 0.02, log=print) -> dict:
+         min_gain: float = 0.02, log=print, groups: int = 1) -> dict:
     """Tune the table rows of buckets 2chi and chi on the workload described.
 
     ``warm`` layers bring the chain to saturation (0: 2 log2(chi) + 4).
@@ -107,8 +138,8 @@ def tune(n: int = 64, chi: int = 256, p: float = 0.1, traj: int = 1, warm: int =
     holds every timed candidate (bucket, field, value, seconds)."""
     if warm <= 0:
         warm = 2 * max(1, int(chi).bit_length()) + 4
-    w = _Workload(n, chi, p, traj, warm, layers, seed, prec, log)
-    table = w.eng.get_tuning()
+    w = _Workload(n, chi, p, traj, warm, layers, seed, prec, log, groups)
+    table = w.engs[0].get_tuning()
     base = w.time_table(table, repeat)
     trials = [{"bucket": None, "field": None, "value": None, "s": round(base, 5)}]
     log(f"autotune: default table {base * 1e3:.1f} ms")
@@ -130,11 +161,13 @@ def tune(n: int = 64, chi: int = 256, p: float = 0.1, traj: int = 1, warm: int =
                     f"(best {best * 1e3:.1f})")
                 if s < best * (1.0 - min_gain):
                     table, best = cand, s
-    w.eng.set_tuning(table)
+    for e in w.engs:
+        e.set_tuning(table)
     return {"table": table, "default_s": round(base, 5), "tuned_s": round(best, 5),
             "trials": trials,
             "workload": {"n": n, "chi": chi, "p": p, "traj": traj, "warm": warm,
-                         "layers": layers, "repeat": repeat, "seed": seed, "prec": prec}}
+                         "layers": layers, "repeat": repeat, "seed": seed, "prec": prec,
+                         "groups": len(w.engs)}}
 
 
 def load_table(path: str, key: str) -> List[dict]:
@@ -158,6 +191,8 @@ def main(argv=None) -> int:
     ap.add_argument("--layers", type=int, default=2)
     ap.add_argument("--repeat", type=int, default=2)
     ap.add_argument("--prec", default="f64", choices=["f64", "f32", "bf16x3"])
+    ap.add_argument("--groups", type=int, default=1,
+                    help="lock-step groups run at once, one engine and host thread each")
     ap.add_argument("--key", default=None, help="entry name in the output file")
     ap.add_argument("--out", required=True, help="JSON file (entries are merged)")
     a = ap.parse_args(argv)
@@ -165,8 +200,9 @@ def main(argv=None) -> int:
     def log(msg):
         print(msg, file=sys.stderr, flush=True)
 
-    res = tune(a.n, a.chi, a.p, a.traj, a.warm, a.layers, a.repeat, prec=a.prec, log=log)
-    key = a.key or f"traj{a.traj}_n{a.n}_chi{a.chi}_{a.prec}"
+    res = tune(a.n, a.chi, a.p, a.traj, a.warm, a.layers, a.repeat, prec=a.prec, log=log,
+               groups=a.groups)
+    key = a.key or f"traj{a.traj}_n{a.n}_chi{a.chi}_{a.prec}_g{res['workload']['groups']}"
     try:
         with open(a.out) as f:
             data = json.load(f)
