@@ -49,11 +49,21 @@ typedef enum {
     TN_SAMPLE_RANGE = 8,  /* r >= total probability: the reference panics on
                              unwrap (mps.rs:1444); the state is projected on
                              the last outcome with non-zero probability */
-    TN_CAPACITY = 9,      /* a Cutoff rank exceeded the handle's chi_cap */
+    TN_CAPACITY = 9,      /* the rank rule asked for more values than the
+                             bond's allocation (chi_cap) holds: the update
+                             kept that many and the state was written; the
+                             reference has no bound (None / Cutoff) */
     TN_DEVICE = 10,       /* HIP runtime error */
     TN_INVALID_ARG = 11,  /* null pointer / bad argument at the C boundary */
-    TN_NO_CONVERGENCE = 12 /* SVD hit its sweep limit (result still written) */
+    TN_NO_CONVERGENCE = 12 /* an SVD stopped at its sweep limit before its
+                              tolerance (the result is still written) */
 } tn_status;
+
+/* Bits of a handle's sticky failure flags (tn_take_flags).  Calls that run
+ * updates return TN_CAPACITY / TN_NO_CONVERGENCE when any update they
+ * enqueued set one (after any measurement status, which comes first); the
+ * handles that set them keep the bits until tn_take_flags. */
+enum { TN_FLAG_NO_CONVERGENCE = 1, TN_FLAG_CAPACITY = 2 };
 
 /* BondDim (mps.rs:157-166).  kind 0 = None (Cutoff at f64::EPSILON),
  * 1 = Const(chi), 2 = Cutoff(eps). */
@@ -99,8 +109,11 @@ int tn_device_count(void);
 
 /* MPS::new_qubits (mps.rs:2033-2039 -> MPS::new, mps.rs:370-391): n qubits in
  * |0...0>.  chi_cap bounds the device allocation per bond; 0 means
- * trunc.chi for Const, else 256.  stream is a hipStream_t or NULL (the
- * library creates one).  n == 0 -> TN_EMPTY_SYSTEM. */
+ * trunc.chi for Const, else the largest exact bond dimension of n qubits,
+ * min(2^{b+1}, 2^{n-b-1}), up to 512.  A None / Cutoff state that needs more
+ * than its bound keeps that many values and the call returns TN_CAPACITY.
+ * stream is a hipStream_t or NULL (the library creates one).  n == 0 ->
+ * TN_EMPTY_SYSTEM. */
 tn_status tn_mps_new_qubits(size_t n, tn_trunc trunc, size_t chi_cap,
                             void* stream, tn_mps** out);
 /* Drop (frees device memory).  A handle made by tn_mps_new_qubits frees its
@@ -148,6 +161,12 @@ tn_status tn_apply_u1(tn_mps* h, size_t k, const double* op);
 /* MPS::apply_unitary2 (mps.rs:1332-1387): contract -> gate -> SVD ->
  * truncate -> split -> renormalise.  op: 4x4 row-major, 32 doubles. */
 tn_status tn_apply_u2(tn_mps* h, size_t k, const double* op);
+/* MPS::apply_unitary2_multi (mps.rs:1393-1411): count 4x4 operators (32
+ * doubles each, back to back) multiplied in order, op[count-1] ... op[0],
+ * then applied as one apply_unitary2 at bond k (count == 0 applies the
+ * identity, which still refactors the bond, as the reference does).  No-op
+ * for n == 1 or k >= n - 1. */
+tn_status tn_apply_u2_multi(tn_mps* h, size_t k, size_t count, const double* ops);
 /* A sequence of apply_uni calls (mps.rs:2093-2108) as in the unitary loop of
  * circuit::apply_bilayer (circuit.rs:637).  dims[i] is 2 or 4 (operator
  * side); ops holds the operators back to back (8 or 32 doubles each).
@@ -289,6 +308,12 @@ tn_status tn_entropies(tn_engine* e, size_t count, tn_mps* const* hs,
  * contracted on the device and copied to out (2^(n+1) doubles, interleaved;
  * cap is out's length in doubles).  n <= 30. */
 tn_status tn_contract(tn_mps* h, double* out, size_t cap);
+
+/* The handle's sticky failure flags (TN_FLAG_* bits) collected since the
+ * last call, then cleared.  No reference counterpart: tells a batch caller
+ * which of its handles had a rank clamped (TN_CAPACITY) or an SVD stopped at
+ * the sweep limit (TN_NO_CONVERGENCE). */
+tn_status tn_take_flags(tn_mps* h, int* flags);
 
 /* Counters of the device work done by this handle (for throughput
  * accounting): number of two-site updates (contract -> SVD -> split) and

@@ -46,6 +46,21 @@ def entropy_from_rho(psi: np.ndarray, n: int, b: int) -> float:
     return float(-np.sum(w * np.log(w)))
 
 
+def renyi_from_rho(psi: np.ndarray, n: int, b: int, a: float) -> float:
+    """ln Tr rho_A^a / (1 - a) from the eigenvalues of the reduced density
+    matrix (eigh), no SVD; a == 1 is the von Neumann limit.  rho_A has rank
+    at most min(2^{b+1}, 2^{N-b-1}); its zero eigenvalues come out of eigh as
+    round-off near 1e-17, which would dominate a < 1, so eigenvalues below
+    1e-13 of the largest are taken as the exact zeros they are."""
+    if a == 1.0:
+        return entropy_from_rho(psi, n, b)
+    m = psi.reshape(2 ** (b + 1), 2 ** (n - b - 1))
+    m = m / np.linalg.norm(m)
+    w = np.linalg.eigvalsh(m @ m.conj().T)
+    w = w[w > 1e-13 * w.max()]
+    return float(math.log(np.sum(w ** a)) / (1.0 - a))
+
+
 def probs(psi: np.ndarray, n: int, k: int) -> Tuple[float, float]:
     t = psi.reshape(2 ** k, 2, 2 ** (n - k - 1))
     p = np.sum(np.abs(t) ** 2, axis=(0, 2))

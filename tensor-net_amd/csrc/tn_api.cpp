@@ -234,6 +234,13 @@ struct tn_engine {
   uint64_t* gen_ids = nullptr; // their counter words, 3 each
   size_t gen_cap = 0;
   bool timing = false;
+  int max_sweeps = kMaxSweeps;  // Jacobi sweep limit (TNMPS_MAX_SWEEPS, 1..kMaxSweeps)
+  // Sticky failure flags (kFlag* bits) of the updates enqueued since the host
+  // last looked: the device word (rank clamps, set by k_finish), its pinned
+  // copy, and the host's own (sweep limit reached).
+  int* d_eflags = nullptr;
+  int* h_eflags = nullptr;
+  int host_flags = 0;
   double perf_host_launch_ms = 0.0;  // host time spent enqueueing Jacobi steps
   hipEvent_t sev[kMaxSweeps] = {};  // per-sweep completion (convergence readback)
   double* h_offs = nullptr;         // pinned copy of offs
@@ -279,6 +286,8 @@ struct tn_engine {
     meas.release();
     ents.release();
     hfree(d_scalar); d_scalar = nullptr; nscalar = 0;
+    hfree(d_eflags); d_eflags = nullptr;
+    hhfree(h_eflags); h_eflags = nullptr;
     hfree(gen_u); hfree(gen_ids); gen_u = nullptr; gen_ids = nullptr; gen_cap = 0;
     for (PerfRec& r : prec) {
       for (hipEvent_t& ev : r.ev)
@@ -335,6 +344,12 @@ struct tn_engine {
       tune.push_back(r);
     }
     factored = env_double("TNMPS_FACTORED", 1) != 0;
+    max_sweeps = std::max(1, std::min(kMaxSweeps, (int)env_double("TNMPS_MAX_SWEEPS", kMaxSweeps)));
+    TN_HIP(hipMalloc(&d_eflags, sizeof(int)));
+    TN_HIP(hipHostMalloc(&h_eflags, sizeof(int), hipHostMallocDefault));
+    *h_eflags = 0;
+    TN_HIP(hipMemcpyAsync(d_eflags, h_eflags, sizeof(int), hipMemcpyHostToDevice, stream));
+    TN_HIP(hipStreamSynchronize(stream));
     tol.read_env();
     trace_sweeps = env_double("TNMPS_TRACE_SWEEPS", 0) != 0;
     // workspace budget: TNMPS_WS_BYTES (default 24 GiB) bounds the tasks per launch
@@ -528,6 +543,11 @@ struct tn_mps {
   // operator clears both; the product state has both.
   std::vector<unsigned char> fresh;
   size_t gam_elems = 0, lam_elems = 0;
+  // Sticky failure flags (kFlag* bits) of this handle's updates: the device
+  // word sits after the bond dimensions (dims[n + 1]); collected ones are
+  // kept here until tn_take_flags.
+  int* dflags() const { return dims + n + 1; }
+  int flags = 0;
 
   ~tn_mps() {
     if (eng) (void)hipStreamSynchronize(eng->stream);
@@ -649,6 +669,8 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
     if (q.has_gate && !q.gdev)
       for (int i = 0; i < 16; ++i) d.gate[i] = make_double2(q.gate[2 * i], q.gate[2 * i + 1]);
     d.gsrc = q.has_gate ? q.gdev : nullptr;
+    d.hflags = h->dflags();
+    d.eflags = e->d_eflags;
     const size_t chiL = h->uL(k), chiR = h->uR(k + 1);
     // a factored request has C = K when K < min(m, n), else the full
     // C = min(m, n) <= K: either way C <= min(ub[k], m, n)
@@ -740,11 +762,12 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
   int launched = 0, checked = 0;
   double last_worst = 1.0;
   bool done = false;
+  const double* last_o = nullptr;  // the last sweep's maxima read back
   if (pr) TN_HIP(hipEventRecord(pr->ev[2], e->stream));
   while (!done) {
     const int depth = last_worst > kAheadOff ? tu.ahead : 1;
     const auto h0 = std::chrono::steady_clock::now();
-    while (launched < kMaxSweeps && launched - checked < depth) {
+    while (launched < e->max_sweeps && launched - checked < depth) {
       if (use_list) launch_active(ja, launched, e->act, e->stream);
       for (int s = 0; s < steps; ++s) {
         if (S == 1) {
@@ -766,6 +789,7 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
     if (checked >= launched) break;  // sweep limit
     TN_HIP(hipEventSynchronize(e->sev[checked]));
     const double* o = e->h_offs + (size_t)checked * nt;
+    last_o = o;
     done = true;
     double worst = 0.0;
     int open = 0;
@@ -786,6 +810,15 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
     ++checked;
   }
   sweeps_run = checked;
+  // The sweep limit stopped tasks whose last sweep was still above the
+  // tolerance: their results are written as they stand and reported.
+  if (!done && last_o) {
+    for (int t = 0; t < nt; ++t)
+      if (!(last_o[t] <= conv_tol)) {
+        tasks[t].h->flags |= kFlagNoConv;
+        e->host_flags |= kFlagNoConv;
+      }
+  }
   if (pr) {
     TN_HIP(hipEventRecord(pr->ev[3], e->stream));
     pr->launches = (size_t)launched * steps * (S == 1 ? 1 : 2);
@@ -946,6 +979,13 @@ tn_status refactor_sweep(tn_mps* h) {
   return TN_OK;
 }
 
+// Enqueue the copy of the engine's device flag word into pinned memory; its
+// value is valid after the next synchronisation of the stream.
+tn_status fetch_eflags(tn_engine* e) {
+  TN_HIP(hipMemcpyAsync(e->h_eflags, e->d_eflags, sizeof(int), hipMemcpyDeviceToHost, e->stream));
+  return TN_OK;
+}
+
 tn_status read_results(tn_engine* e, int base, int count, int* outcomes, double* probs,
                        double* both, int* flags) {
   if (count > 0) {
@@ -962,6 +1002,7 @@ tn_status read_results(tn_engine* e, int base, int count, int* outcomes, double*
       TN_HIP(hipMemcpyAsync(flags, e->d_flag + base, sizeof(int) * count,
                             hipMemcpyDeviceToHost, e->stream));
   }
+  TN_TRY(fetch_eflags(e));
   TN_HIP(hipStreamSynchronize(e->stream));
   return TN_OK;
 }
@@ -1117,9 +1158,59 @@ tn_status check_flags(const std::vector<int>& flags) {
   return TN_OK;
 }
 
+// Allocation bound per bond of a new handle: chi_cap when given, chi for
+// Const(chi), else the largest exact bond dimension min(2^{b+1}, 2^{n-b-1})
+// up to kMaxCols / 2 (the widest SVD the library poses).  A None or Cutoff
+// state whose rank rule asks for more than its bound is truncated to it and
+// the call reports TN_CAPACITY.
+size_t default_cap(size_t n, tn_trunc trunc, size_t chi_cap) {
+  if (chi_cap) return chi_cap;
+  if (trunc.kind == TN_TRUNC_CONST) return std::max<size_t>(trunc.chi, 1);
+  size_t m = 1;
+  for (size_t b = 0; b + 1 < n; ++b) m = std::max(m, exact_bond_max(n, b));
+  return std::min<size_t>(m, (size_t)kMaxCols / 2);
+}
+
+// After a synchronisation that followed fetch_eflags: the failure flags of
+// the updates enqueued since the last collection.  Device flags (a clamped
+// rank) are copied to the handles of `hs` that set them, which keep them
+// until tn_take_flags; the device words are cleared.  Returns TN_CAPACITY for
+// a clamped rank, else TN_NO_CONVERGENCE for an SVD stopped at the sweep
+// limit, else TN_OK.
+tn_status collect_flags(tn_engine* e, const std::vector<tn_mps*>& hs) {
+  int f = *e->h_eflags | e->host_flags;
+  e->host_flags = 0;
+  if (*e->h_eflags) {
+    static const int zero = 0;
+    for (tn_mps* h : hs) {
+      int hf = 0;
+      TN_HIP(hipMemcpy(&hf, h->dflags(), sizeof(int), hipMemcpyDeviceToHost));
+      if (hf) {
+        h->flags |= hf;
+        TN_HIP(hipMemcpy(h->dflags(), &zero, sizeof(int), hipMemcpyHostToDevice));
+      }
+    }
+    *e->h_eflags = 0;
+    TN_HIP(hipMemcpy(e->d_eflags, &zero, sizeof(int), hipMemcpyHostToDevice));
+  }
+  if (f & kFlagCapacity) return TN_CAPACITY;
+  if (f & kFlagNoConv) return TN_NO_CONVERGENCE;
+  return TN_OK;
+}
+
+// Synchronise the engine's stream and collect the failure flags (for calls
+// that return no other results).
+tn_status sync_collect(tn_engine* e, const std::vector<tn_mps*>& hs) {
+  TN_TRY(fetch_eflags(e));
+  TN_HIP(hipStreamSynchronize(e->stream));
+  return collect_flags(e, hs);
+}
+
+// The status of a call: measurement statuses first, then failure flags.
+tn_status first_status(tn_status a, tn_status b) { return a != TN_OK ? a : b; }
+
 tn_status new_handle(tn_engine* e, size_t n, tn_trunc trunc, size_t chi_cap, tn_mps** out) {
-  size_t cap = chi_cap;
-  if (cap == 0) cap = trunc.kind == TN_TRUNC_CONST ? std::max<size_t>(trunc.chi, 1) : 256;
+  const size_t cap = default_cap(n, trunc, chi_cap);
   tn_mps* h = new tn_mps();
   h->n = n;
   h->trunc = trunc;
@@ -1161,20 +1252,22 @@ tn_status new_handle(tn_engine* e, size_t n, tn_trunc trunc, size_t chi_cap, tn_
   hipError_t er;
   if ((er = hipMalloc(&h->gam, sizeof(double2) * h->gam_elems)) != hipSuccess) return fail(er);
   if ((er = hipMalloc(&h->lam, sizeof(double) * h->lam_elems)) != hipSuccess) return fail(er);
-  if ((er = hipMalloc(&h->dims, sizeof(int) * (n + 1))) != hipSuccess) return fail(er);
+  // n + 1 bond dimensions, then the handle's sticky flag word
+  if ((er = hipMalloc(&h->dims, sizeof(int) * (n + 2))) != hipSuccess) return fail(er);
   // |0...0>: Gamma[k] = (1,2,1) with [0,0,0] = 1, lambda = [1] (mps.rs:378-388)
   std::vector<double2> g0(h->gam_elems, make_double2(0.0, 0.0));
   for (size_t k = 0; k < n; ++k) g0[h->goff[k]] = make_double2(1.0, 0.0);
   std::vector<double> l0(h->lam_elems, 0.0);
   for (size_t b = 0; b + 1 < n; ++b) l0[h->loff[b]] = 1.0;
-  std::vector<int> d0(n + 1, 1);
+  std::vector<int> d0(n + 2, 1);
+  d0[n + 1] = 0;
   if ((er = hipMemcpyAsync(h->gam, g0.data(), sizeof(double2) * h->gam_elems,
                            hipMemcpyHostToDevice, e->stream)) != hipSuccess)
     return fail(er);
   if ((er = hipMemcpyAsync(h->lam, l0.data(), sizeof(double) * h->lam_elems,
                            hipMemcpyHostToDevice, e->stream)) != hipSuccess)
     return fail(er);
-  if ((er = hipMemcpyAsync(h->dims, d0.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice,
+  if ((er = hipMemcpyAsync(h->dims, d0.data(), sizeof(int) * (n + 2), hipMemcpyHostToDevice,
                            e->stream)) != hipSuccess)
     return fail(er);
   if ((er = hipStreamSynchronize(e->stream)) != hipSuccess) return fail(er);
@@ -1210,7 +1303,7 @@ tn_status finish_layers(tn_engine* e, const std::vector<tn_mps*>& hv,
     for (size_t i = 0; i < count; ++i)
       ent_out[i] = (hv[i]->n >= 2 && ent_bonds[i] < hv[i]->n - 1) ? tmp[i] : NAN;
   }
-  return check_flags(flags);
+  return first_status(check_flags(flags), collect_flags(e, hv));
 }
 
 tn_status check_trunc(tn_trunc t) {
@@ -1309,8 +1402,7 @@ tn_status tn_mps_new_qubits(size_t n, tn_trunc trunc, size_t chi_cap, void* stre
     g_last_error = "no HIP device visible";
     return TN_DEVICE;
   }
-  size_t cap = chi_cap;
-  if (cap == 0) cap = trunc.kind == TN_TRUNC_CONST ? std::max<size_t>(trunc.chi, 1) : 256;
+  const size_t cap = default_cap(n, trunc, chi_cap);
   size_t maxb = 1;
   for (size_t b = 0; b + 1 < n; ++b) maxb = std::max(maxb, std::min(cap, exact_bond_max(n, b)));
   tn_engine* e = nullptr;
@@ -1406,7 +1498,34 @@ tn_status tn_apply_unis(tn_mps* h, size_t m, const size_t* ks, const int* dims,
   if (!h || (m && (!ks || !ops))) return TN_INVALID_ARG;
   std::vector<std::vector<std::vector<GateOp>>> groups(1);
   TN_TRY(gate_groups(h, m, ks, dims, ops, &groups[0]));
-  return exec_gates(h->eng, {h}, groups);
+  TN_TRY(exec_gates(h->eng, {h}, groups));
+  return sync_collect(h->eng, {h});
+}
+
+tn_status tn_apply_u2_multi(tn_mps* h, size_t k, size_t count, const double* ops) {
+  if (!h || (count && !ops)) return TN_INVALID_ARG;
+  if (h->n == 1 || k >= h->n - 1) return TN_OK;  // mps.rs:1400
+  // acc <- op . acc from the identity, in the listed order (mps.rs:1404-1410)
+  double acc[32] = {};
+  for (int i = 0; i < 4; ++i) acc[2 * (5 * i)] = 1.0;
+  for (size_t q = 0; q < count; ++q) {
+    const double* op = ops + 32 * q;
+    double nxt[32];
+    for (int i = 0; i < 4; ++i)
+      for (int j = 0; j < 4; ++j) {
+        double re = 0.0, im = 0.0;
+        for (int l = 0; l < 4; ++l) {
+          const double ar = op[2 * (4 * i + l)], ai = op[2 * (4 * i + l) + 1];
+          const double br = acc[2 * (4 * l + j)], bi = acc[2 * (4 * l + j) + 1];
+          re += ar * br - ai * bi;
+          im += ar * bi + ai * br;
+        }
+        nxt[2 * (4 * i + j)] = re;
+        nxt[2 * (4 * i + j) + 1] = im;
+      }
+    std::memcpy(acc, nxt, sizeof(acc));
+  }
+  return tn_apply_u2(h, k, acc);
 }
 
 tn_status tn_apply_u1(tn_mps* h, size_t k, const double* op) {
@@ -1449,7 +1568,7 @@ tn_status tn_measure_prob(tn_mps* h, size_t k, double r, int* outcome, double* p
   TN_TRY(read_results(h->eng, 0, 1, &o, &pr, nullptr, &f));
   if (outcome) *outcome = o;
   if (prob) *prob = pr;
-  return (f & 2) ? TN_SAMPLE_RANGE : TN_OK;
+  return first_status((f & 2) ? TN_SAMPLE_RANGE : TN_OK, collect_flags(h->eng, {h}));
 }
 
 tn_status tn_measure_postsel_prob(tn_mps* h, size_t k, size_t p, double* prob) {
@@ -1462,7 +1581,7 @@ tn_status tn_measure_postsel_prob(tn_mps* h, size_t k, size_t p, double* prob) {
   double pr = 0.0;
   TN_TRY(read_results(h->eng, 0, 1, nullptr, &pr, nullptr, &f));
   if (prob) *prob = pr;
-  return (f & 1) ? TN_ZERO_PROB : TN_OK;
+  return first_status((f & 1) ? TN_ZERO_PROB : TN_OK, collect_flags(h->eng, {h}));
 }
 
 tn_status tn_meas_layer(tn_mps* h, size_t m, const size_t* ks, const int* forced,
@@ -1486,7 +1605,7 @@ tn_status tn_meas_layer(tn_mps* h, size_t m, const size_t* ks, const int* forced
     if (out_outcomes) out_outcomes[i] = outs[i];
     if (out_probs) out_probs[i] = probs[i];
   }
-  return check_flags(flags);
+  return first_status(check_flags(flags), collect_flags(h->eng, {h}));
 }
 
 tn_status tn_apply_bilayer(tn_mps* h, size_t nu, const size_t* uks, const int* udims,
@@ -1494,8 +1613,13 @@ tn_status tn_apply_bilayer(tn_mps* h, size_t nu, const size_t* uks, const int* u
                            const double* r, size_t* out_count, size_t* out_ks,
                            int* out_outcomes, double* out_probs) {
   if (!h) return TN_INVALID_ARG;
-  TN_TRY(tn_apply_unis(h, nu, uks, udims, ops));
-  return tn_meas_layer(h, nm, mks, forced, r, out_count, out_ks, out_outcomes, out_probs);
+  // a clamped rank or an SVD at the sweep limit still writes the state:
+  // the measurement layer runs and the status is reported with its own
+  const tn_status su = tn_apply_unis(h, nu, uks, udims, ops);
+  if (su != TN_OK && su != TN_CAPACITY && su != TN_NO_CONVERGENCE) return su;
+  const tn_status sm =
+      tn_meas_layer(h, nm, mks, forced, r, out_count, out_ks, out_outcomes, out_probs);
+  return sm != TN_OK ? sm : su;
 }
 
 tn_status tn_entropy_ry_schmidt(tn_mps* h, double a, size_t b, double* out) {
@@ -1658,7 +1782,7 @@ tn_status tn_run_circuit(tn_mps* h, size_t depth, const size_t* nu, const size_t
   TN_TRY(read_results(e, 0, slot, out_outcomes, out_probs, nullptr, flags.data()));
   if (want_ent)
     TN_HIP(hipMemcpy(ent_out, e->d_scalar, sizeof(double) * (depth + 1), hipMemcpyDeviceToHost));
-  return check_flags(flags);
+  return first_status(check_flags(flags), collect_flags(e, {h}));
 }
 
 tn_status tn_batch_bilayer(tn_engine* e, size_t count, tn_mps* const* hs, const size_t* nu,
@@ -1830,6 +1954,13 @@ tn_status tn_engine_reset_perf(tn_engine* e) {
 tn_status tn_engine_sync(tn_engine* e) {
   if (!e) return TN_INVALID_ARG;
   TN_HIP(hipStreamSynchronize(e->stream));
+  return TN_OK;
+}
+
+tn_status tn_take_flags(tn_mps* h, int* flags) {
+  if (!h || !flags) return TN_INVALID_ARG;
+  *flags = h->flags;
+  h->flags = 0;
   return TN_OK;
 }
 

@@ -41,9 +41,10 @@ enum {
   kInfoSweeps, kInfoChiL, kInfoChiM, kInfoChiR, kInfoMode, kInfoLen = 16
 };
 
-// flag bits in info[kInfoFlags]
-constexpr int kFlagNoConv = 1;     // Jacobi hit kMaxSweeps
-constexpr int kFlagCapacity = 2;   // rank clamped to the bond's allocation
+// failure flag bits (the values of TN_FLAG_* in tnmps.h): in info[kInfoFlags]
+// and the handle's and engine's sticky words
+constexpr int kFlagNoConv = 1;     // Jacobi stopped at the sweep limit (set by the host)
+constexpr int kFlagCapacity = 2;   // rank clamped to the bond's allocation (k_finish)
 
 // One two-site update (mps.rs:846-880 local_refactor, or mps.rs:1332-1387
 // apply_unitary2 when has_gate is set).  All pointers are device addresses.
@@ -75,6 +76,8 @@ struct UpdDesc {
   double teps;
   double2 gate[16];      // 4x4 row-major, qubit k the most significant bit
   const double2* gsrc;   // the gate in device memory instead (generated), or null
+  int* hflags;           // the handle's sticky failure flags (kFlag* bits)
+  int* eflags;           // the engine's: any handle's flags since the host last looked
 };
 
 // The Jacobi step kernels of one launch (all tasks of an update batch).  Task

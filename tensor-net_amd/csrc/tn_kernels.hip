@@ -492,6 +492,12 @@ __global__ __launch_bounds__(1024) void k_finish(const UpdDesc* __restrict__ des
     d.info[kInfoRank] = rank;
     d.info[kInfoFlags] |= flags;
     d.dims[1] = rank;
+    // a clamped rank is reported through the sticky words the host reads
+    // when the call that enqueued this update returns
+    if (flags) {
+      atomicOr(d.hflags, flags);
+      atomicOr(d.eflags, flags);
+    }
   }
 }
 

@@ -27,6 +27,10 @@ TN_DEVICE = 10
 TN_INVALID_ARG = 11
 TN_NO_CONVERGENCE = 12
 
+# sticky failure flag bits (tn_take_flags)
+TN_FLAG_NO_CONVERGENCE = 1
+TN_FLAG_CAPACITY = 2
+
 TRUNC_NONE, TRUNC_CONST, TRUNC_CUTOFF = 0, 1, 2
 PREC_F64, PREC_F32, PREC_BF16X3 = 0, 1, 2
 
@@ -85,6 +89,8 @@ SIGNATURES = {
     "tn_sync": (C.c_int, [_h]),
     "tn_apply_u1": (C.c_int, [_h, _sz, _dp]),
     "tn_apply_u2": (C.c_int, [_h, _sz, _dp]),
+    "tn_apply_u2_multi": (C.c_int, [_h, _sz, _sz, _dp]),
+    "tn_take_flags": (C.c_int, [_h, _ip]),
     "tn_apply_unis": (C.c_int, [_h, _sz, _szp, _ip, _dp]),
     "tn_prob": (C.c_int, [_h, _sz, _sz, _dp]),
     "tn_probs": (C.c_int, [_h, _sz, _dp]),

@@ -221,6 +221,23 @@ class MPS:
         _check(self._lib.tn_apply_u2(self._h, int(k), _dptr(buf)), "apply_unitary2")
         return self
 
+    def apply_unitary2_multi(self, k: int, ops) -> "MPS":
+        """``apply_unitary2_multi`` (mps.rs:1393-1411): the 4x4 operators
+        multiplied in order (last one leftmost), applied at bond k as one
+        ``apply_unitary2``; a no-op for k >= n - 1."""
+        bufs = [_op_buf(op, 4) for op in ops]
+        buf = np.ascontiguousarray(np.concatenate(bufs)) if bufs else np.zeros(32)
+        _check(self._lib.tn_apply_u2_multi(self._h, int(k), len(bufs), _dptr(buf)),
+               "apply_unitary2_multi")
+        return self
+
+    def take_flags(self) -> int:
+        """The handle's sticky failure flags (``TN_FLAG_CAPACITY``,
+        ``TN_FLAG_NO_CONVERGENCE``) since the last call, then cleared."""
+        f = C.c_int()
+        _check(self._lib.tn_take_flags(self._h, C.byref(f)), "take_flags")
+        return f.value
+
     def apply_unis(self, unis: Sequence[Tuple[int, np.ndarray]]) -> "MPS":
         """The unitary loop of ``circuit::apply_bilayer`` (circuit.rs:637-638)
         via ``apply_uni`` (mps.rs:2093-2108)."""

@@ -295,7 +295,95 @@ def test_truncated_trajectory_invariants():
         assert abs(sum(st.probs(k)) - 1.0) < 1e-12
 
 
+# --------------------------------------------------- Renyi entropies
+RENYI_ALPHAS = (0.0, 0.5, 2.0, 3.0, 7.5)
+
+
+def _renyi_two_value(c2: float, a: float) -> float:
+    """S_a of the spectrum (cos t, sin t) with c2 = cos^2 t."""
+    return math.log(c2 ** a + (1.0 - c2) ** a) / (1.0 - a)
+
+
+@pytest.mark.parametrize("a", RENYI_ALPHAS)
+def test_renyi_bell_ghz_closed_form(a):
+    """entropy_ry_schmidt (mps.rs:1268-1282): a flat two-value spectrum has
+    S_a = ln 2 for every a (Bell pair, and every bond of a GHZ state)."""
+    st = VidalMPS(2, None)
+    st.apply_unitary1(0, H)
+    st.apply_unitary2(0, CX)
+    assert abs(st.entropy_ry_schmidt(a, 0) - math.log(2)) < 1e-14
+    g = VidalMPS(6, ("const", 4))
+    g.apply_unitary1(0, H)
+    for k in range(5):
+        g.apply_unitary2(k, CX)
+    for b in range(5):
+        assert abs(g.entropy_ry_schmidt(a, b) - math.log(2)) < 1e-14
+    assert g.entropy_ry_schmidt(a, 5) is None
+
+
+@pytest.mark.parametrize("a", RENYI_ALPHAS)
+def test_renyi_graded_two_value_closed_form(a):
+    """XRot(t) then CX gives lambda = (cos t/2, sin t/2), so
+    S_a = ln(cos^(2a) + sin^(2a)) / (1 - a): a graded spectrum whose value
+    depends on a, and on squaring the Schmidt values before the power."""
+    ang = 3 * math.pi / 5
+    st = VidalMPS(4, ("const", 4))
+    st.apply_unitary1(1, xrot(ang))
+    st.apply_unitary2(1, CX)
+    want = _renyi_two_value(math.cos(ang / 2) ** 2, a)
+    assert abs(st.entropy_ry_schmidt(a, 1) - want) < 1e-13
+    assert st.entropy_ry_schmidt(a, 0) == 0.0 and st.entropy_ry_schmidt(a, 2) == 0.0
+
+
+@pytest.mark.parametrize("n,seed", [(6, 21), (9, 22), (12, 23)])
+def test_renyi_vs_dense_rho(n, seed):
+    """S_a at every bond of a random state equals ln Tr rho_A^a / (1 - a)
+    from the dense reduced density matrix, and S_1 is the von Neumann
+    entropy (the a == 1 branch)."""
+    st, psi = _random_gate_run(n, 5 * n, seed)
+    for b in range(n - 1):
+        for a in RENYI_ALPHAS + (1.0,):
+            want = dense.renyi_from_rho(psi, n, b, a)
+            assert abs(st.entropy_ry_schmidt(a, b) - want) < 1e-11 * max(1.0, abs(want)), \
+                f"bond {b} a={a}"
+
+
+def test_renyi_limits():
+    """S_a is non-increasing in a; S_0 = ln(rank); S_a -> S_1 as a -> 1."""
+    st, _ = _random_gate_run(8, 40, 31)
+    for b in range(7):
+        vals = [st.entropy_ry_schmidt(a, b) for a in (0.0, 0.5, 0.999999, 2.0, 4.0)]
+        assert all(x >= y - 1e-12 for x, y in zip(vals, vals[1:])), vals
+        assert abs(vals[0] - math.log(st.bond_dim(b))) < 1e-12
+        assert abs(vals[2] - st.entropy_vn(b)) < 1e-5
+
+
 # ------------------------------------------------- planted mistakes (CPU)
+def test_planted_renyi_unsquared_fails(monkeypatch):
+    """entropy_ry_schmidt with s^a in place of (s^2)^a (a dropped square)
+    must fail the Renyi pins."""
+    def bad(self, a, b):
+        if a == 1.0:
+            return self.entropy_vn(b)
+        if b >= len(self.L):
+            return None
+        return math.log(sum(float(s) ** a for s in self.L[b])) / (1.0 - a)
+
+    monkeypatch.setattr(VidalMPS, "entropy_ry_schmidt", bad)
+    for a in (0.5, 2.0):
+        with pytest.raises(AssertionError):
+            test_renyi_graded_two_value_closed_form(a)
+        with pytest.raises(AssertionError):
+            test_renyi_vs_dense_rho(6, 21)
+
+
+def test_planted_renyi_sign_fails(monkeypatch):
+    """A wrong prefactor 1 / (a - 1) must fail the closed forms."""
+    orig = VidalMPS.entropy_ry_schmidt
+    monkeypatch.setattr(VidalMPS, "entropy_ry_schmidt",
+                        lambda self, a, b: orig(self, a, b) if a == 1.0 else -orig(self, a, b))
+    with pytest.raises(AssertionError):
+        test_renyi_bell_ghz_closed_form(2.0)
 def test_planted_dropped_lambda_fails(monkeypatch):
     """Dropping the lambda[k-1] row weight in contract_local2 must break the
     dense comparison."""
