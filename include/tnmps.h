@@ -161,6 +161,20 @@ tn_status tn_apply_u1(tn_mps* h, size_t k, const double* op);
 /* MPS::apply_unitary2 (mps.rs:1332-1387): contract -> gate -> SVD ->
  * truncate -> split -> renormalise.  op: 4x4 row-major, 32 doubles. */
 tn_status tn_apply_u2(tn_mps* h, size_t k, const double* op);
+/* tn_apply_u2 without any allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory during the call (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same update, kernels and result as tn_apply_u2 (mps.rs:1332-1387), with
+ * the update's workspace (X, W, per-column arrays, row-split partial Gram
+ * matrices, the descriptor) in a static device array of 10.3 MiB per GPU,
+ * held until the process ends, instead of the engine's.  Limit: the bonds
+ * next to the update (k-1 and k+1) at most 256, so theta is at most
+ * 512 x 512; above that TN_INVALID_ARG (tn_last_error says why) and the state
+ * is untouched.  Every call on one GPU shares that GPU's array: calls that
+ * overlap on one GPU (from several host threads or engines) race.  Blocks
+ * until the update is done.  The handle itself and its engine must exist
+ * (their memory was allocated when they were made). */
+tn_status tn_apply_u2_noalloc(tn_mps* h, size_t k, const double* op);
 /* MPS::apply_unitary2_multi (mps.rs:1393-1411): count 4x4 operators (32
  * doubles each, back to back) multiplied in order, op[count-1] ... op[0],
  * then applied as one apply_unitary2 at bond k (count == 0 applies the

@@ -19,6 +19,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -161,6 +162,68 @@ struct DescRing {
 
 using EntReq = EntDesc;
 
+// ---- static scratch of tn_apply_u2_noalloc -------------------------------
+// One two-site update's workspace with theta up to kScratchCols x
+// kScratchCols (bonds next to the updated one up to kScratchCols / 2), laid
+// out as the engine lays out one slot: X and W (kScratchCols^2 complex
+// each), the per-column arrays, partial norms, per-sweep maxima, info, the
+// active-task list, the row-split partial Gram matrices (both block widths)
+// and the task's descriptor.  Each device holds one copy for the life of the
+// process; its address on a device comes from hipGetSymbolAddress.
+constexpr int kScratchCols = 512;
+constexpr size_t scratch_align(size_t b) { return (b + 255) / 256 * 256; }
+constexpr size_t kScrX = 0;
+constexpr size_t kScrW = kScrX + scratch_align(sizeof(double2) * kScratchCols * kScratchCols);
+constexpr size_t kScrSraw = kScrW + scratch_align(sizeof(double2) * kScratchCols * kScratchCols);
+constexpr size_t kScrSval = kScrSraw + scratch_align(sizeof(double) * kScratchCols);
+constexpr size_t kScrFac = kScrSval + scratch_align(sizeof(double) * kScratchCols);
+constexpr size_t kScrPerm = kScrFac + scratch_align(sizeof(double) * kScratchCols);
+constexpr size_t kScrPart = kScrPerm + scratch_align(sizeof(int) * kScratchCols);
+constexpr size_t kScrOffs = kScrPart + scratch_align(sizeof(double) * 2 * kNPart);
+constexpr size_t kScrInfo = kScrOffs + scratch_align(sizeof(double) * kMaxSweeps);
+constexpr size_t kScrAct = kScrInfo + scratch_align(sizeof(int) * kInfoLen);
+constexpr size_t kScrGpart = kScrAct + scratch_align(sizeof(int) * 2);
+// partial Gram matrices of one task: (pairs + 1) x kMaxSplit slices of
+// (2 bw)^2 values, for the wider of the two block widths
+constexpr size_t gpart_elems(int cols, int bw) {
+  return ((size_t)(cols + bw - 1) / bw / 2 + 1) * kMaxSplit * (size_t)(4 * bw * bw);
+}
+constexpr size_t kScrGpartElems = gpart_elems(kScratchCols, kBW) > gpart_elems(kScratchCols, kBWWide)
+                                      ? gpart_elems(kScratchCols, kBW)
+                                      : gpart_elems(kScratchCols, kBWWide);
+constexpr size_t kScrDesc = kScrGpart + scratch_align(sizeof(double2) * kScrGpartElems);
+constexpr size_t kScratchBytes = kScrDesc + scratch_align(sizeof(UpdDesc));
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+// This device's address of `scratch`, resolved once per device with that
+// device current.  (The array's name in host code is a host-side stand-in:
+// only the address returned here is a device address.)
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+tn_status scratch_address(int dev, void** out) {
+  *out = nullptr;
+  if (dev < 0 || dev >= kMaxDevices) return TN_DEVICE;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    int cur = 0;
+    TN_HIP(hipGetDevice(&cur));
+    if (cur != dev) TN_HIP(hipSetDevice(dev));
+    void* p = nullptr;
+    const hipError_t er = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != dev) (void)hipSetDevice(cur);
+    if (er != hipSuccess) {
+      g_last_error = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(er);
+      return TN_DEVICE;
+    }
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return TN_OK;
+}
+
 }  // namespace
 
 // Device-time accounting of one update batch or measurement launch.  Its
@@ -241,6 +304,7 @@ struct tn_engine {
   int* d_eflags = nullptr;
   int* h_eflags = nullptr;
   int host_flags = 0;
+  unsigned char* scratch_ptr = nullptr;  // this device's `scratch` (tn_apply_u2_noalloc)
   double perf_host_launch_ms = 0.0;  // host time spent enqueueing Jacobi steps
   hipEvent_t sev[kMaxSweeps] = {};  // per-sweep completion (convergence readback)
   double* h_offs = nullptr;         // pinned copy of offs
@@ -345,6 +409,11 @@ struct tn_engine {
     }
     factored = env_double("TNMPS_FACTORED", 1) != 0;
     max_sweeps = std::max(1, std::min(kMaxSweeps, (int)env_double("TNMPS_MAX_SWEEPS", kMaxSweeps)));
+    {
+      void* sp = nullptr;
+      TN_TRY(scratch_address(dev, &sp));
+      scratch_ptr = (unsigned char*)sp;
+    }
     TN_HIP(hipMalloc(&d_eflags, sizeof(int)));
     TN_HIP(hipHostMalloc(&h_eflags, sizeof(int), hipHostMallocDefault));
     *h_eflags = 0;
@@ -620,14 +689,57 @@ struct MTask {
   int slot;
 };
 
+// Device workspace of one launch batch: task t's X (ldx x ldw), W (ldw x
+// ldw), column norms, values, scales, order, partial norms, per-sweep maxima
+// (stride nt) and info, at fixed offsets from these bases; the row-split
+// partial Gram matrices of the first gpart_slots tasks; the active-task
+// list; the host copy of the per-sweep maxima; and, when not null, the
+// device copy of the batch's descriptors (else the engine's ring holds it).
+struct Workspace {
+  double2* X = nullptr;
+  double2* W = nullptr;
+  double* sraw = nullptr;
+  double* sval = nullptr;
+  double* fac = nullptr;
+  int* perm = nullptr;
+  double* part = nullptr;
+  double2* gpart = nullptr;
+  size_t gpart_per = 0;
+  int gpart_slots = 0;
+  double* offs = nullptr;
+  int* info = nullptr;
+  int* act = nullptr;
+  double* h_offs = nullptr;
+  int ldx = 0, ldw = 0;
+  UpdDesc* desc = nullptr;
+};
+
+Workspace engine_workspace(const tn_engine* e) {
+  Workspace w;
+  w.X = e->X; w.W = e->W; w.sraw = e->sraw; w.sval = e->sval; w.fac = e->fac;
+  w.perm = e->perm; w.part = e->part; w.gpart = e->gpart; w.gpart_per = e->gpart_per;
+  w.gpart_slots = e->gpart_slots; w.offs = e->offs; w.info = e->info; w.act = e->act;
+  w.h_offs = e->h_offs; w.ldx = e->ldx; w.ldw = e->ldw;
+  return w;
+}
+
 // Enqueue up to max_slots independent updates.  Within one handle, bonds
 // must pairwise differ by at least 2 so that no update reads what another
-// writes; different handles are independent.
-tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
-  TN_TRY(e->ensure_slots(nt));
+// writes; different handles are independent.  With `ws` the batch runs in
+// that workspace (one task, no allocation, no timing records) instead of the
+// engine's.
+tn_status run_chunk(tn_engine* e, const Task* tasks, int nt, const Workspace* ws = nullptr) {
   UpdDesc* hd = nullptr;
   int rs = 0;
-  TN_TRY(e->upd.next(&hd, &rs));
+  UpdDesc one{};
+  if (ws) {
+    if (nt != 1) return TN_INVALID_ARG;
+    hd = &one;
+  } else {
+    TN_TRY(e->ensure_slots(nt));
+    TN_TRY(e->upd.next(&hd, &rs));
+  }
+  const Workspace w = ws ? *ws : engine_workspace(e);
   size_t maxC = 1, maxR = 1, maxTiles = 1, maxCC = 1, maxChiL = 1, maxChiR = 1;
   bool any_factored = false;
   int mode0 = 0;  // first task's mode, for the sweep trace
@@ -646,20 +758,20 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
     d.lamM = h->lam + h->loff[k];
     d.lamR = (k + 2 >= h->n) ? nullptr : h->lam + h->loff[k + 1];
     d.dims = h->dims + k;
-    d.X = e->X + (size_t)t * e->ldx * e->ldw;
-    d.W = e->W + (size_t)t * e->ldw * e->ldw;
-    d.sraw = e->sraw + (size_t)t * e->ldw;
-    d.sval = e->sval + (size_t)t * e->ldw;
-    d.fac = e->fac + (size_t)t * e->ldw;
-    d.perm = e->perm + (size_t)t * e->ldw;
-    d.part = e->part + (size_t)t * 2 * kNPart;
-    d.offs = e->offs + t;
-    d.info = e->info + (size_t)t * kInfoLen;
+    d.X = w.X + (size_t)t * w.ldx * w.ldw;
+    d.W = w.W + (size_t)t * w.ldw * w.ldw;
+    d.sraw = w.sraw + (size_t)t * w.ldw;
+    d.sval = w.sval + (size_t)t * w.ldw;
+    d.fac = w.fac + (size_t)t * w.ldw;
+    d.perm = w.perm + (size_t)t * w.ldw;
+    d.part = w.part + (size_t)t * 2 * kNPart;
+    d.offs = w.offs + t;
+    d.info = w.info + (size_t)t * kInfoLen;
     // the factored product goes to the second half of X's columns; the SVD
     // of a factored update uses C = K <= chi_max = ldw / 2 of them
-    d.T = d.X + (size_t)(e->ldw / 2) * e->ldx;
-    d.ldx = e->ldx;
-    d.ldw = e->ldw;
+    d.T = d.X + (size_t)(w.ldw / 2) * w.ldx;
+    d.ldx = w.ldx;
+    d.ldw = w.ldw;
     d.has_gate = q.has_gate ? 1 : 0;
     d.prec = h->prec;
     d.tkind = h->trunc.kind;
@@ -685,10 +797,17 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
   const tn_tune& tu = e->tune_for(maxC);
   const size_t tv = (size_t)tu.theta_tile / 2;
   maxTiles = std::max<size_t>(1, ((maxChiL + tv - 1) / tv) * ((maxChiR + tv - 1) / tv));
-  TN_TRY(e->upd.upload(rs, nt, e->stream));
-  const UpdDesc* dd = e->upd.d[rs];
+  const UpdDesc* dd = nullptr;
+  if (ws) {
+    // a pageable source: the copy is staged before the call returns
+    TN_HIP(hipMemcpyAsync(w.desc, hd, sizeof(UpdDesc), hipMemcpyHostToDevice, e->stream));
+    dd = w.desc;
+  } else {
+    TN_TRY(e->upd.upload(rs, nt, e->stream));
+    dd = e->upd.d[rs];
+  }
   PerfRec* pr = nullptr;
-  if (e->timing) {
+  if (e->timing && !ws) {
     TN_TRY(e->perf_begin(0, nt, &pr));
     TN_HIP(hipEventRecord(pr->ev[0], e->stream));
   }
@@ -718,14 +837,14 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
   // (S > 1 needs nt * npairs <= cus, so nt <= cus: the partial Gram buffer
   // covers min(slots, cus) tasks)
   while (S < tu.split_max && (size_t)nt * npairs * S * 2 <= (size_t)2 * e->cus &&
-         maxR >= (size_t)tu.split_rows * S && nt <= e->gpart_slots)
+         maxR >= (size_t)tu.split_rows * S && nt <= w.gpart_slots)
     S *= 2;
   JacobiArgs ja{};
-  ja.X = e->X;
-  ja.W = e->W;
-  ja.gpart = e->gpart;
-  ja.offs = e->offs;
-  ja.info = e->info;
+  ja.X = w.X;
+  ja.W = w.W;
+  ja.gpart = w.gpart;
+  ja.offs = w.offs;
+  ja.info = w.info;
   // Many tasks: the step kernel takes the sweep's unconverged tasks from a
   // list that a one-workgroup kernel builds before the sweep, so tasks that
   // have converged (a lock-step batch runs until its slowest task has) cost
@@ -733,13 +852,13 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
   // sweep read back (or nt), a bound on the list's length since a converged
   // task stays converged.  The arithmetic per task is the same either way.
   const bool use_list = S == 1 && nt >= e->list_min && nt <= 8192;
-  ja.act = use_list ? e->act : nullptr;
+  ja.act = use_list ? w.act : nullptr;
   int open_known = nt;
-  ja.xslot = (size_t)e->ldx * e->ldw;
-  ja.wslot = (size_t)e->ldw * e->ldw;
-  ja.gslot = e->gpart_per;
-  ja.ldx = e->ldx;
-  ja.ldw = e->ldw;
+  ja.xslot = (size_t)w.ldx * w.ldw;
+  ja.wslot = (size_t)w.ldw * w.ldw;
+  ja.gslot = w.gpart_per;
+  ja.ldx = w.ldx;
+  ja.ldw = w.ldw;
   ja.nt = nt;
   ja.nbe = nbe;
   ja.S = S;
@@ -768,7 +887,7 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
     const int depth = last_worst > kAheadOff ? tu.ahead : 1;
     const auto h0 = std::chrono::steady_clock::now();
     while (launched < e->max_sweeps && launched - checked < depth) {
-      if (use_list) launch_active(ja, launched, e->act, e->stream);
+      if (use_list) launch_active(ja, launched, w.act, e->stream);
       for (int s = 0; s < steps; ++s) {
         if (S == 1) {
           launch_jacobi_step(ja, npairs, bw, launched, s, use_list ? open_known : nt,
@@ -779,7 +898,7 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
         }
       }
       TN_HIP(hipGetLastError());
-      TN_HIP(hipMemcpyAsync(e->h_offs + (size_t)launched * nt, e->offs + (size_t)launched * nt,
+      TN_HIP(hipMemcpyAsync(w.h_offs + (size_t)launched * nt, w.offs + (size_t)launched * nt,
                             sizeof(double) * nt, hipMemcpyDeviceToHost, e->stream));
       TN_HIP(hipEventRecord(e->sev[launched], e->stream));
       ++launched;
@@ -788,7 +907,7 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count();
     if (checked >= launched) break;  // sweep limit
     TN_HIP(hipEventSynchronize(e->sev[checked]));
-    const double* o = e->h_offs + (size_t)checked * nt;
+    const double* o = w.h_offs + (size_t)checked * nt;
     last_o = o;
     done = true;
     double worst = 0.0;
@@ -837,7 +956,7 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt) {
   e->sweeps += (unsigned long long)sweeps_run;
   if (pr) {
     TN_HIP(hipEventRecord(pr->ev[5], e->stream));
-    TN_HIP(hipMemcpyAsync(pr->h_info, e->info, sizeof(int) * kInfoLen * nt, hipMemcpyDeviceToHost,
+    TN_HIP(hipMemcpyAsync(pr->h_info, w.info, sizeof(int) * kInfoLen * nt, hipMemcpyDeviceToHost,
                           e->stream));
     pr->gate.resize(nt);
     for (int t = 0; t < nt; ++t) pr->gate[t] = tasks[t].has_gate ? 1 : 0;
@@ -1177,12 +1296,13 @@ size_t default_cap(size_t n, tn_trunc trunc, size_t chi_cap) {
 // until tn_take_flags; the device words are cleared.  Returns TN_CAPACITY for
 // a clamped rank, else TN_NO_CONVERGENCE for an SVD stopped at the sweep
 // limit, else TN_OK.
-tn_status collect_flags(tn_engine* e, const std::vector<tn_mps*>& hs) {
+tn_status collect_flags(tn_engine* e, tn_mps* const* hs, size_t count) {
   int f = *e->h_eflags | e->host_flags;
   e->host_flags = 0;
   if (*e->h_eflags) {
     static const int zero = 0;
-    for (tn_mps* h : hs) {
+    for (size_t i = 0; i < count; ++i) {
+      tn_mps* h = hs[i];
       int hf = 0;
       TN_HIP(hipMemcpy(&hf, h->dflags(), sizeof(int), hipMemcpyDeviceToHost));
       if (hf) {
@@ -1196,6 +1316,9 @@ tn_status collect_flags(tn_engine* e, const std::vector<tn_mps*>& hs) {
   if (f & kFlagCapacity) return TN_CAPACITY;
   if (f & kFlagNoConv) return TN_NO_CONVERGENCE;
   return TN_OK;
+}
+tn_status collect_flags(tn_engine* e, const std::vector<tn_mps*>& hs) {
+  return collect_flags(e, hs.data(), hs.size());
 }
 
 // Synchronise the engine's stream and collect the failure flags (for calls
@@ -1500,6 +1623,51 @@ tn_status tn_apply_unis(tn_mps* h, size_t m, const size_t* ks, const int* dims,
   TN_TRY(gate_groups(h, m, ks, dims, ops, &groups[0]));
   TN_TRY(exec_gates(h->eng, {h}, groups));
   return sync_collect(h->eng, {h});
+}
+
+tn_status tn_apply_u2_noalloc(tn_mps* h, size_t k, const double* op) {
+  if (!h || !op) return TN_INVALID_ARG;
+  if (h->n == 1 || k >= h->n - 1) return TN_OK;  // mps.rs:1335
+  tn_engine* e = h->eng;
+  // theta is at most (2 chiL) x (2 chiR) with the host's bounds on bonds
+  // k-1 and k+1
+  if (2 * h->uL(k) > (size_t)kScratchCols || 2 * h->uR(k + 1) > (size_t)kScratchCols) {
+    g_last_error = "tn_apply_u2_noalloc: bonds next to the update above 256 (theta above "
+                   "512 x 512, the static scratch bound); use tn_apply_u2";
+    return TN_INVALID_ARG;
+  }
+  if (!e->scratch_ptr) {
+    g_last_error = "tn_apply_u2_noalloc: no scratch address on this device";
+    return TN_DEVICE;
+  }
+  double h_offs[kMaxSweeps];
+  unsigned char* b = e->scratch_ptr;
+  Workspace w;
+  w.X = (double2*)(b + kScrX);
+  w.W = (double2*)(b + kScrW);
+  w.sraw = (double*)(b + kScrSraw);
+  w.sval = (double*)(b + kScrSval);
+  w.fac = (double*)(b + kScrFac);
+  w.perm = (int*)(b + kScrPerm);
+  w.part = (double*)(b + kScrPart);
+  w.offs = (double*)(b + kScrOffs);
+  w.info = (int*)(b + kScrInfo);
+  w.act = (int*)(b + kScrAct);
+  w.gpart = (double2*)(b + kScrGpart);
+  w.gpart_per = kScrGpartElems;
+  w.gpart_slots = 1;
+  w.h_offs = h_offs;
+  w.ldx = w.ldw = kScratchCols;
+  w.desc = (UpdDesc*)(b + kScrDesc);
+  Task t;
+  t.h = h;
+  t.k = k;
+  t.has_gate = true;
+  std::memcpy(t.gate, op, sizeof(double) * 32);
+  TN_TRY(run_chunk(e, &t, 1, &w));
+  TN_TRY(fetch_eflags(e));
+  TN_HIP(hipStreamSynchronize(e->stream));
+  return collect_flags(e, &h, 1);
 }
 
 tn_status tn_apply_u2_multi(tn_mps* h, size_t k, size_t count, const double* ops) {

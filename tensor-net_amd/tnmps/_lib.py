@@ -90,6 +90,7 @@ SIGNATURES = {
     "tn_apply_u1": (C.c_int, [_h, _sz, _dp]),
     "tn_apply_u2": (C.c_int, [_h, _sz, _dp]),
     "tn_apply_u2_multi": (C.c_int, [_h, _sz, _sz, _dp]),
+    "tn_apply_u2_noalloc": (C.c_int, [_h, _sz, _dp]),
     "tn_take_flags": (C.c_int, [_h, _ip]),
     "tn_apply_unis": (C.c_int, [_h, _sz, _szp, _ip, _dp]),
     "tn_prob": (C.c_int, [_h, _sz, _sz, _dp]),

@@ -231,6 +231,15 @@ class MPS:
                "apply_unitary2_multi")
         return self
 
+    def apply_unitary2_noalloc(self, k: int, op) -> "MPS":
+        """``apply_unitary2`` with its workspace in the library's static
+        scratch (``tn_apply_u2_noalloc``): no device allocation in the call;
+        bonds k-1 and k+1 at most 256."""
+        buf = _op_buf(op, 4)
+        _check(self._lib.tn_apply_u2_noalloc(self._h, int(k), _dptr(buf)),
+               "apply_unitary2_noalloc")
+        return self
+
     def take_flags(self) -> int:
         """The handle's sticky failure flags (``TN_FLAG_CAPACITY``,
         ``TN_FLAG_NO_CONVERGENCE``) since the last call, then cleared."""
