@@ -4,7 +4,7 @@
 (tensor-net/lib/mps.rs) over the C ABI of ``libtnmps.so``; all arithmetic
 runs in hand-written HIP kernels for gfx950.
 """
-from .mps import MPS, Engine, MPSError  # noqa: F401
+from .mps import MPS, Engine, MPSError, MPSWarning  # noqa: F401
 from . import workload  # noqa: F401
 
-__all__ = ["MPS", "Engine", "MPSError", "workload"]
+__all__ = ["MPS", "Engine", "MPSError", "MPSWarning", "workload"]

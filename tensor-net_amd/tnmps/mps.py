@@ -50,6 +50,23 @@ def _check(code: int, where: str) -> None:
         raise MPSError(code, where)
 
 
+class MPSWarning(UserWarning):
+    """A rank clamped to its bond allocation (TN_CAPACITY) or an SVD stopped
+    at its sweep limit (TN_NO_CONVERGENCE) in a trajectory or batch call: the
+    states are written and the call's results are returned; the handles that
+    set it keep the bits (``MPS.take_flags``)."""
+
+
+def _check_warn(code: int, where: str) -> None:
+    """``_check`` for trajectory and batch calls: the two statuses that still
+    write every state are warnings there, so a batch keeps its results."""
+    if code in (L.TN_CAPACITY, L.TN_NO_CONVERGENCE):
+        import warnings
+        warnings.warn(MPSWarning(f"{where}: {L.status_str(code)}"), stacklevel=3)
+        return
+    _check(code, where)
+
+
 # contraction arithmetic names (tn_set_precision)
 PRECISIONS = {"f64": L.PREC_F64, "f32": L.PREC_F32, "bf16x3": L.PREC_BF16X3}
 
@@ -460,11 +477,11 @@ class MPS:
         cnts = np.zeros(max(depth, 1), dtype=np.uintp)
         oo = np.zeros(tot, dtype=np.int32)
         op = np.zeros(tot, dtype=np.float64)
-        _check(self._lib.tn_run_circuit(self._h, depth, _szptr(nu_a), _szptr(uks_a),
+        _check_warn(self._lib.tn_run_circuit(self._h, depth, _szptr(nu_a), _szptr(uks_a),
                                         _dptr(ops_a), _szptr(nm_a), _szptr(mks_a),
                                         _iptr(fo_a), _dptr(rr_a), int(ent_bond), _dptr(ent),
                                         _szptr(cnts), _iptr(oo), _dptr(op)),
-               "run_circuit")
+                     "run_circuit")
         # processed sites per layer: sorted, deduplicated, in range
         record, probs = [], []
         pos = 0
@@ -600,7 +617,7 @@ class Engine:
         sites = np.zeros(tot, dtype=np.uintp)
         oo = np.zeros(tot, dtype=np.int32)
         op = np.zeros(tot)
-        _check(self._lib.tn_batch_bilayer_gen(self._h, cnt, C.cast(hs, C.POINTER(C.c_void_p)),
+        _check_warn(self._lib.tn_batch_bilayer_gen(self._h, cnt, C.cast(hs, C.POINTER(C.c_void_p)),
                                               int(seed), tid.ctypes.data_as(C.POINTER(C.c_uint64)),
                                               lay.ctypes.data_as(C.POINTER(C.c_uint64)),
                                               _dptr(pm), _szptr(eb), _dptr(ent),
@@ -653,7 +670,7 @@ class Engine:
         tot = max(len(mks), 1)
         oo = np.zeros(tot, dtype=np.int32)
         op = np.zeros(tot)
-        _check(self._lib.tn_batch_bilayer(self._h, cnt, C.cast(hs, C.POINTER(C.c_void_p)),
+        _check_warn(self._lib.tn_batch_bilayer(self._h, cnt, C.cast(hs, C.POINTER(C.c_void_p)),
                                           _szptr(nu_a), _szptr(uks_a), _dptr(ops_a),
                                           _szptr(nm_a), _szptr(mks_a), _iptr(fo_a),
                                           _dptr(rr_a), _szptr(eb), _dptr(ent), _szptr(counts),

@@ -87,19 +87,21 @@ def test_batch_sampled_draws(gpu):
 
 def test_fp32_contraction_error(gpu):
     """TN_PREC_F32: theta in FP32 with FP32 accumulation, SVD in FP64.  The
-    entropy error against the FP64 oracle stays at the FP32 level."""
+    entropy error against the FP64 oracle stays within 1e-6 after every one
+    of the 12 layers, and so do the final Schmidt values."""
     n, depth, p, chi = 12, 12, 0.1, 16
     c = haar_circuit(n, depth, p, 77)
     ref = run_trajectory(n, c.layers, ("const", chi), rs=c.draws)
     st = MPS.new_qubits(n, ("const", chi)).set_precision("f32")
     got = st.run_circuit(c.layers, forced=ref["record"])
-    err = max(abs(a - b) for a, b in zip(got["entropy"], ref["entropy"]))
-    assert err < 1e-4, err
-    assert err > 0.0  # the FP32 path really ran
+    errs = [abs(a - b) for a, b in zip(got["entropy"], ref["entropy"])]
+    print(f"FP32 n={n} chi={chi}: entropy errors per layer", [f"{e:.1e}" for e in errs])
+    assert max(errs) <= 1e-6, errs  # every layer (measured: at most 4.3e-7)
+    assert max(errs) > 0.0  # the FP32 path really ran
     s0 = max(float(np.max(np.abs(st.schmidt(b)[: len(ref["state"].L[b])]
                                  - ref["state"].L[b][: len(st.schmidt(b))])))
              for b in range(n - 1))
-    assert s0 < 1e-4
+    assert s0 <= 1e-6
 
 
 def test_step_and_row_split_jacobi_agree(gpu, monkeypatch):

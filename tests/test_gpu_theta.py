@@ -70,11 +70,12 @@ def _prec_error(prec, seed):
 def test_bf16x3_contraction_error(gpu):
     """TN_PREC_BF16X3: each FP32 operand split into two BF16 halves, three
     BF16 products per real product, FP32 accumulation; about 16 bits of each
-    product are kept, so entropies and Schmidt values stay within 1e-3 of the
-    FP64 oracle, and the error is above the FP32 path's."""
+    product are kept, so entropies and Schmidt values stay within 5e-5 of the
+    FP64 oracle over 12 layers, and the error is above the FP32 path's."""
     ent, lam = _prec_error("bf16x3", 77)
     ent32, lam32 = _prec_error("f32", 77)
-    assert ent < 1e-3 and lam < 1e-3, (ent, lam)
+    print(f"bf16x3 entropy {ent:.2e} schmidt {lam:.2e}; f32 entropy {ent32:.2e} schmidt {lam32:.2e}")
+    assert ent < 5e-5 and lam < 5e-5, (ent, lam)  # measured: 1.1e-5, 6.7e-6
     assert ent > 0.0 and lam > 0.0
     assert max(ent, lam) > max(ent32, lam32)
 

@@ -358,6 +358,27 @@ def test_renyi_limits():
         assert abs(vals[2] - st.entropy_vn(b)) < 1e-5
 
 
+# ------------------------------------------- the alternate LAPACK driver
+def test_gesdd_oracle_matches_gesvd_oracle():
+    """The full-size GPU tests run the oracle with zgesdd for speed: on a
+    truncating trajectory it gives the zgesvd oracle's records, probabilities,
+    entropies and spectra to round-off."""
+    n, chi = 12, 16
+    c = haar_circuit(n, 12, 0.2, 4321)
+    a = run_trajectory(n, c.layers, ("const", chi), rs=c.draws)
+    VidalMPS.svd_driver = "gesdd"
+    try:
+        b = run_trajectory(n, c.layers, ("const", chi), forced=a["record"])
+    finally:
+        VidalMPS.svd_driver = "gesvd"
+    np.testing.assert_allclose(b["entropy"], a["entropy"], atol=1e-12)
+    pa = [p for layer in a["probs"] for _, p in layer]
+    pb = [p for layer in b["probs"] for _, p in layer]
+    np.testing.assert_allclose(pb, pa, atol=1e-12)
+    for bond in range(n - 1):
+        assert_spectrum_close(b["state"].L[bond], a["state"].L[bond], rtol=1e-12)
+
+
 # ------------------------------------------------- planted mistakes (CPU)
 def test_planted_renyi_unsquared_fails(monkeypatch):
     """entropy_ry_schmidt with s^a in place of (s^2)^a (a dropped square)
