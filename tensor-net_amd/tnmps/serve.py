@@ -17,9 +17,10 @@ tick, advances all of them by one layer in lock step (one
 ``batch_bilayer`` call for carried circuits, one ``batch_bilayer_gen``
 call per generator seed), so their gates and sweep steps share launches.
 Requests that have arrived are admitted in arrival order whenever the HBM
-held by resident
-trajectories stays within the budget (slots are counted in bytes of Gamma
-and lambda); finished trajectories leave at once and free their memory.
+held stays within the budget: each worker engine's update workspace (its
+cap, set from the budget when the engines are made) plus the resident
+trajectories' Gamma and lambda; finished trajectories leave at once and free
+their memory.
 
 With ``workers=W > 1`` the server runs W such lock-step loops at once, each
 on its own engine (stream) and host thread.  A lock-step tick lasts as long
@@ -50,6 +51,21 @@ def layer_cost(n: int, chi: int) -> float:
     2n two-site SVDs of order chi_eff^3, chi_eff the largest reachable bond."""
     ce = min(chi, 2 ** min(n // 2, 40))
     return float(n) * float(ce) ** 3
+
+
+GIB = float(1 << 30)
+
+
+def workspace_bytes(chi_max: int, ws_cap: float) -> float:
+    """Device bytes an engine's update workspace can reach under a
+    TNMPS_WS_BYTES cap of ws_cap: its slots (X and W of 2chi x 2chi complex
+    plus per-column arrays, as the library counts them; at least one) and
+    the row-split partial Gram matrices of up to 256 slots (one per CU)."""
+    ld = 2 * chi_max
+    per = 16.0 * (2.0 * ld * ld) + 64.0 * ld + 4096.0
+    slots = max(1.0, float(int(ws_cap // per)))
+    gpart = max(((ld + bw - 1) // bw // 2 + 1) * 8 * 4 * bw * bw for bw in (8, 16)) * 16.0
+    return slots * per + min(slots, 256.0) * gpart
 
 
 def mps_bytes(n: int, chi: int) -> int:
@@ -102,11 +118,28 @@ class Server:
 
     def __init__(self, chi_max: int = 256, hbm_budget: float = 64e9, max_active: int = 1024,
                  log: Optional[Callable[[str], None]] = None, workers: int = 1,
-                 dispatch: str = "least_loaded"):
+                 dispatch: str = "least_loaded", ws_cap: Optional[float] = None):
+        """``ws_cap``: each worker engine's workspace cap (TNMPS_WS_BYTES),
+        by default a quarter of the budget over the workers, at most 24 GiB;
+        what the workspaces can reach counts against the budget from the
+        start."""
         if dispatch not in ("least_loaded", "round_robin"):
             raise ValueError(f"dispatch {dispatch!r}")
         self.dispatch = dispatch
-        self.engines = [Engine(chi_max) for _ in range(max(1, workers))]
+        nw = max(1, workers)
+        if ws_cap is None:
+            ws_cap = min(24 * GIB, 0.25 * hbm_budget / nw)
+        import os
+        old = os.environ.get("TNMPS_WS_BYTES")
+        os.environ["TNMPS_WS_BYTES"] = str(int(ws_cap))
+        try:
+            self.engines = [Engine(chi_max) for _ in range(nw)]
+        finally:
+            if old is None:
+                del os.environ["TNMPS_WS_BYTES"]
+            else:
+                os.environ["TNMPS_WS_BYTES"] = old
+        self.ws_held = nw * workspace_bytes(chi_max, ws_cap)
         self._load = [0.0] * len(self.engines)  # outstanding work per worker
         self.engine = self.engines[0]
         self.chi_max = chi_max
@@ -114,10 +147,10 @@ class Server:
         self.max_active = max_active
         self.ticks = 0
         self.layers_done = 0
-        self.peak_held = 0.0   # largest bytes resident while more than one was active
+        self.peak_held = 0.0   # largest bytes held while more than one was active
         self.log = log
         self._lock = threading.Lock()
-        self._held = 0.0
+        self._held = self.ws_held
         self._nactive = 0
 
     def stats(self):

@@ -7,7 +7,7 @@ import pytest
 
 from oracle.mps_oracle import run_trajectory
 from tnmps import MPS, Engine
-from tnmps.serve import Request, Server, mps_bytes
+from tnmps.serve import Request, Server, mps_bytes, workspace_bytes
 from tnmps.workload import haar_circuit
 
 pytestmark = pytest.mark.gpu
@@ -29,9 +29,13 @@ def _requests():
 
 def test_serving_mix_equals_alone(gpu):
     reqs = _requests()
-    # a budget that holds only part of the stream forces queueing + admission
-    budget = sum(mps_bytes(r.n, r.chi) for r in reqs[:3])
-    srv = Server(chi_max=32, hbm_budget=budget)
+    # a budget that holds the engine's workspace (capped at 8 slots, so
+    # batches are chunked too) and only part of the stream forces queueing
+    # and admission
+    ws_cap = 8 * (16.0 * 2 * 64 * 64 + 64 * 64 + 4096)
+    budget = workspace_bytes(32, ws_cap) + sum(mps_bytes(r.n, r.chi) for r in reqs[:3])
+    srv = Server(chi_max=32, hbm_budget=budget, ws_cap=ws_cap)
+    assert srv.ws_held == workspace_bytes(32, ws_cap)
     # simulated clock: one time unit per tick, so arrivals interleave with layers
     ticks = {"t": 0.0}
 
@@ -105,8 +109,9 @@ def test_serving_workers_equal_alone(gpu, dispatch):
     for i, (n, p, chi, t) in enumerate([(10, 0.2, 16, 0.0), (12, 0.3, 8, 0.1)]):
         reqs.append(Request(rid=20 + i, n=n, p=p, chi=chi, arrival=t, gen_seed=seed,
                             traj_id=200 + i, gen_depth=n))
-    budget = sum(mps_bytes(r.n, r.chi) for r in reqs[:4])
-    srv = Server(chi_max=32, hbm_budget=budget, workers=2, dispatch=dispatch)
+    ws_cap = 16 * (16.0 * 2 * 64 * 64 + 64 * 64 + 4096)
+    budget = 2 * workspace_bytes(32, ws_cap) + sum(mps_bytes(r.n, r.chi) for r in reqs[:4])
+    srv = Server(chi_max=32, hbm_budget=budget, workers=2, dispatch=dispatch, ws_cap=ws_cap)
     srv.run(reqs)
     assert srv.peak_held <= budget
     assert srv.stats()[0] > 0
