@@ -285,6 +285,7 @@ struct tn_engine {
   DescRing<MeasDesc> meas;
   DescRing<EntDesc> ents;
   int mres = 0;  // measurement result slots
+  double* d_mpart = nullptr;   // per-site partial probabilities (2 kMeasParts each)
   int* d_out = nullptr;
   double* d_prob = nullptr;
   double* d_probs = nullptr;
@@ -337,7 +338,8 @@ struct tn_engine {
     slots = 0;
   }
   void release_res() {
-    hfree(d_out); hfree(d_prob); hfree(d_probs); hfree(d_flag); hfree(d_mdims);
+    hfree(d_out); hfree(d_prob); hfree(d_probs); hfree(d_flag); hfree(d_mdims); hfree(d_mpart);
+    d_mpart = nullptr;
     d_out = nullptr; d_prob = d_probs = nullptr; d_flag = nullptr; d_mdims = nullptr;
     n_mdims = 0;
     mres = 0;
@@ -556,6 +558,7 @@ struct tn_engine {
     TN_HIP(hipMalloc(&d_prob, sizeof(double) * ns));
     TN_HIP(hipMalloc(&d_probs, sizeof(double) * 2 * ns));
     TN_HIP(hipMalloc(&d_flag, sizeof(int) * ns));
+    TN_HIP(hipMalloc(&d_mpart, sizeof(double) * 2 * kMeasParts * ns));
     mres = ns;
     return TN_OK;
   }
@@ -1062,7 +1065,11 @@ tn_status run_measures(tn_engine* e, const std::vector<MTask>& ms) {
     for (int i = 0; i < nm; ++i) pr->proj[i] = ms[i].forced != -2 ? 1 : 0;
     TN_HIP(hipEventRecord(pr->ev[0], e->stream));
   }
-  launch_measure(e->meas.d[rs], nm, e->stream);
+  // workgroups per site: at least 8 rows each, from the host's bound on chiL
+  size_t maxL = 1;
+  for (const MTask& q : ms) maxL = std::max(maxL, q.h->uL(q.k));
+  const int G = (int)std::min<size_t>(kMeasParts, std::max<size_t>(1, maxL / 8));
+  launch_measure(e->meas.d[rs], nm, G, e->d_mpart, e->stream);
   TN_HIP(hipGetLastError());
   if (pr) {
     TN_HIP(hipEventRecord(pr->ev[1], e->stream));

@@ -30,10 +30,11 @@ namespace tnmps {
 
 constexpr int kBW = 8;          // Jacobi block width for batched work (columns)
 constexpr int kBWWide = 16;     // Jacobi block width for lone large SVDs
-constexpr int kMaxSweeps = 40;  // outer Jacobi sweep limit
+constexpr int kMaxSweeps = 64;  // outer Jacobi sweep limit (slow graded inputs take up to ~45)
 constexpr int kNPart = 64;      // blocks of the split/renormalise kernels
 constexpr int kMaxCols = 1024;  // largest C (= min(m, n)) the SVD accepts
 constexpr int kMaxSplit = 8;    // workgroups per block pair in the row-split step
+constexpr int kMeasParts = 64;  // most workgroups per measured site
 
 // info[] slots of a task (device ints)
 enum {

@@ -29,7 +29,8 @@ void launch_finish(const UpdDesc* descs, int nt, int sweeps_run, hipStream_t s);
 void launch_fmul(const UpdDesc* descs, int nt, int maxX, int maxRank, hipStream_t s);
 void launch_split(const UpdDesc* descs, int nt, hipStream_t s);
 void launch_renorm(const UpdDesc* descs, int nt, hipStream_t s);
-void launch_measure(const MeasDesc* descs, int nm, hipStream_t s);
+// G workgroups per site (1 .. kMeasParts); parts holds 2 kMeasParts doubles per site.
+void launch_measure(const MeasDesc* descs, int nm, int G, double* parts, hipStream_t s);
 // One workgroup per entropy read-out descs[i], result to out[descs[i].slot].
 void launch_entropy(const EntDesc* descs, int count, double* out, hipStream_t s);
 // One site of the state-vector contraction: out (2P x chiR) from acc

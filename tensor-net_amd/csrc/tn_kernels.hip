@@ -17,7 +17,8 @@
 //   k_split        new Gamma[k], Gamma[k+1] with lambda divisions and partial
 //                  norms (mps.rs:852-876, 1357-1380, local_norm 1147-1202)
 //   k_renorm       local_renormalize of both sites (mps.rs:1206-1209)
-//   k_measure      Born probabilities, sampling or forced outcome, projection
+//   k_measure_part, k_measure_project
+//                  Born probabilities, sampling or forced outcome, projection
 //                  (mps.rs:1027-1134, 1427-1465, 2229-2242)
 //   k_entropy      von Neumann / Renyi entropy of one bond (mps.rs:1251-1282)
 //   k_u1           single-site unitary (mps.rs:1298-1314)
@@ -682,17 +683,30 @@ __global__ __launch_bounds__(256) void k_renorm(const UpdDesc* __restrict__ desc
 }
 
 // ------------------------------------------------------------ measure
-__global__ __launch_bounds__(1024) void k_measure(const MeasDesc* __restrict__ descs) {
-  const MeasDesc& d = descs[blockIdx.x];
+// A site's Born probabilities and projection (mps.rs:1027-1134, 1427-1465)
+// over gridDim.x workgroups per site (grid x, site y): k_measure_part sums
+// p_s = sum_v lamL_v^2 sum_u |Gamma[v, s, u]|^2 lamR_u^2 over the workgroup's
+// rows v; k_measure_project sums the partials in one fixed order (so every
+// workgroup of the site holds bitwise the same p0, p1 and outcome), the
+// first one writes the results, and each projects its own rows.  A wave
+// takes whole rows, its lanes run along u, so (v, 0) and (v, 1) are read
+// contiguously.
+__device__ __forceinline__ void meas_rows(int chiL, int g, int G, int& v0, int& v1) {
+  const int per = (chiL + G - 1) / G;
+  v0 = min(chiL, g * per);
+  v1 = min(chiL, v0 + per);
+}
+
+__global__ __launch_bounds__(256) void k_measure_part(const MeasDesc* __restrict__ descs,
+                                                      double* __restrict__ parts) {
+  const MeasDesc& d = descs[blockIdx.y];
   const int chiL = d.dims[0], chiR = d.dims[1];
-  __shared__ double red[16];
-  __shared__ int s_out;
-  __shared__ double s_prob;
-  // wave w takes rows v = w, w + 16, ...; its lanes run along u, so the two
-  // rows (v, s=0) and (v, s=1) of Gamma[k] are read contiguously
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  __shared__ double red[4];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  int v0, v1;
+  meas_rows(chiL, blockIdx.x, gridDim.x, v0, v1);
   double p0 = 0.0, p1 = 0.0;
-  for (int v = wid; v < chiL; v += nw) {
+  for (int v = v0 + wid; v < v1; v += 4) {
     const double wl = d.lamL ? d.lamL[v] * d.lamL[v] : 1.0;
     const double2* r0 = d.g + (size_t)v * 2 * chiR;
     const double2* r1 = r0 + chiR;
@@ -708,40 +722,71 @@ __global__ __launch_bounds__(1024) void k_measure(const MeasDesc* __restrict__ d
   p0 = block_sum(p0, red);
   p1 = block_sum(p1, red);
   if (threadIdx.x == 0) {
-    int flag = 0, out;
-    double prob;
-    if (d.forced >= 0) {
-      out = d.forced;
-      prob = out == 0 ? p0 : p1;
-    } else {
-      // first j with r < cumulative probability (mps.rs:1441-1446)
-      const double c0 = p0, c1 = p0 + p1;
-      if (d.r < c0) out = 0;
-      else if (d.r < c1) out = 1;
-      else {
-        out = p1 > 0.0 ? 1 : 0;
-        flag |= 2;
-      }
-      prob = out == 0 ? p0 : p1;
+    double* pp = parts + ((size_t)blockIdx.y * kMeasParts + blockIdx.x) * 2;
+    pp[0] = p0;
+    pp[1] = p1;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_measure_project(const MeasDesc* __restrict__ descs,
+                                                         const double* __restrict__ parts) {
+  const MeasDesc& d = descs[blockIdx.y];
+  const int chiL = d.dims[0], chiR = d.dims[1];
+  __shared__ double s_p[2];
+  if (threadIdx.x < 64) {
+    // the site's partials, one per lane, summed across the wave in a fixed order
+    const double* pp = parts + (size_t)blockIdx.y * kMeasParts * 2;
+    double a = 0.0, b = 0.0;
+    if (threadIdx.x < (int)gridDim.x) {
+      a = pp[2 * threadIdx.x];
+      b = pp[2 * threadIdx.x + 1];
     }
-    if (d.forced != -2 && !(prob > 0.0)) flag |= 1;
-    s_out = out;
-    s_prob = prob;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      a += __shfl_xor(a, off, 64);
+      b += __shfl_xor(b, off, 64);
+    }
+    if (threadIdx.x == 0) {
+      s_p[0] = a;
+      s_p[1] = b;
+    }
+  }
+  __syncthreads();
+  const double p0 = s_p[0], p1 = s_p[1];
+  int flag = 0, out;
+  double prob;
+  if (d.forced >= 0) {
+    out = d.forced;
+    prob = out == 0 ? p0 : p1;
+  } else {
+    // first j with r < cumulative probability (mps.rs:1441-1446)
+    const double c0 = p0, c1 = p0 + p1;
+    if (d.r < c0) out = 0;
+    else if (d.r < c1) out = 1;
+    else {
+      out = p1 > 0.0 ? 1 : 0;
+      flag |= 2;
+    }
+    prob = out == 0 ? p0 : p1;
+  }
+  if (d.forced != -2 && !(prob > 0.0)) flag |= 1;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
     d.out_outcome[d.slot] = out;
     d.out_prob[d.slot] = prob;
     d.out_probs[2 * d.slot + 0] = p0;
     d.out_probs[2 * d.slot + 1] = p1;
     d.out_flag[d.slot] = flag;
     if (d.out_dims) {
-      d.out_dims[2 * blockIdx.x + 0] = chiL;
-      d.out_dims[2 * blockIdx.x + 1] = chiR;
+      d.out_dims[2 * blockIdx.y + 0] = chiL;
+      d.out_dims[2 * blockIdx.y + 1] = chiR;
     }
   }
-  __syncthreads();
   if (d.forced == -2) return;
-  const int out = s_out;
-  const double renorm = sqrt(s_prob);
-  for (int v = wid; v < chiL; v += nw) {
+  const double renorm = sqrt(prob);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  int v0, v1;
+  meas_rows(chiL, blockIdx.x, gridDim.x, v0, v1);
+  for (int v = v0 + wid; v < v1; v += 4) {
     double2* keep = d.g + ((size_t)v * 2 + out) * chiR;
     double2* drop = d.g + ((size_t)v * 2 + (1 - out)) * chiR;
     for (int u = lane; u < chiR; u += 64) {
@@ -835,8 +880,10 @@ void launch_split(const UpdDesc* descs, int nt, hipStream_t s) {
 void launch_renorm(const UpdDesc* descs, int nt, hipStream_t s) {
   hipLaunchKernelGGL(k_renorm, dim3(kNPart, nt), dim3(256), 0, s, descs);
 }
-void launch_measure(const MeasDesc* descs, int nm, hipStream_t s) {
-  hipLaunchKernelGGL(k_measure, dim3(nm), dim3(1024), 0, s, descs);
+void launch_measure(const MeasDesc* descs, int nm, int G, double* parts, hipStream_t s) {
+  hipLaunchKernelGGL(k_measure_part, dim3(G, nm), dim3(256), 0, s, descs, parts);
+  hipLaunchKernelGGL(k_measure_project, dim3(G, nm), dim3(256), 0, s, descs,
+                     (const double*)parts);
 }
 void launch_entropy(const EntDesc* descs, int count, double* out, hipStream_t s) {
   if (count <= 0) return;
