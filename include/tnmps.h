@@ -116,6 +116,15 @@ int tn_device_count(void);
  * TN_EMPTY_SYSTEM. */
 tn_status tn_mps_new_qubits(size_t n, tn_trunc trunc, size_t chi_cap,
                             void* stream, tn_mps** out);
+/* MPS::from_vector (mps.rs:516-546 -> factorize, mps.rs:448-508) for n
+ * qubits: psi holds 2^n interleaved complex amplitudes, qubit 0 the slowest
+ * index; it is normalised, then factored site by site, each step an SVD on
+ * the device with process_svd's rank rule (trunc, chi_cap as in
+ * tn_mps_new_qubits).  n == 0 -> TN_EMPTY_SYSTEM; a zero or non-finite
+ * vector, or n > 30 -> TN_INVALID_ARG; a rank above chi_cap -> TN_CAPACITY
+ * (the state is made). */
+tn_status tn_mps_from_vector(size_t n, const double* psi, tn_trunc trunc, size_t chi_cap,
+                             void* stream, tn_mps** out);
 /* Drop (frees device memory).  A handle made by tn_mps_new_qubits frees its
  * private engine with it. */
 void tn_mps_free(tn_mps* h);

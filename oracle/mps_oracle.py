@@ -403,6 +403,47 @@ class VidalMPS:
         return max(self.bond_dims()) if self.L else None
 
 
+def from_vector(n: int, psi, trunc=None, driver: str = "gesvd") -> VidalMPS:
+    """``MPS::from_vector`` (mps.rs:516-546) -> ``factorize`` (mps.rs:448-508)
+    for n qubits: normalise (:531-537), then for each site fuse its index with
+    the previous Schmidt index (:465-470), decompose (local_decomp, :474),
+    Gamma[k] = U[:, :r] with rows divided by the previous Schmidt values
+    (:478-492), q <- the scaled right factor; the last Gamma is the final q
+    with the same division (:499-505).  The SVD is the economy-size one: the
+    rows of V^H beyond min(m, n) that zgesvd's full form adds are never kept
+    (process_svd keeps r <= min(m, n) of them)."""
+    psi = np.asarray(psi, dtype=np.complex128).reshape(-1)
+    if n < 1:
+        raise OracleError("EmptySystem")
+    if psi.size != 2 ** n:
+        raise OracleError("StateIncompatibleShape")
+    norm = math.sqrt(sum(float(abs(a)) ** 2 for a in psi))          # :531-536
+    psi = psi / norm
+    st = VidalMPS(n, trunc)
+    if n == 1:
+        st.G = [psi.reshape(1, 2, 1).copy()]                        # :538-544
+        return st
+    G: List[np.ndarray] = []
+    L: List[np.ndarray] = []
+    udim = 1
+    q = psi.reshape(1, -1)
+    for _ in range(n - 1):
+        m = udim * 2                                                 # :465-470
+        q = q.reshape(m, q.size // m)
+        u, s, vh = sla.svd(q, full_matrices=False, lapack_driver=driver,
+                           check_finite=False)
+        u, s, q, rank = process_svd(u, s, vh, trunc)                 # :474
+        g = u[:, :rank].reshape(udim, 2, rank)                       # :478-483
+        if L:
+            g = g / L[-1][:, None, None]                             # :484-490
+        G.append(np.ascontiguousarray(g))
+        L.append(s)
+        udim = rank
+    G.append(np.ascontiguousarray(q.reshape(udim, 2, 1) / L[-1][:, None, None]))  # :499-505
+    st.G, st.L = G, L
+    return st
+
+
 def apply_bilayer(state: VidalMPS, unis, meas):
     """``circuit::apply_bilayer`` (circuit.rs:624-659).
 

@@ -726,6 +726,131 @@ Workspace engine_workspace(const tn_engine* e) {
   return w;
 }
 
+// The Jacobi sweeps of one launch batch: nt tasks in workspace w, rows and
+// columns bounded by maxR and maxC, launch shapes from tuning row tu, stop
+// tolerance conv_tol.  *sweeps_run receives the sweeps checked, *launches
+// the step kernels launched; no_conv(t) is called for each task the sweep
+// limit stopped above the tolerance.  trace[] (gate, prepass, mode) labels
+// the TNMPS_TRACE_SWEEPS lines.
+template <typename NoConv>
+tn_status jacobi_sweeps(tn_engine* e, const Workspace& w, int nt, size_t maxR, size_t maxC,
+                        const tn_tune& tu, double conv_tol, const int (&trace)[3],
+                        NoConv no_conv, int* sweeps_run, size_t* launches) {
+  const int bw = tu.bw;
+  const int nb = (int)((maxC + bw - 1) / bw);
+  const int nbe = nb + (nb & 1);
+  const int npairs = nbe / 2, steps = nbe - 1;
+  const Tolerances& tol = e->tol;
+  // One launch per round-robin step (two when the rows are split); the host
+  // checks convergence per sweep.  Few pairs (a lone large SVD) leave most
+  // CUs idle: split each pair's rows over S workgroups (gram kernel, then
+  // solve + apply kernel).
+  int S = 1;
+  // (S > 1 needs nt * npairs <= cus, so nt <= cus: the partial Gram buffer
+  // covers min(slots, cus) tasks)
+  while (S < tu.split_max && (size_t)nt * npairs * S * 2 <= (size_t)2 * e->cus &&
+         maxR >= (size_t)tu.split_rows * S && nt <= w.gpart_slots)
+    S *= 2;
+  JacobiArgs ja{};
+  ja.X = w.X;
+  ja.W = w.W;
+  ja.gpart = w.gpart;
+  ja.offs = w.offs;
+  ja.info = w.info;
+  // Many tasks: the step kernel takes the sweep's unconverged tasks from a
+  // list that a one-workgroup kernel builds before the sweep, so tasks that
+  // have converged (a lock-step batch runs until its slowest task has) cost
+  // no workgroups.  The grid's rows are the unconverged count of the last
+  // sweep read back (or nt), a bound on the list's length since a converged
+  // task stays converged.  The arithmetic per task is the same either way.
+  const bool use_list = S == 1 && nt >= e->list_min && nt <= 8192;
+  ja.act = use_list ? w.act : nullptr;
+  int open_known = nt;
+  ja.xslot = (size_t)w.ldx * w.ldw;
+  ja.wslot = (size_t)w.ldw * w.ldw;
+  ja.gslot = w.gpart_per;
+  ja.ldx = w.ldx;
+  ja.ldw = w.ldw;
+  ja.nt = nt;
+  ja.nbe = nbe;
+  ja.S = S;
+  ja.Rh = (int)maxR;
+  ja.Ch = (int)maxC;
+  ja.tol_rot = tol.tol_rot;
+  ja.conv_tol = conv_tol;
+  ja.tiny2 = tol.tiny_norm * tol.tiny_norm;
+  // Sweeps are enqueued one ahead while the last sweep read back is still far
+  // from convergence (max off-diagonal above kAheadOff: with quadratic
+  // convergence the next sweep cannot reach conv_tol, so the one after it is
+  // needed too); each sweep's maxima come back through pinned memory behind
+  // an event.  Near convergence the host waits for every sweep, so no sweep
+  // is enqueued that is not needed; a sweep enqueued after convergence
+  // returns at once on the device.
+  // (1e-3 for conv_tol = 1e-8; scaled by the square root for looser tolerances)
+  const double kAheadOff = std::min(0.5, 1e-3 * std::sqrt(conv_tol / 1e-8));
+  int launched = 0, checked = 0;
+  double last_worst = 1.0;
+  bool done = false;
+  const double* last_o = nullptr;  // the last sweep's maxima read back
+  while (!done) {
+    const int depth = last_worst > kAheadOff ? tu.ahead : 1;
+    const auto h0 = std::chrono::steady_clock::now();
+    while (launched < e->max_sweeps && launched - checked < depth) {
+      if (use_list) launch_active(ja, launched, w.act, e->stream);
+      for (int s = 0; s < steps; ++s) {
+        if (S == 1) {
+          launch_jacobi_step(ja, npairs, bw, launched, s, use_list ? open_known : nt,
+                             e->stream);
+        } else {
+          launch_jacobi_gram(ja, npairs, bw, launched, s, e->stream);
+          launch_jacobi_apply(ja, npairs, bw, launched, s, e->stream);
+        }
+      }
+      TN_HIP(hipGetLastError());
+      TN_HIP(hipMemcpyAsync(w.h_offs + (size_t)launched * nt, w.offs + (size_t)launched * nt,
+                            sizeof(double) * nt, hipMemcpyDeviceToHost, e->stream));
+      TN_HIP(hipEventRecord(e->sev[launched], e->stream));
+      ++launched;
+    }
+    e->perf_host_launch_ms +=
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count();
+    if (checked >= launched) break;  // sweep limit
+    TN_HIP(hipEventSynchronize(e->sev[checked]));
+    const double* o = w.h_offs + (size_t)checked * nt;
+    last_o = o;
+    done = true;
+    double worst = 0.0;
+    int open = 0;
+    for (int t = 0; t < nt; ++t) {
+      if (!(o[t] <= conv_tol)) {
+        done = false;
+        ++open;
+      }
+      worst = std::max(worst, o[t]);
+    }
+    if (e->trace_sweeps)
+      std::fprintf(stderr,
+                   "[tnmps] batch of %d (C <= %zu, gate %d, prepass %d, mode %d): sweep %d max "
+                   "off-diagonal %.3e, %d not converged\n",
+                   nt, maxC, trace[0], trace[1], trace[2], checked, worst, open);
+    last_worst = worst;
+    open_known = std::max(open, 1);
+    ++checked;
+  }
+  // The sweep limit stopped tasks whose last sweep was still above the
+  // tolerance: their results are written as they stand and reported.
+  if (!done && last_o) {
+    for (int t = 0; t < nt; ++t)
+      if (!(last_o[t] <= conv_tol)) {
+        no_conv(t);
+        e->host_flags |= kFlagNoConv;
+      }
+  }
+  *sweeps_run = checked;
+  *launches = (size_t)launched * steps * (S == 1 ? 1 : 2);
+  return TN_OK;
+}
+
 // Enqueue up to max_slots independent updates.  Within one handle, bonds
 // must pairwise differ by at least 2 so that no update reads what another
 // writes; different handles are independent.  With `ws` the batch runs in
@@ -826,124 +951,17 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt, const Workspace* ws
   // Block width 8 by default (measured faster than 16 even for a lone
   // 512-column SVD: the 32x32 inner solve costs more than the halved step
   // count saves); the tuning table may select the wide panels per bucket.
-  const int bw = tu.bw;
-  const int nb = (int)((maxC + bw - 1) / bw);
-  const int nbe = nb + (nb & 1);
-  const int npairs = nbe / 2, steps = nbe - 1;
-  const Tolerances& tol = e->tol;
-  int sweeps_run = 0;
-  // One launch per round-robin step (two when the rows are split); the host
-  // checks convergence per sweep.  Few pairs (a lone large SVD) leave most
-  // CUs idle: split each pair's rows over S workgroups (gram kernel, then
-  // solve + apply kernel).
-  int S = 1;
-  // (S > 1 needs nt * npairs <= cus, so nt <= cus: the partial Gram buffer
-  // covers min(slots, cus) tasks)
-  while (S < tu.split_max && (size_t)nt * npairs * S * 2 <= (size_t)2 * e->cus &&
-         maxR >= (size_t)tu.split_rows * S && nt <= w.gpart_slots)
-    S *= 2;
-  JacobiArgs ja{};
-  ja.X = w.X;
-  ja.W = w.W;
-  ja.gpart = w.gpart;
-  ja.offs = w.offs;
-  ja.info = w.info;
-  // Many tasks: the step kernel takes the sweep's unconverged tasks from a
-  // list that a one-workgroup kernel builds before the sweep, so tasks that
-  // have converged (a lock-step batch runs until its slowest task has) cost
-  // no workgroups.  The grid's rows are the unconverged count of the last
-  // sweep read back (or nt), a bound on the list's length since a converged
-  // task stays converged.  The arithmetic per task is the same either way.
-  const bool use_list = S == 1 && nt >= e->list_min && nt <= 8192;
-  ja.act = use_list ? w.act : nullptr;
-  int open_known = nt;
-  ja.xslot = (size_t)w.ldx * w.ldw;
-  ja.wslot = (size_t)w.ldw * w.ldw;
-  ja.gslot = w.gpart_per;
-  ja.ldx = w.ldx;
-  ja.ldw = w.ldw;
-  ja.nt = nt;
-  ja.nbe = nbe;
-  ja.S = S;
-  ja.Rh = (int)maxR;
-  ja.Ch = (int)maxC;
-  ja.tol_rot = tol.tol_rot;
   // one precision per launch batch (run_tasks)
-  const double conv_tol = tol.for_prec(tasks[0].h->prec);
-  ja.conv_tol = conv_tol;
-  ja.tiny2 = tol.tiny_norm * tol.tiny_norm;
-  // Sweeps are enqueued one ahead while the last sweep read back is still far
-  // from convergence (max off-diagonal above kAheadOff: with quadratic
-  // convergence the next sweep cannot reach conv_tol, so the one after it is
-  // needed too); each sweep's maxima come back through pinned memory behind
-  // an event.  Near convergence the host waits for every sweep, so no sweep
-  // is enqueued that is not needed; a sweep enqueued after convergence
-  // returns at once on the device.
-  // (1e-3 for conv_tol = 1e-8; scaled by the square root for looser tolerances)
-  const double kAheadOff = std::min(0.5, 1e-3 * std::sqrt(conv_tol / 1e-8));
-  int launched = 0, checked = 0;
-  double last_worst = 1.0;
-  bool done = false;
-  const double* last_o = nullptr;  // the last sweep's maxima read back
+  const double conv_tol = e->tol.for_prec(tasks[0].h->prec);
+  int sweeps_run = 0;
+  size_t launches = 0;
+  const int trace[3] = {tasks[0].has_gate ? 1 : 0, tasks[0].pre, mode0};
   if (pr) TN_HIP(hipEventRecord(pr->ev[2], e->stream));
-  while (!done) {
-    const int depth = last_worst > kAheadOff ? tu.ahead : 1;
-    const auto h0 = std::chrono::steady_clock::now();
-    while (launched < e->max_sweeps && launched - checked < depth) {
-      if (use_list) launch_active(ja, launched, w.act, e->stream);
-      for (int s = 0; s < steps; ++s) {
-        if (S == 1) {
-          launch_jacobi_step(ja, npairs, bw, launched, s, use_list ? open_known : nt,
-                             e->stream);
-        } else {
-          launch_jacobi_gram(ja, npairs, bw, launched, s, e->stream);
-          launch_jacobi_apply(ja, npairs, bw, launched, s, e->stream);
-        }
-      }
-      TN_HIP(hipGetLastError());
-      TN_HIP(hipMemcpyAsync(w.h_offs + (size_t)launched * nt, w.offs + (size_t)launched * nt,
-                            sizeof(double) * nt, hipMemcpyDeviceToHost, e->stream));
-      TN_HIP(hipEventRecord(e->sev[launched], e->stream));
-      ++launched;
-    }
-    e->perf_host_launch_ms +=
-        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count();
-    if (checked >= launched) break;  // sweep limit
-    TN_HIP(hipEventSynchronize(e->sev[checked]));
-    const double* o = w.h_offs + (size_t)checked * nt;
-    last_o = o;
-    done = true;
-    double worst = 0.0;
-    int open = 0;
-    for (int t = 0; t < nt; ++t) {
-      if (!(o[t] <= conv_tol)) {
-        done = false;
-        ++open;
-      }
-      worst = std::max(worst, o[t]);
-    }
-    if (e->trace_sweeps)
-      std::fprintf(stderr,
-                   "[tnmps] batch of %d (C <= %zu, gate %d, prepass %d, mode %d): sweep %d max "
-                   "off-diagonal %.3e, %d not converged\n",
-                   nt, maxC, tasks[0].has_gate ? 1 : 0, tasks[0].pre, mode0, checked, worst, open);
-    last_worst = worst;
-    open_known = std::max(open, 1);
-    ++checked;
-  }
-  sweeps_run = checked;
-  // The sweep limit stopped tasks whose last sweep was still above the
-  // tolerance: their results are written as they stand and reported.
-  if (!done && last_o) {
-    for (int t = 0; t < nt; ++t)
-      if (!(last_o[t] <= conv_tol)) {
-        tasks[t].h->flags |= kFlagNoConv;
-        e->host_flags |= kFlagNoConv;
-      }
-  }
+  TN_TRY(jacobi_sweeps(e, w, nt, maxR, maxC, tu, conv_tol, trace,
+                       [&](int t) { tasks[t].h->flags |= kFlagNoConv; }, &sweeps_run, &launches));
   if (pr) {
     TN_HIP(hipEventRecord(pr->ev[3], e->stream));
-    pr->launches = (size_t)launched * steps * (S == 1 ? 1 : 2);
+    pr->launches = launches;
     TN_HIP(hipEventRecord(pr->ev[4], e->stream));
   }
   launch_finish(dd, nt, sweeps_run, e->stream);
@@ -1544,6 +1562,169 @@ tn_status tn_mps_new_qubits(size_t n, tn_trunc trunc, size_t chi_cap, void* stre
   }
   e->auto_free = true;
   return TN_OK;
+}
+
+tn_status tn_mps_from_vector(size_t n, const double* psi, tn_trunc trunc, size_t chi_cap,
+                             void* stream, tn_mps** out) {
+  if (!out) return TN_INVALID_ARG;
+  *out = nullptr;
+  if (n == 0) return TN_EMPTY_SYSTEM;  // mps.rs:527
+  if (!psi || n > 30) return TN_INVALID_ARG;
+  TN_TRY(check_trunc(trunc));
+  const size_t len = (size_t)1 << n;
+  // normalise (mps.rs:531-537); a zero vector has no normalised form
+  double nrm2 = 0.0;
+  for (size_t i = 0; i < 2 * len; ++i) nrm2 += psi[i] * psi[i];
+  if (!(nrm2 > 0.0) || !std::isfinite(nrm2)) {
+    g_last_error = "tn_mps_from_vector: the state vector has no finite non-zero norm";
+    return TN_INVALID_ARG;
+  }
+  const double nrm = std::sqrt(nrm2);
+  tn_mps* h = nullptr;
+  TN_TRY(tn_mps_new_qubits(n, trunc, chi_cap, stream, &h));
+  tn_engine* e = h->eng;
+  // the conjugated normalised vector: q = psi as 2 x 2^(n-1), row-major
+  std::vector<double2> q0(len);
+  for (size_t i = 0; i < len; ++i)
+    q0[i] = make_double2(psi[2 * i] / nrm, -psi[2 * i + 1] / nrm);
+  if (n == 1) {  // mps.rs:538-544: Gamma[0] = the vector
+    std::vector<double2> g(2);
+    for (int i = 0; i < 2; ++i) g[i] = make_double2(q0[i].x, -q0[i].y);
+    const hipError_t er = hipMemcpy(h->gam, g.data(), sizeof(double2) * 2, hipMemcpyHostToDevice);
+    if (er != hipSuccess) {
+      tn_mps_free(h);
+      g_last_error = hipGetErrorString(er);
+      return TN_DEVICE;
+    }
+    *out = h;
+    return TN_OK;
+  }
+  // temporaries of the call: two factor buffers, the SVD workspace of one
+  // task with C <= kMaxCols columns, and its descriptor
+  constexpr int CM = kMaxCols;
+  const size_t gp = std::max(((size_t)CM / kBW / 2 + 1) * kMaxSplit * (size_t)(4 * kBW * kBW),
+                             ((size_t)CM / kBWWide / 2 + 1) * kMaxSplit * (size_t)(4 * kBWWide * kBWWide));
+  double2 *bufA = nullptr, *bufB = nullptr, *W = nullptr, *gpart = nullptr;
+  double *sraw = nullptr, *sval = nullptr, *fac = nullptr, *part = nullptr, *offs = nullptr;
+  int *perm = nullptr, *info = nullptr, *act = nullptr;
+  UpdDesc* dd = nullptr;
+  auto cleanup = [&]() {
+    (void)hipStreamSynchronize(e->stream);
+    hfree(bufA); hfree(bufB); hfree(W); hfree(gpart); hfree(sraw); hfree(sval); hfree(fac);
+    hfree(part); hfree(offs); hfree(perm); hfree(info); hfree(act); hfree(dd);
+  };
+  auto fail = [&](tn_status st) {
+    cleanup();
+    tn_mps_free(h);
+    return st;
+  };
+#define FV_HIP(expr)                                                          \
+  do {                                                                        \
+    hipError_t e__ = (expr);                                                  \
+    if (e__ != hipSuccess) {                                                  \
+      g_last_error = std::string(#expr) + ": " + hipGetErrorString(e__);      \
+      return fail(TN_DEVICE);                                                 \
+    }                                                                         \
+  } while (0)
+  FV_HIP(hipMalloc(&bufA, sizeof(double2) * len));
+  FV_HIP(hipMalloc(&bufB, sizeof(double2) * len));
+  FV_HIP(hipMalloc(&W, sizeof(double2) * (size_t)CM * CM));
+  FV_HIP(hipMalloc(&gpart, sizeof(double2) * gp));
+  FV_HIP(hipMalloc(&sraw, sizeof(double) * CM));
+  FV_HIP(hipMalloc(&sval, sizeof(double) * CM));
+  FV_HIP(hipMalloc(&fac, sizeof(double) * CM));
+  FV_HIP(hipMalloc(&part, sizeof(double) * 2 * kNPart));
+  FV_HIP(hipMalloc(&offs, sizeof(double) * kMaxSweeps));
+  FV_HIP(hipMalloc(&perm, sizeof(int) * CM));
+  FV_HIP(hipMalloc(&info, sizeof(int) * kInfoLen));
+  FV_HIP(hipMalloc(&act, sizeof(int) * 2));
+  FV_HIP(hipMalloc(&dd, sizeof(UpdDesc)));
+  FV_HIP(hipMemcpyAsync(bufA, q0.data(), sizeof(double2) * len, hipMemcpyHostToDevice, e->stream));
+  double h_offs[kMaxSweeps];
+  double2* cur = bufA;  // conj(q), row-major m x ncols
+  double2* oth = bufB;
+  size_t udim = 1, qlen = len;
+  for (size_t k = 0; k + 1 < n; ++k) {
+    // mps.rs:465-470: fuse the site's index with the previous Schmidt index
+    const size_t m = 2 * udim, ncols = qlen / m;
+    const int orient = m <= ncols ? 0 : 1;
+    FvArgs a{};
+    a.m = (int)m;
+    a.ncols = (int)ncols;
+    a.R = (int)std::max(m, ncols);
+    a.C = (int)std::min(m, ncols);
+    a.orient = orient;
+    a.src = cur;
+    a.X = orient == 0 ? cur : oth;
+    a.W = W;
+    a.offs = offs;
+    a.info = info;
+    launch_fv_init(a, e->stream);
+    FV_HIP(hipGetLastError());
+    Workspace w;
+    w.X = a.X; w.W = W; w.sraw = sraw; w.sval = sval; w.fac = fac; w.perm = perm;
+    w.part = part; w.gpart = gpart; w.gpart_per = gp; w.gpart_slots = 1; w.offs = offs;
+    w.info = info; w.act = act; w.h_offs = h_offs; w.ldx = a.R; w.ldw = a.C;
+    const tn_tune& tu = e->tune_for((size_t)a.C);
+    int sweeps = 0;
+    size_t launches = 0;
+    const int trace[3] = {0, 0, 0};
+    const tn_status js = jacobi_sweeps(e, w, 1, (size_t)a.R, (size_t)a.C, tu,
+                                       e->tol.for_prec(TN_PREC_F64), trace,
+                                       [&](int) { h->flags |= kFlagNoConv; }, &sweeps, &launches);
+    if (js != TN_OK) return fail(js);
+    // process_svd (mps.rs:236-293) on the SVD: values, rank, lambda[k], dims[k+1]
+    UpdDesc d;
+    std::memset(&d, 0, sizeof(d));
+    d.X = a.X; d.W = W; d.sraw = sraw; d.sval = sval; d.fac = fac; d.perm = perm;
+    d.part = part; d.offs = offs; d.info = info; d.ldx = a.R; d.ldw = a.C;
+    d.lamM = h->lam + h->loff[k];
+    d.dims = h->dims + k;
+    d.tkind = h->trunc.kind;
+    d.tchi = (int)std::min<size_t>(h->trunc.chi, 1u << 30);
+    d.teps = h->trunc.eps;
+    d.cap_rank = (int)h->bmax[k];
+    d.hflags = h->dflags();
+    d.eflags = e->d_eflags;
+    FV_HIP(hipMemcpyAsync(dd, &d, sizeof(UpdDesc), hipMemcpyHostToDevice, e->stream));
+    launch_finish(dd, 1, sweeps, e->stream);
+    FV_HIP(hipGetLastError());
+    int rank = 0;
+    FV_HIP(hipMemcpyAsync(&rank, h->dims + k + 1, sizeof(int), hipMemcpyDeviceToHost, e->stream));
+    FV_HIP(hipStreamSynchronize(e->stream));
+    // Gamma[k] = U[:, :r] / lambda[k-1] (mps.rs:478-492); q <- the right factor
+    a.rank = rank;
+    a.perm = perm;
+    a.sraw = sraw;
+    a.fac = fac;
+    a.gk = h->gam + h->goff[k];
+    a.lamPrev = k == 0 ? nullptr : h->lam + h->loff[k - 1];
+    a.X2 = orient == 0 ? oth : cur;
+    launch_fv_split(a, e->stream);
+    FV_HIP(hipGetLastError());
+    h->ub[k] = (size_t)rank;
+    cur = a.X2;
+    oth = cur == bufA ? bufB : bufA;
+    udim = (size_t)rank;
+    qlen = udim * ncols;
+  }
+  // mps.rs:499-505: the last Gamma from the final factor
+  launch_fv_last(cur, h->lam + h->loff[n - 2], (int)udim, h->gam + h->goff[n - 1], e->stream);
+  FV_HIP(hipGetLastError());
+#undef FV_HIP
+  // lambda[k-1] Gamma[k] = U has orthonormal columns for k < n-1
+  for (size_t k = 0; k + 1 < n; ++k) h->fresh[k] = kFreshLeft;
+  h->fresh[n - 1] = 0;
+  TN_TRY(fetch_eflags(e));
+  const hipError_t er = hipStreamSynchronize(e->stream);
+  if (er != hipSuccess) {
+    g_last_error = hipGetErrorString(er);
+    return fail(TN_DEVICE);
+  }
+  const tn_status st = collect_flags(e, &h, 1);
+  cleanup();
+  *out = h;
+  return st;
 }
 
 void tn_mps_free(tn_mps* h) { delete h; }

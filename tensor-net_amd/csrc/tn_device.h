@@ -105,6 +105,26 @@ struct JacobiArgs {
   double tiny2;          // columns with squared norm below take no part
 };
 
+// One step of the state-vector factorization (mps.rs:448-508 factorize).
+// The current right factor q (m x ncols, row-major) is held conjugated in
+// its own memory order, which is q^H stored column-major: orientation A
+// decomposes X = q^H (ncols x m) in place; orientation B (m > ncols)
+// decomposes X = q (m x ncols), a transposed copy.
+struct FvArgs {
+  double2* X;            // the SVD input, R x C column-major (ld R)
+  double2* W;            // C x C accumulated rotation (ld C)
+  const double2* src;    // orientation B: conj(q) to transpose into X
+  double2* X2;           // the next factor conj(q_new), row-major r x ncols
+  double2* gk;           // Gamma[k] (m/2, 2, r)
+  const double* lamPrev; // lambda[k-1] or nullptr (k == 0)
+  const int* perm;       // sorted column order (k_finish)
+  const double* sraw;    // column norms of X W
+  const double* fac;     // row scales of the right factor (process_svd)
+  double* offs;          // per-sweep maxima, zeroed by k_fv_init
+  int* info;             // kInfoLen ints, written by k_fv_init
+  int m, ncols, R, C, orient, rank;
+};
+
 // One entropy read-out: lambda of a bond, where its length is, Renyi order a
 // (1: von Neumann), result slot.
 struct EntDesc {

@@ -38,6 +38,14 @@ void launch_entropy(const EntDesc* descs, int count, double* out, hipStream_t s)
 void launch_contract(const double2* acc, size_t P, const int* dims, const double2* g,
                      const double* lamL, size_t max_out, double2* out, hipStream_t s);
 void launch_u1(double2* g, const int* dims, const double* op8, int gx, hipStream_t s);
+// Factorization of a state vector (mps.rs:448-508), one site per call:
+// info and offs of the SVD task, W = I, and for orientation B X = q from
+// conj(q) (k_fv_init); then, after the Jacobi sweeps and k_finish with the
+// rank known, Gamma[k] and the next factor (k_fv_split); the last site from
+// the final factor (k_fv_last, m/2 rows of 2 values).
+void launch_fv_init(const FvArgs& a, hipStream_t s);
+void launch_fv_split(const FvArgs& a, hipStream_t s);
+void launch_fv_last(const double2* q, const double* lam, int udim, double2* g, hipStream_t s);
 // Haar 4x4 unitaries from the counter-based generator (tn_rng.h): unitary u
 // from counter words ids[3u .. 3u+2], written row-major to out + 16 u.
 void launch_haar4(uint64_t seed, const uint64_t* ids, int count, double2* out, hipStream_t s);

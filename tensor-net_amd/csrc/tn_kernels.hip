@@ -841,6 +841,78 @@ __global__ __launch_bounds__(256) void k_contract(const double2* __restrict__ ac
   }
 }
 
+// ------------------------------------------------------ from_vector
+// The SVD task of one factorization step: info (R, C, orientation), zeroed
+// per-sweep maxima, W = I; in orientation B also X = q, the transpose of
+// the conjugated factor in memory.
+__global__ __launch_bounds__(256) void k_fv_init(FvArgs a) {
+  const unsigned tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const unsigned stride = gridDim.x * blockDim.x;
+  if (tid == 0) {
+    for (int i = 0; i < kInfoLen; ++i) a.info[i] = 0;
+    a.info[kInfoM] = a.m;
+    a.info[kInfoN] = a.ncols;
+    a.info[kInfoR] = a.R;
+    a.info[kInfoC] = a.C;
+    a.info[kInfoOrient] = a.orient;
+  }
+  for (unsigned s = tid; s < (unsigned)kMaxSweeps; s += stride) a.offs[s] = 0.0;
+  const unsigned cc = (unsigned)a.C * (unsigned)a.C;
+  for (unsigned e = tid; e < cc; e += stride) {
+    const unsigned i = e % (unsigned)a.C, j = e / (unsigned)a.C;
+    a.W[e] = make_double2(i == j ? 1.0 : 0.0, 0.0);
+  }
+  if (a.orient == 1) {
+    // X[c][i] (column-major, ld m) = q[i][c] = conj(src[i * ncols + c])
+    const size_t tot = (size_t)a.m * a.ncols;
+    for (size_t e = tid; e < tot; e += stride) {
+      const size_t i = e % (size_t)a.m, c = e / (size_t)a.m;
+      a.X[e] = cconj(a.src[i * a.ncols + c]);
+    }
+  }
+}
+
+// Gamma[k][v, s, j] = U[(2v + s), perm_j] / lamPrev_v and the next factor
+// conj(q_new)[j][c] with q_new[j] = fac_j V^H[perm_j] (process_svd's row
+// scale), for the kept j < rank.  Orientation A (X = q^H): U = W,
+// V = X W / sraw.  Orientation B (X = q): U = X W / sraw, V = W.
+__global__ __launch_bounds__(256) void k_fv_split(FvArgs a) {
+  const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  const int r = a.rank;
+  const size_t ng = (size_t)a.m * r;
+  for (size_t e = tid; e < ng; e += stride) {
+    const int j = (int)(e % (size_t)r), i = (int)(e / (size_t)r);  // i = 2v + s
+    const int pj = a.perm[j];
+    double2 u = a.orient == 0 ? a.W[(size_t)pj * a.C + i]
+                              : cdivr(a.X[(size_t)pj * a.R + i], a.sraw[pj]);
+    if (a.lamPrev) u = cdivr(u, a.lamPrev[i >> 1]);
+    a.gk[e] = u;
+  }
+  const size_t nq = (size_t)r * a.ncols;
+  for (size_t e = tid; e < nq; e += stride) {
+    const int c = (int)(e % (size_t)a.ncols), j = (int)(e / (size_t)a.ncols);
+    const int pj = a.perm[j];
+    double2 q;
+    if (a.orient == 0) {
+      // conj(q_new[j][c]) = fac_j (X W)[c][pj] / sraw_pj
+      q = cscale(cdivr(a.X[(size_t)pj * a.R + c], a.sraw[pj]), a.fac[j]);
+    } else {
+      // conj(q_new[j][c]) = fac_j W[c][pj]
+      q = cscale(a.W[(size_t)pj * a.C + c], a.fac[j]);
+    }
+    a.X2[e] = q;
+  }
+}
+
+// Gamma[n-1][v, s, 0] = q[v][s] / lam_v from the conjugated final factor.
+__global__ __launch_bounds__(256) void k_fv_last(const double2* __restrict__ q,
+                                                 const double* __restrict__ lam, int udim,
+                                                 double2* __restrict__ g) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < 2 * udim) g[e] = cdivr(cconj(q[e]), lam[e >> 1]);
+}
+
 // ----------------------------------------------------------------- u1
 struct Gate2 { double2 m[4]; };
 __global__ __launch_bounds__(256) void k_u1(double2* g, const int* dims, Gate2 op) {
@@ -894,6 +966,19 @@ void launch_contract(const double2* acc, size_t P, const int* dims, const double
   const size_t blocks = std::min<size_t>((max_out + 255) / 256, 65536);
   hipLaunchKernelGGL(k_contract, dim3((unsigned)std::max<size_t>(blocks, 1)), dim3(256), 0, s,
                      acc, P, dims, g, lamL, out);
+}
+void launch_fv_init(const FvArgs& a, hipStream_t s) {
+  const size_t work = std::max((size_t)a.C * a.C, a.orient == 1 ? (size_t)a.m * a.ncols : 1);
+  const int gx = (int)std::min<size_t>(std::max<size_t>((work + 255) / 256, 1), 2048);
+  hipLaunchKernelGGL(k_fv_init, dim3(gx), dim3(256), 0, s, a);
+}
+void launch_fv_split(const FvArgs& a, hipStream_t s) {
+  const size_t work = std::max((size_t)a.m * a.rank, (size_t)a.rank * a.ncols);
+  const int gx = (int)std::min<size_t>(std::max<size_t>((work + 255) / 256, 1), 2048);
+  hipLaunchKernelGGL(k_fv_split, dim3(gx), dim3(256), 0, s, a);
+}
+void launch_fv_last(const double2* q, const double* lam, int udim, double2* g, hipStream_t s) {
+  hipLaunchKernelGGL(k_fv_last, dim3((2 * udim + 255) / 256), dim3(256), 0, s, q, lam, udim, g);
 }
 void launch_u1(double2* g, const int* dims, const double* op8, int gx, hipStream_t s) {
   Gate2 op;

@@ -73,6 +73,7 @@ SIGNATURES = {
     "tn_device_count": (C.c_int, []),
     "tn_mps_new_qubits": (C.c_int, [_sz, tn_trunc, _sz, C.c_void_p, _hp]),
     "tn_mps_free": (None, [_h]),
+    "tn_mps_from_vector": (C.c_int, [_sz, _dp, tn_trunc, _sz, C.c_void_p, _hp]),
     "tn_engine_new": (C.c_int, [_sz, C.c_void_p, _hp]),
     "tn_engine_free": (None, [_h]),
     "tn_mps_new_qubits_in": (C.c_int, [_h, _sz, tn_trunc, _sz, _hp]),

@@ -176,6 +176,31 @@ class MPS:
                "new_qubits")
         return cls(h.value, lib)
 
+    @classmethod
+    def from_vector(cls, psi, trunc=None, chi_cap: int = 0, stream=None) -> "MPS":
+        """``MPS::from_vector`` (mps.rs:516-546): the MPS of a qubit state
+        vector (2^n amplitudes, qubit 0 the slowest index), normalised and
+        factored site by site on the device (``factorize``, mps.rs:448-508).
+        A length that is not a power of 2 is ``StateIncompatibleShape``."""
+        a = np.ascontiguousarray(np.asarray(psi, dtype=np.complex128).reshape(-1))
+        n = a.size.bit_length() - 1
+        if a.size == 0:
+            raise MPSError(L.TN_EMPTY_SYSTEM, "from_vector")
+        if a.size != 1 << n or n == 0:
+            raise MPSError(L.TN_STATE_INCOMPATIBLE_SHAPE, "from_vector")
+        lib = L.load()
+        h = C.c_void_p()
+        code = lib.tn_mps_from_vector(n, _dptr(a.view(np.float64)), to_trunc(trunc),
+                                      int(chi_cap), C.c_void_p(stream) if stream else None,
+                                      C.byref(h))
+        if code == L.TN_CAPACITY and h.value:
+            st = cls(h.value, lib)
+            import warnings
+            warnings.warn(MPSWarning(f"from_vector: {L.status_str(code)}"), stacklevel=2)
+            return st
+        _check(code, "from_vector")
+        return cls(h.value, lib)
+
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value:

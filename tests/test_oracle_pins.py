@@ -15,7 +15,7 @@ import pytest
 
 from helpers import CX, CZ, H, I2, S, SQ2, assert_spectrum_close, kron, state_overlap_err, xrot
 from oracle import dense
-from oracle.mps_oracle import (F64_EPS, VidalMPS, process_svd, run_trajectory,
+from oracle.mps_oracle import (F64_EPS, VidalMPS, from_vector, process_svd, run_trajectory,
                                truncation_rank)
 from tnmps.workload import brickwork_sites, haar4, haar_circuit
 
@@ -356,6 +356,68 @@ def test_renyi_limits():
         assert all(x >= y - 1e-12 for x, y in zip(vals, vals[1:])), vals
         assert abs(vals[0] - math.log(st.bond_dim(b))) < 1e-12
         assert abs(vals[2] - st.entropy_vn(b)) < 1e-5
+
+
+# ------------------------------------------------- from_vector / factorize
+def _random_vector(n, seed):
+    rng = np.random.default_rng(seed)
+    return rng.normal(size=2 ** n) + 1j * rng.normal(size=2 ** n)
+
+
+@pytest.mark.parametrize("n", [1, 2, 5, 9])
+def test_from_vector_reproduces_the_vector(n):
+    """factorize (mps.rs:448-508) untruncated: the MPS contracts back to the
+    normalised vector, entry by entry, and every bond holds the dense
+    Schmidt values."""
+    psi = _random_vector(n, 100 + n) * 3.7
+    st = from_vector(n, psi, None)
+    want = psi / np.linalg.norm(psi)
+    np.testing.assert_allclose(st.contract(), want, atol=1e-13)
+    for b in range(n - 1):
+        assert_spectrum_close(st.L[b], dense.schmidt_values(want, n, b), rtol=1e-12,
+                              msg=f"bond {b}")
+
+
+def test_from_vector_canonical_and_product_state():
+    """The factors are left-canonical (lambda[k-1] Gamma[k] has orthonormal
+    columns), and |0...0> factors into the product state of new_qubits."""
+    n = 8
+    st = from_vector(n, _random_vector(n, 7), None)
+    for k in range(n - 1):
+        lL = st.L[k - 1] if k > 0 else np.ones(1)
+        a = (lL[:, None, None] * st.G[k]).reshape(-1, st.G[k].shape[2])
+        assert np.abs(a.conj().T @ a - np.eye(a.shape[1])).max() < 1e-12
+    z = np.zeros(2 ** 5, dtype=complex)
+    z[0] = 1.0
+    p = from_vector(5, z, None)
+    assert p.bond_dims() == [1] * 4 and all(p.probs(k) == [1.0, 0.0] for k in range(5))
+
+
+def test_from_vector_truncated():
+    """Const(chi): bonds at most chi, each spectrum normalised; bond 0 is the
+    dense spectrum's leading values renormalised (nothing truncated before
+    it)."""
+    n, chi = 10, 6
+    psi = _random_vector(n, 8)
+    st = from_vector(n, psi, ("const", chi))
+    assert max(st.bond_dims()) <= chi
+    for b in range(n - 1):
+        assert abs(np.sum(st.L[b] ** 2) - 1.0) < 1e-13
+    sd = dense.schmidt_values(psi / np.linalg.norm(psi), n, 0)
+    np.testing.assert_allclose(st.L[0], sd[:2] / np.linalg.norm(sd[:2]), rtol=1e-12)
+    with pytest.raises(Exception):
+        from_vector(3, np.ones(7), None)
+
+
+def test_planted_from_vector_without_division_fails():
+    """Gamma[k] left undivided by the previous Schmidt values (the division of
+    mps.rs:484-490 dropped) must break the contraction pin."""
+    n = 6
+    psi = _random_vector(n, 9)
+    st = from_vector(n, psi, None)
+    for k in range(1, n):
+        st.G[k] = st.G[k] * st.L[k - 1][:, None, None]
+    assert np.max(np.abs(st.contract() - psi / np.linalg.norm(psi))) > 1e-3
 
 
 # ------------------------------------------- the alternate LAPACK driver
