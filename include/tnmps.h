@@ -332,6 +332,18 @@ tn_status tn_entropies(tn_engine* e, size_t count, tn_mps* const* hs,
  * cap is out's length in doubles).  n <= 30. */
 tn_status tn_contract(tn_mps* h, double* out, size_t cap);
 
+/* MPS::into_matrix_part(a..b) (mps.rs:1926-1943 over the network of
+ * into_network_part, mps.rs:1742-1829): the reduced density matrix of sites
+ * a..b-1, the others traced out through the canonical form (Gamma of the
+ * range, lambda on its inner bonds, lambda^2 on bonds a-1 and b-1), as
+ * 2^(b-a) x 2^(b-a) complex row-major in out (2^(2(b-a)+1) doubles,
+ * interleaved; cap is out's length in doubles).  Site a is the most
+ * significant bit of the row (ket) and column (bra) indices.  b is clamped
+ * to n as the reference clamps part.end; a >= b, more than 12 sites, or a
+ * contracted block of the range above 2^28 complex entries: TN_INVALID_ARG
+ * (the reference has no bound). */
+tn_status tn_matrix_part(tn_mps* h, size_t a, size_t b, double* out, size_t cap);
+
 /* The handle's sticky failure flags (TN_FLAG_* bits) collected since the
  * last call, then cleared.  No reference counterpart: tells a batch caller
  * which of its handles had a rank clamped (TN_CAPACITY) or an SVD stopped at

@@ -61,6 +61,14 @@ def renyi_from_rho(psi: np.ndarray, n: int, b: int, a: float) -> float:
     return float(math.log(np.sum(w ** a)) / (1.0 - a))
 
 
+def rho_part(psi: np.ndarray, n: int, a: int, b: int) -> np.ndarray:
+    """Partial trace over every qubit outside a..b-1: rho[i, j] =
+    sum_{l, r} psi[l, i, r] conj(psi[l, j, r]) with psi read as
+    (2^a, 2^(b-a), 2^(n-b)); qubit a is the most significant bit of i."""
+    m = psi.reshape(2 ** a, 2 ** (b - a), 2 ** (n - b))
+    return np.einsum("lir,ljr->ij", m, m.conj())
+
+
 def probs(psi: np.ndarray, n: int, k: int) -> Tuple[float, float]:
     t = psi.reshape(2 ** k, 2, 2 ** (n - k - 1))
     p = np.sum(np.abs(t) ** 2, axis=(0, 2))

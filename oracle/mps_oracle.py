@@ -391,6 +391,31 @@ class VidalMPS:
         acc = sum((float(sk) * float(sk)) ** a for sk in self.L[b])
         return math.log(acc) / (1.0 - a)
 
+    def matrix_part(self, a: int, b: int) -> np.ndarray:
+        """``MPS::into_matrix_part(a..b)`` (mps.rs:1926-1943) of the network
+        ``into_network_part`` builds (mps.rs:1742-1829): Gamma of sites a..b-1
+        with lambda[k] on the bonds inside the range (:1778-1791), the network
+        and its conjugate, and diag(lambda^2) tracers on bond a-1 (:1801-1813)
+        and on bond b-1 (:1814-1826), which trace out the sites outside the
+        range through the canonical form.  ``part.end`` is clamped to n
+        (:1746).  Indices sorted kets (rows) before bras, each by site, so
+        site a is the most significant bit of the row index (:1930-1942)."""
+        b = min(b, self.n)
+        if not 0 <= a < b:
+            raise OracleError("InvalidArgument")
+        chil = self.G[a].shape[0]
+        t = np.eye(chil, dtype=np.complex128)                     # (alpha, alpha')
+        for k in range(a, b):
+            g = self.G[k]
+            if k > a:
+                g = g * self.L[k - 1][:, None, None]              # lambda inside the range
+            t = (t @ g.reshape(g.shape[0], -1)).reshape(-1, g.shape[2])
+        d = 2 ** (b - a)
+        t = t.reshape(chil, d, -1)                                # (alpha, i, beta)
+        wl = self.L[a - 1] ** 2 if a > 0 else np.ones(1)
+        wr = self.L[b - 1] ** 2 if b < self.n else np.ones(1)
+        return np.einsum("a,b,aib,ajb->ij", wl, wr, t, t.conj())
+
     def bond_dim(self, k: int) -> Optional[int]:
         """mps.rs:1243-1245."""
         return len(self.L[k]) if k < len(self.L) else None

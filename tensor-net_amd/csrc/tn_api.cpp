@@ -101,6 +101,10 @@ void hhfree(void* p) {
 }
 
 constexpr int kRing = 4;  // staging buffers in flight per descriptor kind
+// tn_matrix_part: at most 12 sites (a 4096 x 4096 matrix) and at most 2^28
+// complex entries (4 GiB) in the contracted block of the range
+constexpr size_t kMatrixPartMaxSites = 12;
+constexpr size_t kMatrixPartMaxElems = (size_t)1 << 28;
 
 size_t exact_bond_max(size_t n, size_t b) {
   // min(2^{b+1}, 2^{n-b-1}) without overflow
@@ -2043,6 +2047,63 @@ tn_status tn_contract(tn_mps* h, double* out, size_t cap) {
   }
   if (hipGetLastError() != hipSuccess ||
       hipMemcpyAsync(out, a, sizeof(double2) * len, hipMemcpyDeviceToHost, e->stream) !=
+          hipSuccess ||
+      hipStreamSynchronize(e->stream) != hipSuccess)
+    return done(TN_DEVICE);
+  return done(TN_OK);
+}
+
+tn_status tn_matrix_part(tn_mps* h, size_t a, size_t b, double* out, size_t cap) {
+  if (!h || !out) return TN_INVALID_ARG;
+  const size_t n = h->n;
+  b = std::min(b, n);  // mps.rs:1746
+  if (a >= b || b - a > kMatrixPartMaxSites) return TN_INVALID_ARG;
+  const size_t D = (size_t)1 << (b - a);
+  if (cap < 2 * D * D) return TN_INVALID_ARG;
+  tn_engine* e = h->eng;
+  std::vector<int> d(n + 1);
+  TN_HIP(hipMemcpyAsync(d.data(), h->dims, sizeof(int) * (n + 1), hipMemcpyDeviceToHost,
+                        e->stream));
+  TN_HIP(hipStreamSynchronize(e->stream));
+  // T after site k holds chiL 2^(k-a+1) rows of d[k+1] entries
+  const size_t chiL = (size_t)d[a];
+  size_t most = chiL * chiL;
+  for (size_t k = a; k < b; ++k) most = std::max(most, chiL * ((size_t)2 << (k - a)) * d[k + 1]);
+  if (most > kMatrixPartMaxElems) return TN_INVALID_ARG;
+  std::vector<double2> eye(chiL * chiL, make_double2(0.0, 0.0));
+  for (size_t i = 0; i < chiL; ++i) eye[i * chiL + i] = make_double2(1.0, 0.0);
+  double2 *x = nullptr, *y = nullptr, *rho = nullptr;
+  auto done = [&](tn_status st) {
+    (void)hipStreamSynchronize(e->stream);
+    hfree(x);
+    hfree(y);
+    hfree(rho);
+    return st;
+  };
+  if (hipMalloc(&x, sizeof(double2) * most) != hipSuccess ||
+      hipMalloc(&y, sizeof(double2) * most) != hipSuccess ||
+      hipMalloc(&rho, sizeof(double2) * D * D) != hipSuccess)
+    return done(TN_DEVICE);
+  if (hipMemcpyAsync(x, eye.data(), sizeof(double2) * eye.size(), hipMemcpyHostToDevice,
+                     e->stream) != hipSuccess)
+    return done(TN_DEVICE);
+  // rows (alpha, s_a .. s_k): lambda[a-1] enters as the first site's left
+  // weight on the identity, the bonds inside the range as their sites' left
+  // weights (mps.rs:1778-1791)
+  size_t P = chiL;
+  for (size_t k = a; k < b; ++k) {
+    launch_contract(x, P, h->dims + k, h->gam + h->goff[k],
+                    k == 0 ? nullptr : h->lam + h->loff[k - 1], 2 * P * (size_t)d[k + 1], y,
+                    e->stream);
+    std::swap(x, y);
+    P *= 2;
+  }
+  // the tracers' lambda^2 (mps.rs:1801-1826): bond a-1 through both factors'
+  // lambda[a-1], bond b-1 as the weight of the summed right index
+  launch_rho_part(x, chiL, D, (size_t)d[b], b < n ? h->lam + h->loff[b - 1] : nullptr, rho,
+                  e->stream);
+  if (hipGetLastError() != hipSuccess ||
+      hipMemcpyAsync(out, rho, sizeof(double2) * D * D, hipMemcpyDeviceToHost, e->stream) !=
           hipSuccess ||
       hipStreamSynchronize(e->stream) != hipSuccess)
     return done(TN_DEVICE);

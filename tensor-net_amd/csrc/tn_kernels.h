@@ -44,6 +44,10 @@ void launch_u1(double2* g, const int* dims, const double* op8, int gx, hipStream
 // rank known, Gamma[k] and the next factor (k_fv_split); the last site from
 // the final factor (k_fv_last, m/2 rows of 2 values).
 void launch_fv_init(const FvArgs& a, hipStream_t s);
+// rho (D x D, row-major) = sum_(l, r) lamR[r]^2 T[l][i][r] conj(T[l][j][r]) over
+// T of L x D x R; lamR null means weights 1.
+void launch_rho_part(const double2* t, size_t L, size_t D, size_t R, const double* lamR,
+                     double2* rho, hipStream_t s);
 void launch_fv_split(const FvArgs& a, hipStream_t s);
 void launch_fv_last(const double2* q, const double* lam, int udim, double2* g, hipStream_t s);
 // Haar 4x4 unitaries from the counter-based generator (tn_rng.h): unitary u

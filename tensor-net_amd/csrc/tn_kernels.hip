@@ -841,6 +841,37 @@ __global__ __launch_bounds__(256) void k_contract(const double2* __restrict__ ac
   }
 }
 
+// ------------------------------------------------------ matrix part
+// rho[i][j] = sum_(l, r) w[r] T[l][i][r] conj(T[l][j][r]) over T of L x D x R
+// (row-major), w[r] = lamR[r]^2 (1 without lamR): the reduced density matrix
+// of D = 2^(b-a) states of sites a..b-1 from T = lambda[a-1] Gamma[a] ...
+// Gamma[b-1] and the right boundary's lambda.  16 x 16 tiles of rho per
+// workgroup, the summed index (l, r) staged through LDS 16 at a time.
+__global__ __launch_bounds__(256) void k_rho_part(const double2* __restrict__ t, size_t D,
+                                                  size_t R, size_t E,
+                                                  const double* __restrict__ lamR,
+                                                  double2* __restrict__ rho) {
+  __shared__ double2 mi[16][17], mj[16][17];
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  const size_t i0 = (size_t)blockIdx.y * 16, j0 = (size_t)blockIdx.x * 16;
+  double2 acc = make_double2(0.0, 0.0);
+  for (size_t e0 = 0; e0 < E; e0 += 16) {
+    // thread (ty, tx) stages w T[i0 + ty][e0 + tx] and T[j0 + ty][e0 + tx]
+    const size_t e = e0 + tx;
+    const size_t l = e / R, r = e % R;
+    const size_t ii = i0 + ty, jj = j0 + ty;
+    const double w = (lamR && e < E) ? lamR[r] * lamR[r] : 1.0;
+    mi[ty][tx] = (ii < D && e < E) ? cscale(t[(l * D + ii) * R + r], w) : make_double2(0.0, 0.0);
+    mj[ty][tx] = (jj < D && e < E) ? t[(l * D + jj) * R + r] : make_double2(0.0, 0.0);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 16; ++k) acc = cadd(acc, cmulc(mj[tx][k], mi[ty][k]));
+    __syncthreads();
+  }
+  const size_t i = i0 + ty, j = j0 + tx;
+  if (i < D && j < D) rho[i * D + j] = acc;
+}
+
 // ------------------------------------------------------ from_vector
 // The SVD task of one factorization step: info (R, C, orientation), zeroed
 // per-sweep maxima, W = I; in orientation B also X = q, the transpose of
@@ -966,6 +997,11 @@ void launch_contract(const double2* acc, size_t P, const int* dims, const double
   const size_t blocks = std::min<size_t>((max_out + 255) / 256, 65536);
   hipLaunchKernelGGL(k_contract, dim3((unsigned)std::max<size_t>(blocks, 1)), dim3(256), 0, s,
                      acc, P, dims, g, lamL, out);
+}
+void launch_rho_part(const double2* t, size_t L, size_t D, size_t R, const double* lamR,
+                     double2* rho, hipStream_t s) {
+  const unsigned g = (unsigned)((D + 15) / 16);
+  hipLaunchKernelGGL(k_rho_part, dim3(g, g), dim3(256), 0, s, t, D, R, L * R, lamR, rho);
 }
 void launch_fv_init(const FvArgs& a, hipStream_t s) {
   const size_t work = std::max((size_t)a.C * a.C, a.orient == 1 ? (size_t)a.m * a.ncols : 1);

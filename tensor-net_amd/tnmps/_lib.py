@@ -120,6 +120,7 @@ SIGNATURES = {
     "tn_engine_sync": (C.c_int, [_h]),
     "tn_entropies": (C.c_int, [_h, _sz, _hp, _szp, _dp, _dp]),
     "tn_contract": (C.c_int, [_h, _dp, _sz]),
+    "tn_matrix_part": (C.c_int, [_h, _sz, _sz, _dp, _sz]),
     "tn_rng_words": (C.c_int, [C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, _sz,
                                C.POINTER(C.c_uint64)]),
     "tn_gen_haar4": (C.c_int, [_h, C.c_uint64, _sz, C.POINTER(C.c_uint64), _dp]),

@@ -439,6 +439,22 @@ class MPS:
                "contract")
         return out
 
+    def matrix_part(self, start: int, end: int) -> Tuple[List[Tuple[str, int]], np.ndarray]:
+        """``MPS::into_matrix_part(start..end)`` (mps.rs:1926-1943): the
+        reduced density matrix of sites start..end-1 (end clamped to n), the
+        rest traced out through the canonical form, computed on the device
+        (``tn_matrix_part``, at most 12 sites).  Returns the index list, kets
+        then bras by site as the reference sorts them, and the matrix, site
+        ``start`` the most significant bit of rows and columns."""
+        end = min(end, self.n)
+        w = end - start
+        d = 1 << w if 0 < w <= 12 else 1  # other widths are refused by the call
+        out = np.zeros((d, d), dtype=np.complex128)
+        _check(self._lib.tn_matrix_part(self._h, start, end, _dptr(out.view(np.float64)),
+                                        2 * out.size), "matrix_part")
+        idx = [("Ket", k) for k in range(start, end)] + [("Bra", k) for k in range(start, end)]
+        return idx, out
+
     def stats(self) -> Tuple[int, int]:
         u = C.c_size_t()
         s = C.c_size_t()

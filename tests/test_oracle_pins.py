@@ -420,6 +420,76 @@ def test_planted_from_vector_without_division_fails():
     assert np.max(np.abs(st.contract() - psi / np.linalg.norm(psi))) > 1e-3
 
 
+# ------------------------------------------------ into_matrix_part
+@pytest.mark.parametrize("n,seed", [(6, 21), (9, 22)])
+def test_matrix_part_vs_dense_partial_trace(n, seed):
+    """into_matrix_part (mps.rs:1926-1943) traces out the sites outside the
+    range through the canonical form only; the dense partial trace of the
+    contracted vector is the independent answer, for every range with up
+    to 4 sites, at the chain's ends and inside it."""
+    st, psi = _random_gate_run(n, 6 * n, seed)
+    for a in range(n):
+        for b in range(a + 1, min(n, a + 4) + 1):
+            rho = st.matrix_part(a, b)
+            ref = dense.rho_part(psi, n, a, b)
+            assert np.max(np.abs(rho - ref)) < 1e-12, (a, b)
+
+
+def test_matrix_part_closed_forms():
+    """Product |0...0>: |0><0| on every site.  Bell pair: I/2 on either
+    qubit, the pure projector on both.  GHZ(5): diag(1/2, 0, ..., 0, 1/2)
+    on any range of 2 or more sites.  end beyond n is clamped (mps.rs:1746)."""
+    st = VidalMPS(4)
+    for a in range(4):
+        r = st.matrix_part(a, a + 1)
+        np.testing.assert_allclose(r, np.diag([1.0, 0.0]), atol=1e-15)
+    bell = VidalMPS(2)
+    bell.apply_unitary1(0, H)
+    bell.apply_unitary2(0, CX)
+    np.testing.assert_allclose(bell.matrix_part(0, 1), I2 / 2, atol=1e-15)
+    np.testing.assert_allclose(bell.matrix_part(1, 2), I2 / 2, atol=1e-15)
+    v = np.array([1.0, 0.0, 0.0, 1.0]) / math.sqrt(2.0)
+    np.testing.assert_allclose(bell.matrix_part(0, 2), np.outer(v, v), atol=1e-15)
+    ghz = VidalMPS(5)
+    ghz.apply_unitary1(0, H)
+    for k in range(4):
+        ghz.apply_unitary2(k, CX)
+    for a, b in [(0, 2), (1, 4), (2, 5), (3, 9)]:
+        d = 2 ** (min(b, 5) - a)
+        want = np.zeros((d, d))
+        want[0, 0] = want[-1, -1] = 0.5
+        np.testing.assert_allclose(ghz.matrix_part(a, b), want, atol=1e-15)
+    with pytest.raises(Exception):
+        ghz.matrix_part(3, 3)
+
+
+def test_matrix_part_spectrum_is_the_schmidt_spectrum():
+    """For the range 0..b+1 the eigenvalues of rho are the squared Schmidt
+    values of bond b (and zeros); trace 1, Hermitian."""
+    n = 8
+    st, _ = _random_gate_run(n, 40, 23)
+    for b in range(4):
+        rho = st.matrix_part(0, b + 1)
+        assert np.max(np.abs(rho - rho.conj().T)) < 1e-15
+        assert abs(np.trace(rho) - 1.0) < 1e-13
+        w = np.sort(np.linalg.eigvalsh(rho))[::-1][: len(st.L[b])]
+        np.testing.assert_allclose(w, st.L[b] ** 2, atol=1e-13)
+
+
+def test_planted_matrix_part_without_boundary_weight_fails(monkeypatch):
+    """The lambda^2 tracer on bond b-1 (mps.rs:1814-1826) left out must
+    break the dense pin."""
+    n = 7
+    st, psi = _random_gate_run(n, 40, 24)
+    ref = dense.rho_part(psi, n, 2, 4)
+    saved = st.L[3].copy()
+    st.L[3] = np.ones_like(saved)  # the tracer reads lambda[b-1] only
+    bad = st.matrix_part(2, 4)
+    st.L[3] = saved
+    assert np.max(np.abs(bad - ref)) > 1e-3
+    assert np.max(np.abs(st.matrix_part(2, 4) - ref)) < 1e-12
+
+
 # ------------------------------------------- the alternate LAPACK driver
 def test_gesdd_oracle_matches_gesvd_oracle():
     """The full-size GPU tests run the oracle with zgesdd for speed: on a
