@@ -49,6 +49,7 @@ struct PairSmem {
   double2 Gs[2][PW][LD];            // panel Gram matrix (Gs[0]), double-buffered in the solve
   double2 buf[64 * LD + 2 * PW * LD];
   double red[4];
+  int rr, go;  // round word and decision of the two-wave 16 x 16 solve
   // buf holds the 64-row panel chunk Ps and the accumulated rotation Ws
   // (double-buffered); the per-wave partial Gram matrices of panel_gram
   // overlay both once its row loop is done.
@@ -264,63 +265,202 @@ __device__ __forceinline__ bool rot2(double a, double b, double2 c, double tol2,
   return go;
 }
 
-// One round of the 16 x 16 panel solve (BW = 8), unrolled: ROUND pairs index
-// x with x ^ ROUND (x with a zero bit inserted at ROUND's highest set bit),
-// 8 disjoint rotations.  Threads 0..127 each compute one column of a 2 x 2
-// block of G' = Q^H G Q (the block's row pair x column pair, with both
-// rotations); threads 128..255 the two entries of one row of W' = W Q in a
-// column pair.  Reads buffer CUR, writes CUR ^ 1.
+// rot2 of two independent pairs, the two dependency chains interleaved
+// statement by statement (the same arithmetic, bitwise the same results).
+__device__ __forceinline__ void rot2x2(double a1, double b1, double2 c1, double a2, double b2,
+                                       double2 c2, double tol2, double tiny2, double& cs1,
+                                       double2& sn1, double& cs2, double2& sn2) {
+  const double c21 = cabs2(c1), c22 = cabs2(c2);
+  const bool go1 = a1 > tiny2 && b1 > tiny2 && c21 > tol2 * a1 * b1;
+  const bool go2 = a2 > tiny2 && b2 > tiny2 && c22 > tol2 * a2 * b2;
+  const double d1 = b1 - a1, d2 = b2 - a2;
+  const double c41 = 4.0 * c21, c42 = 4.0 * c22;
+  const double r21 = fma(d1, d1, c41), r22 = fma(d2, d2, c42);
+  const double q1 = __builtin_amdgcn_rsq(r21), q2 = __builtin_amdgcn_rsq(r22);
+  const double s1 = fma(r21, q1, fabs(d1)), s2 = fma(r22, q2, fabs(d2));
+  const double u1 = fma(s1, s1, c41), u2 = fma(s2, s2, c42);
+  const double h1 = __builtin_amdgcn_rsq(u1), h2 = __builtin_amdgcn_rsq(u2);
+  const double e1 = fma(-u1 * h1, h1, 1.0), e2 = fma(-u2 * h2, h2, 1.0);
+  const double ru1 = fma(h1 * e1, fma(e1, 0.375, 0.5), h1);
+  const double ru2 = fma(h2 * e2, fma(e2, 0.375, 0.5), h2);
+  const double f1 = (d1 >= 0.0 ? 2.0 : -2.0) * ru1, f2 = (d2 >= 0.0 ? 2.0 : -2.0) * ru2;
+  cs1 = go1 ? s1 * ru1 : 1.0;
+  cs2 = go2 ? s2 * ru2 : 1.0;
+  sn1 = go1 ? make_double2(f1 * c1.x, f1 * c1.y) : make_double2(0.0, 0.0);
+  sn2 = go2 ? make_double2(f2 * c2.x, f2 * c2.y) : make_double2(0.0, 0.0);
+}
+
+// Orders one wave's LDS accesses across rounds: LDS instructions of a wave
+// are performed in issue order, so a compiler barrier is enough (no wait for
+// the stores to complete).
+__device__ __forceinline__ void wave_sync() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_wave_barrier();
+  asm volatile("" ::: "memory");
+}
+
+// ------------------------------------------------- 16 x 16 solve, two waves
+// One cyclic two-sided Jacobi sweep of a 16 x 16 panel Gram matrix on two
+// waves of the workgroup: wave 0 runs the rounds on G (G[0] -> G[1] -> ...
+// double-buffered) and posts each round's 8 rotations to LDS; wave 1 follows
+// round by round and applies them to W, in place from the identity.  Round r
+// pairs index x with x ^ r (x with a zero bit inserted at r's highest set
+// bit).  The LDS performs a wave's operations in issue order and serves the
+// waves of a workgroup from one memory, so wave 0 orders its rounds with
+// compiler barriers alone, and wave 1, once it reads round r on the round
+// word (written after the round's rotations), reads those rotations.  Wave 0
+// never waits, so wave 1's waits end (they are bounded as well).  The caller
+// clears *rr before a barrier that precedes both waves' calls, and meets the
+// two waves at a barrier afterwards.
+constexpr int kRoundsNone = 100;    // posted when there is nothing to rotate
+constexpr int kSolveSpin = 1 << 22;
+
+struct Solve16 {
+  double2 (*G)[16][17];  // G[0]: the panel Gram matrix
+  double2 (*W)[17];      // 16 rows: the accumulated rotation
+  double* rcs;           // [16][8]: cosine of round r, pair q at rcs[8 r + q]
+  double2* rsn;          // [16][8]: sine
+  int* rr;               // last round posted
+};
+
+// Wave 0, round ROUND: lane l computes the 2 x 2 block (row pair l / 8,
+// column pair l % 8) of G' = Qa^H G Qb and posts column pair l % 8's
+// rotation (the eight lanes of a column pair hold bitwise the same one, so
+// they all write it: no branch between the two rotations' chains).  Reads
+// G[CUR], writes G[CUR ^ 1].
 template <int ROUND, int CUR>
-__device__ __forceinline__ void solve_round16(PairSmem<8>& sm, double tol2, double tiny2) {
+__device__ __forceinline__ void g_round16(const Solve16& sv, double tol2, double tiny2) {
   constexpr int NXT = CUR ^ 1;
   constexpr int h = ROUND >= 8 ? 3 : ROUND >= 4 ? 2 : ROUND >= 2 ? 1 : 0;
   constexpr int lowm = (1 << h) - 1;
-  auto first = [](int p) { return ((p >> h) << (h + 1)) | (p & lowm); };
-  const int tid = threadIdx.x;
-  if (tid < 128) {
-    const int blk = tid >> 1, half = tid & 1;
-    const int x1 = first(blk >> 3), x2 = x1 ^ ROUND;
-    const int y1 = first(blk & 7), y2 = y1 ^ ROUND;
-    const double ax = sm.Gs[CUR][x1][x1].x, bx = sm.Gs[CUR][x2][x2].x;
-    const double2 cx = sm.Gs[CUR][x1][x2];
-    const double ay = sm.Gs[CUR][y1][y1].x, by = sm.Gs[CUR][y2][y2].x;
-    const double2 cy = sm.Gs[CUR][y1][y2];
-    const double2 g11 = sm.Gs[CUR][x1][y1], g12 = sm.Gs[CUR][x1][y2];
-    const double2 g21 = sm.Gs[CUR][x2][y1], g22 = sm.Gs[CUR][x2][y2];
-    double csa, csb;
-    double2 sna, snb;
-    rot2(ax, bx, cx, tol2, tiny2, csa, sna);
-    rot2(ay, by, cy, tol2, tiny2, csb, snb);
-    // column yo of G Qb = g[., y1] qa + g[., y2] qb
-    const int yo = half ? y2 : y1;
-    const double2 qa = half ? snb : make_double2(csb, 0.0);
-    const double2 qb = half ? make_double2(csb, 0.0) : make_double2(-snb.x, snb.y);
-    const double2 t1 = cadd(cmul(g11, qa), cmul(g12, qb));
-    const double2 t2 = cadd(cmul(g21, qa), cmul(g22, qb));
-    sm.Gs[NXT][x1][yo] = cadd(cscale(t1, csa), cmul(make_double2(-sna.x, -sna.y), t2));
-    sm.Gs[NXT][x2][yo] = cadd(cmul(cconj(sna), t1), cscale(t2, csa));
-  } else {
-    const int w = tid - 128;
-    const int i = w >> 3;
-    const int y1 = first(w & 7), y2 = y1 ^ ROUND;
-    const double ay = sm.Gs[CUR][y1][y1].x, by = sm.Gs[CUR][y2][y2].x;
-    const double2 cy = sm.Gs[CUR][y1][y2];
-    const double2 w1 = sm.Ws(CUR, i, y1), w2 = sm.Ws(CUR, i, y2);
-    double csb;
-    double2 snb;
-    rot2(ay, by, cy, tol2, tiny2, csb, snb);
-    sm.Ws(NXT, i, y1) = cadd(cscale(w1, csb), cmul(w2, make_double2(-snb.x, snb.y)));
-    sm.Ws(NXT, i, y2) = cadd(cmul(w1, snb), cscale(w2, csb));
-  }
+  auto first = [](int q) { return ((q >> h) << (h + 1)) | (q & lowm); };
+  // the lane index passes through an empty asm so that the round's LDS
+  // addresses are formed here, not hoisted out of a caller's loop (15
+  // rounds of them would not fit the registers)
+  int lane = threadIdx.x & 63;
+  asm volatile("" : "+v"(lane));
+  double2 (*G)[17] = sv.G[CUR];
+  double2 (*Gn)[17] = sv.G[NXT];
+  const int x1 = first(lane >> 3), x2 = x1 ^ ROUND;
+  const int y1 = first(lane & 7), y2 = y1 ^ ROUND;
+  const double ax = G[x1][x1].x, bx = G[x2][x2].x;
+  const double2 cx = G[x1][x2];
+  const double ay = G[y1][y1].x, by = G[y2][y2].x;
+  const double2 cy = G[y1][y2];
+  const double2 g11 = G[x1][y1], g12 = G[x1][y2];
+  const double2 g21 = G[x2][y1], g22 = G[x2][y2];
+  // the round's loads all issue before the rotations' arithmetic
+  __builtin_amdgcn_sched_barrier(0);
+  double csa, csb;
+  double2 sna, snb;
+  rot2x2(ax, bx, cx, ay, by, cy, tol2, tiny2, csa, sna, csb, snb);
+  // T = G[block] Qb, then G' = Qa^H T, with Q = [[cs, sn], [-conj(sn), cs]]
+  const double2 msb = make_double2(-snb.x, snb.y);  // -conj(sn_b)
+  const double2 t11 = cadd(cscale(g11, csb), cmul(g12, msb));
+  const double2 t12 = cadd(cmul(g11, snb), cscale(g12, csb));
+  const double2 t21 = cadd(cscale(g21, csb), cmul(g22, msb));
+  const double2 t22 = cadd(cmul(g21, snb), cscale(g22, csb));
+  const double2 nsa = make_double2(-sna.x, -sna.y);  // -sn_a
+  const double2 csa_ = cconj(sna);                    // conj(sn_a)
+  Gn[x1][y1] = cadd(cscale(t11, csa), cmul(nsa, t21));
+  Gn[x1][y2] = cadd(cscale(t12, csa), cmul(nsa, t22));
+  Gn[x2][y1] = cadd(cmul(csa_, t11), cscale(t21, csa));
+  Gn[x2][y2] = cadd(cmul(csa_, t12), cscale(t22, csa));
+  sv.rcs[8 * ROUND + (lane & 7)] = csb;
+  sv.rsn[8 * ROUND + (lane & 7)] = snb;
 }
 
 template <int ROUND, int CUR>
-__device__ __forceinline__ void solve_rounds16(PairSmem<8>& sm, double tol2, double tiny2) {
+__device__ __forceinline__ void g_rounds16(const Solve16& sv, double tol2, double tiny2) {
   if constexpr (ROUND < 16) {
-    solve_round16<ROUND, CUR>(sm, tol2, tiny2);
-    __syncthreads();
-    solve_rounds16<ROUND + 1, CUR ^ 1>(sm, tol2, tiny2);
+    g_round16<ROUND, CUR>(sv, tol2, tiny2);
+    wave_sync();
+    __hip_atomic_store(sv.rr, ROUND, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    wave_sync();
+    g_rounds16<ROUND + 1, CUR ^ 1>(sv, tol2, tiny2);
   }
+}
+
+// Wave 1, round ROUND: lane l rotates rows l / 8 and l / 8 + 8 of W in
+// column pair l % 8 (W' = W Qb), in place.
+template <int ROUND>
+__device__ __forceinline__ void w_round16(const Solve16& sv) {
+  constexpr int h = ROUND >= 8 ? 3 : ROUND >= 4 ? 2 : ROUND >= 2 ? 1 : 0;
+  constexpr int lowm = (1 << h) - 1;
+  auto first = [](int q) { return ((q >> h) << (h + 1)) | (q & lowm); };
+  int lane = threadIdx.x & 63;
+  asm volatile("" : "+v"(lane));
+  const int q = lane & 7, y1 = first(q), y2 = y1 ^ ROUND;
+  const int i0 = lane >> 3, i1 = i0 + 8;
+  const double cs = sv.rcs[8 * ROUND + q];
+  const double2 sn = sv.rsn[8 * ROUND + q];
+  const double2 msn = make_double2(-sn.x, sn.y);  // -conj(sn)
+  const double2 w01 = sv.W[i0][y1], w02 = sv.W[i0][y2];
+  const double2 w11 = sv.W[i1][y1], w12 = sv.W[i1][y2];
+  sv.W[i0][y1] = cadd(cscale(w01, cs), cmul(w02, msn));
+  sv.W[i0][y2] = cadd(cmul(w01, sn), cscale(w02, cs));
+  sv.W[i1][y1] = cadd(cscale(w11, cs), cmul(w12, msn));
+  sv.W[i1][y2] = cadd(cmul(w11, sn), cscale(w12, cs));
+}
+
+__device__ __forceinline__ void wait_round(const int* rr, int r) {
+  for (int it = 0; it < kSolveSpin; ++it) {
+    if (__hip_atomic_load(rr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) >= r) break;
+    __builtin_amdgcn_s_sleep(1);
+  }
+  wave_sync();
+}
+
+template <int ROUND>
+__device__ __forceinline__ void w_rounds16(const Solve16& sv) {
+  if constexpr (ROUND < 16) {
+    wait_round(sv.rr, ROUND);
+    w_round16<ROUND>(sv);
+    wave_sync();
+    w_rounds16<ROUND + 1>(sv);
+  }
+}
+
+// Wave 0's part: the largest relative off-diagonal of G[0] (recorded into
+// *rec when not null and positive), then the rounds unless it is at most
+// tol_rot.  Returns whether there is a rotation to apply.
+__device__ bool solve16_wave0(const Solve16& sv, double* rec, double tol_rot, double tiny2) {
+  const int lane = threadIdx.x & 63;
+  double r2 = 0.0;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int e = lane + 64 * q, gi = e >> 4, gj = e & 15;
+    if (gi < gj) {
+      const double ga = sv.G[0][gi][gi].x, gb = sv.G[0][gj][gj].x;
+      if (ga > tiny2 && gb > tiny2) r2 = fmax(r2, cabs2(sv.G[0][gi][gj]) / (ga * gb));
+    }
+  }
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) r2 = fmax(r2, __shfl_xor(r2, o, 64));
+  const double off = sqrt(r2);
+  if (rec && lane == 0 && off > 0.0)
+    atomicMax((unsigned long long*)rec, (unsigned long long)__double_as_longlong(off));
+  if (!(off > tol_rot)) {
+    __hip_atomic_store(sv.rr, kRoundsNone, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    return false;
+  }
+  g_rounds16<1, 0>(sv, tol_rot * tol_rot, tiny2);
+  return true;
+}
+
+// Wave 1's part: W from the identity through the posted rounds.
+__device__ void solve16_wave1(const Solve16& sv) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int e = lane + 64 * q;
+    sv.W[e >> 4][e & 15] = make_double2((e >> 4) == (e & 15) ? 1.0 : 0.0, 0.0);
+  }
+  wave_sync();
+  wait_round(sv.rr, 1);
+  if (__hip_atomic_load(sv.rr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == kRoundsNone)
+    return;
+  w_rounds16<1>(sv);
 }
 
 // From the panel Gram matrix in sm.Gs[0]: record the largest relative
@@ -328,15 +468,35 @@ __device__ __forceinline__ void solve_rounds16(PairSmem<8>& sm, double tol2, dou
 // orthogonal apply one cyclic two-sided Jacobi sweep to G (G <- Q^H G Q),
 // accumulating the rotation in sm.Ws[wsel].  Round r = 1 .. PW-1 pairs index
 // x with x ^ r: PW/2 disjoint rotations per round, every pair once per sweep.
-// Each work item computes the rotations it needs from the round's Gram
-// buffer, so a round needs one barrier.  Returns false (for the whole block)
-// when there is nothing to apply.  A function of sm.Gs[0] alone: blocks
-// holding the same G compute bitwise the same rotation.
+// 16 x 16 panels (BW = 8) take the two-wave solve above; wider ones spread a
+// round's work items over the workgroup, each computing the rotations it
+// needs from the round's Gram buffer, one barrier per round.  Returns false
+// (for the whole block) when there is nothing to apply.  A function of
+// sm.Gs[0] alone: blocks holding the same G compute bitwise the same
+// rotation.
 template <int BW>
 __device__ bool panel_solve(double* rec, double tol_rot, double tiny2, PairSmem<BW>& sm,
                             int& wsel) {
   constexpr int PW = 2 * BW;
   const int tid = threadIdx.x;
+  if constexpr (BW == 8) {
+    // waves 0 and 1 (the panel chunk buffer holds the posted rotations: the
+    // chunks are in registers meanwhile); W lands in Ws[0]
+    const Solve16 sv{sm.Gs, (double2(*)[17]) & sm.Ws(0, 0, 0), (double*)&sm.Ps(0, 0),
+                     &sm.Ps(8, 0), &sm.rr};
+    if (tid == 0) sm.rr = 0;
+    __syncthreads();
+    const int wid = tid >> 6;
+    if (wid == 0) {
+      const bool go = solve16_wave0(sv, rec, tol_rot, tiny2);
+      if (tid == 0) sm.go = go ? 1 : 0;
+    } else if (wid == 1) {
+      solve16_wave1(sv);
+    }
+    __syncthreads();
+    wsel = 0;
+    return sm.go != 0;
+  } else {
   const double tol2 = tol_rot * tol_rot;
   double ratio2 = 0.0;
   for (int e = tid; e < PW * PW; e += 256) {
@@ -351,14 +511,6 @@ __device__ bool panel_solve(double* rec, double tol_rot, double tiny2, PairSmem<
   if (rec && tid == 0 && off > 0.0)
     atomicMax((unsigned long long*)rec, (unsigned long long)__double_as_longlong(off));
   if (off <= tol_rot) return false;
-
-  if constexpr (BW == 8) {
-    // 15 rounds, unrolled: every index is a compile-time function of the
-    // thread id, and the last round writes buffer 1
-    solve_rounds16<1, 0>(sm, tol2, tiny2);
-    wsel = 1;
-    return true;
-  }
   // Work items of a round: the (PW/2)^2 2x2 blocks of G on the round's row
   // pair x column pair (G' = Qa^H G Qb touches only its own block), then the
   // PW x PW/2 (row, column pair) items of W' = W Qb.  Pair a of round r is
@@ -415,6 +567,7 @@ __device__ bool panel_solve(double* rec, double tol_rot, double tiny2, PairSmem<
   }
   wsel = cur;
   return true;
+  }
 }
 
 // Rows [a, b) of the panel of M (leading dimension ld, columns below cend)
