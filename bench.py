@@ -286,8 +286,8 @@ def run_batched(args) -> dict:
                      "flops": "LAPACK-style count of the SVDs posed, over the wall time",
                      "alg_tflops_whole_update": round(perf["alg_flops"] / wall / 1e12, 4),
                      "busiest_stream_jacobi_s": round(jac_s, 3)},
-        "breakdown": {k: round(perf[k], 2) for k in ("ms_theta", "ms_jacobi", "ms_post",
-                                                     "ms_measure", "ms_host_launch")},
+        "breakdown": {k: round(perf[k], 2) for k in ("ms_theta", "ms_qr", "ms_jacobi", "ms_post",
+                                                     "ms_measure", "ms_host_launch", "qr_tasks")},
         "memory_kernels": memory_kernels(perf, G),
         "contraction_kernel": contraction_kernel(perf, args.prec, G),
     }
@@ -687,8 +687,8 @@ def run_single(args, pool, workers: int, host: dict) -> dict:
         "launches": perf["jacobi_launches"],
         "svd_flops": perf["svd_flops"],
     }
-    breakdown = {k: round(perf[k], 3) for k in ("ms_theta", "ms_jacobi", "ms_post", "ms_measure",
-                                                "ms_host_launch")}
+    breakdown = {k: round(perf[k], 3) for k in ("ms_theta", "ms_qr", "ms_jacobi", "ms_post",
+                                                "ms_measure", "ms_host_launch", "qr_tasks")}
     breakdown["sweeps_per_update_batch"] = round(perf["sweeps"] / max(perf["updates"], 1), 3)
     breakdown["alg_tflops_whole_update"] = round(perf["alg_flops"] / wall / 1e12, 4)
 

@@ -94,6 +94,10 @@ typedef struct {
     int split_max;   /* 1..8: most workgroups per block pair (row-split steps) */
     int split_rows;  /* >= 64: least X rows per row-split workgroup */
     int ahead;       /* 1..3: sweeps enqueued ahead while far from convergence */
+    int qr;          /* 0 or 1: precondition the SVD with a column-sorted
+                        Householder QR and run the Jacobi sweeps on R^H
+                        (taken when every SVD of the batch has at most 512
+                        rows; the SVD computed is the same to rounding) */
 } tn_tune;
 
 /* Human-readable name of a status code (static string). */
@@ -175,7 +179,8 @@ tn_status tn_apply_u2(tn_mps* h, size_t k, const double* op);
  * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
  * Same update, kernels and result as tn_apply_u2 (mps.rs:1332-1387), with
  * the update's workspace (X, W, per-column arrays, row-split partial Gram
- * matrices, the descriptor) in a static device array of 10.3 MiB per GPU,
+ * matrices, the QR preconditioner's buffers, the descriptor) in a static
+ * device array of 14.3 MiB per GPU,
  * held until the process ends, instead of the engine's.  Limit: the bonds
  * next to the update (k-1 and k+1) at most 256, so theta is at most
  * 512 x 512; above that TN_INVALID_ARG (tn_last_error says why) and the state
@@ -390,6 +395,11 @@ typedef struct {
      * gate). */
     double ms_contract;
     double contract_flops;
+    /* The QR preconditioner: SVD tasks that ran it (counted with timing on
+     * or off), and the event span of its kernels (sort, QR, and the final
+     * X <- Q (W Sigma); part of neither ms_theta nor ms_jacobi). */
+    size_t qr_tasks;
+    double ms_qr;
 } tn_perf;
 
 /* Turn device-time accounting on (1) or off (0).  On adds one stream

@@ -23,7 +23,7 @@ HIPCC_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     f"-I{REPO / 'include'}",
 ]
-SOURCES = ["tn_kernels.hip", "tn_jacobi.hip", "tn_api.cpp"]
+SOURCES = ["tn_kernels.hip", "tn_jacobi.hip", "tn_qr.hip", "tn_api.cpp"]
 
 
 def build(verbose: bool = False) -> Path:

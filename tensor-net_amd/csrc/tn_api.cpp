@@ -173,8 +173,9 @@ using EntReq = EntDesc;
 // kScratchCols (bonds next to the updated one up to kScratchCols / 2), laid
 // out as the engine lays out one slot: X and W (kScratchCols^2 complex
 // each), the per-column arrays, partial norms, per-sweep maxima, info, the
-// active-task list, the row-split partial Gram matrices (both block widths)
-// and the task's descriptor.  Each device holds one copy for the life of the
+// active-task list, the row-split partial Gram matrices (both block widths),
+// the QR preconditioner's factored copy of X, panel T factors and sort
+// order, and the task's descriptor.  Each device holds one copy for the life of the
 // process; its address on a device comes from hipGetSymbolAddress.
 constexpr int kScratchCols = 512;
 constexpr size_t scratch_align(size_t b) { return (b + 255) / 256 * 256; }
@@ -197,7 +198,11 @@ constexpr size_t gpart_elems(int cols, int bw) {
 constexpr size_t kScrGpartElems = gpart_elems(kScratchCols, kBW) > gpart_elems(kScratchCols, kBWWide)
                                       ? gpart_elems(kScratchCols, kBW)
                                       : gpart_elems(kScratchCols, kBWWide);
-constexpr size_t kScrDesc = kScrGpart + scratch_align(sizeof(double2) * kScrGpartElems);
+constexpr size_t kScrQrE = kScrGpart + scratch_align(sizeof(double2) * kScrGpartElems);
+constexpr size_t kScrQrTElems = (size_t)(kScratchCols / kQrNb) * kQrNb * kQrNb;
+constexpr size_t kScrQrT = kScrQrE + scratch_align(sizeof(double2) * kScratchCols * kScratchCols);
+constexpr size_t kScrQrPerm = kScrQrT + scratch_align(sizeof(double2) * kScrQrTElems);
+constexpr size_t kScrDesc = kScrQrPerm + scratch_align(sizeof(int) * kScratchCols);
 constexpr size_t kScratchBytes = kScrDesc + scratch_align(sizeof(UpdDesc));
 
 static __device__ __align__(256) unsigned char scratch[kScratchBytes];
@@ -238,8 +243,10 @@ tn_status scratch_address(int dev, void** out) {
 // no host synchronisation to the stream.
 struct PerfRec {
   // theta begin/end, sweeps begin/end, post begin/end, split begin, renorm
-  // begin, contraction end (winit begin)
-  hipEvent_t ev[9] = {};
+  // begin, contraction end (QR begin), QR end (winit begin); ev[3]..ev[4]
+  // spans the QR's last kernel
+  hipEvent_t ev[10] = {};
+  bool qr = false;        // the batch ran the QR preconditioner
   int* h_info = nullptr;  // pinned copy of the batch's info (update batch) or the
                           // measured sites' chiL, chiR (measurement launch)
   int cap = 0;            // tasks (or sites) h_info holds
@@ -278,6 +285,13 @@ struct tn_engine {
   double* offs = nullptr;
   int* info = nullptr;
   int* act = nullptr;    // active-task list of the current sweep: count, then tasks
+  // QR preconditioner (tn_qr.hip): factored copy of X, panel T factors and
+  // sort order per slot
+  double2* qrE = nullptr;
+  double2* qrT = nullptr;
+  int* qrPerm = nullptr;
+  size_t qr_tslot = 0;   // elements of qrT per slot
+  int qr_min = 32;       // least C for which a batch takes the QR path (TNMPS_QR_MIN)
   int cus = 0;           // compute units of the device
   int list_min = 64;     // tasks from which the step kernel runs over act (TNMPS_LIST_MIN)
   int split_max = kMaxSplit;  // cap on workgroups per block pair (TNMPS_SPLIT_MAX)
@@ -334,6 +348,9 @@ struct tn_engine {
   void release_ws() {
     hfree(X); hfree(W); hfree(sraw); hfree(sval); hfree(fac); hfree(perm);
     hfree(part); hfree(offs); hfree(info); hfree(gpart); hfree(act);
+    hfree(qrE); hfree(qrT); hfree(qrPerm);
+    qrE = qrT = nullptr;
+    qrPerm = nullptr;
     hhfree(h_offs);
     h_offs = nullptr;
     gpart = nullptr;
@@ -413,9 +430,12 @@ struct tn_engine {
       r.split_max = split_max;
       r.split_rows = split_rows;
       r.ahead = std::max(1, std::min(3, (int)env_double("TNMPS_AHEAD", 2)));
+      r.qr = env_double("TNMPS_QR", 1) != 0 ? 1 : 0;
       tune.push_back(r);
     }
     factored = env_double("TNMPS_FACTORED", 1) != 0;
+    qr_min = std::max(kQrNb, (int)env_double("TNMPS_QR_MIN", 32));
+    qr_tslot = (size_t)((ldw + kQrNb - 1) / kQrNb) * kQrNb * kQrNb;
     max_sweeps = std::max(1, std::min(kMaxSweeps, (int)env_double("TNMPS_MAX_SWEEPS", kMaxSweeps)));
     {
       void* sp = nullptr;
@@ -431,7 +451,8 @@ struct tn_engine {
     trace_sweeps = env_double("TNMPS_TRACE_SWEEPS", 0) != 0;
     // workspace budget: TNMPS_WS_BYTES (default 24 GiB) bounds the tasks per launch
     const double budget = env_double("TNMPS_WS_BYTES", 24.0 * (1ull << 30));
-    const double per = 16.0 * ((double)ldx * ldw + (double)ldw * ldw) + 64.0 * ldw + 4096.0;
+    const double per = 16.0 * (2.0 * ldx * ldw + (double)ldw * ldw + (double)qr_tslot) +
+                       68.0 * ldw + 4096.0;
     max_slots = (int)std::max(1.0, std::min(8192.0, budget / per));
     return TN_OK;
   }
@@ -452,8 +473,9 @@ struct tn_engine {
       }
       perf.measure_sites += (size_t)r.nt;
     } else {
-      perf.ms_theta += span(r, 0, 1);
+      perf.ms_theta += span(r, 0, 8) + span(r, 9, 1);
       perf.ms_contract += span(r, 0, 8);
+      perf.ms_qr += span(r, 8, 9) + span(r, 3, 4);
       perf.ms_jacobi += span(r, 2, 3);
       perf.ms_post += span(r, 4, 5);
       perf.ms_split += span(r, 6, 7);
@@ -539,6 +561,9 @@ struct tn_engine {
     gpart_slots = std::min(ns, std::max(cus, 1));
     TN_HIP(hipMalloc(&gpart, sizeof(double2) * gpart_per * gpart_slots));
     TN_HIP(hipMalloc(&act, sizeof(int) * (ns + 1)));
+    TN_HIP(hipMalloc(&qrE, sizeof(double2) * xe * ns));
+    TN_HIP(hipMalloc(&qrT, sizeof(double2) * qr_tslot * ns));
+    TN_HIP(hipMalloc(&qrPerm, sizeof(int) * ldw * ns));
     slots = ns;
     return upd.ensure(ns, stream);
   }
@@ -719,6 +744,10 @@ struct Workspace {
   int* info = nullptr;
   int* act = nullptr;
   double* h_offs = nullptr;
+  double2* qrE = nullptr;   // or null: no QR preconditioner in this workspace
+  double2* qrT = nullptr;
+  int* qrPerm = nullptr;
+  size_t qr_tslot = 0;
   int ldx = 0, ldw = 0;
   UpdDesc* desc = nullptr;
 };
@@ -729,6 +758,7 @@ Workspace engine_workspace(const tn_engine* e) {
   w.perm = e->perm; w.part = e->part; w.gpart = e->gpart; w.gpart_per = e->gpart_per;
   w.gpart_slots = e->gpart_slots; w.offs = e->offs; w.info = e->info; w.act = e->act;
   w.h_offs = e->h_offs; w.ldx = e->ldx; w.ldw = e->ldw;
+  w.qrE = e->qrE; w.qrT = e->qrT; w.qrPerm = e->qrPerm; w.qr_tslot = e->qr_tslot;
   return w;
 }
 
@@ -951,6 +981,30 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt, const Workspace* ws
   const int gx_w = (int)std::min<size_t>((maxCC + 255) / 256, 1024);
   launch_theta(dd, nt, tu.theta_tile, gx_theta, e->stream);
   if (pr) TN_HIP(hipEventRecord(pr->ev[8], e->stream));
+  // QR preconditioner: the Jacobi sweeps then run on the C x C Y = R^H
+  const bool use_qr = tu.qr && w.qrE && maxR <= (size_t)kQrMaxRows && maxC >= (size_t)e->qr_min;
+  QrArgs qa{};
+  if (use_qr) {
+    qa.X = w.X;
+    qa.W = w.W;
+    qa.E = w.qrE;
+    qa.T = w.qrT;
+    qa.cn = w.sraw;
+    qa.perm0 = w.qrPerm;
+    qa.info = w.info;
+    qa.xslot = (size_t)w.ldx * w.ldw;
+    qa.wslot = (size_t)w.ldw * w.ldw;
+    qa.tslot = w.qr_tslot;
+    qa.ldx = w.ldx;
+    qa.ldw = w.ldw;
+    qa.nt = nt;
+    launch_qr(qa, (int)maxR, (int)maxC, e->stream);
+    e->perf.qr_tasks += (size_t)nt;
+  }
+  if (pr) {
+    TN_HIP(hipEventRecord(pr->ev[9], e->stream));
+    pr->qr = use_qr;
+  }
   launch_winit(dd, nt, gx_w, e->stream);
   TN_HIP(hipGetLastError());
   if (pr) TN_HIP(hipEventRecord(pr->ev[1], e->stream));
@@ -963,13 +1017,14 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt, const Workspace* ws
   size_t launches = 0;
   const int trace[3] = {tasks[0].has_gate ? 1 : 0, tasks[0].pre, mode0};
   if (pr) TN_HIP(hipEventRecord(pr->ev[2], e->stream));
-  TN_TRY(jacobi_sweeps(e, w, nt, maxR, maxC, tu, conv_tol, trace,
+  TN_TRY(jacobi_sweeps(e, w, nt, use_qr ? maxC : maxR, maxC, tu, conv_tol, trace,
                        [&](int t) { tasks[t].h->flags |= kFlagNoConv; }, &sweeps_run, &launches));
   if (pr) {
     TN_HIP(hipEventRecord(pr->ev[3], e->stream));
     pr->launches = launches;
-    TN_HIP(hipEventRecord(pr->ev[4], e->stream));
   }
+  if (use_qr) launch_qr_final(qa, (int)maxC, e->stream);
+  if (pr) TN_HIP(hipEventRecord(pr->ev[4], e->stream));
   launch_finish(dd, nt, sweeps_run, e->stream);
   if (any_factored) launch_fmul(dd, nt, (int)maxR, (int)maxC, e->stream);
   if (pr) TN_HIP(hipEventRecord(pr->ev[6], e->stream));
@@ -1780,6 +1835,7 @@ tn_status tn_engine_set_tuning(tn_engine* e, const tn_tune* rows, size_t count) 
     if (r.split_max < 1 || r.split_max > kMaxSplit) return TN_INVALID_ARG;
     if (r.split_rows < 64) return TN_INVALID_ARG;
     if (r.ahead < 1 || r.ahead > 3) return TN_INVALID_ARG;
+    if (r.qr != 0 && r.qr != 1) return TN_INVALID_ARG;
   }
   e->tune.assign(rows, rows + count);
   return TN_OK;
@@ -1849,6 +1905,10 @@ tn_status tn_apply_u2_noalloc(tn_mps* h, size_t k, const double* op) {
   w.act = (int*)(b + kScrAct);
   w.gpart = (double2*)(b + kScrGpart);
   w.gpart_per = kScrGpartElems;
+  w.qrE = (double2*)(b + kScrQrE);
+  w.qrT = (double2*)(b + kScrQrT);
+  w.qrPerm = (int*)(b + kScrQrPerm);
+  w.qr_tslot = kScrQrTElems;
   w.gpart_slots = 1;
   w.h_offs = h_offs;
   w.ldx = w.ldw = kScratchCols;

@@ -35,11 +35,15 @@ constexpr int kNPart = 64;      // blocks of the split/renormalise kernels
 constexpr int kMaxCols = 1024;  // largest C (= min(m, n)) the SVD accepts
 constexpr int kMaxSplit = 8;    // workgroups per block pair in the row-split step
 constexpr int kMeasParts = 64;  // most workgroups per measured site
+constexpr int kQrNb = 16;       // Householder panel width of the QR preconditioner
+constexpr int kQrMaxRows = 512; // most rows R of an SVD the QR preconditioner takes
 
 // info[] slots of a task (device ints)
 enum {
   kInfoM = 0, kInfoN, kInfoR, kInfoC, kInfoOrient, kInfoRank, kInfoFlags,
-  kInfoSweeps, kInfoChiL, kInfoChiM, kInfoChiR, kInfoMode, kInfoLen = 16
+  kInfoSweeps, kInfoChiL, kInfoChiM, kInfoChiR, kInfoMode,
+  kInfoRJ,  // rows of the Jacobi problem: R, or C after the QR preconditioner
+  kInfoLen = 16
 };
 
 // failure flag bits (the values of TN_FLAG_* in tnmps.h): in info[kInfoFlags]
@@ -103,6 +107,21 @@ struct JacobiArgs {
   double tol_rot;        // rotate a pair when |g_ij| > tol_rot sqrt(g_ii g_jj)
   double conv_tol;       // converged after a sweep whose max ratio <= conv_tol
   double tiny2;          // columns with squared norm below take no part
+};
+
+// The QR preconditioner of one launch batch (tn_qr.hip): task t's X and W
+// as in JacobiArgs, its factored copy E (same shape as X), panel T factors
+// and sort order.
+struct QrArgs {
+  double2* X;            // X + t * xslot (ldx rows): theta; Y = R^H; finally U Sigma
+  double2* W;            // W + t * wslot (ldw rows)
+  double2* E;            // E + t * xslot: X with sorted columns, factored in place
+  double2* T;            // T + t * tslot: panel jp's 16 x 16 T (row-major) at 256 jp
+  double* cn;            // squared column norms, ldw per task
+  int* perm0;            // columns by decreasing norm, ldw per task
+  int* info;             // kInfoLen ints per task
+  size_t xslot, wslot, tslot;
+  int ldx, ldw, nt;
 };
 
 // One step of the state-vector factorization (mps.rs:448-508 factorize).

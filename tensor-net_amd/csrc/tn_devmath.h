@@ -54,4 +54,29 @@ __device__ __forceinline__ double block_max(double v, double* buf) {
   return s;
 }
 
+// In-place bitonic sort of key[0 .. P2) (P2 a power of two) by decreasing
+// key, ties by increasing idx, over the workgroup; every thread takes part
+// and the sorted arrays are visible to all of them on return.
+__device__ __forceinline__ void bitonic_desc(double* key, int* idx, int P2) {
+  for (int k = 2; k <= P2; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = threadIdx.x; i < P2; i += blockDim.x) {
+        const int l = i ^ j;
+        if (l > i) {
+          const bool desc = ((i & k) == 0);
+          const double ki = key[i], kl = key[l];
+          const int ii = idx[i], il = idx[l];
+          // "i before l" in the final order
+          const bool i_first = (ki > kl) || (ki == kl && ii < il);
+          if (desc ? !i_first : i_first) {
+            key[i] = kl; key[l] = ki;
+            idx[i] = il; idx[l] = ii;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
 }  // namespace tnmps

@@ -654,7 +654,7 @@ template <int BW>
 __device__ __forceinline__ void step_task(const JacobiArgs& a, int t, int p, int sweep, int step,
                                           PairSmem<BW>& sm) {
   const int* inf = a.info + (size_t)t * kInfoLen;
-  const int R = inf[kInfoR], C = inf[kInfoC];
+  const int R = inf[kInfoRJ], C = inf[kInfoC];
   const bool done = converged(a, t, sweep);
   int lcol[BW / 2], ocol[BW / 2];
   thread_cols<BW>(a.nbe, step, p, lcol, ocol);
@@ -726,7 +726,7 @@ __global__ __launch_bounds__(256, BW == 8 ? 2 : 1) void k_jacobi_gram(JacobiArgs
   __shared__ PairSmem<BW> sm;
   const int t = blockIdx.y, p = blockIdx.x / a.S, q = blockIdx.x - p * a.S;
   const int* inf = a.info + (size_t)t * kInfoLen;
-  const int R = inf[kInfoR], C = inf[kInfoC];
+  const int R = inf[kInfoRJ], C = inf[kInfoC];
   const bool done = converged(a, t, sweep);
   int lcol[BW / 2], ocol[BW / 2];
   thread_cols<BW>(a.nbe, step, p, lcol, ocol);
@@ -753,7 +753,7 @@ __global__ __launch_bounds__(256, BW == 8 ? 2 : 1) void k_jacobi_apply(JacobiArg
   __shared__ PairSmem<BW> sm;
   const int t = blockIdx.y, p = blockIdx.x / a.S, q = blockIdx.x - p * a.S;
   const int* inf = a.info + (size_t)t * kInfoLen;
-  const int R = inf[kInfoR], C = inf[kInfoC];
+  const int R = inf[kInfoRJ], C = inf[kInfoC];
   const bool done = converged(a, t, sweep);
   int lcol[BW / 2], ocol[BW / 2];
   thread_cols<BW>(a.nbe, step, p, lcol, ocol);

@@ -23,6 +23,12 @@ void launch_jacobi_gram(const JacobiArgs& a, int npairs, int bw, int sweep, int 
                         hipStream_t s);
 void launch_jacobi_apply(const JacobiArgs& a, int npairs, int bw, int sweep, int step,
                          hipStream_t s);
+// QR preconditioner (tn_qr.hip): sort order, blocked Householder QR of the
+// sorted X into E, and Y = R^H into X's slot (info[kInfoRJ] = C); after the
+// Jacobi sweeps on Y, X <- Q (W Sigma) and W <- P U~ Sigma^-1.  Every task
+// has R <= kQrMaxRows; maxC bounds the tasks' C.
+void launch_qr(const QrArgs& a, int maxR, int maxC, hipStream_t s);
+void launch_qr_final(const QrArgs& a, int maxC, hipStream_t s);
 void launch_finish(const UpdDesc* descs, int nt, int sweeps_run, hipStream_t s);
 // Factored updates: T = kept singular vectors x isometric factor; grid bounds
 // maxX >= max(m, n) and maxRank >= rank over the tasks.

@@ -335,6 +335,7 @@ __global__ __launch_bounds__(256) void k_theta(const UpdDesc* __restrict__ descs
       d.info[kInfoM] = ts.m;
       d.info[kInfoN] = ts.n;
       d.info[kInfoR] = ts.R;
+      d.info[kInfoRJ] = ts.R;
       d.info[kInfoC] = ts.C;
       d.info[kInfoOrient] = orient;
       d.info[kInfoMode] = ts.mode;
@@ -422,26 +423,8 @@ __global__ __launch_bounds__(1024) void k_finish(const UpdDesc* __restrict__ des
     idx[i] = i;
   }
   __syncthreads();
-  // bitonic sort: descending key, ascending index on ties
-  for (int k = 2; k <= P2; k <<= 1) {
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int i = tid; i < P2; i += blockDim.x) {
-        const int l = i ^ j;
-        if (l > i) {
-          const bool desc = ((i & k) == 0);
-          const double ki = key[i], kl = key[l];
-          const int ii = idx[i], il = idx[l];
-          // "i before l" in the final order
-          const bool i_first = (ki > kl) || (ki == kl && ii < il);
-          if (desc ? !i_first : i_first) {
-            key[i] = kl; key[l] = ki;
-            idx[i] = il; idx[l] = ii;
-          }
-        }
-      }
-      __syncthreads();
-    }
-  }
+  // descending key, ascending index on ties
+  bitonic_desc(key, idx, P2);
   // norm of all values (mps.rs:243-248)
   double v = 0.0;
   for (int i = tid; i < C; i += blockDim.x) v += key[i] * key[i];
@@ -884,6 +867,7 @@ __global__ __launch_bounds__(256) void k_fv_init(FvArgs a) {
     a.info[kInfoM] = a.m;
     a.info[kInfoN] = a.ncols;
     a.info[kInfoR] = a.R;
+    a.info[kInfoRJ] = a.R;
     a.info[kInfoC] = a.C;
     a.info[kInfoOrient] = a.orient;
   }

@@ -1,0 +1,608 @@
+// Householder QR preconditioning of the Jacobi SVD for gfx950 (the SVD stands
+// in for LAPACK zgesvd, tensor-net/lib/mps.rs:299).
+//
+// For X (R x C, R >= C, column-major) the library can decompose, instead of
+// X itself, the C x C lower-triangular Y = R^H of a QR factorisation of X
+// with its columns sorted by decreasing norm:
+//   X P = Q R,   Y = R^H,   Y W = U~ (one-sided Jacobi on Y, unchanged),
+// so  X = (Q W) Sigma (P U~ Sigma^-1)^H  with Sigma the column norms of U~.
+// The SVD's consumers expect X W = U Sigma with W the right singular
+// vectors, so the last kernel writes
+//   X <- Q (W Sigma)    and    W <- P U~ Sigma^-1
+// (a zero column of U~ gives a zero column of W; the rank rule drops it).
+// This is the QR preconditioner of Drmac and Veselic (one-sided Jacobi on
+// R^H of a column-sorted QR): the rows of R are graded like the sorted
+// columns, and Jacobi on such input needs fewer sweeps.  For factored
+// updates (C = K < R) the Jacobi problem also shrinks from R to C rows.
+//
+// The QR is blocked Householder in the LAPACK convention (zgeqrf: H_i =
+// I - tau_i v_i v_i^H, Q = H_1 ... H_C; panels of kQrNb columns, Q_panel =
+// I - V T V^H with T upper triangular as zlarft builds it).  Per panel:
+//   k_qr_panel   one workgroup per task factors the panel's columns in LDS
+//                (reflector c: wave c forms it, then every later column's
+//                wave applies it; one workgroup barrier per reflector) and
+//                builds T
+//   k_qr_trail   the trailing columns, 16 per workgroup, on the FP64 matrix
+//                cores: A <- A - V (T^H (V^H A))
+// The first panel and trailing update read X through the sort permutation
+// and write the factored matrix E (the V below the diagonal, R on and above
+// it) into their own buffer, so no separate gather pass runs.
+//
+// Lane layout of v_mfma_f64_16x16x4f64 (checked on the device): lane l holds
+// A[l % 16][l / 16] and B[l / 16][l % 16]; D[i][j] is element i / 4 of lane
+// j + 16 (i % 4), i.e. element e of lane l is D[l / 16 + 4 e][l % 16].
+#include "tn_device.h"
+#include "tn_devmath.h"
+#include "tn_kernels.h"
+
+namespace tnmps {
+
+namespace {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ v4d mfma16(double a, double b, v4d c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ double2 wave_csum(double2 v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    v.x += __shfl_xor(v.x, off, 64);
+    v.y += __shfl_xor(v.y, off, 64);
+  }
+  return v;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Orders one wave's LDS accesses (a wave's LDS operations are performed in
+// issue order; the barrier keeps the compiler from reordering them).
+__device__ __forceinline__ void wave_lds_sync() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_wave_barrier();
+  asm volatile("" ::: "memory");
+}
+
+struct TaskQr {
+  int R, C;
+  double2* X;
+  double2* W;
+  double2* E;
+  double2* T;
+  int* perm0;
+  double* cn;
+};
+
+__device__ __forceinline__ TaskQr task_qr(const QrArgs& a, int t) {
+  const int* inf = a.info + (size_t)t * kInfoLen;
+  TaskQr q;
+  q.R = inf[kInfoR];
+  q.C = inf[kInfoC];
+  q.X = a.X + (size_t)t * a.xslot;
+  q.W = a.W + (size_t)t * a.wslot;
+  q.E = a.E + (size_t)t * a.xslot;
+  q.T = a.T + (size_t)t * a.tslot;
+  q.perm0 = a.perm0 + (size_t)t * a.ldw;
+  q.cn = a.cn + (size_t)t * a.ldw;
+  return q;
+}
+
+// Column c of the matrix a panel or trailing update reads: panel 0's pass
+// reads X through the sort order, later ones the factored matrix E.
+__device__ __forceinline__ const double2* src_col(const QrArgs& a, const TaskQr& q, int jp,
+                                                  int c) {
+  return jp == 0 ? q.X + (size_t)q.perm0[c] * a.ldx : q.E + (size_t)c * a.ldx;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------ column norms
+// Squared column norms of X (16 columns per workgroup, 4 per wave).
+__global__ __launch_bounds__(256) void k_qr_norms(QrArgs a) {
+  const TaskQr q = task_qr(a, blockIdx.y);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int j = blockIdx.x * 16 + wid * 4 + k;
+    if (j >= q.C) break;
+    const double2* col = q.X + (size_t)j * a.ldx;
+    double s = 0.0;
+    for (int r = lane; r < q.R; r += 64) s += cabs2(col[r]);
+    s = wave_sum(s);
+    if (lane == 0) q.cn[j] = s;
+  }
+}
+
+// ------------------------------------------------------------ sort order
+// perm0: columns by decreasing norm, ties by increasing index (one
+// workgroup per task).
+__global__ __launch_bounds__(1024) void k_qr_sort(QrArgs a) {
+  const TaskQr q = task_qr(a, blockIdx.x);
+  __shared__ double key[kMaxCols];
+  __shared__ int idx[kMaxCols];
+  int P2 = 1;
+  while (P2 < q.C) P2 <<= 1;
+  for (int i = threadIdx.x; i < P2; i += blockDim.x) {
+    key[i] = (i < q.C) ? q.cn[i] : -1.0;
+    idx[i] = i;
+  }
+  __syncthreads();
+  bitonic_desc(key, idx, P2);
+  for (int i = threadIdx.x; i < q.C; i += blockDim.x) q.perm0[i] = idx[i];
+}
+
+// ------------------------------------------------------- panel factorisation
+// Panel jp (columns g0 = 16 jp .. g0 + 15, rows g0 .. R-1) of every task with
+// C > g0, held in LDS column by column (R - g0 <= kQrMaxRows).  Wave w owns
+// panel column w; lane l its rows l, l + 64, ...
+//
+// Reflector c (zlarfg): alpha = P[c][c], xnorm = |P[c][c+1:]|,
+//   beta = -sign(Re alpha) sqrt(|alpha|^2 + xnorm^2),
+//   tau = (beta - alpha) / beta,  v = P[c][c+1:] / (alpha - beta),  P[c][c] = beta
+// (tau = 0 and nothing changes when xnorm = 0 and Im alpha = 0).  Then every
+// later column k: w = v^H P[k][c:], P[k][c:] -= conj(tau) w v  (H^H applied).
+// T (zlarft, forward, columnwise): T[i][i] = tau_i,
+//   T[j][i] = -tau_i sum_{l=j}^{i-1} T[j][l] (V^H V)[l][i]   (j < i).
+__global__ __launch_bounds__(1024) void k_qr_panel(QrArgs a, int jp) {
+  __shared__ double2 P[kQrNb][kQrMaxRows];
+  __shared__ double2 taus[kQrNb];
+  __shared__ double2 g[kQrNb][kQrNb + 1];
+  const TaskQr q = task_qr(a, blockIdx.x);
+  const int g0 = kQrNb * jp;
+  if (g0 >= q.C || q.R > kQrMaxRows) return;  // (the host takes this path for R <= kQrMaxRows)
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int nr = q.R - g0;
+  const int ncol = min(kQrNb, q.C - g0);
+  if (wid < ncol) {
+    const double2* src = src_col(a, q, jp, g0 + wid) + g0;
+    for (int r = lane; r < nr; r += 64) P[wid][r] = src[r];
+  }
+  __syncthreads();
+  for (int c = 0; c < ncol; ++c) {
+    if (wid == c) {
+      double s = 0.0;
+      for (int r = c + 1 + lane; r < nr; r += 64) s += cabs2(P[c][r]);
+      s = wave_sum(s);
+      const double2 alpha = P[c][c];
+      double2 tau = make_double2(0.0, 0.0);
+      if (s > 0.0 || alpha.y != 0.0) {
+        const double mag = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + s);
+        const double beta = alpha.x >= 0.0 ? -mag : mag;
+        tau = make_double2((beta - alpha.x) / beta, -alpha.y / beta);
+        // 1 / (alpha - beta)
+        const double2 d = make_double2(alpha.x - beta, alpha.y);
+        const double id2 = 1.0 / cabs2(d);
+        const double2 sc = make_double2(d.x * id2, -d.y * id2);
+        for (int r = c + 1 + lane; r < nr; r += 64) P[c][r] = cmul(P[c][r], sc);
+        if (lane == 0) P[c][c] = make_double2(beta, 0.0);
+      }
+      if (lane == 0) taus[c] = tau;
+    }
+    __syncthreads();
+    if (wid > c && wid < ncol) {
+      const double2 tau = taus[c];
+      double2 w = make_double2(0.0, 0.0);
+      for (int r = c + lane; r < nr; r += 64) {
+        const double2 v = r == c ? make_double2(1.0, 0.0) : P[c][r];
+        w = cadd(w, cmulc(v, P[wid][r]));
+      }
+      w = wave_csum(w);
+      const double2 f = cmulc(tau, w);  // conj(tau) w
+      for (int r = c + lane; r < nr; r += 64) {
+        const double2 v = r == c ? make_double2(1.0, 0.0) : P[c][r];
+        const double2 x = P[wid][r];
+        const double2 fv = cmul(f, v);
+        P[wid][r] = make_double2(x.x - fv.x, x.y - fv.y);
+      }
+    }
+  }
+  __syncthreads();
+  // (V^H V)[j][i] for j < i over rows r >= i (v_i[i] = 1, v_i[r < i] = 0)
+  if (wid < ncol) {
+    const int i = wid;
+    for (int j = 0; j < i; ++j) {
+      double2 acc = make_double2(0.0, 0.0);
+      for (int r = i + lane; r < nr; r += 64) {
+        const double2 vi = r == i ? make_double2(1.0, 0.0) : P[i][r];
+        acc = cadd(acc, cmulc(P[j][r], vi));
+      }
+      acc = wave_csum(acc);
+      if (lane == 0) g[j][i] = acc;
+    }
+  }
+  __syncthreads();
+  // T row j by lane j of wave 0 (a row depends only on its own earlier
+  // entries), written row-major to the task's T slot of the panel
+  if (wid == 0 && lane < kQrNb) {
+    const int j = lane;
+    double2 trow[kQrNb];
+#pragma unroll
+    for (int i = 0; i < kQrNb; ++i) {
+      double2 v = make_double2(0.0, 0.0);
+      if (i < ncol) {
+        if (i == j) {
+          v = taus[i];
+        } else if (i > j) {
+          double2 s = make_double2(0.0, 0.0);
+#pragma unroll
+          for (int l = 0; l < kQrNb; ++l)
+            if (l >= j && l < i) s = cadd(s, cmul(trow[l], g[l][i]));
+          const double2 ts = cmul(taus[i], s);
+          v = make_double2(-ts.x, -ts.y);
+        }
+      }
+      trow[i] = v;
+    }
+    double2* T = q.T + (size_t)jp * kQrNb * kQrNb + j * kQrNb;
+#pragma unroll
+    for (int i = 0; i < kQrNb; ++i) T[i] = trow[i];
+  }
+  if (wid < ncol) {
+    double2* dst = q.E + (size_t)(g0 + wid) * a.ldx + g0;
+    for (int r = lane; r < nr; r += 64) dst[r] = P[wid][r];
+  }
+}
+
+// ------------------------------------------------------- trailing update
+// Trailing columns c0 .. c0 + 15 (c0 = 16 (jp + 1) + 16 blockIdx.x) of
+// every task, rows g0 .. R-1:  A <- A - V (T^H (V^H A)).  Both products run
+// on the FP64 matrix cores with the rows staged through LDS 64 at a time
+// (thread: row tid & 63, columns tid / 64 + 4 k):
+//   V^H A:  wave w takes rows 16 w .. 16 w + 15 of a chunk, four at a time,
+//           as K:  Re += Vr^T Ar + Vi^T Ai,  Im += Vr^T Ai - Vi^T Ar;
+//   V M:    wave w computes rows 16 w .. 16 w + 15 of a chunk with the
+//           panel columns as K:  Re = Vr Mr - Vi Mi,  Im = Vr Mi + Vi Mr.
+// The four waves' partial V^H A are summed in a fixed order.
+namespace {
+struct TrailSmem {
+  double2 Vs[64][kQrNb + 1];
+  double2 As[64][kQrNb + 1];
+  double2 part[4][kQrNb][kQrNb + 1];
+  double2 Ms[kQrNb][kQrNb + 1];
+  double2 Ts[kQrNb][kQrNb + 1];
+};
+
+// V[r][i] of panel jp for local row r (global g0 + r) and panel column i:
+// 1 on the diagonal, 0 above it, E below it; 0 past the panel's ncol
+// reflectors and past its nr rows.
+__device__ __forceinline__ double2 vload(const double2* E, size_t ldx, int g0, int nr, int ncol,
+                                         int r, int i) {
+  if (r >= nr || r < i || i >= ncol) return make_double2(0.0, 0.0);
+  if (r == i) return make_double2(1.0, 0.0);
+  return E[(size_t)(g0 + i) * ldx + g0 + r];
+}
+}  // namespace
+
+__global__ __launch_bounds__(256) void k_qr_trail(QrArgs a, int jp) {
+  __shared__ TrailSmem sm;
+  const TaskQr q = task_qr(a, blockIdx.y);
+  const int g0 = kQrNb * jp;
+  const int c0 = g0 + kQrNb + kQrNb * blockIdx.x;
+  if (c0 >= q.C) return;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int nr = q.R - g0;
+  const int ncv = min(kQrNb, q.C - g0);  // reflectors in the panel
+  const int rr = tid & 63;
+  const double2* acol[4];
+  int ic[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    ic[k] = wid + 4 * k;
+    const int c = c0 + ic[k];
+    acol[k] = c < q.C ? src_col(a, q, jp, c) + g0 : nullptr;
+  }
+  // T of the panel
+  {
+    const int i = tid >> 4, j = tid & 15;
+    sm.Ts[i][j] = q.T[(size_t)jp * kQrNb * kQrNb + i * kQrNb + j];
+  }
+  // pass 1: V^H A
+  v4d gre = {0.0, 0.0, 0.0, 0.0}, gim = {0.0, 0.0, 0.0, 0.0};
+  for (int r0 = 0; r0 < nr; r0 += 64) {
+    const int r = r0 + rr;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      sm.Vs[rr][ic[k]] = vload(q.E, a.ldx, g0, nr, ncv, r, ic[k]);
+      sm.As[rr][ic[k]] = (acol[k] && r < nr) ? acol[k][r] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int gg = 0; gg < 4; ++gg) {
+      const int row = 16 * wid + 4 * gg + (lane >> 4), col = lane & 15;
+      const double2 v = sm.Vs[row][col], x = sm.As[row][col];
+      gre = mfma16(v.x, x.x, gre);
+      gre = mfma16(v.y, x.y, gre);
+      gim = mfma16(v.x, x.y, gim);
+      gim = mfma16(-v.y, x.x, gim);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int e = 0; e < 4; ++e)
+    sm.part[wid][(lane >> 4) + 4 * e][lane & 15] = make_double2(gre[e], gim[e]);
+  __syncthreads();
+  {
+    // M = T^H (V^H A): M[i][j] = sum_{l <= i} conj(T[l][i]) S[l][j]
+    const int i = tid >> 4, j = tid & 15;
+    double2 acc = make_double2(0.0, 0.0);
+    for (int l = 0; l <= i; ++l) {
+      double2 s = sm.part[0][l][j];
+      s = cadd(s, sm.part[1][l][j]);
+      s = cadd(s, sm.part[2][l][j]);
+      s = cadd(s, sm.part[3][l][j]);
+      acc = cadd(acc, cmulc(sm.Ts[l][i], s));
+    }
+    sm.Ms[i][j] = acc;
+  }
+  __syncthreads();
+  double2 mk[4];
+#pragma unroll
+  for (int kk = 0; kk < 4; ++kk) mk[kk] = sm.Ms[4 * kk + (lane >> 4)][lane & 15];
+  // pass 2: A - V M, written to E
+  const int oc = c0 + (lane & 15);
+  double2* dst = oc < q.C ? q.E + (size_t)oc * a.ldx + g0 : nullptr;
+  for (int r0 = 0; r0 < nr; r0 += 64) {
+    const int r = r0 + rr;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      sm.Vs[rr][ic[k]] = vload(q.E, a.ldx, g0, nr, ncv, r, ic[k]);
+      sm.As[rr][ic[k]] = (acol[k] && r < nr) ? acol[k][r] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    v4d ore = {0.0, 0.0, 0.0, 0.0}, oim = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) {
+      const double2 v = sm.Vs[16 * wid + (lane & 15)][4 * kk + (lane >> 4)];
+      ore = mfma16(v.x, mk[kk].x, ore);
+      ore = mfma16(-v.y, mk[kk].y, ore);
+      oim = mfma16(v.x, mk[kk].y, oim);
+      oim = mfma16(v.y, mk[kk].x, oim);
+    }
+    if (dst) {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int lr = 16 * wid + (lane >> 4) + 4 * e;
+        if (r0 + lr < nr) {
+          const double2 x = sm.As[lr][lane & 15];
+          dst[r0 + lr] = make_double2(x.x - ore[e], x.y - oim[e]);
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------ Y = R^H
+// Y (C x C, column-major in X's slot, ld ldx) = R^H: Y[j][i] = conj(R[i][j])
+// = conj(E[j][i] stored at column j, row i) for i <= j, zero above the
+// diagonal.  32 x 32 tiles through LDS; the Jacobi problem's row count
+// becomes C.
+__global__ __launch_bounds__(256) void k_qr_formy(QrArgs a, int ntile) {
+  __shared__ double2 tile[32][33];
+  const TaskQr q = task_qr(a, blockIdx.y);
+  const int bi = blockIdx.x % ntile, bj = blockIdx.x / ntile;
+  if (blockIdx.x == 0 && threadIdx.x == 0) a.info[(size_t)blockIdx.y * kInfoLen + kInfoRJ] = q.C;
+  const int i0 = 32 * bi, j0 = 32 * bj;  // R rows i, columns j
+  if (i0 >= q.C || j0 >= q.C) return;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int jj = ty; jj < 32; jj += 8) {
+    const int i = i0 + tx, j = j0 + jj;
+    tile[jj][tx] = (i < q.C && j < q.C && i <= j) ? q.E[(size_t)j * a.ldx + i]
+                                                  : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  // Y column i (= R row i), rows j
+  for (int ii = ty; ii < 32; ii += 8) {
+    const int i = i0 + ii, j = j0 + tx;
+    if (i < q.C && j < q.C) q.X[(size_t)i * a.ldx + j] = cconj(tile[tx][ii]);
+  }
+}
+
+// ------------------------------------------------------------ final
+// After the Jacobi sweeps on Y (X's slot holds U~ = Y W): columns j0 .. j0+15
+// of every task, one workgroup of 8 waves.
+//   sigma_j = |U~[:, j]|;  W[:, j] <- P U~[:, j] / sigma_j  (row perm0[i] <- row i);
+//   Z = [W[:, j] sigma_j ; 0] (R x 16)  <-  Q Z, applying the panels last to
+//   first: Z <- Z - V (T (V^H Z))  on the FP64 matrix cores;  X[:, j] <- Z.
+// Z stays in registers in the matrix-core output layout: wave w holds row
+// tiles t = w, w + 8, w + 16, w + 24 (16 rows each; R <= kQrMaxRows), lane
+// l element e of tile t is Z[16 t + l / 16 + 4 e][j0 + l % 16].  That layout
+// is also the B operand of V^H Z computed as (Z^H V)^H, so only V changes
+// layout: it is loaded with lanes along rows (the A operand of V M) and
+// transposed through a per-wave LDS tile (the B operand of Z^H V).
+namespace {
+constexpr int kFinWaves = 8;
+constexpr int kFinTiles = kQrMaxRows / 16 / kFinWaves;  // row tiles per wave
+struct FinSmem {
+  double2 part[kFinWaves][kQrNb][kQrNb + 1];
+  double2 Ms[kQrNb][kQrNb + 1];
+  double2 Ts[kQrNb][kQrNb + 1];
+  double2 vt[kFinWaves][kQrNb][kQrNb + 1];
+  double s2[kFinWaves][kQrNb];
+  double sig[kQrNb];
+};
+}  // namespace
+
+__global__ __launch_bounds__(512) void k_qr_final(QrArgs a) {
+  __shared__ FinSmem sm;
+  const TaskQr q = task_qr(a, blockIdx.y);
+  const int j0 = kQrNb * blockIdx.x;
+  if (j0 >= q.C || q.R > kQrMaxRows) return;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int col = j0 + (lane & 15);
+  const bool colok = col < q.C;
+  // U~ column tile: squared norms, kept for the new W
+  double2 u[kFinTiles][4];
+  double s2 = 0.0;
+#pragma unroll
+  for (int s = 0; s < kFinTiles; ++s) {
+    const int t = wid + kFinWaves * s;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int row = 16 * t + (lane >> 4) + 4 * e;
+      u[s][e] = (colok && row < q.C) ? q.X[(size_t)col * a.ldx + row] : make_double2(0.0, 0.0);
+      s2 += cabs2(u[s][e]);
+    }
+  }
+  s2 += __shfl_xor(s2, 16, 64);
+  s2 += __shfl_xor(s2, 32, 64);
+  // W column tile, the start of Z
+  v4d zr[kFinTiles], zi[kFinTiles];
+#pragma unroll
+  for (int s = 0; s < kFinTiles; ++s) {
+    const int t = wid + kFinWaves * s;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int row = 16 * t + (lane >> 4) + 4 * e;
+      const double2 w = (colok && row < q.C) ? q.W[(size_t)col * a.ldw + row]
+                                             : make_double2(0.0, 0.0);
+      zr[s][e] = w.x;
+      zi[s][e] = w.y;
+    }
+  }
+  if (lane < 16) sm.s2[wid][lane] = s2;
+  __syncthreads();
+  if (tid < kQrNb) {
+    double v = 0.0;
+#pragma unroll
+    for (int w = 0; w < kFinWaves; ++w) v += sm.s2[w][tid];
+    sm.sig[tid] = sqrt(v);
+  }
+  __syncthreads();
+  const double sg = sm.sig[lane & 15];
+  const double isg = sg > 0.0 ? 1.0 / sg : 0.0;
+#pragma unroll
+  for (int s = 0; s < kFinTiles; ++s) {
+    zr[s] *= sg;
+    zi[s] *= sg;
+  }
+  // every read of this tile of W (in Z) precedes the writes below
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < kFinTiles; ++s) {
+    const int t = wid + kFinWaves * s;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int row = 16 * t + (lane >> 4) + 4 * e;
+      if (colok && row < q.C)
+        q.W[(size_t)col * a.ldw + q.perm0[row]] = cscale(u[s][e], isg);
+    }
+  }
+  // Z <- Q Z, panels last to first
+  const int npan = (q.C + kQrNb - 1) / kQrNb;
+  for (int jp = npan - 1; jp >= 0; --jp) {
+    const int g0 = kQrNb * jp;
+    const int nr = q.R - g0;
+    const int ncol = min(kQrNb, q.C - g0);
+    __syncthreads();  // the previous panel's Ms and Ts reads are done
+    if (tid < kQrNb * kQrNb) {
+      const int i = tid >> 4, j = tid & 15;
+      sm.Ts[i][j] = q.T[(size_t)jp * kQrNb * kQrNb + i * kQrNb + j];
+    }
+    // Z^H V over the wave's row tiles at or below the panel
+    v4d hre = {0.0, 0.0, 0.0, 0.0}, him = {0.0, 0.0, 0.0, 0.0};
+    double2 vrow[kFinTiles][4];
+#pragma unroll
+    for (int s = 0; s < kFinTiles; ++s) {
+      const int t = wid + kFinWaves * s;
+      // V in rows: lane r + 16 k holds V[16 t + r][4 kk + k]
+#pragma unroll
+      for (int kk = 0; kk < 4; ++kk) {
+        const int lr = 16 * t + (lane & 15) - g0;
+        const int i = 4 * kk + (lane >> 4);
+        vrow[s][kk] = (t >= jp) ? vload(q.E, a.ldx, g0, nr, ncol, lr, i) : make_double2(0.0, 0.0);
+      }
+      if (t >= jp && 16 * t < q.R) {
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk)
+          sm.vt[wid][lane & 15][4 * kk + (lane >> 4)] = vrow[s][kk];
+        wave_lds_sync();
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          // B operand: lane i + 16 k holds V[16 t + 4 e + k][i]
+          const double2 v = sm.vt[wid][4 * e + (lane >> 4)][lane & 15];
+          // A operand conj(Z)^T: lane j + 16 k holds Z[16 t + 4 e + k][j] (element e)
+          hre = mfma16(zr[s][e], v.x, hre);
+          hre = mfma16(zi[s][e], v.y, hre);
+          him = mfma16(zr[s][e], v.y, him);
+          him = mfma16(-zi[s][e], v.x, him);
+        }
+        wave_lds_sync();
+      }
+    }
+    // H[j][i] = (Z^H V)[j][i] = conj((V^H Z)[i][j])
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+      sm.part[wid][(lane >> 4) + 4 * e][lane & 15] = make_double2(hre[e], him[e]);
+    __syncthreads();
+    if (tid < kQrNb * kQrNb) {
+      // M = T (V^H Z): M[i][j] = sum_{l >= i} T[i][l] conj(H[j][l])
+      const int i = tid >> 4, j = tid & 15;
+      double2 acc = make_double2(0.0, 0.0);
+      for (int l = i; l < kQrNb; ++l) {
+        double2 h = sm.part[0][j][l];
+#pragma unroll
+        for (int w = 1; w < kFinWaves; ++w) h = cadd(h, sm.part[w][j][l]);
+        acc = cadd(acc, cmul(sm.Ts[i][l], cconj(h)));
+      }
+      sm.Ms[i][j] = acc;
+    }
+    __syncthreads();
+    double2 mk[4];
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) mk[kk] = sm.Ms[4 * kk + (lane >> 4)][lane & 15];
+    // Z -= V M
+#pragma unroll
+    for (int s = 0; s < kFinTiles; ++s) {
+      const int t = wid + kFinWaves * s;
+      if (t >= jp && 16 * t < q.R) {
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) {
+          const double2 v = vrow[s][kk];
+          zr[s] = mfma16(-v.x, mk[kk].x, zr[s]);
+          zr[s] = mfma16(v.y, mk[kk].y, zr[s]);
+          zi[s] = mfma16(-v.x, mk[kk].y, zi[s]);
+          zi[s] = mfma16(-v.y, mk[kk].x, zi[s]);
+        }
+      }
+    }
+  }
+  // X[:, j] <- Z (rows < R)
+  if (colok) {
+#pragma unroll
+    for (int s = 0; s < kFinTiles; ++s) {
+      const int t = wid + kFinWaves * s;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int row = 16 * t + (lane >> 4) + 4 * e;
+        if (row < q.R) q.X[(size_t)col * a.ldx + row] = make_double2(zr[s][e], zi[s][e]);
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------- launch
+void launch_qr(const QrArgs& a, int maxR, int maxC, hipStream_t s) {
+  const int ctiles = (maxC + kQrNb - 1) / kQrNb;
+  hipLaunchKernelGGL(k_qr_norms, dim3(ctiles, a.nt), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_qr_sort, dim3(a.nt), dim3(1024), 0, s, a);
+  for (int jp = 0; jp < ctiles; ++jp) {
+    hipLaunchKernelGGL(k_qr_panel, dim3(a.nt), dim3(1024), 0, s, a, jp);
+    const int trail = ctiles - jp - 1;
+    if (trail > 0) hipLaunchKernelGGL(k_qr_trail, dim3(trail, a.nt), dim3(256), 0, s, a, jp);
+  }
+  const int yt = (maxC + 31) / 32;
+  hipLaunchKernelGGL(k_qr_formy, dim3(yt * yt, a.nt), dim3(256), 0, s, a, yt);
+  (void)maxR;
+}
+
+void launch_qr_final(const QrArgs& a, int maxC, hipStream_t s) {
+  const int ctiles = (maxC + kQrNb - 1) / kQrNb;
+  hipLaunchKernelGGL(k_qr_final, dim3(ctiles, a.nt), dim3(512), 0, s, a);
+}
+
+}  // namespace tnmps

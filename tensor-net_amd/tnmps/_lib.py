@@ -41,10 +41,13 @@ class tn_trunc(C.Structure):
 
 class tn_tune(C.Structure):
     _fields_ = [("c_max", C.c_int), ("theta_tile", C.c_int), ("bw", C.c_int),
-                ("split_max", C.c_int), ("split_rows", C.c_int), ("ahead", C.c_int)]
+                ("split_max", C.c_int), ("split_rows", C.c_int), ("ahead", C.c_int),
+                ("qr", C.c_int)]
 
 
-TUNE_FIELDS = ("c_max", "theta_tile", "bw", "split_max", "split_rows", "ahead")
+TUNE_FIELDS = ("c_max", "theta_tile", "bw", "split_max", "split_rows", "ahead", "qr")
+# fields a table may leave out, and the value they then take
+TUNE_DEFAULTS = {"qr": 1}
 
 
 class tn_perf(C.Structure):
@@ -56,7 +59,8 @@ class tn_perf(C.Structure):
                 ("factored", C.c_size_t), ("ms_split", C.c_double), ("ms_renorm", C.c_double),
                 ("bytes_split", C.c_double), ("bytes_renorm", C.c_double),
                 ("bytes_measure", C.c_double), ("measure_sites", C.c_size_t),
-                ("ms_contract", C.c_double), ("contract_flops", C.c_double)]
+                ("ms_contract", C.c_double), ("contract_flops", C.c_double),
+                ("qr_tasks", C.c_size_t), ("ms_qr", C.c_double)]
 
 
 _sz = C.c_size_t

@@ -41,6 +41,7 @@ CANDIDATES: Dict[str, Sequence[int]] = {
     "split_rows": (64, 128, 256),
     "bw": (8, 16),
     "ahead": (1, 2, 3),
+    "qr": (0, 1),
 }
 
 
@@ -175,10 +176,14 @@ def load_table(path: str, key: str) -> List[dict]:
     with open(path) as f:
         data = json.load(f)
     rows = data[key]["table"]
+    out = []
     for r in rows:
-        if set(r) != set(L.TUNE_FIELDS):
+        missing = set(L.TUNE_FIELDS) - set(r)
+        if set(r) - set(L.TUNE_FIELDS) or missing - set(L.TUNE_DEFAULTS):
             raise ValueError(f"{path}:{key}: row fields {sorted(r)}")
-    return rows
+        # a table written before a field existed takes that field's default
+        out.append({f: r[f] if f in r else L.TUNE_DEFAULTS[f] for f in L.TUNE_FIELDS})
+    return out
 
 
 def main(argv=None) -> int:

@@ -72,11 +72,15 @@ PRECISIONS = {"f64": L.PREC_F64, "f32": L.PREC_F32, "bf16x3": L.PREC_BF16X3}
 
 
 def _tune_array(rows):
-    """ctypes array of tn_tune from dicts or field tuples."""
+    """ctypes array of tn_tune from dicts or field tuples (fields missing
+    from a row take their value in TUNE_DEFAULTS)."""
     rows = list(rows)
     arr = (L.tn_tune * max(1, len(rows)))()
     for i, r in enumerate(rows):
-        vals = [r[f] for f in L.TUNE_FIELDS] if isinstance(r, dict) else list(r)
+        if isinstance(r, dict):
+            vals = [r[f] if f in r else L.TUNE_DEFAULTS[f] for f in L.TUNE_FIELDS]
+        else:
+            vals = list(r) + [L.TUNE_DEFAULTS[f] for f in L.TUNE_FIELDS[len(r):]]
         for f, v in zip(L.TUNE_FIELDS, vals):
             setattr(arr[i], f, int(v))
     return arr, len(rows)

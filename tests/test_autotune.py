@@ -8,7 +8,7 @@ from tnmps.autotune import CANDIDATES, bucket_of, load_table
 
 
 def _rows():
-    return [dict(c_max=c, theta_tile=32, bw=8, split_max=8, split_rows=128, ahead=2)
+    return [dict(c_max=c, theta_tile=32, bw=8, split_max=8, split_rows=128, ahead=2, qr=1)
             for c in (16, 32, 64, 128, 256, 512, 1024)]
 
 
@@ -29,11 +29,16 @@ def test_candidates_are_valid_fields():
     assert all(1 <= v <= 8 for v in CANDIDATES["split_max"])
     assert all(v >= 64 for v in CANDIDATES["split_rows"])
     assert all(1 <= v <= 3 for v in CANDIDATES["ahead"])
+    assert set(CANDIDATES["qr"]) <= {0, 1}
 
 
 def test_load_table(tmp_path):
     p = tmp_path / "t.json"
     p.write_text(json.dumps({"k": {"table": _rows()}}))
+    assert load_table(str(p), "k") == _rows()
+    # a row without a field that has a default (an older table) takes it
+    old = [{f: v for f, v in r.items() if f != "qr"} for r in _rows()]
+    p.write_text(json.dumps({"k": {"table": old}}))
     assert load_table(str(p), "k") == _rows()
     bad = _rows()
     bad[0]["extra"] = 1

@@ -20,9 +20,9 @@ pytestmark = pytest.mark.gpu
 BUCKETS = (16, 32, 64, 128, 256, 512, 1024)
 
 
-def table(tile, bw=8, split_max=8, split_rows=128, ahead=2):
+def table(tile, bw=8, split_max=8, split_rows=128, ahead=2, qr=1):
     return [dict(c_max=c, theta_tile=tile, bw=bw, split_max=split_max, split_rows=split_rows,
-                 ahead=ahead) for c in BUCKETS]
+                 ahead=ahead, qr=qr) for c in BUCKETS]
 
 
 def test_theta_tiles_match_oracle_and_each_other(gpu):
@@ -84,12 +84,12 @@ def test_tuning_table_roundtrip_and_validation(gpu):
     eng = Engine(32)
     rows = eng.get_tuning()
     assert [r["c_max"] for r in rows] == list(BUCKETS)
-    new = table(64, bw=16, split_max=2, split_rows=256, ahead=3)[:3]
+    new = table(64, bw=16, split_max=2, split_rows=256, ahead=3, qr=0)[:3]
     eng.set_tuning(new)
     assert eng.get_tuning() == new
     bad = [dict(new[0], theta_tile=48), dict(new[0], bw=12), dict(new[0], split_max=0),
            dict(new[0], split_max=9), dict(new[0], split_rows=32), dict(new[0], ahead=0),
-           dict(new[0], ahead=4), dict(new[0], c_max=0)]
+           dict(new[0], ahead=4), dict(new[0], c_max=0), dict(new[0], qr=2)]
     for r in bad:
         with pytest.raises(MPSError):
             eng.set_tuning([r])
