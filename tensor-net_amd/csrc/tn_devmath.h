@@ -54,6 +54,50 @@ __device__ __forceinline__ double block_max(double v, double* buf) {
   return s;
 }
 
+// Cross-lane moves and wave sums on the DPP network (no LDS): CTRL is a
+// DPP control word (quad_perm 0x00-0xff, row_half_mirror 0x141, row_mirror
+// 0x140).  A double moves as its two 32-bit halves.
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+__device__ __forceinline__ double read_lane(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Sum over the 64 lanes of a wave, returned to every lane, in a fixed order
+// (xor 1, xor 2, half-row mirror, row mirror: each 16-lane row then holds
+// its row sum; the four row sums are added as (r0 + r1) + (r2 + r3)).  Every
+// lane of the wave must be active.
+__device__ __forceinline__ double wave_sum_dpp(double v) {
+  v += dpp_mov<0xB1>(v);
+  v += dpp_mov<0x4E>(v);
+  v += dpp_mov<0x141>(v);
+  v += dpp_mov<0x140>(v);
+  return (read_lane(v, 0) + read_lane(v, 16)) + (read_lane(v, 32) + read_lane(v, 48));
+}
+
+// The same for both parts of a complex value, the two chains interleaved.
+__device__ __forceinline__ double2 wave_csum_dpp(double2 v) {
+  v.x += dpp_mov<0xB1>(v.x);
+  v.y += dpp_mov<0xB1>(v.y);
+  v.x += dpp_mov<0x4E>(v.x);
+  v.y += dpp_mov<0x4E>(v.y);
+  v.x += dpp_mov<0x141>(v.x);
+  v.y += dpp_mov<0x141>(v.y);
+  v.x += dpp_mov<0x140>(v.x);
+  v.y += dpp_mov<0x140>(v.y);
+  return make_double2((read_lane(v.x, 0) + read_lane(v.x, 16)) + (read_lane(v.x, 32) + read_lane(v.x, 48)),
+                      (read_lane(v.y, 0) + read_lane(v.y, 16)) + (read_lane(v.y, 32) + read_lane(v.y, 48)));
+}
+
 // In-place bitonic sort of key[0 .. P2) (P2 a power of two) by decreasing
 // key, ties by increasing idx, over the workgroup; every thread takes part
 // and the sorted arrays are visible to all of them on return.

@@ -44,21 +44,6 @@ __device__ __forceinline__ v4d mfma16(double a, double b, v4d c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
-__device__ __forceinline__ double2 wave_csum(double2 v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    v.x += __shfl_xor(v.x, off, 64);
-    v.y += __shfl_xor(v.y, off, 64);
-  }
-  return v;
-}
-
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
-
 // Orders one wave's LDS accesses (a wave's LDS operations are performed in
 // issue order; the barrier keeps the compiler from reordering them).
 __device__ __forceinline__ void wave_lds_sync() {
@@ -112,7 +97,7 @@ __global__ __launch_bounds__(256) void k_qr_norms(QrArgs a) {
     const double2* col = q.X + (size_t)j * a.ldx;
     double s = 0.0;
     for (int r = lane; r < q.R; r += 64) s += cabs2(col[r]);
-    s = wave_sum(s);
+    s = wave_sum_dpp(s);
     if (lane == 0) q.cn[j] = s;
   }
 }
@@ -137,17 +122,19 @@ __global__ __launch_bounds__(1024) void k_qr_sort(QrArgs a) {
 
 // ------------------------------------------------------- panel factorisation
 // Panel jp (columns g0 = 16 jp .. g0 + 15, rows g0 .. R-1) of every task with
-// C > g0, held in LDS column by column (R - g0 <= kQrMaxRows).  Wave w owns
-// panel column w; lane l its rows l, l + 64, ...
+// C > g0 (R - g0 <= kQrMaxRows).  Wave w holds panel column w in registers,
+// lane l its rows l + 64 m; a reflector's vector goes through LDS.
 //
 // Reflector c (zlarfg): alpha = P[c][c], xnorm = |P[c][c+1:]|,
 //   beta = -sign(Re alpha) sqrt(|alpha|^2 + xnorm^2),
 //   tau = (beta - alpha) / beta,  v = P[c][c+1:] / (alpha - beta),  P[c][c] = beta
 // (tau = 0 and nothing changes when xnorm = 0 and Im alpha = 0).  Then every
 // later column k: w = v^H P[k][c:], P[k][c:] -= conj(tau) w v  (H^H applied).
+// One workgroup barrier per reflector; the wave sums run on the DPP network.
 // T (zlarft, forward, columnwise): T[i][i] = tau_i,
 //   T[j][i] = -tau_i sum_{l=j}^{i-1} T[j][l] (V^H V)[l][i]   (j < i).
 __global__ __launch_bounds__(1024) void k_qr_panel(QrArgs a, int jp) {
+  constexpr int RPL = kQrMaxRows / 64;  // rows per lane
   __shared__ double2 P[kQrNb][kQrMaxRows];
   __shared__ double2 taus[kQrNb];
   __shared__ double2 g[kQrNb][kQrNb + 1];
@@ -157,17 +144,24 @@ __global__ __launch_bounds__(1024) void k_qr_panel(QrArgs a, int jp) {
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const int nr = q.R - g0;
   const int ncol = min(kQrNb, q.C - g0);
+  double2 x[RPL];
   if (wid < ncol) {
     const double2* src = src_col(a, q, jp, g0 + wid) + g0;
-    for (int r = lane; r < nr; r += 64) P[wid][r] = src[r];
+#pragma unroll
+    for (int m = 0; m < RPL; ++m) {
+      const int r = lane + 64 * m;
+      x[m] = r < nr ? src[r] : make_double2(0.0, 0.0);
+    }
   }
-  __syncthreads();
   for (int c = 0; c < ncol; ++c) {
     if (wid == c) {
       double s = 0.0;
-      for (int r = c + 1 + lane; r < nr; r += 64) s += cabs2(P[c][r]);
-      s = wave_sum(s);
-      const double2 alpha = P[c][c];
+#pragma unroll
+      for (int m = 0; m < RPL; ++m)
+        if (lane + 64 * m > c) s += cabs2(x[m]);
+      s = wave_sum_dpp(s);
+      // row c < 64 sits in lane c, slot 0
+      const double2 alpha = make_double2(read_lane(x[0].x, c), read_lane(x[0].y, c));
       double2 tau = make_double2(0.0, 0.0);
       if (s > 0.0 || alpha.y != 0.0) {
         const double mag = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + s);
@@ -177,40 +171,64 @@ __global__ __launch_bounds__(1024) void k_qr_panel(QrArgs a, int jp) {
         const double2 d = make_double2(alpha.x - beta, alpha.y);
         const double id2 = 1.0 / cabs2(d);
         const double2 sc = make_double2(d.x * id2, -d.y * id2);
-        for (int r = c + 1 + lane; r < nr; r += 64) P[c][r] = cmul(P[c][r], sc);
-        if (lane == 0) P[c][c] = make_double2(beta, 0.0);
+#pragma unroll
+        for (int m = 0; m < RPL; ++m)
+          if (lane + 64 * m > c) x[m] = cmul(x[m], sc);
+        if (lane == c) x[0] = make_double2(beta, 0.0);
       }
+#pragma unroll
+      for (int m = 0; m < RPL; ++m)
+        if (lane + 64 * m < nr) P[c][lane + 64 * m] = x[m];
       if (lane == 0) taus[c] = tau;
     }
     __syncthreads();
     if (wid > c && wid < ncol) {
       const double2 tau = taus[c];
+      double2 v[RPL];
       double2 w = make_double2(0.0, 0.0);
-      for (int r = c + lane; r < nr; r += 64) {
-        const double2 v = r == c ? make_double2(1.0, 0.0) : P[c][r];
-        w = cadd(w, cmulc(v, P[wid][r]));
+#pragma unroll
+      for (int m = 0; m < RPL; ++m) {
+        const int r = lane + 64 * m;
+        v[m] = (r > c && r < nr) ? P[c][r] : make_double2(r == c ? 1.0 : 0.0, 0.0);
+        w = cadd(w, cmulc(v[m], x[m]));
       }
-      w = wave_csum(w);
+      w = wave_csum_dpp(w);
       const double2 f = cmulc(tau, w);  // conj(tau) w
-      for (int r = c + lane; r < nr; r += 64) {
-        const double2 v = r == c ? make_double2(1.0, 0.0) : P[c][r];
-        const double2 x = P[wid][r];
-        const double2 fv = cmul(f, v);
-        P[wid][r] = make_double2(x.x - fv.x, x.y - fv.y);
+#pragma unroll
+      for (int m = 0; m < RPL; ++m) {
+        const double2 fv = cmul(f, v[m]);
+        x[m] = make_double2(x[m].x - fv.x, x[m].y - fv.y);
+      }
+    }
+  }
+  // every column final: to LDS for the T factor, and to E
+  if (wid < ncol) {
+    double2* dst = q.E + (size_t)(g0 + wid) * a.ldx + g0;
+#pragma unroll
+    for (int m = 0; m < RPL; ++m) {
+      const int r = lane + 64 * m;
+      if (r < nr) {
+        P[wid][r] = x[m];
+        dst[r] = x[m];
       }
     }
   }
   __syncthreads();
-  // (V^H V)[j][i] for j < i over rows r >= i (v_i[i] = 1, v_i[r < i] = 0)
+  // (V^H V)[j][i] for j < i over rows r >= i (v_i[i] = 1, v_i[r < i] = 0):
+  // wave i, its own column from registers
   if (wid < ncol) {
     const int i = wid;
     for (int j = 0; j < i; ++j) {
       double2 acc = make_double2(0.0, 0.0);
-      for (int r = i + lane; r < nr; r += 64) {
-        const double2 vi = r == i ? make_double2(1.0, 0.0) : P[i][r];
-        acc = cadd(acc, cmulc(P[j][r], vi));
+#pragma unroll
+      for (int m = 0; m < RPL; ++m) {
+        const int r = lane + 64 * m;
+        if (r >= i && r < nr) {
+          const double2 vi = r == i ? make_double2(1.0, 0.0) : x[m];
+          acc = cadd(acc, cmulc(P[j][r], vi));
+        }
       }
-      acc = wave_csum(acc);
+      acc = wave_csum_dpp(acc);
       if (lane == 0) g[j][i] = acc;
     }
   }
@@ -241,32 +259,30 @@ __global__ __launch_bounds__(1024) void k_qr_panel(QrArgs a, int jp) {
 #pragma unroll
     for (int i = 0; i < kQrNb; ++i) T[i] = trow[i];
   }
-  if (wid < ncol) {
-    double2* dst = q.E + (size_t)(g0 + wid) * a.ldx + g0;
-    for (int r = lane; r < nr; r += 64) dst[r] = P[wid][r];
-  }
 }
 
-// ------------------------------------------------------- trailing update
-// Trailing columns c0 .. c0 + 15 (c0 = 16 (jp + 1) + 16 blockIdx.x) of
-// every task, rows g0 .. R-1:  A <- A - V (T^H (V^H A)).  Both products run
-// on the FP64 matrix cores with the rows staged through LDS 64 at a time
-// (thread: row tid & 63, columns tid / 64 + 4 k):
-//   V^H A:  wave w takes rows 16 w .. 16 w + 15 of a chunk, four at a time,
-//           as K:  Re += Vr^T Ar + Vi^T Ai,  Im += Vr^T Ai - Vi^T Ar;
-//   V M:    wave w computes rows 16 w .. 16 w + 15 of a chunk with the
-//           panel columns as K:  Re = Vr Mr - Vi Mi,  Im = Vr Mi + Vi Mr.
-// The four waves' partial V^H A are summed in a fixed order.
+// ------------------------------------------------- one panel on registers
+// The trailing update and the final Q application run one panel's
+// reflectors, Z <- (I - V op(T) V^H) Z, on a 16-column tile Z held in
+// registers by a workgroup of kQW waves, in the matrix-core output layout:
+// wave w holds row tiles t = w, w + kQW, ... (16 rows each; R <=
+// kQrMaxRows), and element e of lane l of tile t is Z[16 t + l / 16 + 4 e]
+// [l % 16].  That layout is also the operand of V^H Z computed as
+// (Z^H V)^H, so only V changes layout: it is loaded with lanes along rows
+// (the A operand of V M) and transposed through a per-wave LDS tile (the B
+// operand of Z^H V).  The kQW partial Z^H V are summed in a fixed order.
 namespace {
-struct TrailSmem {
-  double2 Vs[64][kQrNb + 1];
-  double2 As[64][kQrNb + 1];
-  double2 part[4][kQrNb][kQrNb + 1];
+constexpr int kQW = 8;
+constexpr int kQT = kQrMaxRows / 16 / kQW;  // row tiles per wave
+
+struct PanelSmem {
+  double2 part[kQW][kQrNb][kQrNb + 1];
   double2 Ms[kQrNb][kQrNb + 1];
   double2 Ts[kQrNb][kQrNb + 1];
+  double2 vt[kQW][kQrNb][kQrNb + 1];
 };
 
-// V[r][i] of panel jp for local row r (global g0 + r) and panel column i:
+// V[r][i] of a panel for local row r (global g0 + r) and panel column i:
 // 1 on the diagonal, 0 above it, E below it; 0 past the panel's ncol
 // reflectors and past its nr rows.
 __device__ __forceinline__ double2 vload(const double2* E, size_t ldx, int g0, int nr, int ncol,
@@ -275,66 +291,75 @@ __device__ __forceinline__ double2 vload(const double2* E, size_t ldx, int g0, i
   if (r == i) return make_double2(1.0, 0.0);
   return E[(size_t)(g0 + i) * ldx + g0 + r];
 }
-}  // namespace
 
-__global__ __launch_bounds__(256) void k_qr_trail(QrArgs a, int jp) {
-  __shared__ TrailSmem sm;
-  const TaskQr q = task_qr(a, blockIdx.y);
-  const int g0 = kQrNb * jp;
-  const int c0 = g0 + kQrNb + kQrNb * blockIdx.x;
-  if (c0 >= q.C) return;
+// Panel jp's V for the wave's row tiles, lanes along rows: lane r + 16 k of
+// slot (s, kk) holds V[16 t + r][4 kk + k] (zero for tiles above the panel).
+__device__ __forceinline__ void load_vrows(const double2* E, size_t ldx, int R, int C, int jp,
+                                           double2 (&v)[kQT][4]) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int g0 = kQrNb * jp, nr = R - g0, ncol = min(kQrNb, C - g0);
+#pragma unroll
+  for (int s = 0; s < kQT; ++s) {
+    const int t = wid + kQW * s;
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk)
+      v[s][kk] = (t >= jp) ? vload(E, ldx, g0, nr, ncol, 16 * t + (lane & 15) - g0,
+                                   4 * kk + (lane >> 4))
+                           : make_double2(0.0, 0.0);
+  }
+}
+
+// Z <- (I - V op(T) V^H) Z with op(T) = T^H (ADJ: Q_panel^H, the QR's
+// trailing update) or T (Q_panel, the final application of Q).  Tp: the
+// panel's T, row-major.  Begins with a workgroup barrier.
+template <bool ADJ>
+__device__ void panel_update(const double2 (&vrow)[kQT][4], const double2* Tp, int jp, int R,
+                             v4d (&zr)[kQT], v4d (&zi)[kQT], PanelSmem& sm) {
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-  const int nr = q.R - g0;
-  const int ncv = min(kQrNb, q.C - g0);  // reflectors in the panel
-  const int rr = tid & 63;
-  const double2* acol[4];
-  int ic[4];
+  __syncthreads();  // the previous use of sm is done
+  if (tid < kQrNb * kQrNb) sm.Ts[tid >> 4][tid & 15] = Tp[tid];
+  // H = Z^H V over the wave's row tiles at or below the panel
+  v4d hre = {0.0, 0.0, 0.0, 0.0}, him = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    ic[k] = wid + 4 * k;
-    const int c = c0 + ic[k];
-    acol[k] = c < q.C ? src_col(a, q, jp, c) + g0 : nullptr;
-  }
-  // T of the panel
-  {
-    const int i = tid >> 4, j = tid & 15;
-    sm.Ts[i][j] = q.T[(size_t)jp * kQrNb * kQrNb + i * kQrNb + j];
-  }
-  // pass 1: V^H A
-  v4d gre = {0.0, 0.0, 0.0, 0.0}, gim = {0.0, 0.0, 0.0, 0.0};
-  for (int r0 = 0; r0 < nr; r0 += 64) {
-    const int r = r0 + rr;
+  for (int s = 0; s < kQT; ++s) {
+    const int t = wid + kQW * s;
+    if (t >= jp && 16 * t < R) {
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      sm.Vs[rr][ic[k]] = vload(q.E, a.ldx, g0, nr, ncv, r, ic[k]);
-      sm.As[rr][ic[k]] = (acol[k] && r < nr) ? acol[k][r] : make_double2(0.0, 0.0);
+      for (int kk = 0; kk < 4; ++kk) sm.vt[wid][lane & 15][4 * kk + (lane >> 4)] = vrow[s][kk];
+      wave_lds_sync();
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        // B operand: lane i + 16 k holds V[16 t + 4 e + k][i]; A operand
+        // conj(Z)^T: lane j + 16 k holds Z[16 t + 4 e + k][j] (element e)
+        const double2 v = sm.vt[wid][4 * e + (lane >> 4)][lane & 15];
+        hre = mfma16(zr[s][e], v.x, hre);
+        hre = mfma16(zi[s][e], v.y, hre);
+        him = mfma16(zr[s][e], v.y, him);
+        him = mfma16(-zi[s][e], v.x, him);
+      }
+      wave_lds_sync();
     }
-    __syncthreads();
-#pragma unroll
-    for (int gg = 0; gg < 4; ++gg) {
-      const int row = 16 * wid + 4 * gg + (lane >> 4), col = lane & 15;
-      const double2 v = sm.Vs[row][col], x = sm.As[row][col];
-      gre = mfma16(v.x, x.x, gre);
-      gre = mfma16(v.y, x.y, gre);
-      gim = mfma16(v.x, x.y, gim);
-      gim = mfma16(-v.y, x.x, gim);
-    }
-    __syncthreads();
   }
 #pragma unroll
   for (int e = 0; e < 4; ++e)
-    sm.part[wid][(lane >> 4) + 4 * e][lane & 15] = make_double2(gre[e], gim[e]);
+    sm.part[wid][(lane >> 4) + 4 * e][lane & 15] = make_double2(hre[e], him[e]);
   __syncthreads();
-  {
-    // M = T^H (V^H A): M[i][j] = sum_{l <= i} conj(T[l][i]) S[l][j]
+  if (tid < kQrNb * kQrNb) {
+    // H summed over the waves in a fixed order, into part[0]
+    const int j = tid >> 4, l = tid & 15;
+    double2 h = sm.part[0][j][l];
+#pragma unroll
+    for (int w = 1; w < kQW; ++w) h = cadd(h, sm.part[w][j][l]);
+    sm.part[0][j][l] = h;
+  }
+  __syncthreads();
+  if (tid < kQrNb * kQrNb) {
+    // (V^H Z)[l][j] = conj(H[j][l]);  M = op(T) V^H Z
     const int i = tid >> 4, j = tid & 15;
     double2 acc = make_double2(0.0, 0.0);
-    for (int l = 0; l <= i; ++l) {
-      double2 s = sm.part[0][l][j];
-      s = cadd(s, sm.part[1][l][j]);
-      s = cadd(s, sm.part[2][l][j]);
-      s = cadd(s, sm.part[3][l][j]);
-      acc = cadd(acc, cmulc(sm.Ts[l][i], s));
+    for (int l = ADJ ? 0 : i; l < (ADJ ? i + 1 : kQrNb); ++l) {
+      const double2 t = ADJ ? cconj(sm.Ts[l][i]) : sm.Ts[i][l];
+      acc = cadd(acc, cmul(t, cconj(sm.part[0][j][l])));
     }
     sm.Ms[i][j] = acc;
   }
@@ -342,37 +367,65 @@ __global__ __launch_bounds__(256) void k_qr_trail(QrArgs a, int jp) {
   double2 mk[4];
 #pragma unroll
   for (int kk = 0; kk < 4; ++kk) mk[kk] = sm.Ms[4 * kk + (lane >> 4)][lane & 15];
-  // pass 2: A - V M, written to E
-  const int oc = c0 + (lane & 15);
-  double2* dst = oc < q.C ? q.E + (size_t)oc * a.ldx + g0 : nullptr;
-  for (int r0 = 0; r0 < nr; r0 += 64) {
-    const int r = r0 + rr;
+  // Z -= V M:  A operand V (lane r + 16 k: V[r][4 kk + k]), B operand M
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      sm.Vs[rr][ic[k]] = vload(q.E, a.ldx, g0, nr, ncv, r, ic[k]);
-      sm.As[rr][ic[k]] = (acol[k] && r < nr) ? acol[k][r] : make_double2(0.0, 0.0);
-    }
-    __syncthreads();
-    v4d ore = {0.0, 0.0, 0.0, 0.0}, oim = {0.0, 0.0, 0.0, 0.0};
+  for (int s = 0; s < kQT; ++s) {
+    const int t = wid + kQW * s;
+    if (t >= jp && 16 * t < R) {
 #pragma unroll
-    for (int kk = 0; kk < 4; ++kk) {
-      const double2 v = sm.Vs[16 * wid + (lane & 15)][4 * kk + (lane >> 4)];
-      ore = mfma16(v.x, mk[kk].x, ore);
-      ore = mfma16(-v.y, mk[kk].y, ore);
-      oim = mfma16(v.x, mk[kk].y, oim);
-      oim = mfma16(v.y, mk[kk].x, oim);
-    }
-    if (dst) {
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int lr = 16 * wid + (lane >> 4) + 4 * e;
-        if (r0 + lr < nr) {
-          const double2 x = sm.As[lr][lane & 15];
-          dst[r0 + lr] = make_double2(x.x - ore[e], x.y - oim[e]);
-        }
+      for (int kk = 0; kk < 4; ++kk) {
+        const double2 v = vrow[s][kk];
+        zr[s] = mfma16(-v.x, mk[kk].x, zr[s]);
+        zr[s] = mfma16(v.y, mk[kk].y, zr[s]);
+        zi[s] = mfma16(-v.x, mk[kk].y, zi[s]);
+        zi[s] = mfma16(-v.y, mk[kk].x, zi[s]);
       }
     }
-    __syncthreads();
+  }
+}
+}  // namespace
+
+// ------------------------------------------------------- trailing update
+// Trailing columns c0 .. c0 + 15 (c0 = 16 (jp + 1) + 16 blockIdx.x) of
+// every task, rows g0 .. R-1:  A <- A - V (T^H (V^H A)).  The tile and the
+// panel's V load at entry (every load in flight at once), the update runs on
+// registers (panel_update), and the tile goes to E.
+__global__ __launch_bounds__(512) void k_qr_trail(QrArgs a, int jp) {
+  __shared__ PanelSmem sm;
+  const TaskQr q = task_qr(a, blockIdx.y);
+  const int g0 = kQrNb * jp;
+  const int c0 = g0 + kQrNb + kQrNb * blockIdx.x;
+  if (c0 >= q.C || q.R > kQrMaxRows) return;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int col = c0 + (lane & 15);
+  const bool colok = col < q.C;
+  const double2* src = colok ? src_col(a, q, jp, col) : nullptr;
+  v4d zr[kQT], zi[kQT];
+#pragma unroll
+  for (int s = 0; s < kQT; ++s) {
+    const int t = wid + kQW * s;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int row = 16 * t + (lane >> 4) + 4 * e;
+      const double2 x = (colok && t >= jp && row < q.R) ? src[row] : make_double2(0.0, 0.0);
+      zr[s][e] = x.x;
+      zi[s][e] = x.y;
+    }
+  }
+  double2 vrow[kQT][4];
+  load_vrows(q.E, a.ldx, q.R, q.C, jp, vrow);
+  panel_update<true>(vrow, q.T + (size_t)jp * kQrNb * kQrNb, jp, q.R, zr, zi, sm);
+  if (colok) {
+    double2* dst = q.E + (size_t)col * a.ldx;
+#pragma unroll
+    for (int s = 0; s < kQT; ++s) {
+      const int t = wid + kQW * s;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int row = 16 * t + (lane >> 4) + 4 * e;
+        if (t >= jp && row < q.R) dst[row] = make_double2(zr[s][e], zi[s][e]);
+      }
+    }
   }
 }
 
@@ -404,25 +457,14 @@ __global__ __launch_bounds__(256) void k_qr_formy(QrArgs a, int ntile) {
 
 // ------------------------------------------------------------ final
 // After the Jacobi sweeps on Y (X's slot holds U~ = Y W): columns j0 .. j0+15
-// of every task, one workgroup of 8 waves.
+// of every task, one workgroup of kQW waves.
 //   sigma_j = |U~[:, j]|;  W[:, j] <- P U~[:, j] / sigma_j  (row perm0[i] <- row i);
 //   Z = [W[:, j] sigma_j ; 0] (R x 16)  <-  Q Z, applying the panels last to
-//   first: Z <- Z - V (T (V^H Z))  on the FP64 matrix cores;  X[:, j] <- Z.
-// Z stays in registers in the matrix-core output layout: wave w holds row
-// tiles t = w, w + 8, w + 16, w + 24 (16 rows each; R <= kQrMaxRows), lane
-// l element e of tile t is Z[16 t + l / 16 + 4 e][j0 + l % 16].  That layout
-// is also the B operand of V^H Z computed as (Z^H V)^H, so only V changes
-// layout: it is loaded with lanes along rows (the A operand of V M) and
-// transposed through a per-wave LDS tile (the B operand of Z^H V).
+//   first (panel_update);  X[:, j] <- Z.
 namespace {
-constexpr int kFinWaves = 8;
-constexpr int kFinTiles = kQrMaxRows / 16 / kFinWaves;  // row tiles per wave
 struct FinSmem {
-  double2 part[kFinWaves][kQrNb][kQrNb + 1];
-  double2 Ms[kQrNb][kQrNb + 1];
-  double2 Ts[kQrNb][kQrNb + 1];
-  double2 vt[kFinWaves][kQrNb][kQrNb + 1];
-  double s2[kFinWaves][kQrNb];
+  PanelSmem p;
+  double s2[kQW][kQrNb];
   double sig[kQrNb];
 };
 }  // namespace
@@ -436,11 +478,11 @@ __global__ __launch_bounds__(512) void k_qr_final(QrArgs a) {
   const int col = j0 + (lane & 15);
   const bool colok = col < q.C;
   // U~ column tile: squared norms, kept for the new W
-  double2 u[kFinTiles][4];
+  double2 u[kQT][4];
   double s2 = 0.0;
 #pragma unroll
-  for (int s = 0; s < kFinTiles; ++s) {
-    const int t = wid + kFinWaves * s;
+  for (int s = 0; s < kQT; ++s) {
+    const int t = wid + kQW * s;
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       const int row = 16 * t + (lane >> 4) + 4 * e;
@@ -451,10 +493,10 @@ __global__ __launch_bounds__(512) void k_qr_final(QrArgs a) {
   s2 += __shfl_xor(s2, 16, 64);
   s2 += __shfl_xor(s2, 32, 64);
   // W column tile, the start of Z
-  v4d zr[kFinTiles], zi[kFinTiles];
+  v4d zr[kQT], zi[kQT];
 #pragma unroll
-  for (int s = 0; s < kFinTiles; ++s) {
-    const int t = wid + kFinWaves * s;
+  for (int s = 0; s < kQT; ++s) {
+    const int t = wid + kQW * s;
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       const int row = 16 * t + (lane >> 4) + 4 * e;
@@ -469,22 +511,22 @@ __global__ __launch_bounds__(512) void k_qr_final(QrArgs a) {
   if (tid < kQrNb) {
     double v = 0.0;
 #pragma unroll
-    for (int w = 0; w < kFinWaves; ++w) v += sm.s2[w][tid];
+    for (int w = 0; w < kQW; ++w) v += sm.s2[w][tid];
     sm.sig[tid] = sqrt(v);
   }
   __syncthreads();
   const double sg = sm.sig[lane & 15];
   const double isg = sg > 0.0 ? 1.0 / sg : 0.0;
 #pragma unroll
-  for (int s = 0; s < kFinTiles; ++s) {
+  for (int s = 0; s < kQT; ++s) {
     zr[s] *= sg;
     zi[s] *= sg;
   }
   // every read of this tile of W (in Z) precedes the writes below
   __syncthreads();
 #pragma unroll
-  for (int s = 0; s < kFinTiles; ++s) {
-    const int t = wid + kFinWaves * s;
+  for (int s = 0; s < kQT; ++s) {
+    const int t = wid + kQW * s;
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       const int row = 16 * t + (lane >> 4) + 4 * e;
@@ -495,87 +537,15 @@ __global__ __launch_bounds__(512) void k_qr_final(QrArgs a) {
   // Z <- Q Z, panels last to first
   const int npan = (q.C + kQrNb - 1) / kQrNb;
   for (int jp = npan - 1; jp >= 0; --jp) {
-    const int g0 = kQrNb * jp;
-    const int nr = q.R - g0;
-    const int ncol = min(kQrNb, q.C - g0);
-    __syncthreads();  // the previous panel's Ms and Ts reads are done
-    if (tid < kQrNb * kQrNb) {
-      const int i = tid >> 4, j = tid & 15;
-      sm.Ts[i][j] = q.T[(size_t)jp * kQrNb * kQrNb + i * kQrNb + j];
-    }
-    // Z^H V over the wave's row tiles at or below the panel
-    v4d hre = {0.0, 0.0, 0.0, 0.0}, him = {0.0, 0.0, 0.0, 0.0};
-    double2 vrow[kFinTiles][4];
-#pragma unroll
-    for (int s = 0; s < kFinTiles; ++s) {
-      const int t = wid + kFinWaves * s;
-      // V in rows: lane r + 16 k holds V[16 t + r][4 kk + k]
-#pragma unroll
-      for (int kk = 0; kk < 4; ++kk) {
-        const int lr = 16 * t + (lane & 15) - g0;
-        const int i = 4 * kk + (lane >> 4);
-        vrow[s][kk] = (t >= jp) ? vload(q.E, a.ldx, g0, nr, ncol, lr, i) : make_double2(0.0, 0.0);
-      }
-      if (t >= jp && 16 * t < q.R) {
-#pragma unroll
-        for (int kk = 0; kk < 4; ++kk)
-          sm.vt[wid][lane & 15][4 * kk + (lane >> 4)] = vrow[s][kk];
-        wave_lds_sync();
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          // B operand: lane i + 16 k holds V[16 t + 4 e + k][i]
-          const double2 v = sm.vt[wid][4 * e + (lane >> 4)][lane & 15];
-          // A operand conj(Z)^T: lane j + 16 k holds Z[16 t + 4 e + k][j] (element e)
-          hre = mfma16(zr[s][e], v.x, hre);
-          hre = mfma16(zi[s][e], v.y, hre);
-          him = mfma16(zr[s][e], v.y, him);
-          him = mfma16(-zi[s][e], v.x, him);
-        }
-        wave_lds_sync();
-      }
-    }
-    // H[j][i] = (Z^H V)[j][i] = conj((V^H Z)[i][j])
-#pragma unroll
-    for (int e = 0; e < 4; ++e)
-      sm.part[wid][(lane >> 4) + 4 * e][lane & 15] = make_double2(hre[e], him[e]);
-    __syncthreads();
-    if (tid < kQrNb * kQrNb) {
-      // M = T (V^H Z): M[i][j] = sum_{l >= i} T[i][l] conj(H[j][l])
-      const int i = tid >> 4, j = tid & 15;
-      double2 acc = make_double2(0.0, 0.0);
-      for (int l = i; l < kQrNb; ++l) {
-        double2 h = sm.part[0][j][l];
-#pragma unroll
-        for (int w = 1; w < kFinWaves; ++w) h = cadd(h, sm.part[w][j][l]);
-        acc = cadd(acc, cmul(sm.Ts[i][l], cconj(h)));
-      }
-      sm.Ms[i][j] = acc;
-    }
-    __syncthreads();
-    double2 mk[4];
-#pragma unroll
-    for (int kk = 0; kk < 4; ++kk) mk[kk] = sm.Ms[4 * kk + (lane >> 4)][lane & 15];
-    // Z -= V M
-#pragma unroll
-    for (int s = 0; s < kFinTiles; ++s) {
-      const int t = wid + kFinWaves * s;
-      if (t >= jp && 16 * t < q.R) {
-#pragma unroll
-        for (int kk = 0; kk < 4; ++kk) {
-          const double2 v = vrow[s][kk];
-          zr[s] = mfma16(-v.x, mk[kk].x, zr[s]);
-          zr[s] = mfma16(v.y, mk[kk].y, zr[s]);
-          zi[s] = mfma16(-v.x, mk[kk].y, zi[s]);
-          zi[s] = mfma16(-v.y, mk[kk].x, zi[s]);
-        }
-      }
-    }
+    double2 vrow[kQT][4];
+    load_vrows(q.E, a.ldx, q.R, q.C, jp, vrow);
+    panel_update<false>(vrow, q.T + (size_t)jp * kQrNb * kQrNb, jp, q.R, zr, zi, sm.p);
   }
   // X[:, j] <- Z (rows < R)
   if (colok) {
 #pragma unroll
-    for (int s = 0; s < kFinTiles; ++s) {
-      const int t = wid + kFinWaves * s;
+    for (int s = 0; s < kQT; ++s) {
+      const int t = wid + kQW * s;
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         const int row = 16 * t + (lane >> 4) + 4 * e;
@@ -593,7 +563,7 @@ void launch_qr(const QrArgs& a, int maxR, int maxC, hipStream_t s) {
   for (int jp = 0; jp < ctiles; ++jp) {
     hipLaunchKernelGGL(k_qr_panel, dim3(a.nt), dim3(1024), 0, s, a, jp);
     const int trail = ctiles - jp - 1;
-    if (trail > 0) hipLaunchKernelGGL(k_qr_trail, dim3(trail, a.nt), dim3(256), 0, s, a, jp);
+    if (trail > 0) hipLaunchKernelGGL(k_qr_trail, dim3(trail, a.nt), dim3(512), 0, s, a, jp);
   }
   const int yt = (maxC + 31) / 32;
   hipLaunchKernelGGL(k_qr_formy, dim3(yt * yt, a.nt), dim3(256), 0, s, a, yt);
