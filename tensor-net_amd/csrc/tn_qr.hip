@@ -121,6 +121,12 @@ __global__ __launch_bounds__(1024) void k_qr_sort(QrArgs a) {
 }
 
 // ------------------------------------------------------- panel factorisation
+// Columns (below the diagonal) with norm at most kQrNullCol get no
+// reflector: far below f64::EPSILON times the SVD input's norm (about 1),
+// they cannot change a kept singular value, and beta |alpha - beta|^2
+// stays a normal number above it.
+constexpr double kQrNullCol = 1e-90;
+
 // Panel jp (columns g0 = 16 jp .. g0 + 15, rows g0 .. R-1) of every task with
 // C > g0 (R - g0 <= kQrMaxRows).  Wave w holds panel column w in registers,
 // lane l its rows l + 64 m; a reflector's vector goes through LDS.
@@ -128,14 +134,18 @@ __global__ __launch_bounds__(1024) void k_qr_sort(QrArgs a) {
 // Reflector c (zlarfg): alpha = P[c][c], xnorm = |P[c][c+1:]|,
 //   beta = -sign(Re alpha) sqrt(|alpha|^2 + xnorm^2),
 //   tau = (beta - alpha) / beta,  v = P[c][c+1:] / (alpha - beta),  P[c][c] = beta
-// (tau = 0 and nothing changes when xnorm = 0 and Im alpha = 0).  Then every
+// (tau = 0 and nothing changes when xnorm = 0 and Im alpha = 0, or when
+// the column is at most kQrNullCol).  Then every
 // later column k: w = v^H P[k][c:], P[k][c:] -= conj(tau) w v  (H^H applied).
 // One workgroup barrier per reflector; the wave sums run on the DPP network.
+// The waves whose reflectors are done form V^H V for T meanwhile.
 // T (zlarft, forward, columnwise): T[i][i] = tau_i,
 //   T[j][i] = -tau_i sum_{l=j}^{i-1} T[j][l] (V^H V)[l][i]   (j < i).
 __global__ __launch_bounds__(1024) void k_qr_panel(QrArgs a, int jp) {
   constexpr int RPL = kQrMaxRows / 64;  // rows per lane
-  __shared__ double2 P[kQrNb][kQrMaxRows];
+  // reflector c's vector, in buffer c & 1 (a buffer is rewritten two
+  // reflectors later, after every wave has passed the barrier between)
+  __shared__ double2 vbuf[2][kQrMaxRows];
   __shared__ double2 taus[kQrNb];
   __shared__ double2 g[kQrNb][kQrNb + 1];
   const TaskQr q = task_qr(a, blockIdx.x);
@@ -153,24 +163,34 @@ __global__ __launch_bounds__(1024) void k_qr_panel(QrArgs a, int jp) {
       x[m] = r < nr ? src[r] : make_double2(0.0, 0.0);
     }
   }
+  // squared norm of the wave's column below its diagonal, ready when its
+  // reflector is due (wave c + 1 forms it while applying reflector c)
+  double s_own = 0.0;
+  if (wid == 0) {
+    double sl = 0.0;
+#pragma unroll
+    for (int m = 0; m < RPL; ++m)
+      if (lane + 64 * m > 0) sl += cabs2(x[m]);
+    s_own = wave_sum_dpp(sl);
+  }
   for (int c = 0; c < ncol; ++c) {
     if (wid == c) {
-      double s = 0.0;
-#pragma unroll
-      for (int m = 0; m < RPL; ++m)
-        if (lane + 64 * m > c) s += cabs2(x[m]);
-      s = wave_sum_dpp(s);
+      const double s = s_own;
       // row c < 64 sits in lane c, slot 0
       const double2 alpha = make_double2(read_lane(x[0].x, c), read_lane(x[0].y, c));
       double2 tau = make_double2(0.0, 0.0);
-      if (s > 0.0 || alpha.y != 0.0) {
-        const double mag = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + s);
+      const double mag = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + s);
+      // a column below kQrNullCol is numerically null (the input has norm
+      // near 1): H = I keeps the arithmetic below clear of underflow
+      if ((s > 0.0 || alpha.y != 0.0) && mag > kQrNullCol) {
         const double beta = alpha.x >= 0.0 ? -mag : mag;
-        tau = make_double2((beta - alpha.x) / beta, -alpha.y / beta);
-        // 1 / (alpha - beta)
+        // d = alpha - beta; one division gives 1 / beta and 1 / |d|^2
         const double2 d = make_double2(alpha.x - beta, alpha.y);
-        const double id2 = 1.0 / cabs2(d);
-        const double2 sc = make_double2(d.x * id2, -d.y * id2);
+        const double d2 = cabs2(d);
+        const double rr = 1.0 / (beta * d2);
+        const double ibeta = d2 * rr, id2 = beta * rr;
+        tau = make_double2((beta - alpha.x) * ibeta, -alpha.y * ibeta);
+        const double2 sc = make_double2(d.x * id2, -d.y * id2);  // 1 / d
 #pragma unroll
         for (int m = 0; m < RPL; ++m)
           if (lane + 64 * m > c) x[m] = cmul(x[m], sc);
@@ -178,7 +198,7 @@ __global__ __launch_bounds__(1024) void k_qr_panel(QrArgs a, int jp) {
       }
 #pragma unroll
       for (int m = 0; m < RPL; ++m)
-        if (lane + 64 * m < nr) P[c][lane + 64 * m] = x[m];
+        if (lane + 64 * m < nr) vbuf[c & 1][lane + 64 * m] = x[m];
       if (lane == 0) taus[c] = tau;
     }
     __syncthreads();
@@ -189,47 +209,41 @@ __global__ __launch_bounds__(1024) void k_qr_panel(QrArgs a, int jp) {
 #pragma unroll
       for (int m = 0; m < RPL; ++m) {
         const int r = lane + 64 * m;
-        v[m] = (r > c && r < nr) ? P[c][r] : make_double2(r == c ? 1.0 : 0.0, 0.0);
+        v[m] = (r > c && r < nr) ? vbuf[c & 1][r] : make_double2(r == c ? 1.0 : 0.0, 0.0);
         w = cadd(w, cmulc(v[m], x[m]));
       }
       w = wave_csum_dpp(w);
       const double2 f = cmulc(tau, w);  // conj(tau) w
+      double sl = 0.0;
 #pragma unroll
       for (int m = 0; m < RPL; ++m) {
         const double2 fv = cmul(f, v[m]);
         x[m] = make_double2(x[m].x - fv.x, x[m].y - fv.y);
+        if (lane + 64 * m > c + 1) sl += cabs2(x[m]);
       }
+      if (wid == c + 1) s_own = wave_sum_dpp(sl);
+    } else if (wid < c) {
+      // waves done with their reflector: (V^H V)[wid][c] over rows r >= c
+      // (v_c[c] = 1, v_c[r < c] = 0), off the reflector chain
+      double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+      for (int m = 0; m < RPL; ++m) {
+        const int r = lane + 64 * m;
+        if (r >= c && r < nr) {
+          const double2 vc = r == c ? make_double2(1.0, 0.0) : vbuf[c & 1][r];
+          acc = cadd(acc, cmulc(x[m], vc));
+        }
+      }
+      acc = wave_csum_dpp(acc);
+      if (lane == 0) g[wid][c] = acc;
     }
   }
-  // every column final: to LDS for the T factor, and to E
   if (wid < ncol) {
     double2* dst = q.E + (size_t)(g0 + wid) * a.ldx + g0;
 #pragma unroll
     for (int m = 0; m < RPL; ++m) {
       const int r = lane + 64 * m;
-      if (r < nr) {
-        P[wid][r] = x[m];
-        dst[r] = x[m];
-      }
-    }
-  }
-  __syncthreads();
-  // (V^H V)[j][i] for j < i over rows r >= i (v_i[i] = 1, v_i[r < i] = 0):
-  // wave i, its own column from registers
-  if (wid < ncol) {
-    const int i = wid;
-    for (int j = 0; j < i; ++j) {
-      double2 acc = make_double2(0.0, 0.0);
-#pragma unroll
-      for (int m = 0; m < RPL; ++m) {
-        const int r = lane + 64 * m;
-        if (r >= i && r < nr) {
-          const double2 vi = r == i ? make_double2(1.0, 0.0) : x[m];
-          acc = cadd(acc, cmulc(P[j][r], vi));
-        }
-      }
-      acc = wave_csum_dpp(acc);
-      if (lane == 0) g[j][i] = acc;
+      if (r < nr) dst[r] = x[m];
     }
   }
   __syncthreads();
@@ -516,7 +530,8 @@ __global__ __launch_bounds__(512) void k_qr_final(QrArgs a) {
   }
   __syncthreads();
   const double sg = sm.sig[lane & 15];
-  const double isg = sg > 0.0 ? 1.0 / sg : 0.0;
+  // (columns of U~ that small are dropped by the rank rule; their W stays 0)
+  const double isg = sg > 1e-150 ? 1.0 / sg : 0.0;
 #pragma unroll
   for (int s = 0; s < kQT; ++s) {
     zr[s] *= sg;
