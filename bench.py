@@ -10,10 +10,12 @@ are at chi_max (saturated()).  The metric is two-site updates per second
 (contract -> SVD -> truncate -> split, gates and refactors alike).
 
 The same run also measures, under "mixed_batched", the metric's other side:
-the FP32-contraction batched path at the same N, chi and p (256 trajectories
-in 4 lock-step groups, saturated the same way), its ratio to the FP64
-single-trajectory value of this run, and the entropy error of its first
-trajectories against an FP64 replay of their outcome records.  The CPU
+the batched path at the same N, chi and p (256 trajectories in 4 lock-step
+groups), with the FP32 contraction (saturated the same way) and then in FP64
+throughout on the same circuits and number of warmup layers; each rate's
+ratio to the FP64 single-trajectory value of this run, and the FP32 run's
+entropy error against an FP64 replay of its first trajectories' outcome
+records.  The CPU
 baseline runs the oracle in one process per host core (up to 16), each on
 its own copy of the saturated state, in a pool forked before the GPU is
 touched.
@@ -553,11 +555,11 @@ def main() -> int:
                     help="CPU-baseline processes (0: one per core, at most 16)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true",
-                    help="single: leave out the FP32 batched measurement (mixed_batched)")
+                    help="single: leave out the batched FP32 and FP64 measurement (mixed_batched)")
     ap.add_argument("--mixed-traj", type=int, default=256,
-                    help="single: trajectories of the FP32 batched measurement")
-    ap.add_argument("--mixed-steps", type=int, default=2,
-                    help="single: timed layers of the FP32 batched measurement")
+                    help="single: trajectories of the batched measurement")
+    ap.add_argument("--mixed-steps", type=int, default=1,
+                    help="single: timed layers of each batched measurement")
     ap.add_argument("--workload", default="single",
                     choices=["single", "batched", "mixed", "serving", "gates"])
     ap.add_argument("--traj", type=int, default=0, help="trajectories (batched, mixed)")
@@ -626,8 +628,9 @@ def main() -> int:
 
 
 def run_single(args, pool, workers: int, host: dict) -> dict:
-    """The metric's line: the FP64 single trajectory, then the FP32 batched
-    path of the same N, chi and p (mixed_batched), then the CPU baseline."""
+    """The metric's line: the FP64 single trajectory, then the batched path
+    of the same N, chi and p in FP32 and FP64 (mixed_batched), then the CPU
+    baseline."""
     from tnmps import MPS
     from tnmps.workload import haar_circuit
 
@@ -746,35 +749,50 @@ def run_single(args, pool, workers: int, host: dict) -> dict:
 
 
 def mixed_batched(args, single_value: float) -> dict:
-    """The FP32-contraction batched path at the single line's N, chi and p:
-    --mixed-traj trajectories in 4 lock-step groups, saturated as the single
-    line is, --mixed-steps timed layers; its ratio to this run's FP64
-    single-trajectory value, and the half-chain entropy error of its first
-    trajectories against an FP64 replay of their outcome records."""
+    """The batched path at the single line's N, chi and p: --mixed-traj
+    trajectories in 4 lock-step groups, first with the FP32 contraction
+    (warmed up as the single line is, until saturated), then with FP64
+    throughout on the same circuits and the same number of warmup layers, so
+    both time the same layers; --mixed-steps timed layers each.  Each rate's
+    ratio to this run's FP64 single-trajectory value, and the FP32 run's
+    half-chain entropy error against an FP64 replay of its first
+    trajectories' outcome records."""
     import copy
 
     a = copy.copy(args)
     a.n, a.chi = args.n or 64, args.chi or 256
-    a.traj, a.prec, a.groups = args.mixed_traj, "f32", 4
+    a.traj, a.groups = args.mixed_traj, 4
     a.pgrid = f"{args.p},{args.p},1"
     a.depth = a.n
-    a.warmup, a.steps = 1, args.mixed_steps
-    log(f"mixed batched: {a.traj} FP32 trajectories N={a.n} chi={a.chi} p={args.p}")
-    r = run_batched(a)
-    rep = r["config"]["fp64_replay"]
+    a.steps = args.mixed_steps
+    legs = {}
+    warm = 1
+    for prec in ("f32", "f64"):
+        a.prec, a.warmup = prec, warm
+        log(f"batched leg: {a.traj} trajectories N={a.n} chi={a.chi} p={args.p} {prec}")
+        r = run_batched(a)
+        warm = r["warmup"]
+        leg = {"value": r["value"], "unit": "two-site updates/s",
+               "ratio_vs_fp64_single": round(r["value"] / single_value, 3) if single_value > 0
+               else None,
+               "jacobi_achieved_TFLOPs": r["roofline"]["achieved"],
+               "saturated": r["config"]["saturated"]}
+        if prec == "f32":
+            leg["fp64_replay"] = r["config"]["fp64_replay"]
+            leg["memory_kernels"] = r["memory_kernels"]
+        legs[prec] = leg
+    rep = legs["f32"]["fp64_replay"]
     err = rep["max_half_chain_entropy_abs_err_vs_fp64"] if rep else None
-    ratio = r["value"] / single_value if single_value > 0 else None
+    ratio = legs["f32"]["ratio_vs_fp64_single"]
     return {
-        "value": r["value"], "unit": "two-site updates/s",
-        "workload": r["config"]["workload"] + "; FP32 contraction, FP64 SVD",
-        "trajectories": a.traj, "groups": r["config"]["groups"],
-        "warmup": r["warmup"], "steps": r["steps"], "saturated": r["config"]["saturated"],
-        "ratio_vs_fp64_single": round(ratio, 3) if ratio else None,
-        "fp64_replay": rep,
+        "workload": f"{a.traj} trajectories N={a.n} depth={a.depth} p={args.p} Const({a.chi}), "
+                    f"4 lock-step groups on 4 streams; step = one layer of every trajectory "
+                    f"after {warm} warmup layers (saturated)",
+        "trajectories": a.traj, "groups": 4, "warmup": warm, "steps": a.steps,
+        "f32": legs["f32"], "f64": legs["f64"],
+        "value": legs["f32"]["value"], "ratio_vs_fp64_single": ratio,
         "target": ">= 20x the FP64 single-trajectory value with entropy error <= 1e-6",
         "target_met": bool(ratio and ratio >= 20.0 and err is not None and err <= 1e-6),
-        "jacobi_achieved_TFLOPs": r["roofline"]["achieved"],
-        "memory_kernels": r["memory_kernels"],
     }
 
 

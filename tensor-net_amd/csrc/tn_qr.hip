@@ -17,13 +17,14 @@
 //
 // The QR is blocked Householder in the LAPACK convention (zgeqrf: H_i =
 // I - tau_i v_i v_i^H, Q = H_1 ... H_C; panels of kQrNb columns, Q_panel =
-// I - V T V^H with T upper triangular as zlarft builds it).  Per panel:
-//   k_qr_panel   one workgroup per task factors the panel's columns in LDS
-//                (reflector c: wave c forms it, then every later column's
-//                wave applies it; one workgroup barrier per reflector) and
-//                builds T
-//   k_qr_trail   the trailing columns, 16 per workgroup, on the FP64 matrix
-//                cores: A <- A - V (T^H (V^H A))
+// I - V T V^H with T upper triangular as zlarft builds it).  One launch per
+// panel (k_qr_step): block 0 of every task updates the next panel's columns
+// with the current panel and factors them at once, in registers (reflector
+// c: its column's wave forms it, then every wave applies it to its later
+// columns; one workgroup barrier per reflector), and builds T; the other
+// blocks update the trailing columns, 16 per workgroup, on the FP64 matrix
+// cores: A <- A - V (T^H (V^H A)).  So the next panel's factorisation runs
+// beside the rest of the trailing update.
 // The first panel and trailing update read X through the sort permutation
 // and write the factored matrix E (the V below the diagonal, R on and above
 // it) into their own buffer, so no separate gather pass runs.
@@ -120,160 +121,12 @@ __global__ __launch_bounds__(1024) void k_qr_sort(QrArgs a) {
   for (int i = threadIdx.x; i < q.C; i += blockDim.x) q.perm0[i] = idx[i];
 }
 
-// ------------------------------------------------------- panel factorisation
+// ------------------------------------------------------- null columns
 // Columns (below the diagonal) with norm at most kQrNullCol get no
 // reflector: far below f64::EPSILON times the SVD input's norm (about 1),
 // they cannot change a kept singular value, and beta |alpha - beta|^2
 // stays a normal number above it.
 constexpr double kQrNullCol = 1e-90;
-
-// Panel jp (columns g0 = 16 jp .. g0 + 15, rows g0 .. R-1) of every task with
-// C > g0 (R - g0 <= kQrMaxRows).  Wave w holds panel column w in registers,
-// lane l its rows l + 64 m; a reflector's vector goes through LDS.
-//
-// Reflector c (zlarfg): alpha = P[c][c], xnorm = |P[c][c+1:]|,
-//   beta = -sign(Re alpha) sqrt(|alpha|^2 + xnorm^2),
-//   tau = (beta - alpha) / beta,  v = P[c][c+1:] / (alpha - beta),  P[c][c] = beta
-// (tau = 0 and nothing changes when xnorm = 0 and Im alpha = 0, or when
-// the column is at most kQrNullCol).  Then every
-// later column k: w = v^H P[k][c:], P[k][c:] -= conj(tau) w v  (H^H applied).
-// One workgroup barrier per reflector; the wave sums run on the DPP network.
-// The waves whose reflectors are done form V^H V for T meanwhile.
-// T (zlarft, forward, columnwise): T[i][i] = tau_i,
-//   T[j][i] = -tau_i sum_{l=j}^{i-1} T[j][l] (V^H V)[l][i]   (j < i).
-__global__ __launch_bounds__(1024) void k_qr_panel(QrArgs a, int jp) {
-  constexpr int RPL = kQrMaxRows / 64;  // rows per lane
-  // reflector c's vector, in buffer c & 1 (a buffer is rewritten two
-  // reflectors later, after every wave has passed the barrier between)
-  __shared__ double2 vbuf[2][kQrMaxRows];
-  __shared__ double2 taus[kQrNb];
-  __shared__ double2 g[kQrNb][kQrNb + 1];
-  const TaskQr q = task_qr(a, blockIdx.x);
-  const int g0 = kQrNb * jp;
-  if (g0 >= q.C || q.R > kQrMaxRows) return;  // (the host takes this path for R <= kQrMaxRows)
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  const int nr = q.R - g0;
-  const int ncol = min(kQrNb, q.C - g0);
-  double2 x[RPL];
-  if (wid < ncol) {
-    const double2* src = src_col(a, q, jp, g0 + wid) + g0;
-#pragma unroll
-    for (int m = 0; m < RPL; ++m) {
-      const int r = lane + 64 * m;
-      x[m] = r < nr ? src[r] : make_double2(0.0, 0.0);
-    }
-  }
-  // squared norm of the wave's column below its diagonal, ready when its
-  // reflector is due (wave c + 1 forms it while applying reflector c)
-  double s_own = 0.0;
-  if (wid == 0) {
-    double sl = 0.0;
-#pragma unroll
-    for (int m = 0; m < RPL; ++m)
-      if (lane + 64 * m > 0) sl += cabs2(x[m]);
-    s_own = wave_sum_dpp(sl);
-  }
-  for (int c = 0; c < ncol; ++c) {
-    if (wid == c) {
-      const double s = s_own;
-      // row c < 64 sits in lane c, slot 0
-      const double2 alpha = make_double2(read_lane(x[0].x, c), read_lane(x[0].y, c));
-      double2 tau = make_double2(0.0, 0.0);
-      const double mag = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + s);
-      // a column below kQrNullCol is numerically null (the input has norm
-      // near 1): H = I keeps the arithmetic below clear of underflow
-      if ((s > 0.0 || alpha.y != 0.0) && mag > kQrNullCol) {
-        const double beta = alpha.x >= 0.0 ? -mag : mag;
-        // d = alpha - beta; one division gives 1 / beta and 1 / |d|^2
-        const double2 d = make_double2(alpha.x - beta, alpha.y);
-        const double d2 = cabs2(d);
-        const double rr = 1.0 / (beta * d2);
-        const double ibeta = d2 * rr, id2 = beta * rr;
-        tau = make_double2((beta - alpha.x) * ibeta, -alpha.y * ibeta);
-        const double2 sc = make_double2(d.x * id2, -d.y * id2);  // 1 / d
-#pragma unroll
-        for (int m = 0; m < RPL; ++m)
-          if (lane + 64 * m > c) x[m] = cmul(x[m], sc);
-        if (lane == c) x[0] = make_double2(beta, 0.0);
-      }
-#pragma unroll
-      for (int m = 0; m < RPL; ++m)
-        if (lane + 64 * m < nr) vbuf[c & 1][lane + 64 * m] = x[m];
-      if (lane == 0) taus[c] = tau;
-    }
-    __syncthreads();
-    if (wid > c && wid < ncol) {
-      const double2 tau = taus[c];
-      double2 v[RPL];
-      double2 w = make_double2(0.0, 0.0);
-#pragma unroll
-      for (int m = 0; m < RPL; ++m) {
-        const int r = lane + 64 * m;
-        v[m] = (r > c && r < nr) ? vbuf[c & 1][r] : make_double2(r == c ? 1.0 : 0.0, 0.0);
-        w = cadd(w, cmulc(v[m], x[m]));
-      }
-      w = wave_csum_dpp(w);
-      const double2 f = cmulc(tau, w);  // conj(tau) w
-      double sl = 0.0;
-#pragma unroll
-      for (int m = 0; m < RPL; ++m) {
-        const double2 fv = cmul(f, v[m]);
-        x[m] = make_double2(x[m].x - fv.x, x[m].y - fv.y);
-        if (lane + 64 * m > c + 1) sl += cabs2(x[m]);
-      }
-      if (wid == c + 1) s_own = wave_sum_dpp(sl);
-    } else if (wid < c) {
-      // waves done with their reflector: (V^H V)[wid][c] over rows r >= c
-      // (v_c[c] = 1, v_c[r < c] = 0), off the reflector chain
-      double2 acc = make_double2(0.0, 0.0);
-#pragma unroll
-      for (int m = 0; m < RPL; ++m) {
-        const int r = lane + 64 * m;
-        if (r >= c && r < nr) {
-          const double2 vc = r == c ? make_double2(1.0, 0.0) : vbuf[c & 1][r];
-          acc = cadd(acc, cmulc(x[m], vc));
-        }
-      }
-      acc = wave_csum_dpp(acc);
-      if (lane == 0) g[wid][c] = acc;
-    }
-  }
-  if (wid < ncol) {
-    double2* dst = q.E + (size_t)(g0 + wid) * a.ldx + g0;
-#pragma unroll
-    for (int m = 0; m < RPL; ++m) {
-      const int r = lane + 64 * m;
-      if (r < nr) dst[r] = x[m];
-    }
-  }
-  __syncthreads();
-  // T row j by lane j of wave 0 (a row depends only on its own earlier
-  // entries), written row-major to the task's T slot of the panel
-  if (wid == 0 && lane < kQrNb) {
-    const int j = lane;
-    double2 trow[kQrNb];
-#pragma unroll
-    for (int i = 0; i < kQrNb; ++i) {
-      double2 v = make_double2(0.0, 0.0);
-      if (i < ncol) {
-        if (i == j) {
-          v = taus[i];
-        } else if (i > j) {
-          double2 s = make_double2(0.0, 0.0);
-#pragma unroll
-          for (int l = 0; l < kQrNb; ++l)
-            if (l >= j && l < i) s = cadd(s, cmul(trow[l], g[l][i]));
-          const double2 ts = cmul(taus[i], s);
-          v = make_double2(-ts.x, -ts.y);
-        }
-      }
-      trow[i] = v;
-    }
-    double2* T = q.T + (size_t)jp * kQrNb * kQrNb + j * kQrNb;
-#pragma unroll
-    for (int i = 0; i < kQrNb; ++i) T[i] = trow[i];
-  }
-}
 
 // ------------------------------------------------- one panel on registers
 // The trailing update and the final Q application run one panel's
@@ -399,17 +252,170 @@ __device__ void panel_update(const double2 (&vrow)[kQT][4], const double2* Tp, i
 }
 }  // namespace
 
-// ------------------------------------------------------- trailing update
-// Trailing columns c0 .. c0 + 15 (c0 = 16 (jp + 1) + 16 blockIdx.x) of
-// every task, rows g0 .. R-1:  A <- A - V (T^H (V^H A)).  The tile and the
-// panel's V load at entry (every load in flight at once), the update runs on
-// registers (panel_update), and the tile goes to E.
-__global__ __launch_bounds__(512) void k_qr_trail(QrArgs a, int jp) {
-  __shared__ PanelSmem sm;
-  const TaskQr q = task_qr(a, blockIdx.y);
+// ------------------------------------------- QR step with look-ahead
+// One launch per panel: block 0 of every task updates the next panel's
+// columns with the current panel and factors them at once (so the next
+// panel's factorisation runs beside the rest of the trailing update), the
+// other blocks update the trailing columns beyond.  Launch jp = -1 factors
+// panel 0 only.
+namespace {
+struct FactorSmem {
+  // reflector c's vector, in buffer c & 1 (a buffer is rewritten two
+  // reflectors later, after every wave has passed the barrier between)
+  double2 vbuf[2][kQrMaxRows];
+  double2 taus[kQrNb];
+  double2 g[kQrNb][kQrNb + 1];
+};
+
+constexpr int kRPL = kQrMaxRows / 64;  // rows per lane
+
+// Reflector c from the owning wave's column xc (zlarfg): alpha = x[c],
+// xnorm = |x[c+1:]|, beta = -sign(Re alpha) sqrt(|alpha|^2 + xnorm^2),
+// tau = (beta - alpha) / beta, v = x[c+1:] / (alpha - beta), x[c] = beta
+// (tau = 0 and nothing changes when xnorm = 0 and Im alpha = 0, or when the
+// column is at most kQrNullCol); one division gives 1 / beta and
+// 1 / |alpha - beta|^2.  Posted to fs.vbuf[c & 1] and fs.taus[c].
+__device__ __forceinline__ void reflect(double2 (&xc)[kRPL], double s, int c, int nr,
+                                        FactorSmem& fs) {
+  const int lane = threadIdx.x & 63;
+  const double2 alpha = make_double2(read_lane(xc[0].x, c), read_lane(xc[0].y, c));
+  double2 tau = make_double2(0.0, 0.0);
+  const double mag = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + s);
+  if ((s > 0.0 || alpha.y != 0.0) && mag > kQrNullCol) {
+    const double beta = alpha.x >= 0.0 ? -mag : mag;
+    const double2 d = make_double2(alpha.x - beta, alpha.y);
+    const double d2 = cabs2(d);
+    const double rr = 1.0 / (beta * d2);
+    const double ibeta = d2 * rr, id2 = beta * rr;
+    tau = make_double2((beta - alpha.x) * ibeta, -alpha.y * ibeta);
+    const double2 sc = make_double2(d.x * id2, -d.y * id2);  // 1 / d
+#pragma unroll
+    for (int m = 0; m < kRPL; ++m)
+      if (lane + 64 * m > c) xc[m] = cmul(xc[m], sc);
+    if (lane == c) xc[0] = make_double2(beta, 0.0);
+  }
+#pragma unroll
+  for (int m = 0; m < kRPL; ++m)
+    if (lane + 64 * m < nr) fs.vbuf[c & 1][lane + 64 * m] = xc[m];
+  if (lane == 0) fs.taus[c] = tau;
+}
+
+// Panel jp of task q factored by the 512-thread workgroup (R - 16 jp <=
+// kQrMaxRows rows): wave w holds
+// panel columns w and w + 8 in registers (lane l: rows l + 64 m).  Per
+// reflector c the owning wave forms it; after one barrier every wave forms
+// w = v_c^H x for both its columns (one pass, the two sums interleaved on
+// the DPP network), updates the later columns (x -= conj(tau) w v) and,
+// for columns before c, records (V^H V)[col][c] = conj(w) for T.  The
+// column next in line forms its norm during its update.  T (zlarft,
+// forward, columnwise): T[i][i] = tau_i,
+//   T[j][i] = -tau_i sum_{l=j}^{i-1} T[j][l] (V^H V)[l][i]   (j < i),
+// one row per lane of wave 0 (a row depends only on its own earlier entries).
+__device__ void factor_panel8(const QrArgs& a, const TaskQr& q, int jp, FactorSmem& fs) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const int g0 = kQrNb * jp;
-  const int c0 = g0 + kQrNb + kQrNb * blockIdx.x;
-  if (c0 >= q.C || q.R > kQrMaxRows) return;
+  const int nr = q.R - g0;
+  const int ncol = min(kQrNb, q.C - g0);
+  double2 x[2][kRPL];
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const int col = wid + 8 * j;
+    const double2* src = col < ncol ? src_col(a, q, jp, g0 + col) + g0 : nullptr;
+#pragma unroll
+    for (int m = 0; m < kRPL; ++m) {
+      const int r = lane + 64 * m;
+      x[j][m] = (src && r < nr) ? src[r] : make_double2(0.0, 0.0);
+    }
+  }
+  double s_own[2] = {0.0, 0.0};
+  if (wid == 0) {
+    double sl = 0.0;
+#pragma unroll
+    for (int m = 0; m < kRPL; ++m)
+      if (lane + 64 * m > 0) sl += cabs2(x[0][m]);
+    s_own[0] = wave_sum_dpp(sl);
+  }
+  for (int c = 0; c < ncol; ++c) {
+    if (wid == (c & 7)) {
+      if (c < 8)
+        reflect(x[0], s_own[0], c, nr, fs);
+      else
+        reflect(x[1], s_own[1], c, nr, fs);
+    }
+    __syncthreads();
+    const double2 tau = fs.taus[c];
+    double2 v[kRPL];
+    double2 w0 = make_double2(0.0, 0.0), w1 = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int m = 0; m < kRPL; ++m) {
+      const int r = lane + 64 * m;
+      v[m] = (r > c && r < nr) ? fs.vbuf[c & 1][r] : make_double2(r == c ? 1.0 : 0.0, 0.0);
+      w0 = cadd(w0, cmulc(v[m], x[0][m]));
+      w1 = cadd(w1, cmulc(v[m], x[1][m]));
+    }
+    w0 = wave_csum_dpp(w0);
+    w1 = wave_csum_dpp(w1);
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int col = wid + 8 * j;
+      const double2 w = j == 0 ? w0 : w1;
+      if (col > c && col < ncol) {
+        const double2 f = cmulc(tau, w);  // conj(tau) w
+        double sl = 0.0;
+#pragma unroll
+        for (int m = 0; m < kRPL; ++m) {
+          const double2 fv = cmul(f, v[m]);
+          x[j][m] = make_double2(x[j][m].x - fv.x, x[j][m].y - fv.y);
+          if (lane + 64 * m > c + 1) sl += cabs2(x[j][m]);
+        }
+        if (col == c + 1) s_own[j] = wave_sum_dpp(sl);
+      } else if (col < c && lane == 0) {
+        fs.g[col][c] = cconj(w);  // (V^H V)[col][c] over rows >= c
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const int col = wid + 8 * j;
+    if (col < ncol) {
+      double2* dst = q.E + (size_t)(g0 + col) * a.ldx + g0;
+#pragma unroll
+      for (int m = 0; m < kRPL; ++m) {
+        const int r = lane + 64 * m;
+        if (r < nr) dst[r] = x[j][m];
+      }
+    }
+  }
+  __syncthreads();
+  if (wid == 0 && lane < kQrNb) {
+    const int j = lane;
+    double2 trow[kQrNb];
+#pragma unroll
+    for (int i = 0; i < kQrNb; ++i) {
+      double2 v = make_double2(0.0, 0.0);
+      if (i < ncol) {
+        if (i == j) {
+          v = fs.taus[i];
+        } else if (i > j) {
+          double2 sm = make_double2(0.0, 0.0);
+#pragma unroll
+          for (int l = 0; l < kQrNb; ++l)
+            if (l >= j && l < i) sm = cadd(sm, cmul(trow[l], fs.g[l][i]));
+          const double2 ts = cmul(fs.taus[i], sm);
+          v = make_double2(-ts.x, -ts.y);
+        }
+      }
+      trow[i] = v;
+    }
+    double2* T = q.T + (size_t)jp * kQrNb * kQrNb + j * kQrNb;
+#pragma unroll
+    for (int i = 0; i < kQrNb; ++i) T[i] = trow[i];
+  }
+}
+
+// Columns c0 .. c0 + 15 of task q, rows 16 jp .. R-1: A <- A - V (T^H (V^H A))
+// with panel jp, on registers (panel_update), written to E.
+__device__ void trail_tile(const QrArgs& a, const TaskQr& q, int jp, int c0, PanelSmem& sm) {
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const int col = c0 + (lane & 15);
   const bool colok = col < q.C;
@@ -441,6 +447,27 @@ __global__ __launch_bounds__(512) void k_qr_trail(QrArgs a, int jp) {
       }
     }
   }
+}
+}  // namespace
+
+__global__ __launch_bounds__(512) void k_qr_step(QrArgs a, int jp) {
+  __shared__ PanelSmem sm;
+  __shared__ FactorSmem fs;
+  const TaskQr q = task_qr(a, blockIdx.y);
+  if (q.R > kQrMaxRows) return;  // (the host takes this path for R <= kQrMaxRows)
+  const int np = jp + 1;  // the panel block 0 factors
+  if (blockIdx.x == 0) {
+    if (kQrNb * np >= q.C) return;
+    if (jp >= 0) {
+      trail_tile(a, q, jp, kQrNb * np, sm);
+      __syncthreads();  // the updated columns, in E, are read back below
+    }
+    factor_panel8(a, q, np, fs);
+    return;
+  }
+  const int c0 = kQrNb * (np + blockIdx.x);
+  if (c0 >= q.C) return;
+  trail_tile(a, q, jp, c0, sm);
 }
 
 // ------------------------------------------------------------ Y = R^H
@@ -575,10 +602,11 @@ void launch_qr(const QrArgs& a, int maxR, int maxC, hipStream_t s) {
   const int ctiles = (maxC + kQrNb - 1) / kQrNb;
   hipLaunchKernelGGL(k_qr_norms, dim3(ctiles, a.nt), dim3(256), 0, s, a);
   hipLaunchKernelGGL(k_qr_sort, dim3(a.nt), dim3(1024), 0, s, a);
-  for (int jp = 0; jp < ctiles; ++jp) {
-    hipLaunchKernelGGL(k_qr_panel, dim3(a.nt), dim3(1024), 0, s, a, jp);
-    const int trail = ctiles - jp - 1;
-    if (trail > 0) hipLaunchKernelGGL(k_qr_trail, dim3(trail, a.nt), dim3(512), 0, s, a, jp);
+  // launch jp factors panel jp + 1 (block 0) and updates the columns after
+  // it with panel jp
+  for (int jp = -1; jp + 1 < ctiles; ++jp) {
+    const int rest = jp < 0 ? 0 : ctiles - jp - 2;
+    hipLaunchKernelGGL(k_qr_step, dim3(1 + rest, a.nt), dim3(512), 0, s, a, jp);
   }
   const int yt = (maxC + 31) / 32;
   hipLaunchKernelGGL(k_qr_formy, dim3(yt * yt, a.nt), dim3(256), 0, s, a, yt);
