@@ -252,8 +252,10 @@ struct PerfRec {
   int cap = 0;            // tasks (or sites) h_info holds
   std::vector<char> proj; // measurement launch: the site is projected
   std::vector<char> gate; // has_gate per task
+  std::vector<int> slots; // pool records: task t's info sits at slot slots[t] of h_info
   int kind = 0;           // 0 update batch, 1 measurement launch (ev[0], ev[1];
-                          // ev[2] once the sites' dimensions are on the host)
+                          // ev[2] once the sites' dimensions are on the host),
+                          // 2 one pool sweep (ev[0], ev[1])
   int nt = 0, n_updates = 0, sweeps = 0;
   size_t launches = 0;
   bool pending = false;
@@ -294,6 +296,15 @@ struct tn_engine {
   int qr_min = 32;       // least C for which a batch takes the QR path (TNMPS_QR_MIN)
   int cus = 0;           // compute units of the device
   int list_min = 64;     // tasks from which the step kernel runs over act (TNMPS_LIST_MIN)
+  // SVD task pool (run_pool): handles from which a batch's measurement
+  // schedules run as a pool (TNMPS_POOL_MIN; 0, the default, keeps the
+  // lock-step path: on the measured workloads the two ran at the same rate),
+  // its pinned active list and slot maps and their device copies
+  int pool_min = 0;
+  int pool_cap = 0;
+  int* h_pact = nullptr;
+  int* h_pmap[2] = {};
+  int* d_pmap[2] = {};
   int split_max = kMaxSplit;  // cap on workgroups per block pair (TNMPS_SPLIT_MAX)
   int split_rows = 128;       // least X rows per row-split workgroup (TNMPS_SPLIT_ROWS)
   int bw_env = 0;             // forced Jacobi block width (TNMPS_BW), 0 = automatic
@@ -375,6 +386,12 @@ struct tn_engine {
     meas.release();
     ents.release();
     hfree(d_scalar); d_scalar = nullptr; nscalar = 0;
+    hhfree(h_pact); h_pact = nullptr;
+    for (int i = 0; i < 2; ++i) {
+      hhfree(h_pmap[i]); hfree(d_pmap[i]);
+      h_pmap[i] = nullptr; d_pmap[i] = nullptr;
+    }
+    pool_cap = 0;
     hfree(d_eflags); d_eflags = nullptr;
     hhfree(h_eflags); h_eflags = nullptr;
     hfree(gen_u); hfree(gen_ids); gen_u = nullptr; gen_ids = nullptr; gen_cap = 0;
@@ -412,6 +429,7 @@ struct tn_engine {
     split_max = std::max(1, std::min(kMaxSplit, (int)env_double("TNMPS_SPLIT_MAX", kMaxSplit)));
     split_rows = std::max(64, (int)env_double("TNMPS_SPLIT_ROWS", 128));
     list_min = std::max(1, (int)env_double("TNMPS_LIST_MIN", 64));
+    pool_min = std::max(0, (int)env_double("TNMPS_POOL_MIN", 0));
     size_t gp = 0;
     for (int bw : {kBW, kBWWide}) {
       const size_t nbmax = (size_t)(ldw + bw - 1) / bw;
@@ -463,9 +481,16 @@ struct tn_engine {
     return ms;
   }
 
+  // The event a record's last span ends with.
+  static int last_ev(const PerfRec& r) { return r.kind == 1 ? 2 : r.kind == 2 ? 1 : 5; }
+
   // Add a completed record to the counters.
   void perf_process(PerfRec& r) {
-    if (r.kind == 1) {
+    if (r.kind == 2) {
+      perf.ms_jacobi += span(r, 0, 1);
+      perf.jacobi_launches += r.launches;
+      perf.sweeps += 1;
+    } else if (r.kind == 1) {
       perf.ms_measure += span(r, 0, 1);
       for (int i = 0; i < r.nt; ++i) {
         const double cells = (double)r.h_info[2 * i] * (double)r.h_info[2 * i + 1];
@@ -486,7 +511,7 @@ struct tn_engine {
       for (int t = 0; t < r.nt; ++t) {
         // flops of the SVD posed (R x C; LAPACK-style count of a complex SVD
         // with both factors) plus the contraction or the factor product
-        const int* inf = r.h_info + (size_t)t * kInfoLen;
+        const int* inf = r.h_info + (size_t)(r.slots.empty() ? t : r.slots[t]) * kInfoLen;
         const double m = inf[kInfoM], n = inf[kInfoN], K = inf[kInfoChiM];
         const double M = inf[kInfoR], N = inf[kInfoC], rank = inf[kInfoRank];
         const double fsvd = 4.0 * (4.0 * M * M * N + 8.0 * M * N * N + 9.0 * N * N * N);
@@ -510,7 +535,7 @@ struct tn_engine {
     for (int i = 0; i < kPerfRing; ++i) {
       PerfRec& r = prec[(prec_next + i) % kPerfRing];
       if (!r.pending) continue;
-      const hipEvent_t last = r.ev[r.kind == 1 ? 2 : 5];
+      const hipEvent_t last = r.ev[last_ev(r)];
       if (block) {
         TN_HIP(hipEventSynchronize(last));
       } else if (hipEventQuery(last) != hipSuccess) {
@@ -525,7 +550,7 @@ struct tn_engine {
   tn_status perf_begin(int kind, int nt, PerfRec** out) {
     PerfRec& r = prec[prec_next];
     if (r.pending) {
-      TN_HIP(hipEventSynchronize(r.ev[r.kind == 1 ? 2 : 5]));
+      TN_HIP(hipEventSynchronize(r.ev[last_ev(r)]));
       perf_process(r);
     }
     if (nt > r.cap) {
@@ -537,6 +562,9 @@ struct tn_engine {
     }
     r.kind = kind;
     r.nt = nt;
+    r.slots.clear();
+    r.launches = 0;
+    r.n_updates = r.sweeps = 0;
     prec_next = (prec_next + 1) % kPerfRing;
     *out = &r;
     return TN_OK;
@@ -566,6 +594,25 @@ struct tn_engine {
     TN_HIP(hipMalloc(&qrPerm, sizeof(int) * ldw * ns));
     slots = ns;
     return upd.ensure(ns, stream);
+  }
+
+  tn_status ensure_pool(int need) {
+    if (need <= pool_cap) return TN_OK;
+    TN_HIP(hipStreamSynchronize(stream));
+    hhfree(h_pact);
+    h_pact = nullptr;
+    for (int i = 0; i < 2; ++i) {
+      hhfree(h_pmap[i]); hfree(d_pmap[i]);
+      h_pmap[i] = nullptr; d_pmap[i] = nullptr;
+    }
+    pool_cap = 0;
+    TN_HIP(hipHostMalloc(&h_pact, sizeof(int) * (need + 1), hipHostMallocDefault));
+    for (int i = 0; i < 2; ++i) {
+      TN_HIP(hipHostMalloc(&h_pmap[i], sizeof(int) * need, hipHostMallocDefault));
+      TN_HIP(hipMalloc(&d_pmap[i], sizeof(int) * need));
+    }
+    pool_cap = need;
+    return TN_OK;
   }
 
   tn_status ensure_mdims(int need) {
@@ -887,6 +934,95 @@ tn_status jacobi_sweeps(tn_engine* e, const Workspace& w, int nt, size_t maxR, s
   return TN_OK;
 }
 
+// The host's bounds on one task's SVD: columns C, rows R, and the outer
+// bonds chiL, chiR.
+struct TaskBounds {
+  size_t C, R, chiL, chiR;
+};
+
+// Descriptor of update q in workspace slot t of w, and its bounds.
+TaskBounds make_desc(const tn_engine* e, const Workspace& w, const Task& q, int t, UpdDesc& d) {
+  tn_mps* h = q.h;
+  const size_t k = q.k;
+  std::memset(&d, 0, sizeof(d));
+  d.mode = q.has_gate ? 0 : (q.pre ? q.pre : factor_mode(e, h, k));
+  d.gk = h->gam + h->goff[k];
+  d.gk1 = h->gam + h->goff[k + 1];
+  d.lamL = k == 0 ? nullptr : h->lam + h->loff[k - 1];
+  d.lamM = h->lam + h->loff[k];
+  d.lamR = (k + 2 >= h->n) ? nullptr : h->lam + h->loff[k + 1];
+  d.dims = h->dims + k;
+  d.X = w.X + (size_t)t * w.ldx * w.ldw;
+  d.W = w.W + (size_t)t * w.ldw * w.ldw;
+  d.sraw = w.sraw + (size_t)t * w.ldw;
+  d.sval = w.sval + (size_t)t * w.ldw;
+  d.fac = w.fac + (size_t)t * w.ldw;
+  d.perm = w.perm + (size_t)t * w.ldw;
+  d.part = w.part + (size_t)t * 2 * kNPart;
+  d.offs = w.offs + t;
+  d.info = w.info + (size_t)t * kInfoLen;
+  // the factored product goes to the second half of X's columns; the SVD
+  // of a factored update uses C = K <= chi_max = ldw / 2 of them
+  d.T = d.X + (size_t)(w.ldw / 2) * w.ldx;
+  d.ldx = w.ldx;
+  d.ldw = w.ldw;
+  d.has_gate = q.has_gate ? 1 : 0;
+  d.prec = h->prec;
+  d.tkind = h->trunc.kind;
+  d.tchi = (int)std::min<size_t>(h->trunc.chi, 1u << 30);
+  d.cap_rank = (int)h->bmax[k];
+  d.teps = h->trunc.eps;
+  if (q.has_gate && !q.gdev)
+    for (int i = 0; i < 16; ++i) d.gate[i] = make_double2(q.gate[2 * i], q.gate[2 * i + 1]);
+  d.gsrc = q.has_gate ? q.gdev : nullptr;
+  d.hflags = h->dflags();
+  d.eflags = e->d_eflags;
+  TaskBounds b;
+  b.chiL = h->uL(k);
+  b.chiR = h->uR(k + 1);
+  // a factored request has C = K when K < min(m, n), else the full
+  // C = min(m, n) <= K: either way C <= min(ub[k], m, n)
+  b.C = std::min(2 * b.chiL, 2 * b.chiR);
+  if (d.mode != 0) b.C = std::min(b.C, h->ub[k]);
+  b.R = std::max(2 * b.chiL, 2 * b.chiR);
+  return b;
+}
+
+// The host's view of a handle after update q: the bound on bond k and the
+// freshness of sites k, k + 1.
+void after_update(const Task& q) {
+  tn_mps* h = q.h;
+  const size_t k = q.k;
+  const size_t chiL = h->uL(k), chiR = h->uR(k + 1);
+  // a refactor's rank is at most min(K, m, n) <= ub[k]; a gate's min(m, n)
+  const size_t b = std::min({2 * chiL, 2 * chiR, rank_bound(h), h->bmax[k]});
+  h->ub[k] = q.has_gate ? b : std::min(b, h->ub[k]);
+  // the prepass makes only its decomposed factor's site fresh
+  h->fresh[k] = (q.pre == 1) ? 0 : kFreshLeft;
+  h->fresh[k + 1] = (q.pre == 2) ? 0 : kFreshRight;
+}
+
+// QR preconditioner arguments for nt tasks of workspace w (task i in slot
+// map[i], or slot i without a map).
+QrArgs qr_args(const Workspace& w, int nt, const int* map) {
+  QrArgs qa{};
+  qa.X = w.X;
+  qa.W = w.W;
+  qa.E = w.qrE;
+  qa.T = w.qrT;
+  qa.cn = w.sraw;
+  qa.perm0 = w.qrPerm;
+  qa.info = w.info;
+  qa.map = map;
+  qa.xslot = (size_t)w.ldx * w.ldw;
+  qa.wslot = (size_t)w.ldw * w.ldw;
+  qa.tslot = w.qr_tslot;
+  qa.ldx = w.ldx;
+  qa.ldw = w.ldw;
+  qa.nt = nt;
+  return qa;
+}
+
 // Enqueue up to max_slots independent updates.  Within one handle, bonds
 // must pairwise differ by at least 2 so that no update reads what another
 // writes; different handles are independent.  With `ws` the batch runs in
@@ -908,55 +1044,14 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt, const Workspace* ws
   bool any_factored = false;
   int mode0 = 0;  // first task's mode, for the sweep trace
   for (int t = 0; t < nt; ++t) {
-    const Task& q = tasks[t];
-    tn_mps* h = q.h;
-    const size_t k = q.k;
-    UpdDesc& d = hd[t];
-    std::memset(&d, 0, sizeof(d));
-    d.mode = q.has_gate ? 0 : (q.pre ? q.pre : factor_mode(e, h, k));
-    if (t == 0) mode0 = d.mode;
-    any_factored = any_factored || d.mode != 0;
-    d.gk = h->gam + h->goff[k];
-    d.gk1 = h->gam + h->goff[k + 1];
-    d.lamL = k == 0 ? nullptr : h->lam + h->loff[k - 1];
-    d.lamM = h->lam + h->loff[k];
-    d.lamR = (k + 2 >= h->n) ? nullptr : h->lam + h->loff[k + 1];
-    d.dims = h->dims + k;
-    d.X = w.X + (size_t)t * w.ldx * w.ldw;
-    d.W = w.W + (size_t)t * w.ldw * w.ldw;
-    d.sraw = w.sraw + (size_t)t * w.ldw;
-    d.sval = w.sval + (size_t)t * w.ldw;
-    d.fac = w.fac + (size_t)t * w.ldw;
-    d.perm = w.perm + (size_t)t * w.ldw;
-    d.part = w.part + (size_t)t * 2 * kNPart;
-    d.offs = w.offs + t;
-    d.info = w.info + (size_t)t * kInfoLen;
-    // the factored product goes to the second half of X's columns; the SVD
-    // of a factored update uses C = K <= chi_max = ldw / 2 of them
-    d.T = d.X + (size_t)(w.ldw / 2) * w.ldx;
-    d.ldx = w.ldx;
-    d.ldw = w.ldw;
-    d.has_gate = q.has_gate ? 1 : 0;
-    d.prec = h->prec;
-    d.tkind = h->trunc.kind;
-    d.tchi = (int)std::min<size_t>(h->trunc.chi, 1u << 30);
-    d.cap_rank = (int)h->bmax[k];
-    d.teps = h->trunc.eps;
-    if (q.has_gate && !q.gdev)
-      for (int i = 0; i < 16; ++i) d.gate[i] = make_double2(q.gate[2 * i], q.gate[2 * i + 1]);
-    d.gsrc = q.has_gate ? q.gdev : nullptr;
-    d.hflags = h->dflags();
-    d.eflags = e->d_eflags;
-    const size_t chiL = h->uL(k), chiR = h->uR(k + 1);
-    // a factored request has C = K when K < min(m, n), else the full
-    // C = min(m, n) <= K: either way C <= min(ub[k], m, n)
-    size_t C = std::min(2 * chiL, 2 * chiR);
-    if (d.mode != 0) C = std::min(C, h->ub[k]);
-    maxC = std::max(maxC, C);
-    maxR = std::max(maxR, std::max(2 * chiL, 2 * chiR));
-    maxCC = std::max(maxCC, C * C);
-    maxChiL = std::max(maxChiL, chiL);
-    maxChiR = std::max(maxChiR, chiR);
+    const TaskBounds tb = make_desc(e, w, tasks[t], t, hd[t]);
+    if (t == 0) mode0 = hd[t].mode;
+    any_factored = any_factored || hd[t].mode != 0;
+    maxC = std::max(maxC, tb.C);
+    maxR = std::max(maxR, tb.R);
+    maxCC = std::max(maxCC, tb.C * tb.C);
+    maxChiL = std::max(maxChiL, tb.chiL);
+    maxChiR = std::max(maxChiR, tb.chiR);
   }
   const tn_tune& tu = e->tune_for(maxC);
   const size_t tv = (size_t)tu.theta_tile / 2;
@@ -983,21 +1078,8 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt, const Workspace* ws
   if (pr) TN_HIP(hipEventRecord(pr->ev[8], e->stream));
   // QR preconditioner: the Jacobi sweeps then run on the C x C Y = R^H
   const bool use_qr = tu.qr && w.qrE && maxR <= (size_t)kQrMaxRows && maxC >= (size_t)e->qr_min;
-  QrArgs qa{};
+  const QrArgs qa = qr_args(w, nt, nullptr);
   if (use_qr) {
-    qa.X = w.X;
-    qa.W = w.W;
-    qa.E = w.qrE;
-    qa.T = w.qrT;
-    qa.cn = w.sraw;
-    qa.perm0 = w.qrPerm;
-    qa.info = w.info;
-    qa.xslot = (size_t)w.ldx * w.ldw;
-    qa.wslot = (size_t)w.ldw * w.ldw;
-    qa.tslot = w.qr_tslot;
-    qa.ldx = w.ldx;
-    qa.ldw = w.ldw;
-    qa.nt = nt;
     launch_qr(qa, (int)maxR, (int)maxC, e->stream);
     e->perf.qr_tasks += (size_t)nt;
   }
@@ -1047,18 +1129,7 @@ tn_status run_chunk(tn_engine* e, const Task* tasks, int nt, const Workspace* ws
     pr->pending = true;
     TN_TRY(e->perf_drain(false));
   }
-  for (int t = 0; t < nt; ++t) {
-    const Task& q = tasks[t];
-    tn_mps* h = q.h;
-    const size_t k = q.k;
-    const size_t chiL = h->uL(k), chiR = h->uR(k + 1);
-    // a refactor's rank is at most min(K, m, n) <= ub[k]; a gate's min(m, n)
-    const size_t b = std::min({2 * chiL, 2 * chiR, rank_bound(h), h->bmax[k]});
-    h->ub[k] = q.has_gate ? b : std::min(b, h->ub[k]);
-    // the prepass makes only its decomposed factor's site fresh
-    h->fresh[k] = (q.pre == 1) ? 0 : kFreshLeft;
-    h->fresh[k + 1] = (q.pre == 2) ? 0 : kFreshRight;
-  }
+  for (int t = 0; t < nt; ++t) after_update(tasks[t]);
   return TN_OK;
 }
 
@@ -1328,9 +1399,298 @@ tn_status exec_gates(tn_engine* e, const std::vector<tn_mps*>& hs,
   return TN_OK;
 }
 
-// Advance several handles through their measurement schedules in lock step.
+// ---- SVD task pool ---------------------------------------------------------
+// The measurement schedules of many handles as a pool of SVD tasks instead
+// of lock-step batches.  A handle has at most one refactor in flight (its
+// schedule is sequential); every pool iteration runs one Jacobi sweep over
+// all tasks in flight; a task whose sweep stayed at or below its tolerance
+// is finished at once, and its handle's next operations (a measurement, the
+// next refactor or that refactor's gauge prepass) start in the freed slot on
+// the next iteration.  A lock-step batch runs until its slowest task has
+// converged, so its last sweeps hold a few tasks; the pool keeps its slots
+// filled.  Each task's arithmetic is the lock-step path's but for the
+// round-robin block count (from the widest task in flight) and the launch
+// shape (one workgroup per block pair), so results agree with solo runs to
+// rounding.  The per-sweep maxima live in row 0 of the workspace's offs
+// (slot t at offs[t]), cleared before every sweep.
+struct PoolSlot {
+  Task task{};
+  int handle = -1;   // index into the pool's handles; -1: free
+  int sweeps = 0;
+  bool qr = false;
+  double tol = 0.0;
+  TaskBounds b{};
+  UpdDesc desc{};
+};
+
+// Contraction (or factored fill), QR preconditioner and W = I for newly
+// admitted tasks (slots adm).
+tn_status pool_admit(tn_engine* e, const Workspace& w, std::vector<PoolSlot>& sl,
+                     const std::vector<int>& adm) {
+  const int n = (int)adm.size();
+  UpdDesc* hd = nullptr;
+  int rs = 0;
+  TN_TRY(e->upd.next(&hd, &rs));
+  size_t maxC = 1, maxR = 1, maxCC = 1, maxChiL = 1, maxChiR = 1;
+  bool any_factored = false;
+  for (int j = 0; j < n; ++j) {
+    PoolSlot& p = sl[adm[j]];
+    p.b = make_desc(e, w, p.task, adm[j], hd[j]);
+    p.desc = hd[j];
+    p.tol = e->tol.for_prec(p.task.h->prec);
+    any_factored = any_factored || hd[j].mode != 0;
+    maxC = std::max(maxC, p.b.C);
+    maxR = std::max(maxR, p.b.R);
+    maxCC = std::max(maxCC, p.b.C * p.b.C);
+    maxChiL = std::max(maxChiL, p.b.chiL);
+    maxChiR = std::max(maxChiR, p.b.chiR);
+    e->h_pmap[0][j] = adm[j];
+  }
+  TN_TRY(e->upd.upload(rs, n, e->stream));
+  const UpdDesc* dd = e->upd.d[rs];
+  const tn_tune& tu = e->tune_for(maxC);
+  const bool use_qr = tu.qr && maxR <= (size_t)kQrMaxRows && maxC >= (size_t)e->qr_min;
+  for (int s : adm) sl[s].qr = use_qr;
+  PerfRec* pr = nullptr;
+  if (e->timing) {
+    TN_TRY(e->perf_begin(0, 0, &pr));
+    TN_HIP(hipEventRecord(pr->ev[0], e->stream));
+  }
+  const size_t tv = (size_t)tu.theta_tile / 2;
+  const size_t maxTiles =
+      std::max<size_t>(1, ((maxChiL + tv - 1) / tv) * ((maxChiR + tv - 1) / tv));
+  const int gx_theta = (int)std::min<size_t>(std::max<size_t>(maxTiles, any_factored ? 256 : 1),
+                                             1024);
+  const int gx_w = (int)std::min<size_t>((maxCC + 255) / 256, 1024);
+  launch_theta(dd, n, tu.theta_tile, gx_theta, e->stream);
+  if (pr) TN_HIP(hipEventRecord(pr->ev[8], e->stream));
+  if (use_qr) {
+    TN_HIP(hipMemcpyAsync(e->d_pmap[0], e->h_pmap[0], sizeof(int) * n, hipMemcpyHostToDevice,
+                          e->stream));
+    launch_qr(qr_args(w, n, e->d_pmap[0]), (int)maxR, (int)maxC, e->stream);
+    e->perf.qr_tasks += (size_t)n;
+  }
+  if (pr) TN_HIP(hipEventRecord(pr->ev[9], e->stream));
+  launch_winit(dd, n, gx_w, e->stream);
+  TN_HIP(hipGetLastError());
+  if (pr) {
+    for (int i : {1, 2, 3, 4, 6, 7, 5}) TN_HIP(hipEventRecord(pr->ev[i], e->stream));
+    pr->pending = true;
+    TN_TRY(e->perf_drain(false));
+  }
+  return TN_OK;
+}
+
+// One Jacobi sweep over the tasks in slots act (ascending); their maxima
+// come back in w.h_offs[slot].
+tn_status pool_sweep(tn_engine* e, const Workspace& w, const std::vector<PoolSlot>& sl,
+                     const std::vector<int>& act, int cap) {
+  const int n = (int)act.size();
+  size_t maxC = 1, maxRJ = 1;
+  e->h_pact[0] = n;
+  for (int j = 0; j < n; ++j) {
+    const PoolSlot& p = sl[act[j]];
+    e->h_pact[1 + j] = act[j];
+    maxC = std::max(maxC, p.b.C);
+    maxRJ = std::max(maxRJ, p.qr ? p.b.C : p.b.R);
+  }
+  TN_HIP(hipMemcpyAsync(w.act, e->h_pact, sizeof(int) * (n + 1), hipMemcpyHostToDevice,
+                        e->stream));
+  TN_HIP(hipMemsetAsync(w.offs, 0, sizeof(double) * cap, e->stream));
+  const int nb = (int)((maxC + kBW - 1) / kBW);
+  const int nbe = nb + (nb & 1);
+  JacobiArgs ja{};
+  ja.X = w.X;
+  ja.W = w.W;
+  ja.gpart = w.gpart;
+  ja.offs = w.offs;
+  ja.info = w.info;
+  ja.act = w.act;
+  ja.xslot = (size_t)w.ldx * w.ldw;
+  ja.wslot = (size_t)w.ldw * w.ldw;
+  ja.gslot = w.gpart_per;
+  ja.ldx = w.ldx;
+  ja.ldw = w.ldw;
+  ja.nt = cap;
+  ja.nbe = nbe;
+  ja.S = 1;
+  ja.Rh = (int)maxRJ;
+  ja.Ch = (int)maxC;
+  ja.tol_rot = e->tol.tol_rot;
+  ja.conv_tol = 0.0;  // (sweep index 0: no task counts as converged in the kernels)
+  ja.tiny2 = e->tol.tiny_norm * e->tol.tiny_norm;
+  PerfRec* pr = nullptr;
+  if (e->timing) {
+    TN_TRY(e->perf_begin(2, 0, &pr));
+    TN_HIP(hipEventRecord(pr->ev[0], e->stream));
+  }
+  const auto h0 = std::chrono::steady_clock::now();
+  for (int s = 0; s < nbe - 1; ++s) launch_jacobi_step(ja, nbe / 2, kBW, 0, s, n, e->stream);
+  TN_HIP(hipGetLastError());
+  e->perf_host_launch_ms +=
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count();
+  if (pr) {
+    TN_HIP(hipEventRecord(pr->ev[1], e->stream));
+    pr->launches = (size_t)(nbe - 1);
+    pr->pending = true;
+  }
+  TN_HIP(hipMemcpyAsync(w.h_offs, w.offs, sizeof(double) * cap, hipMemcpyDeviceToHost,
+                        e->stream));
+  TN_HIP(hipEventRecord(e->sev[0], e->stream));
+  TN_HIP(hipEventSynchronize(e->sev[0]));
+  e->sweeps += 1;
+  if (pr) TN_TRY(e->perf_drain(false));
+  return TN_OK;
+}
+
+// Singular values, rank rule, the QR's final step, factor products, split
+// and renormalisation for the finished tasks in slots ret.
+tn_status pool_retire(tn_engine* e, const Workspace& w, std::vector<PoolSlot>& sl,
+                      const std::vector<int>& ret, int cap) {
+  const int n = (int)ret.size();
+  UpdDesc* hd = nullptr;
+  int rs = 0;
+  TN_TRY(e->upd.next(&hd, &rs));
+  size_t maxC = 1, maxR = 1;
+  int nq = 0, maxsw = 0, n_updates = 0;
+  bool any_factored = false;
+  for (int j = 0; j < n; ++j) {
+    const PoolSlot& p = sl[ret[j]];
+    hd[j] = p.desc;
+    maxC = std::max(maxC, p.b.C);
+    maxR = std::max(maxR, p.b.R);
+    maxsw = std::max(maxsw, p.sweeps);
+    any_factored = any_factored || p.desc.mode != 0;
+    n_updates += p.task.pre ? 0 : 1;
+    if (p.qr) e->h_pmap[1][nq++] = ret[j];
+  }
+  TN_TRY(e->upd.upload(rs, n, e->stream));
+  const UpdDesc* dd = e->upd.d[rs];
+  PerfRec* pr = nullptr;
+  if (e->timing) {
+    TN_TRY(e->perf_begin(0, cap, &pr));
+    for (int i : {0, 8, 9, 1, 2, 3}) TN_HIP(hipEventRecord(pr->ev[i], e->stream));
+  }
+  if (nq) {
+    TN_HIP(hipMemcpyAsync(e->d_pmap[1], e->h_pmap[1], sizeof(int) * nq, hipMemcpyHostToDevice,
+                          e->stream));
+    launch_qr_final(qr_args(w, nq, e->d_pmap[1]), (int)maxC, e->stream);
+  }
+  if (pr) TN_HIP(hipEventRecord(pr->ev[4], e->stream));
+  launch_finish(dd, n, maxsw, e->stream);
+  if (any_factored) launch_fmul(dd, n, (int)maxR, (int)maxC, e->stream);
+  if (pr) TN_HIP(hipEventRecord(pr->ev[6], e->stream));
+  launch_split(dd, n, e->stream);
+  if (pr) TN_HIP(hipEventRecord(pr->ev[7], e->stream));
+  launch_renorm(dd, n, e->stream);
+  TN_HIP(hipGetLastError());
+  e->updates += n_updates;
+  if (pr) {
+    TN_HIP(hipEventRecord(pr->ev[5], e->stream));
+    TN_HIP(hipMemcpyAsync(pr->h_info, w.info, sizeof(int) * kInfoLen * cap,
+                          hipMemcpyDeviceToHost, e->stream));
+    pr->nt = n;
+    pr->slots = ret;
+    pr->gate.assign(n, 0);
+    pr->n_updates = n_updates;
+    pr->pending = true;
+    TN_TRY(e->perf_drain(false));
+  }
+  for (int s : ret) after_update(sl[s].task);
+  return TN_OK;
+}
+
+tn_status run_pool(tn_engine* e, const std::vector<tn_mps*>& hs,
+                   const std::vector<std::vector<Op>>& ops) {
+  const int H = (int)hs.size();
+  const int cap = std::min(H, e->max_slots);
+  TN_TRY(e->ensure_slots(cap));
+  TN_TRY(e->ensure_pool(cap));
+  const Workspace w = engine_workspace(e);
+  std::vector<size_t> pos(H, 0);
+  std::vector<int> busy(H, -1);
+  std::vector<char> pre_done(H, 0);  // the gauge prepass of the refactor at pos has run
+  std::vector<PoolSlot> sl(cap);
+  std::vector<int> freeq;
+  for (int s = cap - 1; s >= 0; --s) freeq.push_back(s);
+  int inflight = 0;
+  for (;;) {
+    // idle handles: a measurement due next runs first (it clears the
+    // freshness its site had, which the next refactor's mode reads) ...
+    std::vector<MTask> ms;
+    for (int i = 0; i < H; ++i) {
+      if (busy[i] >= 0 || pos[i] >= ops[i].size() || !ops[i][pos[i]].meas) continue;
+      const Op& o = ops[i][pos[i]];
+      ms.push_back({hs[i], o.k, o.forced, o.r, o.slot});
+      ++pos[i];
+    }
+    TN_TRY(run_measures(e, ms));
+    // ... then a refactor due next (or its gauge prepass) takes a free slot
+    std::vector<int> adm;
+    bool pending_ops = false;
+    for (int i = 0; i < H; ++i) {
+      if (busy[i] >= 0 || pos[i] >= ops[i].size()) continue;
+      pending_ops = true;
+      if (ops[i][pos[i]].meas || freeq.empty()) continue;  // next iteration
+      Task t{};
+      t.h = hs[i];
+      t.k = ops[i][pos[i]].k;
+      t.has_gate = false;
+      t.pre = pre_done[i] ? 0 : prepass_mode(e, hs[i], t.k);
+      const int s = freeq.back();
+      freeq.pop_back();
+      sl[s].task = t;
+      sl[s].handle = i;
+      sl[s].sweeps = 0;
+      busy[i] = s;
+      adm.push_back(s);
+    }
+    if (!adm.empty()) TN_TRY(pool_admit(e, w, sl, adm));
+    inflight += (int)adm.size();
+    if (inflight == 0) {
+      if (!pending_ops) break;
+      continue;
+    }
+    std::vector<int> act;
+    for (int s = 0; s < cap; ++s)
+      if (sl[s].handle >= 0) act.push_back(s);
+    TN_TRY(pool_sweep(e, w, sl, act, cap));
+    std::vector<int> ret;
+    for (int s : act) {
+      PoolSlot& p = sl[s];
+      ++p.sweeps;
+      if (!(w.h_offs[s] <= p.tol)) {
+        if (p.sweeps < e->max_sweeps) continue;
+        // the sweep limit: written as it stands and reported
+        p.task.h->flags |= kFlagNoConv;
+        e->host_flags |= kFlagNoConv;
+      }
+      ret.push_back(s);
+    }
+    if (!ret.empty()) TN_TRY(pool_retire(e, w, sl, ret, cap));
+    for (int s : ret) {
+      const int i = sl[s].handle;
+      if (sl[s].task.pre) {
+        pre_done[i] = 1;
+      } else {
+        pre_done[i] = 0;
+        ++pos[i];
+      }
+      busy[i] = -1;
+      sl[s].handle = -1;
+      freeq.push_back(s);
+      --inflight;
+    }
+  }
+  return TN_OK;
+}
+
+// Advance several handles through their measurement schedules: as an SVD
+// task pool from pool_min handles on (when pool_min is set), else in lock
+// step.
 tn_status exec_ops(tn_engine* e, const std::vector<tn_mps*>& hs,
                    const std::vector<std::vector<Op>>& ops) {
+  if (e->pool_min >= 2 && (int)hs.size() >= e->pool_min) return run_pool(e, hs, ops);
   size_t depth = 0;
   for (const auto& o : ops) depth = std::max(depth, o.size());
   for (size_t j = 0; j < depth; ++j) {

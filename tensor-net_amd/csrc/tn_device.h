@@ -109,9 +109,9 @@ struct JacobiArgs {
   double tiny2;          // columns with squared norm below take no part
 };
 
-// The QR preconditioner of one launch batch (tn_qr.hip): task t's X and W
-// as in JacobiArgs, its factored copy E (same shape as X), panel T factors
-// and sort order.
+// The QR preconditioner of one launch batch (tn_qr.hip): the X and W of the
+// task in workspace slot t as in JacobiArgs, its factored copy E (same shape
+// as X), panel T factors and sort order.
 struct QrArgs {
   double2* X;            // X + t * xslot (ldx rows): theta; Y = R^H; finally U Sigma
   double2* W;            // W + t * wslot (ldw rows)
@@ -120,8 +120,9 @@ struct QrArgs {
   double* cn;            // squared column norms, ldw per task
   int* perm0;            // columns by decreasing norm, ldw per task
   int* info;             // kInfoLen ints per task
+  const int* map;        // or null: launch task i works in workspace slot map[i] (else slot i)
   size_t xslot, wslot, tslot;
-  int ldx, ldw, nt;
+  int ldx, ldw, nt;      // nt: tasks in the launch
 };
 
 // One step of the state-vector factorization (mps.rs:448-508 factorize).

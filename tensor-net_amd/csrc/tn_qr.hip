@@ -63,7 +63,11 @@ struct TaskQr {
   double* cn;
 };
 
-__device__ __forceinline__ TaskQr task_qr(const QrArgs& a, int t) {
+// Workspace slot of launch task i (the slot map, or i itself).
+__device__ __forceinline__ int qr_slot(const QrArgs& a, int i) { return a.map ? a.map[i] : i; }
+
+__device__ __forceinline__ TaskQr task_qr(const QrArgs& a, int i) {
+  const int t = qr_slot(a, i);
   const int* inf = a.info + (size_t)t * kInfoLen;
   TaskQr q;
   q.R = inf[kInfoR];
@@ -479,7 +483,8 @@ __global__ __launch_bounds__(256) void k_qr_formy(QrArgs a, int ntile) {
   __shared__ double2 tile[32][33];
   const TaskQr q = task_qr(a, blockIdx.y);
   const int bi = blockIdx.x % ntile, bj = blockIdx.x / ntile;
-  if (blockIdx.x == 0 && threadIdx.x == 0) a.info[(size_t)blockIdx.y * kInfoLen + kInfoRJ] = q.C;
+  if (blockIdx.x == 0 && threadIdx.x == 0)
+    a.info[(size_t)qr_slot(a, blockIdx.y) * kInfoLen + kInfoRJ] = q.C;
   const int i0 = 32 * bi, j0 = 32 * bj;  // R rows i, columns j
   if (i0 >= q.C || j0 >= q.C) return;
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
