@@ -827,7 +827,10 @@ tn_status jacobi_sweeps(tn_engine* e, const Workspace& w, int nt, size_t maxR, s
   // One launch per round-robin step (two when the rows are split); the host
   // checks convergence per sweep.  Few pairs (a lone large SVD) leave most
   // CUs idle: split each pair's rows over S workgroups (gram kernel, then
-  // solve + apply kernel).
+  // solve + apply kernel).  The split takes two launches because every
+  // workgroup's Gram matrix needs all of the pair's rows, which the other
+  // workgroups of the pair overwrite in their apply: a kernel boundary
+  // orders the reads before the writes.
   int S = 1;
   // (S > 1 needs nt * npairs <= cus, so nt <= cus: the partial Gram buffer
   // covers min(slots, cus) tasks)
@@ -872,6 +875,7 @@ tn_status jacobi_sweeps(tn_engine* e, const Workspace& w, int nt, size_t maxR, s
   // (1e-3 for conv_tol = 1e-8; scaled by the square root for looser tolerances)
   const double kAheadOff = std::min(0.5, 1e-3 * std::sqrt(conv_tol / 1e-8));
   int launched = 0, checked = 0;
+  size_t step_launches = 0;
   double last_worst = 1.0;
   bool done = false;
   const double* last_o = nullptr;  // the last sweep's maxima read back
@@ -884,9 +888,11 @@ tn_status jacobi_sweeps(tn_engine* e, const Workspace& w, int nt, size_t maxR, s
         if (S == 1) {
           launch_jacobi_step(ja, npairs, bw, launched, s, use_list ? open_known : nt,
                              e->stream);
+          ++step_launches;
         } else {
           launch_jacobi_gram(ja, npairs, bw, launched, s, e->stream);
           launch_jacobi_apply(ja, npairs, bw, launched, s, e->stream);
+          step_launches += 2;
         }
       }
       TN_HIP(hipGetLastError());
@@ -930,7 +936,7 @@ tn_status jacobi_sweeps(tn_engine* e, const Workspace& w, int nt, size_t maxR, s
       }
   }
   *sweeps_run = checked;
-  *launches = (size_t)launched * steps * (S == 1 ? 1 : 2);
+  *launches = step_launches;
   return TN_OK;
 }
 

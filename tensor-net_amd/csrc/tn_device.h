@@ -102,7 +102,7 @@ struct JacobiArgs {
   int ldx, ldw;
   int nt;                // tasks in the launch
   int nbe;               // round-robin blocks (even) of the widest task
-  int S;                 // workgroups per block pair (row split; 1 otherwise)
+  int S;                 // workgroups per block pair, a power of two (row split; 1 otherwise)
   int Rh, Ch;            // bounds on every task's R and C
   double tol_rot;        // rotate a pair when |g_ij| > tol_rot sqrt(g_ii g_jj)
   double conv_tol;       // converged after a sweep whose max ratio <= conv_tol

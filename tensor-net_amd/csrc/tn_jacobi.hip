@@ -30,12 +30,17 @@
 #include "tn_devmath.h"
 #include "tn_kernels.h"
 
+#include <algorithm>
 #include <cfloat>
 
 namespace tnmps {
 
+// Block at position i (1 .. nbe-1) of the circle at `step` (0 .. nbe-2):
+// 1 + (i - 1 + step) mod (nbe - 1), the sum being below 2 (nbe - 1).
 __device__ __forceinline__ int rr_player(int i, int step, int nbe) {
-  return 1 + (i - 1 + step) % (nbe - 1);
+  int v = i - 1 + step;
+  if (v >= nbe - 1) v -= nbe - 1;
+  return 1 + v;
 }
 
 // Columns whose squared norm is below a.tiny2 (by default norm 1.5e-16,
@@ -59,12 +64,11 @@ struct PairSmem {
   __device__ double2& part(int w, int i, int j) { return buf[(w * PW + i) * LD + j]; }
 };
 
-// Column of X (and W) held by panel column cc (0..PW-1) of pair p at `step`.
+// Column of X (and W) held by panel column cc (0..PW-1) of the pair of
+// blocks (ba, bb).
 template <int BW>
-__device__ __forceinline__ int pair_col(int nbe, int step, int p, int cc) {
-  const int blk = (cc < BW) ? ((p == 0) ? 0 : rr_player(p, step, nbe))
-                            : rr_player(nbe - 1 - p, step, nbe);
-  return blk * BW + (cc & (BW - 1));
+__device__ __forceinline__ int pair_col(int ba, int bb, int cc) {
+  return ((cc < BW) ? ba : bb) * BW + (cc & (BW - 1));
 }
 
 // One 64-row chunk of a panel in registers: thread tid holds row tid & 63 of
@@ -83,11 +87,14 @@ template <int BW>
 __device__ __forceinline__ void thread_cols(int nbe, int step, int p, int (&lcol)[BW / 2],
                                             int (&ocol)[BW / 2]) {
   const int wid = threadIdx.x >> 6;
+  // pair p of `step` holds blocks ba and bb (the same for the whole workgroup)
+  const int ba = (p == 0) ? 0 : rr_player(p, step, nbe);
+  const int bb = rr_player(nbe - 1 - p, step, nbe);
 #pragma unroll
   for (int q = 0; q < BW / 2; ++q) {
-    lcol[q] = pair_col<BW>(nbe, step, p, wid + 4 * q);
-    ocol[q] = (BW == 8) ? pair_col<BW>(nbe, step, p, threadIdx.x & 15)
-                        : pair_col<BW>(nbe, step, p, wid * (BW / 2) + q);
+    lcol[q] = pair_col<BW>(ba, bb, wid + 4 * q);
+    ocol[q] = (BW == 8) ? pair_col<BW>(ba, bb, threadIdx.x & 15)
+                        : pair_col<BW>(ba, bb, wid * (BW / 2) + q);
   }
 }
 
@@ -636,18 +643,160 @@ __device__ void panel_apply(double2* M, size_t ld, int a, int b, int cend,
   }
 }
 
-// Row range of slice q of S over n rows (slices are multiples of 64 rows).
+// ------------------------------------------- register-resident panels (BW = 8)
+// The 16-column panel rows a workgroup works on, held in registers for the
+// whole step: chunk c of thread tid is row a + 64 c + (tid & 63) of panel
+// columns (tid >> 6) + 4 q, q < 4 (the Chunk layout).  Chunk 0 loads at
+// kernel entry and the other chunks as soon as the task is known to be
+// unconverged, so a step waits on global memory about once instead of once
+// per chunk, and the Gram matrix and the rotation read X once.  The
+// arithmetic and its order are those of panel_gram / panel_apply.
+template <int NC>
+struct Panel8 {
+  double2 v[NC][4];
+};
+
+// Chunks C0 .. C1-1 of rows [a, rend), columns below cend.
+template <int NC, int C0, int C1>
+__device__ __forceinline__ void load_panel8(const double2* __restrict__ M, size_t ld, int a,
+                                            int rend, int cend, const int (&lcol)[4],
+                                            Panel8<NC>& pn) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int c = C0; c < C1; ++c) {
+    const int r = a + 64 * c + lane;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      pn.v[c][q] =
+          (lcol[q] < cend && r < rend) ? M[(size_t)lcol[q] * ld + r] : make_double2(0.0, 0.0);
+  }
+}
+
+// Zero chunk 0's elements outside rows [a, rend) and columns below cend
+// (it was loaded for the host's bounds, before the task's own arrived).
+template <int NC>
+__device__ __forceinline__ void mask_panel8(int a, int rend, int cend, const int (&lcol)[4],
+                                            Panel8<NC>& pn) {
+  const int r = a + (threadIdx.x & 63);
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+    if (!(lcol[q] < cend && r < rend)) pn.v[0][q] = make_double2(0.0, 0.0);
+}
+
+template <int NC>
+__device__ __forceinline__ void put_panel8(const Panel8<NC>& pn, int c, PairSmem<8>& sm) {
+  const int rr = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) sm.Ps(rr, wid + 4 * q) = pn.v[c][q];
+}
+
+// sm.Gs[0] = P^H P over rows [a, b) of the panel held in pn (b - a <= 64 NC).
+template <int NC>
+__device__ void panel_gram8r(int a, int b, const Panel8<NC>& pn, PairSmem<8>& sm) {
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  // two accumulators per part (even and odd row quads), so that consecutive
+  // matrix-core operations do not wait on each other
+  v4d gre[2] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
+  v4d gim[2] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    if (a + 64 * c >= b) break;
+    put_panel8<NC>(pn, c, sm);
+    __syncthreads();
+    double2 pv[4];
+#pragma unroll
+    for (int g = 0; g < 4; ++g) pv[g] = sm.Ps(16 * wid + 4 * g + (lane >> 4), lane & 15);
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+      const double2 p = pv[g];
+      gre[g & 1] = mfma16(p.x, p.x, gre[g & 1]);
+      gim[g & 1] = mfma16(p.x, p.y, gim[g & 1]);
+      gre[g & 1] = mfma16(p.y, p.y, gre[g & 1]);
+      gim[g & 1] = mfma16(-p.y, p.x, gim[g & 1]);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int v = 0; v < 4; ++v)
+    sm.part(wid, (lane >> 4) + 4 * v, lane & 15) =
+        make_double2(gre[0][v] + gre[1][v], gim[0][v] + gim[1][v]);
+  __syncthreads();
+  {
+    const int gi = tid >> 4, gj = tid & 15;
+    double2 g = sm.part(0, gi, gj);
+    g = cadd(g, sm.part(1, gi, gj));
+    g = cadd(g, sm.part(2, gi, gj));
+    g = cadd(g, sm.part(3, gi, gj));
+    sm.Gs[0][gi][gj] = g;
+  }
+  __syncthreads();
+}
+
+// Rows [a + 64 c0, b) of the panel of M (columns below cend) <- pn * Ws[wsel]
+// (pn's chunk c holds rows a + 64 c).
+template <int NC>
+__device__ void panel_apply8r(double2* M, size_t ld, int a, int b, int cend, int oc,
+                              const Panel8<NC>& pn, PairSmem<8>& sm, int wsel, int c0 = 0) {
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  double2 wk[4];
+#pragma unroll
+  for (int kk = 0; kk < 4; ++kk) wk[kk] = sm.Ws(wsel, 4 * kk + (lane >> 4), lane & 15);
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    const int r0 = a + 64 * c;
+    if (r0 >= b) break;
+    if (c < c0) continue;
+    put_panel8<NC>(pn, c, sm);
+    __syncthreads();
+    v4d ore = {0.0, 0.0, 0.0, 0.0}, oim = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) {
+      const double2 p = sm.Ps(16 * wid + (lane & 15), 4 * kk + (lane >> 4));
+      ore = mfma16(p.x, wk[kk].x, ore);
+      ore = mfma16(-p.y, wk[kk].y, ore);
+      oim = mfma16(p.x, wk[kk].y, oim);
+      oim = mfma16(p.y, wk[kk].x, oim);
+    }
+    if (oc < cend) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int r = r0 + 16 * wid + (lane >> 4) + 4 * v;
+        if (r < b) M[(size_t)oc * ld + r] = make_double2(ore[v], oim[v]);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Row range of slice q of S (a power of two) over n rows (slices are
+// multiples of 64 rows).
 __device__ __forceinline__ void split_rows(int n, int S, int q, int& a, int& b) {
-  const int per = (((n + S - 1) / S) + 63) / 64 * 64;
+  const int sl = __builtin_ctz(S);
+  const int per = (((n + S - 1) >> sl) + 63) / 64 * 64;
   a = min(n, q * per);
   b = min(n, a + per);
 }
 
-// Convergence state of task t before `sweep`: true when its previous sweep
-// stayed at or below conv_tol.
-__device__ __forceinline__ bool converged(const JacobiArgs& a, int t, int sweep) {
-  return sweep > 0 && a.offs[(size_t)(sweep - 1) * a.nt + t] <= a.conv_tol;
+// Block pair p and row slice q of a row-split workgroup.
+__device__ __forceinline__ void split_block(int S, int& p, int& q) {
+  p = (int)blockIdx.x >> __builtin_ctz(S);
+  q = (int)blockIdx.x & (S - 1);
 }
+
+// Convergence state of task t before `sweep`: true when its previous sweep
+// stayed at or below conv_tol.  The load is unconditional (row 0 at sweep
+// 0, where it is not used), so no branch separates it from the kernel's
+// other entry loads.
+__device__ __forceinline__ bool converged(const JacobiArgs& a, int t, int sweep) {
+  const double o = a.offs[(size_t)(sweep > 0 ? sweep - 1 : 0) * a.nt + t];
+  return sweep > 0 && o <= a.conv_tol;
+}
+
+// Keeps the compiler from moving later memory reads above this point: the
+// register-resident kernels issue their panel loads, then read the task's
+// R, C and convergence state, whose wait then does not hold the panel loads
+// back.
+__device__ __forceinline__ void issue_fence() { asm volatile("" ::: "memory"); }
 
 // Block pair p of task t at one step: Gram matrix, solve, apply to X and W.
 template <int BW>
@@ -724,7 +873,9 @@ __global__ __launch_bounds__(256, BW == 8 ? 2 : 1) void k_jacobi_gram(JacobiArgs
                                                                       int step) {
   constexpr int PW = 2 * BW;
   __shared__ PairSmem<BW> sm;
-  const int t = blockIdx.y, p = blockIdx.x / a.S, q = blockIdx.x - p * a.S;
+  const int t = blockIdx.y;
+  int p, q;
+  split_block(a.S, p, q);
   const int* inf = a.info + (size_t)t * kInfoLen;
   const int R = inf[kInfoRJ], C = inf[kInfoC];
   const bool done = converged(a, t, sweep);
@@ -751,7 +902,9 @@ __global__ __launch_bounds__(256, BW == 8 ? 2 : 1) void k_jacobi_apply(JacobiArg
                                                                        int step) {
   constexpr int PW = 2 * BW, NE = (PW * PW + 255) / 256;
   __shared__ PairSmem<BW> sm;
-  const int t = blockIdx.y, p = blockIdx.x / a.S, q = blockIdx.x - p * a.S;
+  const int t = blockIdx.y;
+  int p, q;
+  split_block(a.S, p, q);
   const int* inf = a.info + (size_t)t * kInfoLen;
   const int R = inf[kInfoRJ], C = inf[kInfoC];
   const bool done = converged(a, t, sweep);
@@ -796,34 +949,166 @@ __global__ __launch_bounds__(256, BW == 8 ? 2 : 1) void k_jacobi_apply(JacobiArg
   panel_apply<BW>(W, (size_t)a.ldw, wa, we, C, lcol, ocol, cw, sm, wsel);
 }
 
+// k_jacobi_step for BW = 8 with the panels in registers: X's (R <= 64 NCX
+// rows) and, when NCW > 0, W's (C <= 64 NCW rows); with NCW = 0 W's panel
+// streams through panel_apply.  The same arithmetic as k_jacobi_step<8>.
+template <int NCX, int NCW>
+__global__ __launch_bounds__(256, 2) void k_jacobi_step_r(JacobiArgs a, int sweep, int step) {
+  __shared__ PairSmem<8> sm;
+  const int i = blockIdx.y;
+  const int cnt = a.act ? a.act[0] : a.nt;
+  const int t = a.act ? a.act[1 + i] : i;
+  if (i >= cnt) return;
+  int lcol[4], ocol[4];
+  thread_cols<8>(a.nbe, step, blockIdx.x, lcol, ocol);
+  double2* X = a.X + (size_t)t * a.xslot;
+  double2* W = a.W + (size_t)t * a.wslot;
+  Panel8<NCX> px;
+  load_panel8<NCX, 0, 1>(X, (size_t)a.ldx, 0, a.Rh, a.ldw, lcol, px);
+  issue_fence();
+  const int* inf = a.info + (size_t)t * kInfoLen;
+  const int R = inf[kInfoRJ], C = inf[kInfoC];
+  if (converged(a, t, sweep)) return;
+  load_panel8<NCX, 1, NCX>(X, (size_t)a.ldx, 0, R, C, lcol, px);
+  Panel8<(NCW > 0 ? NCW : 1)> pw;
+  if constexpr (NCW > 0) load_panel8<NCW, 0, NCW>(W, (size_t)a.ldw, 0, C, C, lcol, pw);
+  mask_panel8<NCX>(0, R, C, lcol, px);
+  panel_gram8r<NCX>(0, R, px, sm);
+  Chunk<8> cw;
+  if constexpr (NCW == 0) load_chunk<8>(W, (size_t)a.ldw, 0, C, C, lcol, cw);
+  int wsel = 0;
+  if (!panel_solve<8>(a.offs + (size_t)sweep * a.nt + t, a.tol_rot, a.tiny2, sm, wsel)) return;
+  panel_apply8r<NCX>(X, (size_t)a.ldx, 0, R, C, ocol[0], px, sm, wsel);
+  if constexpr (NCW > 0)
+    panel_apply8r<NCW>(W, (size_t)a.ldw, 0, C, C, ocol[0], pw, sm, wsel);
+  else
+    panel_apply<8>(W, (size_t)a.ldw, 0, C, C, lcol, ocol, cw, sm, wsel);
+}
+
+// k_jacobi_gram for BW = 8 with the slice's rows (at most 64 NC) in registers.
+template <int NC>
+__global__ __launch_bounds__(256, 2) void k_jacobi_gram_r(JacobiArgs a, int sweep, int step) {
+  constexpr int PW = 16;
+  __shared__ PairSmem<8> sm;
+  const int t = blockIdx.y;
+  int p, q;
+  split_block(a.S, p, q);
+  int lcol[4], ocol[4];
+  thread_cols<8>(a.nbe, step, p, lcol, ocol);
+  int xa, xb;
+  split_rows(a.Rh, a.S, q, xa, xb);
+  const double2* X = a.X + (size_t)t * a.xslot;
+  Panel8<NC> px;
+  load_panel8<NC, 0, 1>(X, (size_t)a.ldx, xa, xb, a.ldw, lcol, px);
+  issue_fence();
+  const int* inf = a.info + (size_t)t * kInfoLen;
+  const int R = inf[kInfoRJ], C = inf[kInfoC];
+  if (converged(a, t, sweep)) return;
+  const int xe = min(xb, R);
+  load_panel8<NC, 1, NC>(X, (size_t)a.ldx, xa, xe, C, lcol, px);
+  mask_panel8<NC>(xa, xe, C, lcol, px);
+  panel_gram8r<NC>(xa, xe, px, sm);
+  double2* gp = a.gpart + (size_t)t * a.gslot + ((size_t)p * a.S + q) * (PW * PW);
+  for (int e = threadIdx.x; e < PW * PW; e += 256) gp[e] = sm.Gs[0][e / PW][e % PW];
+}
+
+// k_jacobi_apply for BW = 8 with the slice's rows of X and of W (at most
+// 64 NC each) in registers.
+template <int NC>
+__global__ __launch_bounds__(256, 2) void k_jacobi_apply_r(JacobiArgs a, int sweep, int step) {
+  constexpr int PW = 16;
+  __shared__ PairSmem<8> sm;
+  const int t = blockIdx.y;
+  int p, q;
+  split_block(a.S, p, q);
+  int lcol[4], ocol[4];
+  thread_cols<8>(a.nbe, step, p, lcol, ocol);
+  int xa, xb, wa, wb;
+  split_rows(a.Rh, a.S, q, xa, xb);
+  split_rows(a.Ch, a.S, q, wa, wb);
+  double2* X = a.X + (size_t)t * a.xslot;
+  double2* W = a.W + (size_t)t * a.wslot;
+  const double2* gp = a.gpart + (size_t)t * a.gslot + (size_t)p * a.S * (PW * PW);
+  Panel8<NC> px, pw;
+  load_panel8<NC, 0, 1>(X, (size_t)a.ldx, xa, xb, a.ldw, lcol, px);
+  load_panel8<NC, 0, 1>(W, (size_t)a.ldw, wa, wb, a.ldw, lcol, pw);
+  double2 g[kMaxSplit];
+#pragma unroll
+  for (int s = 0; s < kMaxSplit; ++s)
+    g[s] = s < a.S ? gp[(size_t)s * (PW * PW) + threadIdx.x] : make_double2(0.0, 0.0);
+  issue_fence();
+  const int* inf = a.info + (size_t)t * kInfoLen;
+  const int R = inf[kInfoRJ], C = inf[kInfoC];
+  if (converged(a, t, sweep)) return;
+  const int xe = min(xb, R), we = min(wb, C);
+  load_panel8<NC, 1, NC>(X, (size_t)a.ldx, xa, xe, C, lcol, px);
+  load_panel8<NC, 1, NC>(W, (size_t)a.ldw, wa, we, C, lcol, pw);
+  {
+    double2 sum = g[0];
+#pragma unroll
+    for (int s = 1; s < kMaxSplit; ++s) sum = cadd(sum, g[s]);
+    sm.Gs[0][threadIdx.x / PW][threadIdx.x % PW] = sum;
+  }
+  __syncthreads();
+  mask_panel8<NC>(xa, xe, C, lcol, px);
+  mask_panel8<NC>(wa, we, C, lcol, pw);
+  int wsel = 0;
+  if (!panel_solve<8>(q == 0 ? a.offs + (size_t)sweep * a.nt + t : nullptr, a.tol_rot, a.tiny2,
+                      sm, wsel))
+    return;
+  panel_apply8r<NC>(X, (size_t)a.ldx, xa, xe, C, ocol[0], px, sm, wsel);
+  panel_apply8r<NC>(W, (size_t)a.ldw, wa, we, C, ocol[0], pw, sm, wsel);
+}
+
 // ------------------------------------------------------------- launch
+// 64-row chunks of the widest slice when n rows split into S slices.
+static int slice_chunks(int n, int S) { return (((n + S - 1) / S) + 63) / 64; }
+
+// Block width 8 takes the register-resident kernels when the panel rows fit
+// (X's up to 512 rows, W's up to 256; W streams beyond that), the streaming
+// ones otherwise.
 void launch_jacobi_step(const JacobiArgs& a, int npairs, int bw, int sweep, int step, int gy,
                         hipStream_t s) {
+  const dim3 grid(npairs, gy), blk(256);
   if (bw == 16)
-    hipLaunchKernelGGL(k_jacobi_step<16>, dim3(npairs, gy), dim3(256), 0, s, a, sweep, step);
+    hipLaunchKernelGGL(k_jacobi_step<16>, grid, blk, 0, s, a, sweep, step);
+  else if (a.Rh <= 128 && a.Ch <= 128)
+    hipLaunchKernelGGL((k_jacobi_step_r<2, 2>), grid, blk, 0, s, a, sweep, step);
+  else if (a.Rh <= 256 && a.Ch <= 256)
+    hipLaunchKernelGGL((k_jacobi_step_r<4, 4>), grid, blk, 0, s, a, sweep, step);
+  else if (a.Rh <= 512)
+    hipLaunchKernelGGL((k_jacobi_step_r<8, 0>), grid, blk, 0, s, a, sweep, step);
   else
-    hipLaunchKernelGGL(k_jacobi_step<8>, dim3(npairs, gy), dim3(256), 0, s, a, sweep, step);
+    hipLaunchKernelGGL(k_jacobi_step<8>, grid, blk, 0, s, a, sweep, step);
 }
 void launch_active(const JacobiArgs& a, int sweep, int* act, hipStream_t s) {
   hipLaunchKernelGGL(k_active, dim3(1), dim3(1024), 0, s, a, sweep, act);
 }
 void launch_jacobi_gram(const JacobiArgs& a, int npairs, int bw, int sweep, int step,
                         hipStream_t s) {
+  const dim3 grid(npairs * a.S, a.nt), blk(256);
+  const int nc = slice_chunks(a.Rh, a.S);
   if (bw == 16)
-    hipLaunchKernelGGL(k_jacobi_gram<16>, dim3(npairs * a.S, a.nt), dim3(256), 0, s, a, sweep,
-                       step);
+    hipLaunchKernelGGL(k_jacobi_gram<16>, grid, blk, 0, s, a, sweep, step);
+  else if (nc <= 2)
+    hipLaunchKernelGGL(k_jacobi_gram_r<2>, grid, blk, 0, s, a, sweep, step);
+  else if (nc <= 4)
+    hipLaunchKernelGGL(k_jacobi_gram_r<4>, grid, blk, 0, s, a, sweep, step);
   else
-    hipLaunchKernelGGL(k_jacobi_gram<8>, dim3(npairs * a.S, a.nt), dim3(256), 0, s, a, sweep,
-                       step);
+    hipLaunchKernelGGL(k_jacobi_gram<8>, grid, blk, 0, s, a, sweep, step);
 }
 void launch_jacobi_apply(const JacobiArgs& a, int npairs, int bw, int sweep, int step,
                          hipStream_t s) {
+  const dim3 grid(npairs * a.S, a.nt), blk(256);
+  const int nc = std::max(slice_chunks(a.Rh, a.S), slice_chunks(a.Ch, a.S));
   if (bw == 16)
-    hipLaunchKernelGGL(k_jacobi_apply<16>, dim3(npairs * a.S, a.nt), dim3(256), 0, s, a, sweep,
-                       step);
+    hipLaunchKernelGGL(k_jacobi_apply<16>, grid, blk, 0, s, a, sweep, step);
+  else if (nc <= 2)
+    hipLaunchKernelGGL(k_jacobi_apply_r<2>, grid, blk, 0, s, a, sweep, step);
+  else if (nc <= 4)
+    hipLaunchKernelGGL(k_jacobi_apply_r<4>, grid, blk, 0, s, a, sweep, step);
   else
-    hipLaunchKernelGGL(k_jacobi_apply<8>, dim3(npairs * a.S, a.nt), dim3(256), 0, s, a, sweep,
-                       step);
+    hipLaunchKernelGGL(k_jacobi_apply<8>, grid, blk, 0, s, a, sweep, step);
 }
 
 }  // namespace tnmps
