@@ -92,7 +92,8 @@ typedef struct {
     int theta_tile;  /* 32 or 64: theta tile edge on the matrix cores */
     int bw;          /* 8 or 16: Jacobi block width (columns) */
     int split_max;   /* 1..8: most workgroups per block pair (row-split steps) */
-    int split_rows;  /* >= 64: least X rows per row-split workgroup */
+    int split_rows;  /* >= 64: the row split S doubles while the rows are at
+                        least split_rows x S (slices of split_rows / 2 rows or more) */
     int ahead;       /* 1..3: sweeps enqueued ahead while far from convergence */
     int qr;          /* 0 or 1: precondition the SVD with a column-sorted
                         Householder QR and run the Jacobi sweeps on R^H

@@ -306,7 +306,7 @@ struct tn_engine {
   int* h_pmap[2] = {};
   int* d_pmap[2] = {};
   int split_max = kMaxSplit;  // cap on workgroups per block pair (TNMPS_SPLIT_MAX)
-  int split_rows = 128;       // least X rows per row-split workgroup (TNMPS_SPLIT_ROWS)
+  int split_rows = 128;       // the split doubles while rows >= split_rows x S (TNMPS_SPLIT_ROWS)
   int bw_env = 0;             // forced Jacobi block width (TNMPS_BW), 0 = automatic
   std::vector<tn_tune> tune;  // launch shapes per C bucket (tn_engine_set_tuning)
   bool factored = true;       // factored updates where a factor is isometric (TNMPS_FACTORED)
