@@ -682,7 +682,7 @@ def run_single(args, pool, workers: int, host: dict) -> dict:
     roof = {
         "bound": "mfma",
         "kernel": "SVD: QR preconditioner (k_qr_*) + Jacobi steps (k_jacobi_gram_r<2> + "
-                  "k_jacobi_apply_r<2> row-split, k_jacobi_step_r<8, 0> gate batches)",
+                  "k_jacobi_apply_r<2> row-split, k_jacobi_step<8> gate batches)",
         "achieved": round(achieved, 4),
         "peak": FP64_PEAK_TFLOPS,
         "unit": "TFLOP/s",

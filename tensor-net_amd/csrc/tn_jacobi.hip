@@ -949,10 +949,11 @@ __global__ __launch_bounds__(256, BW == 8 ? 2 : 1) void k_jacobi_apply(JacobiArg
   panel_apply<BW>(W, (size_t)a.ldw, wa, we, C, lcol, ocol, cw, sm, wsel);
 }
 
-// k_jacobi_step for BW = 8 with the panels in registers: X's (R <= 64 NCX
-// rows) and, when NCW > 0, W's (C <= 64 NCW rows); with NCW = 0 W's panel
-// streams through panel_apply.  The same arithmetic as k_jacobi_step<8>.
-template <int NCX, int NCW>
+// k_jacobi_step for BW = 8 with X's panel (R <= 64 NCX rows) in registers;
+// W's panel streams through panel_apply (its chunk 0 loads during the
+// solve).  The same arithmetic as k_jacobi_step<8>.  Holding W's panel too
+// cost a wave per SIMD of occupancy and measured slower in batched runs.
+template <int NCX>
 __global__ __launch_bounds__(256, 2) void k_jacobi_step_r(JacobiArgs a, int sweep, int step) {
   __shared__ PairSmem<8> sm;
   const int i = blockIdx.y;
@@ -970,19 +971,14 @@ __global__ __launch_bounds__(256, 2) void k_jacobi_step_r(JacobiArgs a, int swee
   const int R = inf[kInfoRJ], C = inf[kInfoC];
   if (converged(a, t, sweep)) return;
   load_panel8<NCX, 1, NCX>(X, (size_t)a.ldx, 0, R, C, lcol, px);
-  Panel8<(NCW > 0 ? NCW : 1)> pw;
-  if constexpr (NCW > 0) load_panel8<NCW, 0, NCW>(W, (size_t)a.ldw, 0, C, C, lcol, pw);
   mask_panel8<NCX>(0, R, C, lcol, px);
   panel_gram8r<NCX>(0, R, px, sm);
   Chunk<8> cw;
-  if constexpr (NCW == 0) load_chunk<8>(W, (size_t)a.ldw, 0, C, C, lcol, cw);
+  load_chunk<8>(W, (size_t)a.ldw, 0, C, C, lcol, cw);
   int wsel = 0;
   if (!panel_solve<8>(a.offs + (size_t)sweep * a.nt + t, a.tol_rot, a.tiny2, sm, wsel)) return;
   panel_apply8r<NCX>(X, (size_t)a.ldx, 0, R, C, ocol[0], px, sm, wsel);
-  if constexpr (NCW > 0)
-    panel_apply8r<NCW>(W, (size_t)a.ldw, 0, C, C, ocol[0], pw, sm, wsel);
-  else
-    panel_apply<8>(W, (size_t)a.ldw, 0, C, C, lcol, ocol, cw, sm, wsel);
+  panel_apply<8>(W, (size_t)a.ldw, 0, C, C, lcol, ocol, cw, sm, wsel);
 }
 
 // k_jacobi_gram for BW = 8 with the slice's rows (at most 64 NC) in registers.
@@ -1064,20 +1060,21 @@ __global__ __launch_bounds__(256, 2) void k_jacobi_apply_r(JacobiArgs a, int swe
 // 64-row chunks of the widest slice when n rows split into S slices.
 static int slice_chunks(int n, int S) { return (((n + S - 1) / S) + 63) / 64; }
 
-// Block width 8 takes the register-resident kernels when the panel rows fit
-// (X's up to 512 rows, W's up to 256; W streams beyond that), the streaming
-// ones otherwise.
+// Block width 8 takes the register-resident kernels when X's panel rows fit
+// (one workgroup per pair: up to 256 rows; row split: up to 256 rows per
+// slice of X and of W), the streaming ones otherwise.  The gate batches'
+// 512-row panels stream: their steps are HBM-bound, and the streaming
+// kernel's four workgroups per CU measured as fast as the register-resident
+// one's two.
 void launch_jacobi_step(const JacobiArgs& a, int npairs, int bw, int sweep, int step, int gy,
                         hipStream_t s) {
   const dim3 grid(npairs, gy), blk(256);
   if (bw == 16)
     hipLaunchKernelGGL(k_jacobi_step<16>, grid, blk, 0, s, a, sweep, step);
-  else if (a.Rh <= 128 && a.Ch <= 128)
-    hipLaunchKernelGGL((k_jacobi_step_r<2, 2>), grid, blk, 0, s, a, sweep, step);
-  else if (a.Rh <= 256 && a.Ch <= 256)
-    hipLaunchKernelGGL((k_jacobi_step_r<4, 4>), grid, blk, 0, s, a, sweep, step);
-  else if (a.Rh <= 512)
-    hipLaunchKernelGGL((k_jacobi_step_r<8, 0>), grid, blk, 0, s, a, sweep, step);
+  else if (a.Rh <= 128)
+    hipLaunchKernelGGL(k_jacobi_step_r<2>, grid, blk, 0, s, a, sweep, step);
+  else if (a.Rh <= 256)
+    hipLaunchKernelGGL(k_jacobi_step_r<4>, grid, blk, 0, s, a, sweep, step);
   else
     hipLaunchKernelGGL(k_jacobi_step<8>, grid, blk, 0, s, a, sweep, step);
 }
