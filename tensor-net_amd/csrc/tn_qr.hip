@@ -391,29 +391,69 @@ __device__ void factor_panel8(const QrArgs& a, const TaskQr& q, int jp, FactorSm
     }
   }
   __syncthreads();
-  if (wid == 0 && lane < kQrNb) {
-    const int j = lane;
-    double2 trow[kQrNb];
+  // T in two 8 x 8 diagonal blocks and the block above the diagonal,
+  // T = [[T1, T12], [0, T2]], T12 = -T1 (V^H V)[0:8][8:16] T2 (the product
+  // of two compact-WY factors), on wave 0 through LDS (the reflector
+  // buffers, free once every wave has passed the barrier above): lane j < 16
+  // forms row j of its diagonal block by the recurrence, then 64 lanes form
+  // P = T1 G12 and T12 = -P T2 with at most 8 terms each.  Columns at or
+  // past ncol are zero, and no product reads a (V^H V) entry of them.
+  if (wid == 0) {
+    double2(*tt)[kQrNb + 1] = reinterpret_cast<double2(*)[kQrNb + 1]>(&fs.vbuf[0][0]);
+    double2(*pb)[9] = reinterpret_cast<double2(*)[9]>(&fs.vbuf[1][0]);
+    if (lane < kQrNb) {
+      const int j = lane, o = j & 8;
+      double2 trow[8];
 #pragma unroll
-    for (int i = 0; i < kQrNb; ++i) {
-      double2 v = make_double2(0.0, 0.0);
-      if (i < ncol) {
-        if (i == j) {
-          v = fs.taus[i];
-        } else if (i > j) {
-          double2 sm = make_double2(0.0, 0.0);
+      for (int ii = 0; ii < 8; ++ii) {
+        const int i = o + ii;
+        double2 v = make_double2(0.0, 0.0);
+        if (i < ncol) {
+          if (i == j) {
+            v = fs.taus[i];
+          } else if (i > j) {
+            double2 sm = make_double2(0.0, 0.0);
 #pragma unroll
-          for (int l = 0; l < kQrNb; ++l)
-            if (l >= j && l < i) sm = cadd(sm, cmul(trow[l], fs.g[l][i]));
-          const double2 ts = cmul(fs.taus[i], sm);
-          v = make_double2(-ts.x, -ts.y);
+            for (int ll = 0; ll < 8; ++ll)
+              if (o + ll >= j && ll < ii) sm = cadd(sm, cmul(trow[ll], fs.g[o + ll][i]));
+            const double2 ts = cmul(fs.taus[i], sm);
+            v = make_double2(-ts.x, -ts.y);
+          }
         }
+        trow[ii] = v;
       }
-      trow[i] = v;
-    }
-    double2* T = q.T + (size_t)jp * kQrNb * kQrNb + j * kQrNb;
 #pragma unroll
-    for (int i = 0; i < kQrNb; ++i) T[i] = trow[i];
+      for (int ii = 0; ii < 8; ++ii) {
+        tt[j][o + ii] = trow[ii];
+        tt[j][(o ^ 8) + ii] = make_double2(0.0, 0.0);
+      }
+    }
+    wave_lds_sync();
+    const int r = lane >> 3, m = lane & 7;
+    {
+      double2 acc = make_double2(0.0, 0.0);
+      if (8 + m < ncol) {
+#pragma unroll
+        for (int l = 0; l < 8; ++l)
+          if (l >= r) acc = cadd(acc, cmul(tt[r][l], fs.g[l][8 + m]));
+      }
+      pb[r][m] = acc;
+    }
+    wave_lds_sync();
+    {
+      double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        if (k <= m) acc = cadd(acc, cmul(pb[r][k], tt[8 + k][8 + m]));
+      tt[r][8 + m] = make_double2(-acc.x, -acc.y);
+    }
+    wave_lds_sync();
+    double2* T = q.T + (size_t)jp * kQrNb * kQrNb;
+#pragma unroll
+    for (int e4 = 0; e4 < 4; ++e4) {
+      const int e = lane + 64 * e4;
+      T[e] = tt[e >> 4][e & 15];
+    }
   }
 }
 
