@@ -81,6 +81,19 @@ __device__ __forceinline__ TaskQr task_qr(const QrArgs& a, int i) {
   return q;
 }
 
+// *p when ok, else 0.  The load issues either way (from `safe`, a valid
+// address, when not ok) and the result is masked bitwise with a mask the
+// compiler cannot see through, so a run of these issues back to back: as a
+// conditional load (which the compiler makes of a plain select), each got a
+// branch and a wait for its data of its own.
+__device__ __forceinline__ double2 ld_or_zero(const double2* p, bool ok, const double2* safe) {
+  const double2 v = *(ok ? p : safe);
+  unsigned long long m = ok ? ~0ull : 0ull;
+  asm volatile("" : "+v"(m));
+  return make_double2(__longlong_as_double(__double_as_longlong(v.x) & (long long)m),
+                      __longlong_as_double(__double_as_longlong(v.y) & (long long)m));
+}
+
 // Column c of the matrix a panel or trailing update reads: panel 0's pass
 // reads X through the sort order, later ones the factored matrix E.
 __device__ __forceinline__ const double2* src_col(const QrArgs& a, const TaskQr& q, int jp,
@@ -156,11 +169,12 @@ struct PanelSmem {
 // V[r][i] of a panel for local row r (global g0 + r) and panel column i:
 // 1 on the diagonal, 0 above it, E below it; 0 past the panel's ncol
 // reflectors and past its nr rows.
+// (zero as well where `use` is false)
 __device__ __forceinline__ double2 vload(const double2* E, size_t ldx, int g0, int nr, int ncol,
-                                         int r, int i) {
-  if (r >= nr || r < i || i >= ncol) return make_double2(0.0, 0.0);
-  if (r == i) return make_double2(1.0, 0.0);
-  return E[(size_t)(g0 + i) * ldx + g0 + r];
+                                         int r, int i, bool use = true) {
+  const bool below = use && r < nr && r > i && i < ncol;
+  const double2 v = ld_or_zero(E + (size_t)(g0 + i) * ldx + g0 + r, below, E);
+  return (use && r == i && i < ncol && r < nr) ? make_double2(1.0, 0.0) : v;
 }
 
 // Panel jp's V for the wave's row tiles, lanes along rows: lane r + 16 k of
@@ -174,9 +188,8 @@ __device__ __forceinline__ void load_vrows(const double2* E, size_t ldx, int R, 
     const int t = wid + kQW * s;
 #pragma unroll
     for (int kk = 0; kk < 4; ++kk)
-      v[s][kk] = (t >= jp) ? vload(E, ldx, g0, nr, ncol, 16 * t + (lane & 15) - g0,
-                                   4 * kk + (lane >> 4))
-                           : make_double2(0.0, 0.0);
+      v[s][kk] = vload(E, ldx, g0, nr, ncol, 16 * t + (lane & 15) - g0, 4 * kk + (lane >> 4),
+                       t >= jp);
   }
 }
 
@@ -328,7 +341,7 @@ __device__ void factor_panel8(const QrArgs& a, const TaskQr& q, int jp, FactorSm
 #pragma unroll
     for (int m = 0; m < kRPL; ++m) {
       const int r = lane + 64 * m;
-      x[j][m] = (src && r < nr) ? src[r] : make_double2(0.0, 0.0);
+      x[j][m] = ld_or_zero(src + r, src && r < nr, q.E);
     }
   }
   double s_own[2] = {0.0, 0.0};
@@ -471,7 +484,7 @@ __device__ void trail_tile(const QrArgs& a, const TaskQr& q, int jp, int c0, Pan
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       const int row = 16 * t + (lane >> 4) + 4 * e;
-      const double2 x = (colok && t >= jp && row < q.R) ? src[row] : make_double2(0.0, 0.0);
+      const double2 x = ld_or_zero(src + row, colok && t >= jp && row < q.R, q.E);
       zr[s][e] = x.x;
       zi[s][e] = x.y;
     }
@@ -572,7 +585,7 @@ __global__ __launch_bounds__(512) void k_qr_final(QrArgs a) {
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       const int row = 16 * t + (lane >> 4) + 4 * e;
-      u[s][e] = (colok && row < q.C) ? q.X[(size_t)col * a.ldx + row] : make_double2(0.0, 0.0);
+      u[s][e] = ld_or_zero(q.X + (size_t)col * a.ldx + row, colok && row < q.C, q.X);
       s2 += cabs2(u[s][e]);
     }
   }
