@@ -599,8 +599,7 @@ __global__ __launch_bounds__(512) void k_qr_final(QrArgs a) {
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       const int row = 16 * t + (lane >> 4) + 4 * e;
-      const double2 w = (colok && row < q.C) ? q.W[(size_t)col * a.ldw + row]
-                                             : make_double2(0.0, 0.0);
+      const double2 w = ld_or_zero(q.W + (size_t)col * a.ldw + row, colok && row < q.C, q.W);
       zr[s][e] = w.x;
       zi[s][e] = w.y;
     }
@@ -622,6 +621,17 @@ __global__ __launch_bounds__(512) void k_qr_final(QrArgs a) {
     zr[s] *= sg;
     zi[s] *= sg;
   }
+  // the scatter targets, loaded together (index 0 where unused)
+  int prow[kQT][4];
+#pragma unroll
+  for (int s = 0; s < kQT; ++s) {
+    const int t = wid + kQW * s;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int row = 16 * t + (lane >> 4) + 4 * e;
+      prow[s][e] = q.perm0[(colok && row < q.C) ? row : 0];
+    }
+  }
   // every read of this tile of W (in Z) precedes the writes below
   __syncthreads();
 #pragma unroll
@@ -630,8 +640,7 @@ __global__ __launch_bounds__(512) void k_qr_final(QrArgs a) {
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       const int row = 16 * t + (lane >> 4) + 4 * e;
-      if (colok && row < q.C)
-        q.W[(size_t)col * a.ldw + q.perm0[row]] = cscale(u[s][e], isg);
+      if (colok && row < q.C) q.W[(size_t)col * a.ldw + prow[s][e]] = cscale(u[s][e], isg);
     }
   }
   // Z <- Q Z, panels last to first
