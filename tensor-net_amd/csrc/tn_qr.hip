@@ -202,8 +202,10 @@ __device__ void panel_update(const double2 (&vrow)[kQT][4], const double2* Tp, i
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   __syncthreads();  // the previous use of sm is done
   if (tid < kQrNb * kQrNb) sm.Ts[tid >> 4][tid & 15] = Tp[tid];
-  // H = Z^H V over the wave's row tiles at or below the panel
-  v4d hre = {0.0, 0.0, 0.0, 0.0}, him = {0.0, 0.0, 0.0, 0.0};
+  // H = Z^H V over the wave's row tiles at or below the panel, with three
+  // real products instead of four: P1 = Zr^T Vr, P2 = Zi^T Vi,
+  // P3 = (Zr - Zi)^T (Vr + Vi); Re H = P1 + P2, Im H = P3 - P1 + P2
+  v4d p1 = {0.0, 0.0, 0.0, 0.0}, p2 = {0.0, 0.0, 0.0, 0.0}, p3 = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
   for (int s = 0; s < kQT; ++s) {
     const int t = wid + kQW * s;
@@ -216,14 +218,14 @@ __device__ void panel_update(const double2 (&vrow)[kQT][4], const double2* Tp, i
         // B operand: lane i + 16 k holds V[16 t + 4 e + k][i]; A operand
         // conj(Z)^T: lane j + 16 k holds Z[16 t + 4 e + k][j] (element e)
         const double2 v = sm.vt[wid][4 * e + (lane >> 4)][lane & 15];
-        hre = mfma16(zr[s][e], v.x, hre);
-        hre = mfma16(zi[s][e], v.y, hre);
-        him = mfma16(zr[s][e], v.y, him);
-        him = mfma16(-zi[s][e], v.x, him);
+        p1 = mfma16(zr[s][e], v.x, p1);
+        p2 = mfma16(zi[s][e], v.y, p2);
+        p3 = mfma16(zr[s][e] - zi[s][e], v.x + v.y, p3);
       }
       wave_lds_sync();
     }
   }
+  const v4d hre = p1 + p2, him = p3 - p1 + p2;
 #pragma unroll
   for (int e = 0; e < 4; ++e)
     sm.part[wid][(lane >> 4) + 4 * e][lane & 15] = make_double2(hre[e], him[e]);
@@ -249,21 +251,29 @@ __device__ void panel_update(const double2 (&vrow)[kQT][4], const double2* Tp, i
   }
   __syncthreads();
   double2 mk[4];
+  double mks[4];
 #pragma unroll
-  for (int kk = 0; kk < 4; ++kk) mk[kk] = sm.Ms[4 * kk + (lane >> 4)][lane & 15];
-  // Z -= V M:  A operand V (lane r + 16 k: V[r][4 kk + k]), B operand M
+  for (int kk = 0; kk < 4; ++kk) {
+    mk[kk] = sm.Ms[4 * kk + (lane >> 4)][lane & 15];
+    mks[kk] = mk[kk].x + mk[kk].y;
+  }
+  // Z -= V M:  A operand V (lane r + 16 k: V[r][4 kk + k]), B operand M;
+  // three real products per tile: Q1 = Vr Mr, Q2 = Vi Mi,
+  // Q3 = (Vr + Vi)(Mr + Mi); Re VM = Q1 - Q2, Im VM = Q3 - Q1 - Q2
 #pragma unroll
   for (int s = 0; s < kQT; ++s) {
     const int t = wid + kQW * s;
     if (t >= jp && 16 * t < R) {
+      v4d q1 = {0.0, 0.0, 0.0, 0.0}, q2 = {0.0, 0.0, 0.0, 0.0}, q3 = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
       for (int kk = 0; kk < 4; ++kk) {
         const double2 v = vrow[s][kk];
-        zr[s] = mfma16(-v.x, mk[kk].x, zr[s]);
-        zr[s] = mfma16(v.y, mk[kk].y, zr[s]);
-        zi[s] = mfma16(-v.x, mk[kk].y, zi[s]);
-        zi[s] = mfma16(-v.y, mk[kk].x, zi[s]);
+        q1 = mfma16(v.x, mk[kk].x, q1);
+        q2 = mfma16(v.y, mk[kk].y, q2);
+        q3 = mfma16(v.x + v.y, mks[kk], q3);
       }
+      zr[s] -= q1 - q2;
+      zi[s] -= q3 - q1 - q2;
     }
   }
 }
