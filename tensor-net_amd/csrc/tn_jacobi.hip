@@ -135,6 +135,56 @@ __device__ __forceinline__ v4d mfma16(double a, double b, v4d c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
+// Panel Gram matrix G = P^H P of 16 columns on the matrix cores with three
+// real products per row quad instead of four: Re G = Pr^T Pr + Pi^T Pi and
+// M = Pr^T Pi, from which Im G = M - M^T (exactly antisymmetric, zero on
+// the diagonal).  Lane l holds P[k][l % 16] of the quad's row k = l / 16.
+__device__ __forceinline__ void gram8_acc(double2 p, v4d& re, v4d& m) {
+  re = mfma16(p.x, p.x, re);
+  m = mfma16(p.x, p.y, m);
+  re = mfma16(p.y, p.y, re);
+}
+
+// The four waves' (Re G, M) parts summed in a fixed order into sm.Gs[0],
+// with Im G = S - S^T for S the summed M.
+template <typename Smem>
+__device__ __forceinline__ void gram8_finish(v4d re, v4d m, Smem& sm) {
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+#pragma unroll
+  for (int v = 0; v < 4; ++v)
+    sm.part(wid, (lane >> 4) + 4 * v, lane & 15) = make_double2(re[v], m[v]);
+  __syncthreads();
+  {
+    const int gi = tid >> 4, gj = tid & 15;
+    double2 g = sm.part(0, gi, gj);
+    double mt = sm.part(0, gj, gi).y;
+#pragma unroll
+    for (int w = 1; w < 4; ++w) {
+      g = cadd(g, sm.part(w, gi, gj));
+      mt += sm.part(w, gj, gi).y;
+    }
+    sm.Gs[0][gi][gj] = make_double2(g.x, g.y - mt);
+  }
+  __syncthreads();
+}
+
+// O += P Wl for one 16-row tile of 16 columns, three real products instead
+// of four: T1 = Pr Wr, T2 = Pi Wi, T3 = (Pr + Pi)(Wr + Wi); Re O = T1 - T2,
+// Im O = T3 - T1 - T2 (wks = Wr + Wi).  Lane l holds P[l % 16][4 kk + l / 16]
+// in p[kk].
+__device__ __forceinline__ void apply8_tile(const double2 (&p)[4], const double2 (&wk)[4],
+                                            const double (&wks)[4], v4d& ore, v4d& oim) {
+  v4d t1 = {0.0, 0.0, 0.0, 0.0}, t2 = {0.0, 0.0, 0.0, 0.0}, t3 = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int kk = 0; kk < 4; ++kk) {
+    t1 = mfma16(p[kk].x, wk[kk].x, t1);
+    t2 = mfma16(p[kk].y, wk[kk].y, t2);
+    t3 = mfma16(p[kk].x + p[kk].y, wks[kk], t3);
+  }
+  ore = t1 - t2;
+  oim = t3 - t1 - t2;
+}
+
 // sm.Gs[0] = P^H P over rows [a, b) of the panel of X; `ck` holds rows
 // [a, a + 64).  Thread = (row slice rs, 4x4 tile of G); a row slice takes
 // every RS-th row of each 64-row chunk.  Slices are reduced by shuffles
@@ -144,37 +194,19 @@ __device__ void panel_gram(const double2* __restrict__ X, size_t ldx, int a, int
                            const int (&lcol)[BW / 2], Chunk<BW>& ck, PairSmem<BW>& sm) {
   if constexpr (BW == 8) {
     // 16-column panels on the matrix cores: wave w takes rows 16 w .. 16 w + 15
-    // of each chunk, four at a time, as the K dimension of
-    //   Re G += Pr^T Pr + Pi^T Pi,   Im G += Pr^T Pi - Pi^T Pr.
-    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    v4d gre = {0.0, 0.0, 0.0, 0.0}, gim = {0.0, 0.0, 0.0, 0.0};
+    // of each chunk, four at a time, as the K dimension (gram8_acc)
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    v4d gre = {0.0, 0.0, 0.0, 0.0}, gm = {0.0, 0.0, 0.0, 0.0};
     for (int r0 = a; r0 < b; r0 += 64) {
       put_chunk<BW>(ck, sm);
       __syncthreads();
       if (r0 + 64 < b) load_chunk<BW>(X, ldx, r0 + 64, b, cend, lcol, ck);
 #pragma unroll
-      for (int g = 0; g < 4; ++g) {
-        const double2 p = sm.Ps(16 * wid + 4 * g + (lane >> 4), lane & 15);
-        gre = mfma16(p.x, p.x, gre);
-        gre = mfma16(p.y, p.y, gre);
-        gim = mfma16(p.x, p.y, gim);
-        gim = mfma16(-p.y, p.x, gim);
-      }
+      for (int g = 0; g < 4; ++g)
+        gram8_acc(sm.Ps(16 * wid + 4 * g + (lane >> 4), lane & 15), gre, gm);
       __syncthreads();
     }
-#pragma unroll
-    for (int v = 0; v < 4; ++v)
-      sm.part(wid, (lane >> 4) + 4 * v, lane & 15) = make_double2(gre[v], gim[v]);
-    __syncthreads();
-    {
-      const int gi = tid >> 4, gj = tid & 15;
-      double2 g = sm.part(0, gi, gj);
-      g = cadd(g, sm.part(1, gi, gj));
-      g = cadd(g, sm.part(2, gi, gj));
-      g = cadd(g, sm.part(3, gi, gj));
-      sm.Gs[0][gi][gj] = g;
-    }
-    __syncthreads();
+    gram8_finish(gre, gm, sm);
     return;
   }
   constexpr int PW = 2 * BW, TPR = PW / 4, NTILE = TPR * TPR, RS = 256 / NTILE;
@@ -590,22 +622,22 @@ __device__ void panel_apply(double2* M, size_t ld, int a, int b, int cend,
     //   Re O = Pr Wr - Pi Wi,   Im O = Pr Wi + Pi Wr.
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     double2 wk[4];
+    double wks[4];
 #pragma unroll
-    for (int kk = 0; kk < 4; ++kk) wk[kk] = sm.Ws(wsel, 4 * kk + (lane >> 4), lane & 15);
+    for (int kk = 0; kk < 4; ++kk) {
+      wk[kk] = sm.Ws(wsel, 4 * kk + (lane >> 4), lane & 15);
+      wks[kk] = wk[kk].x + wk[kk].y;
+    }
     const int oc = ocol[0];
     for (int r0 = a; r0 < b; r0 += 64) {
       put_chunk<BW>(ck, sm);
       __syncthreads();
       if (r0 + 64 < b) load_chunk<BW>(M, ld, r0 + 64, b, cend, lcol, ck);
-      v4d ore = {0.0, 0.0, 0.0, 0.0}, oim = {0.0, 0.0, 0.0, 0.0};
+      v4d ore, oim;
+      double2 pk[4];
 #pragma unroll
-      for (int kk = 0; kk < 4; ++kk) {
-        const double2 p = sm.Ps(16 * wid + (lane & 15), 4 * kk + (lane >> 4));
-        ore = mfma16(p.x, wk[kk].x, ore);
-        ore = mfma16(-p.y, wk[kk].y, ore);
-        oim = mfma16(p.x, wk[kk].y, oim);
-        oim = mfma16(p.y, wk[kk].x, oim);
-      }
+      for (int kk = 0; kk < 4; ++kk) pk[kk] = sm.Ps(16 * wid + (lane & 15), 4 * kk + (lane >> 4));
+      apply8_tile(pk, wk, wks, ore, oim);
       if (oc < cend) {
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
@@ -697,7 +729,7 @@ __device__ void panel_gram8r(int a, int b, const Panel8<NC>& pn, PairSmem<8>& sm
   // two accumulators per part (even and odd row quads), so that consecutive
   // matrix-core operations do not wait on each other
   v4d gre[2] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
-  v4d gim[2] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
+  v4d gm[2] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
     if (a + 64 * c >= b) break;
@@ -707,29 +739,10 @@ __device__ void panel_gram8r(int a, int b, const Panel8<NC>& pn, PairSmem<8>& sm
 #pragma unroll
     for (int g = 0; g < 4; ++g) pv[g] = sm.Ps(16 * wid + 4 * g + (lane >> 4), lane & 15);
 #pragma unroll
-    for (int g = 0; g < 4; ++g) {
-      const double2 p = pv[g];
-      gre[g & 1] = mfma16(p.x, p.x, gre[g & 1]);
-      gim[g & 1] = mfma16(p.x, p.y, gim[g & 1]);
-      gre[g & 1] = mfma16(p.y, p.y, gre[g & 1]);
-      gim[g & 1] = mfma16(-p.y, p.x, gim[g & 1]);
-    }
+    for (int g = 0; g < 4; ++g) gram8_acc(pv[g], gre[g & 1], gm[g & 1]);
     __syncthreads();
   }
-#pragma unroll
-  for (int v = 0; v < 4; ++v)
-    sm.part(wid, (lane >> 4) + 4 * v, lane & 15) =
-        make_double2(gre[0][v] + gre[1][v], gim[0][v] + gim[1][v]);
-  __syncthreads();
-  {
-    const int gi = tid >> 4, gj = tid & 15;
-    double2 g = sm.part(0, gi, gj);
-    g = cadd(g, sm.part(1, gi, gj));
-    g = cadd(g, sm.part(2, gi, gj));
-    g = cadd(g, sm.part(3, gi, gj));
-    sm.Gs[0][gi][gj] = g;
-  }
-  __syncthreads();
+  gram8_finish(gre[0] + gre[1], gm[0] + gm[1], sm);
 }
 
 // Rows [a + 64 c0, b) of the panel of M (columns below cend) <- pn * Ws[wsel]
@@ -739,8 +752,12 @@ __device__ void panel_apply8r(double2* M, size_t ld, int a, int b, int cend, int
                               const Panel8<NC>& pn, PairSmem<8>& sm, int wsel, int c0 = 0) {
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   double2 wk[4];
+  double wks[4];
 #pragma unroll
-  for (int kk = 0; kk < 4; ++kk) wk[kk] = sm.Ws(wsel, 4 * kk + (lane >> 4), lane & 15);
+  for (int kk = 0; kk < 4; ++kk) {
+    wk[kk] = sm.Ws(wsel, 4 * kk + (lane >> 4), lane & 15);
+    wks[kk] = wk[kk].x + wk[kk].y;
+  }
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
     const int r0 = a + 64 * c;
@@ -748,15 +765,11 @@ __device__ void panel_apply8r(double2* M, size_t ld, int a, int b, int cend, int
     if (c < c0) continue;
     put_panel8<NC>(pn, c, sm);
     __syncthreads();
-    v4d ore = {0.0, 0.0, 0.0, 0.0}, oim = {0.0, 0.0, 0.0, 0.0};
+    v4d ore, oim;
+    double2 pk[4];
 #pragma unroll
-    for (int kk = 0; kk < 4; ++kk) {
-      const double2 p = sm.Ps(16 * wid + (lane & 15), 4 * kk + (lane >> 4));
-      ore = mfma16(p.x, wk[kk].x, ore);
-      ore = mfma16(-p.y, wk[kk].y, ore);
-      oim = mfma16(p.x, wk[kk].y, oim);
-      oim = mfma16(p.y, wk[kk].x, oim);
-    }
+    for (int kk = 0; kk < 4; ++kk) pk[kk] = sm.Ps(16 * wid + (lane & 15), 4 * kk + (lane >> 4));
+    apply8_tile(pk, wk, wks, ore, oim);
     if (oc < cend) {
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
