@@ -53,7 +53,7 @@ METRIC = "gate+SVD updates/s @chi=256 FP64 vs mixed; trajectories/s + p99 latenc
 # rocprofv3 FETCH_SIZE and WRITE_SIZE (KiB per dispatch, one counter per run)
 # scaled by what the same counters read for a plain 1 GiB device copy
 # (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0); profiles/jacobi_single_traj_chi256_hbm_traffic.md.
-PMC_KIB = {"k_jacobi_apply_r<2>": (1698.1, 508.0), "k_jacobi_gram_r<2>": (537.8, 242.4)}
+PMC_KIB = {"k_jacobi_apply_r<2>": (1698.3, 508.0), "k_jacobi_gram_r<2>": (532.1, 242.4)}
 PMC_SCALE = (2.0, 1.0)
 TRAFFIC_BYTES = {k: 1024.0 * (f * PMC_SCALE[0] + w * PMC_SCALE[1]) for k, (f, w) in PMC_KIB.items()}
 TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of the Jacobi kernels on this workload, "
