@@ -362,11 +362,12 @@ struct Solve16 {
   int* rr;               // last round posted
 };
 
-// Wave 0, round ROUND: lane l computes the 2 x 2 block (row pair l / 8,
-// column pair l % 8) of G' = Qa^H G Qb and posts column pair l % 8's
-// rotation (the eight lanes of a column pair hold bitwise the same one, so
-// they all write it: no branch between the two rotations' chains).  Reads
-// G[CUR], writes G[CUR ^ 1].
+// Wave 0, round ROUND: lane l computes the rotation of column pair l % 8
+// (the eight lanes of a column pair compute bitwise the same one), takes
+// the rotation of row pair l / 8 from lane l / 8 (which computed it as its
+// column pair's) through a lane shuffle, computes the 2 x 2 block (row pair
+// l / 8, column pair l % 8) of G' = Qa^H G Qb and posts its column pair's
+// rotation.  Reads G[CUR], writes G[CUR ^ 1].
 template <int ROUND, int CUR>
 __device__ __forceinline__ void g_round16(const Solve16& sv, double tol2, double tiny2) {
   constexpr int NXT = CUR ^ 1;
@@ -380,19 +381,20 @@ __device__ __forceinline__ void g_round16(const Solve16& sv, double tol2, double
   asm volatile("" : "+v"(lane));
   double2 (*G)[17] = sv.G[CUR];
   double2 (*Gn)[17] = sv.G[NXT];
-  const int x1 = first(lane >> 3), x2 = x1 ^ ROUND;
+  const int pr = lane >> 3;
+  const int x1 = first(pr), x2 = x1 ^ ROUND;
   const int y1 = first(lane & 7), y2 = y1 ^ ROUND;
-  const double ax = G[x1][x1].x, bx = G[x2][x2].x;
-  const double2 cx = G[x1][x2];
   const double ay = G[y1][y1].x, by = G[y2][y2].x;
   const double2 cy = G[y1][y2];
   const double2 g11 = G[x1][y1], g12 = G[x1][y2];
   const double2 g21 = G[x2][y1], g22 = G[x2][y2];
-  // the round's loads all issue before the rotations' arithmetic
+  // the round's loads all issue before the rotation's arithmetic
   __builtin_amdgcn_sched_barrier(0);
-  double csa, csb;
-  double2 sna, snb;
-  rot2x2(ax, bx, cx, ay, by, cy, tol2, tiny2, csa, sna, csb, snb);
+  double csb;
+  double2 snb;
+  rot2(ay, by, cy, tol2, tiny2, csb, snb);
+  const double csa = __shfl(csb, pr, 64);
+  const double2 sna = make_double2(__shfl(snb.x, pr, 64), __shfl(snb.y, pr, 64));
   // T = G[block] Qb, then G' = Qa^H T, with Q = [[cs, sn], [-conj(sn), cs]]
   const double2 msb = make_double2(-snb.x, snb.y);  // -conj(sn_b)
   const double2 t11 = cadd(cscale(g11, csb), cmul(g12, msb));
