@@ -368,17 +368,19 @@ struct Solve16 {
 // column pair's) through a lane shuffle, computes the 2 x 2 block (row pair
 // l / 8, column pair l % 8) of G' = Qa^H G Qb and posts its column pair's
 // rotation.  Reads G[CUR], writes G[CUR ^ 1].
-template <int ROUND, int CUR>
+template <int ROUND, int CUR, bool HOIST>
 __device__ __forceinline__ void g_round16(const Solve16& sv, double tol2, double tiny2) {
   constexpr int NXT = CUR ^ 1;
   constexpr int h = ROUND >= 8 ? 3 : ROUND >= 4 ? 2 : ROUND >= 2 ? 1 : 0;
   constexpr int lowm = (1 << h) - 1;
   auto first = [](int q) { return ((q >> h) << (h + 1)) | (q & lowm); };
-  // the lane index passes through an empty asm so that the round's LDS
-  // addresses are formed here, not hoisted out of a caller's loop (15
-  // rounds of them would not fit the registers)
+  // unless HOIST, the lane index passes through an empty asm so that the
+  // round's LDS addresses are formed here, not hoisted out of the 15 rounds
+  // (their registers cost the one-per-pair kernel occupancy; the row-split
+  // apply kernel, with few workgroups, takes them and saves the address
+  // arithmetic of every round)
   int lane = threadIdx.x & 63;
-  asm volatile("" : "+v"(lane));
+  if constexpr (!HOIST) asm volatile("" : "+v"(lane));
   double2 (*G)[17] = sv.G[CUR];
   double2 (*Gn)[17] = sv.G[NXT];
   const int pr = lane >> 3;
@@ -411,26 +413,26 @@ __device__ __forceinline__ void g_round16(const Solve16& sv, double tol2, double
   sv.rsn[8 * ROUND + (lane & 7)] = snb;
 }
 
-template <int ROUND, int CUR>
+template <int ROUND, int CUR, bool HOIST>
 __device__ __forceinline__ void g_rounds16(const Solve16& sv, double tol2, double tiny2) {
   if constexpr (ROUND < 16) {
-    g_round16<ROUND, CUR>(sv, tol2, tiny2);
+    g_round16<ROUND, CUR, HOIST>(sv, tol2, tiny2);
     wave_sync();
     __hip_atomic_store(sv.rr, ROUND, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     wave_sync();
-    g_rounds16<ROUND + 1, CUR ^ 1>(sv, tol2, tiny2);
+    g_rounds16<ROUND + 1, CUR ^ 1, HOIST>(sv, tol2, tiny2);
   }
 }
 
 // Wave 1, round ROUND: lane l rotates rows l / 8 and l / 8 + 8 of W in
 // column pair l % 8 (W' = W Qb), in place.
-template <int ROUND>
+template <int ROUND, bool HOIST>
 __device__ __forceinline__ void w_round16(const Solve16& sv) {
   constexpr int h = ROUND >= 8 ? 3 : ROUND >= 4 ? 2 : ROUND >= 2 ? 1 : 0;
   constexpr int lowm = (1 << h) - 1;
   auto first = [](int q) { return ((q >> h) << (h + 1)) | (q & lowm); };
   int lane = threadIdx.x & 63;
-  asm volatile("" : "+v"(lane));
+  if constexpr (!HOIST) asm volatile("" : "+v"(lane));
   const int q = lane & 7, y1 = first(q), y2 = y1 ^ ROUND;
   const int i0 = lane >> 3, i1 = i0 + 8;
   const double cs = sv.rcs[8 * ROUND + q];
@@ -452,19 +454,20 @@ __device__ __forceinline__ void wait_round(const int* rr, int r) {
   wave_sync();
 }
 
-template <int ROUND>
+template <int ROUND, bool HOIST>
 __device__ __forceinline__ void w_rounds16(const Solve16& sv) {
   if constexpr (ROUND < 16) {
     wait_round(sv.rr, ROUND);
-    w_round16<ROUND>(sv);
+    w_round16<ROUND, HOIST>(sv);
     wave_sync();
-    w_rounds16<ROUND + 1>(sv);
+    w_rounds16<ROUND + 1, HOIST>(sv);
   }
 }
 
 // Wave 0's part: the largest relative off-diagonal of G[0] (recorded into
 // *rec when not null and positive), then the rounds unless it is at most
 // tol_rot.  Returns whether there is a rotation to apply.
+template <bool HOIST>
 __device__ bool solve16_wave0(const Solve16& sv, double* rec, double tol_rot, double tiny2) {
   const int lane = threadIdx.x & 63;
   double r2 = 0.0;
@@ -485,11 +488,12 @@ __device__ bool solve16_wave0(const Solve16& sv, double* rec, double tol_rot, do
     __hip_atomic_store(sv.rr, kRoundsNone, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     return false;
   }
-  g_rounds16<1, 0>(sv, tol_rot * tol_rot, tiny2);
+  g_rounds16<1, 0, HOIST>(sv, tol_rot * tol_rot, tiny2);
   return true;
 }
 
 // Wave 1's part: W from the identity through the posted rounds.
+template <bool HOIST>
 __device__ void solve16_wave1(const Solve16& sv) {
   const int lane = threadIdx.x & 63;
 #pragma unroll
@@ -501,7 +505,7 @@ __device__ void solve16_wave1(const Solve16& sv) {
   wait_round(sv.rr, 1);
   if (__hip_atomic_load(sv.rr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == kRoundsNone)
     return;
-  w_rounds16<1>(sv);
+  w_rounds16<1, HOIST>(sv);
 }
 
 // From the panel Gram matrix in sm.Gs[0]: record the largest relative
@@ -515,7 +519,7 @@ __device__ void solve16_wave1(const Solve16& sv) {
 // (for the whole block) when there is nothing to apply.  A function of
 // sm.Gs[0] alone: blocks holding the same G compute bitwise the same
 // rotation.
-template <int BW>
+template <int BW, bool HOIST = false>
 __device__ bool panel_solve(double* rec, double tol_rot, double tiny2, PairSmem<BW>& sm,
                             int& wsel) {
   constexpr int PW = 2 * BW;
@@ -529,10 +533,10 @@ __device__ bool panel_solve(double* rec, double tol_rot, double tiny2, PairSmem<
     __syncthreads();
     const int wid = tid >> 6;
     if (wid == 0) {
-      const bool go = solve16_wave0(sv, rec, tol_rot, tiny2);
+      const bool go = solve16_wave0<HOIST>(sv, rec, tol_rot, tiny2);
       if (tid == 0) sm.go = go ? 1 : 0;
     } else if (wid == 1) {
-      solve16_wave1(sv);
+      solve16_wave1<HOIST>(sv);
     }
     __syncthreads();
     wsel = 0;
@@ -1064,8 +1068,8 @@ __global__ __launch_bounds__(256, 2) void k_jacobi_apply_r(JacobiArgs a, int swe
   mask_panel8<NC>(xa, xe, C, lcol, px);
   mask_panel8<NC>(wa, we, C, lcol, pw);
   int wsel = 0;
-  if (!panel_solve<8>(q == 0 ? a.offs + (size_t)sweep * a.nt + t : nullptr, a.tol_rot, a.tiny2,
-                      sm, wsel))
+  if (!panel_solve<8, true>(q == 0 ? a.offs + (size_t)sweep * a.nt + t : nullptr, a.tol_rot,
+                            a.tiny2, sm, wsel))
     return;
   panel_apply8r<NC>(X, (size_t)a.ldx, xa, xe, C, ocol[0], px, sm, wsel);
   panel_apply8r<NC>(W, (size_t)a.ldw, wa, we, C, ocol[0], pw, sm, wsel);
