@@ -181,7 +181,7 @@ tn_status tn_apply_u2(tn_mps* h, size_t k, const double* op);
  * Same update, kernels and result as tn_apply_u2 (mps.rs:1332-1387), with
  * the update's workspace (X, W, per-column arrays, row-split partial Gram
  * matrices, the QR preconditioner's buffers, the descriptor) in a static
- * device array of 14.3 MiB per GPU,
+ * device array of 18.3 MiB per GPU,
  * held until the process ends, instead of the engine's.  Limit: the bonds
  * next to the update (k-1 and k+1) at most 256, so theta is at most
  * 512 x 512; above that TN_INVALID_ARG (tn_last_error says why) and the state

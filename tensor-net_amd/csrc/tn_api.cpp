@@ -202,7 +202,8 @@ constexpr size_t kScrQrE = kScrGpart + scratch_align(sizeof(double2) * kScrGpart
 constexpr size_t kScrQrTElems = (size_t)(kScratchCols / kQrNb) * kQrNb * kQrNb;
 constexpr size_t kScrQrT = kScrQrE + scratch_align(sizeof(double2) * kScratchCols * kScratchCols);
 constexpr size_t kScrQrPerm = kScrQrT + scratch_align(sizeof(double2) * kScrQrTElems);
-constexpr size_t kScrDesc = kScrQrPerm + scratch_align(sizeof(int) * kScratchCols);
+constexpr size_t kScrXb = kScrQrPerm + scratch_align(sizeof(int) * kScratchCols);
+constexpr size_t kScrDesc = kScrXb + scratch_align(sizeof(double2) * kScratchCols * kScratchCols);
 constexpr size_t kScratchBytes = kScrDesc + scratch_align(sizeof(UpdDesc));
 
 static __device__ __align__(256) unsigned char scratch[kScratchBytes];
@@ -293,6 +294,11 @@ struct tn_engine {
   double2* qrT = nullptr;
   int* qrPerm = nullptr;
   size_t qr_tslot = 0;   // elements of qrT per slot
+  // second X of the first xb_slots slots: the one-launch row-split step
+  // ping-pongs X between the two (a lone SVD or a few)
+  double2* Xb = nullptr;
+  int xb_slots = 0;
+  bool split_fused = true;  // one-launch row-split steps (TNMPS_SPLIT_FUSED)
   int qr_min = 32;       // least C for which a batch takes the QR path (TNMPS_QR_MIN)
   int cus = 0;           // compute units of the device
   int list_min = 64;     // tasks from which the step kernel runs over act (TNMPS_LIST_MIN)
@@ -359,9 +365,10 @@ struct tn_engine {
   void release_ws() {
     hfree(X); hfree(W); hfree(sraw); hfree(sval); hfree(fac); hfree(perm);
     hfree(part); hfree(offs); hfree(info); hfree(gpart); hfree(act);
-    hfree(qrE); hfree(qrT); hfree(qrPerm);
-    qrE = qrT = nullptr;
+    hfree(qrE); hfree(qrT); hfree(qrPerm); hfree(Xb);
+    qrE = qrT = Xb = nullptr;
     qrPerm = nullptr;
+    xb_slots = 0;
     hhfree(h_offs);
     h_offs = nullptr;
     gpart = nullptr;
@@ -428,6 +435,7 @@ struct tn_engine {
     cus = prop.multiProcessorCount;
     split_max = std::max(1, std::min(kMaxSplit, (int)env_double("TNMPS_SPLIT_MAX", kMaxSplit)));
     split_rows = std::max(64, (int)env_double("TNMPS_SPLIT_ROWS", 128));
+    split_fused = env_double("TNMPS_SPLIT_FUSED", 1) != 0;
     list_min = std::max(1, (int)env_double("TNMPS_LIST_MIN", 64));
     pool_min = std::max(0, (int)env_double("TNMPS_POOL_MIN", 0));
     size_t gp = 0;
@@ -592,6 +600,8 @@ struct tn_engine {
     TN_HIP(hipMalloc(&qrE, sizeof(double2) * xe * ns));
     TN_HIP(hipMalloc(&qrT, sizeof(double2) * qr_tslot * ns));
     TN_HIP(hipMalloc(&qrPerm, sizeof(int) * ldw * ns));
+    xb_slots = std::min(ns, kFusedSlots);
+    TN_HIP(hipMalloc(&Xb, sizeof(double2) * xe * xb_slots));
     slots = ns;
     return upd.ensure(ns, stream);
   }
@@ -795,6 +805,8 @@ struct Workspace {
   double2* qrT = nullptr;
   int* qrPerm = nullptr;
   size_t qr_tslot = 0;
+  double2* Xb = nullptr;    // or null: row-split steps take two launches
+  int xb_slots = 0;
   int ldx = 0, ldw = 0;
   UpdDesc* desc = nullptr;
 };
@@ -806,6 +818,10 @@ Workspace engine_workspace(const tn_engine* e) {
   w.gpart_slots = e->gpart_slots; w.offs = e->offs; w.info = e->info; w.act = e->act;
   w.h_offs = e->h_offs; w.ldx = e->ldx; w.ldw = e->ldw;
   w.qrE = e->qrE; w.qrT = e->qrT; w.qrPerm = e->qrPerm; w.qr_tslot = e->qr_tslot;
+  if (e->split_fused) {
+    w.Xb = e->Xb;
+    w.xb_slots = e->xb_slots;
+  }
   return w;
 }
 
@@ -839,6 +855,7 @@ tn_status jacobi_sweeps(tn_engine* e, const Workspace& w, int nt, size_t maxR, s
     S *= 2;
   JacobiArgs ja{};
   ja.X = w.X;
+  ja.Xb = nullptr;
   ja.W = w.W;
   ja.gpart = w.gpart;
   ja.offs = w.offs;
@@ -865,6 +882,10 @@ tn_status jacobi_sweeps(tn_engine* e, const Workspace& w, int nt, size_t maxR, s
   ja.tol_rot = tol.tol_rot;
   ja.conv_tol = conv_tol;
   ja.tiny2 = tol.tiny_norm * tol.tiny_norm;
+  // A row split of few tasks takes one launch per step when the workspace
+  // has their second X buffers and the panels fit the kernel's registers.
+  const bool fused = S > 1 && w.Xb && nt <= w.xb_slots && jacobi_fused_fits(ja, bw);
+  if (fused) ja.Xb = w.Xb;
   // Sweeps are enqueued one ahead while the last sweep read back is still far
   // from convergence (max off-diagonal above kAheadOff: with quadratic
   // convergence the next sweep cannot reach conv_tol, so the one after it is
@@ -888,6 +909,9 @@ tn_status jacobi_sweeps(tn_engine* e, const Workspace& w, int nt, size_t maxR, s
         if (S == 1) {
           launch_jacobi_step(ja, npairs, bw, launched, s, use_list ? open_known : nt,
                              e->stream);
+          ++step_launches;
+        } else if (fused) {
+          launch_jacobi_fused(ja, npairs, launched, s, e->stream);
           ++step_launches;
         } else {
           launch_jacobi_gram(ja, npairs, bw, launched, s, e->stream);
@@ -934,6 +958,24 @@ tn_status jacobi_sweeps(tn_engine* e, const Workspace& w, int nt, size_t maxR, s
         no_conv(t);
         e->host_flags |= kFlagNoConv;
       }
+  }
+  // One-launch row split: a task's X is in Xb when it ran an odd number of
+  // sweeps (a sweep is an odd number of steps, each writing the other
+  // buffer).  It ran through the first sweep whose maxima were at or below
+  // the tolerance (the device skips it after that), else every sweep
+  // launched.
+  if (fused) {
+    for (int t = 0; t < nt; ++t) {
+      int ran = launched;
+      for (int j = 0; j < checked; ++j)
+        if (w.h_offs[(size_t)j * nt + t] <= conv_tol) {
+          ran = j + 1;
+          break;
+        }
+      if (ran & 1)
+        TN_HIP(hipMemcpyAsync(w.X + (size_t)t * ja.xslot, w.Xb + (size_t)t * ja.xslot,
+                              sizeof(double2) * ja.xslot, hipMemcpyDeviceToDevice, e->stream));
+    }
   }
   *sweeps_run = checked;
   *launches = step_launches;
@@ -2276,6 +2318,10 @@ tn_status tn_apply_u2_noalloc(tn_mps* h, size_t k, const double* op) {
   w.qrPerm = (int*)(b + kScrQrPerm);
   w.qr_tslot = kScrQrTElems;
   w.gpart_slots = 1;
+  if (e->split_fused) {
+    w.Xb = (double2*)(b + kScrXb);
+    w.xb_slots = 1;
+  }
   w.h_offs = h_offs;
   w.ldx = w.ldw = kScratchCols;
   w.desc = (UpdDesc*)(b + kScrDesc);

@@ -34,6 +34,7 @@ constexpr int kMaxSweeps = 64;  // outer Jacobi sweep limit (slow graded inputs 
 constexpr int kNPart = 64;      // blocks of the split/renormalise kernels
 constexpr int kMaxCols = 1024;  // largest C (= min(m, n)) the SVD accepts
 constexpr int kMaxSplit = 8;    // workgroups per block pair in the row-split step
+constexpr int kFusedSlots = 4;  // tasks whose row-split steps can take one launch (second X buffers)
 constexpr int kMeasParts = 64;  // most workgroups per measured site
 constexpr int kQrNb = 16;       // Householder panel width of the QR preconditioner
 constexpr int kQrMaxRows = 512; // most rows R of an SVD the QR preconditioner takes
@@ -91,6 +92,7 @@ struct UpdDesc {
 // entry.
 struct JacobiArgs {
   double2* X;            // X of task t at X + t * xslot (ldx rows)
+  double2* Xb;           // or null: a second X (same slots) for the one-launch row split
   double2* W;            // W of task t at W + t * wslot (ldw rows)
   double2* gpart;        // row-split partial Gram matrices, gslot per task:
                          // [pair][slice][PW * PW]

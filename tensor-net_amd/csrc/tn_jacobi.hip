@@ -721,6 +721,36 @@ __device__ __forceinline__ void mask_panel8(int a, int rend, int cend, const int
     if (!(lcol[q] < cend && r < rend)) pn.v[0][q] = make_double2(0.0, 0.0);
 }
 
+// Zero every chunk's elements outside rows [a, rend) and columns below cend.
+template <int NC>
+__device__ __forceinline__ void mask_panel8_all(int a, int rend, int cend, const int (&lcol)[4],
+                                                Panel8<NC>& pn) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    const int r = a + 64 * c + lane;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (!(lcol[q] < cend && r < rend)) pn.v[c][q] = make_double2(0.0, 0.0);
+  }
+}
+
+// Rows [64 c0, b) of pn's columns (below cend) written to M unchanged (pn's
+// chunk c holds rows 64 c).
+template <int NC>
+__device__ __forceinline__ void panel_copy8r(double2* M, size_t ld, int b, int cend,
+                                             const int (&lcol)[4], const Panel8<NC>& pn, int c0) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    if (c < c0) continue;
+    const int r = 64 * c + lane;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (lcol[q] < cend && r < b) M[(size_t)lcol[q] * ld + r] = pn.v[c][q];
+  }
+}
+
 template <int NC>
 __device__ __forceinline__ void put_panel8(const Panel8<NC>& pn, int c, PairSmem<8>& sm) {
   const int rr = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -1075,6 +1105,59 @@ __global__ __launch_bounds__(256, 2) void k_jacobi_apply_r(JacobiArgs a, int swe
   panel_apply8r<NC>(W, (size_t)a.ldw, wa, we, C, ocol[0], pw, sm, wsel);
 }
 
+// Row-split step in one launch: workgroup (p, q) forms the Gram matrix of
+// pair p's whole panel (R <= 64 NCG rows, in registers; every workgroup of
+// the pair forms the same G in the same order, as k_jacobi_step_r does),
+// solves it, and writes its slice q of the rotated rows of X (chunks of the
+// same registers) and of W (at most 64 NCW rows).  X ping-pongs between two
+// buffers: step g = sweep (nbe - 1) + step reads X (g even) or Xb (g odd)
+// and writes the other, so no workgroup can read rows that another
+// workgroup of its pair has already rotated (a workgroup may start after
+// another has finished; with one buffer, that read raced with the write).
+// A pair with nothing to rotate copies its slice across.  A task that has
+// converged writes nothing more: its X is in Xb when it ran an odd number of
+// sweeps (a sweep is an odd number of steps), and the host copies it back.
+// Each slice of W is read and written by its own workgroup only.
+template <int NCG, int NCW>
+__global__ __launch_bounds__(256, 2) void k_jacobi_fused_r(JacobiArgs a, int sweep, int step) {
+  __shared__ PairSmem<8> sm;
+  const int t = blockIdx.y;
+  int p, q;
+  split_block(a.S, p, q);
+  int lcol[4], ocol[4];
+  thread_cols<8>(a.nbe, step, p, lcol, ocol);
+  int xa, xb, wa, wb;
+  split_rows(a.Rh, a.S, q, xa, xb);
+  split_rows(a.Ch, a.S, q, wa, wb);
+  const int g = sweep * (a.nbe - 1) + step;
+  const double2* Xi = ((g & 1) ? a.Xb : a.X) + (size_t)t * a.xslot;
+  double2* Xo = ((g & 1) ? a.X : a.Xb) + (size_t)t * a.xslot;
+  double2* W = a.W + (size_t)t * a.wslot;
+  // a row-split launch holds few tasks: every chunk loads at entry, for the
+  // host's bounds, and is masked once the task's R and C have arrived
+  Panel8<NCG> px;
+  Panel8<NCW> pw;
+  load_panel8<NCG, 0, NCG>(Xi, (size_t)a.ldx, 0, a.Rh, a.ldw, lcol, px);
+  load_panel8<NCW, 0, NCW>(W, (size_t)a.ldw, wa, wb, a.ldw, lcol, pw);
+  issue_fence();
+  const int* inf = a.info + (size_t)t * kInfoLen;
+  const int R = inf[kInfoRJ], C = inf[kInfoC];
+  if (converged(a, t, sweep)) return;
+  const int xe = min(xb, R), we = min(wb, C);
+  mask_panel8_all<NCG>(0, R, C, lcol, px);
+  mask_panel8_all<NCW>(wa, we, C, lcol, pw);
+  panel_gram8r<NCG>(0, R, px, sm);
+  int wsel = 0;
+  if (panel_solve<8, true>(q == 0 ? a.offs + (size_t)sweep * a.nt + t : nullptr, a.tol_rot,
+                           a.tiny2, sm, wsel)) {
+    // slices start on 64-row boundaries: rows [xa, xe) are chunks xa / 64 ..
+    panel_apply8r<NCG>(Xo, (size_t)a.ldx, 0, xe, C, ocol[0], px, sm, wsel, xa >> 6);
+    panel_apply8r<NCW>(W, (size_t)a.ldw, wa, we, C, ocol[0], pw, sm, wsel);
+  } else {
+    panel_copy8r<NCG>(Xo, (size_t)a.ldx, xe, C, lcol, px, xa >> 6);
+  }
+}
+
 // ------------------------------------------------------------- launch
 // 64-row chunks of the widest slice when n rows split into S slices.
 static int slice_chunks(int n, int S) { return (((n + S - 1) / S) + 63) / 64; }
@@ -1112,6 +1195,17 @@ void launch_jacobi_gram(const JacobiArgs& a, int npairs, int bw, int sweep, int 
     hipLaunchKernelGGL(k_jacobi_gram_r<4>, grid, blk, 0, s, a, sweep, step);
   else
     hipLaunchKernelGGL(k_jacobi_gram<8>, grid, blk, 0, s, a, sweep, step);
+}
+bool jacobi_fused_fits(const JacobiArgs& a, int bw) {
+  // (panels of up to 256 rows: a 512-row one spills from the registers)
+  return bw == 8 && a.S > 1 && a.Rh <= 256 && slice_chunks(a.Ch, a.S) <= 4;
+}
+void launch_jacobi_fused(const JacobiArgs& a, int npairs, int sweep, int step, hipStream_t s) {
+  const dim3 grid(npairs * a.S, a.nt), blk(256);
+  if (slice_chunks(a.Ch, a.S) <= 2)
+    hipLaunchKernelGGL((k_jacobi_fused_r<4, 2>), grid, blk, 0, s, a, sweep, step);
+  else
+    hipLaunchKernelGGL((k_jacobi_fused_r<4, 4>), grid, blk, 0, s, a, sweep, step);
 }
 void launch_jacobi_apply(const JacobiArgs& a, int npairs, int bw, int sweep, int step,
                          hipStream_t s) {

@@ -23,6 +23,13 @@ void launch_jacobi_gram(const JacobiArgs& a, int npairs, int bw, int sweep, int 
                         hipStream_t s);
 void launch_jacobi_apply(const JacobiArgs& a, int npairs, int bw, int sweep, int step,
                          hipStream_t s);
+// Row-split step in one launch (every workgroup of a pair forms the whole
+// panel's Gram matrix and writes its slice of the rotated rows into the
+// other X buffer: step g = sweep (nbe - 1) + step reads a.X when g is even,
+// a.Xb when odd); jacobi_fused_fits says whether the panels fit its
+// registers.
+bool jacobi_fused_fits(const JacobiArgs& a, int bw);
+void launch_jacobi_fused(const JacobiArgs& a, int npairs, int sweep, int step, hipStream_t s);
 // QR preconditioner (tn_qr.hip): sort order, blocked Householder QR of the
 // sorted X into E, and Y = R^H into X's slot (info[kInfoRJ] = C); after the
 // Jacobi sweeps on Y, X <- Q (W Sigma) and W <- P U~ Sigma^-1.  Every task
