@@ -53,7 +53,7 @@ METRIC = "gate+SVD updates/s @chi=256 FP64 vs mixed; trajectories/s + p99 latenc
 # rocprofv3 FETCH_SIZE and WRITE_SIZE (KiB per dispatch, one counter per run)
 # scaled by what the same counters read for a plain 1 GiB device copy
 # (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0); profiles/jacobi_single_traj_chi256_hbm_traffic.md.
-PMC_KIB = {"k_jacobi_apply_r<2>": (1698.3, 508.0), "k_jacobi_gram_r<2>": (532.1, 242.4)}
+PMC_KIB = {"k_jacobi_fused_r<4, 2>": (2723.5, 621.6)}
 PMC_SCALE = (2.0, 1.0)
 TRAFFIC_BYTES = {k: 1024.0 * (f * PMC_SCALE[0] + w * PMC_SCALE[1]) for k, (f, w) in PMC_KIB.items()}
 TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of the Jacobi kernels on this workload, "
@@ -671,23 +671,23 @@ def run_single(args, pool, workers: int, host: dict) -> dict:
     value = updates / wall
 
     # The dominant kernels, the SVD's (the QR preconditioner's kernels and the
-    # Jacobi step kernels: row-split gram + solve / apply, or one-workgroup
-    # steps), are credited with the algorithmic R-SVD flops of the SVDs they
-    # perform, over the HIP-event spans of the QR and of the Jacobi sweeps on
-    # the library's stream (dispatch gaps between dependent launches
-    # included).  The traffic is that of the most frequent step dispatch,
-    # the mean of a row-split gram and apply.
+    # Jacobi step kernels: one-launch row-split steps, or one workgroup per
+    # block pair), are credited with the algorithmic R-SVD flops of the SVDs
+    # they perform, over the HIP-event spans of the QR and of the Jacobi
+    # sweeps on the library's stream (dispatch gaps between dependent
+    # launches included).  The traffic is that of the most frequent step
+    # dispatch, a lone refactor's row-split step.
     svd_s = (perf["ms_jacobi"] + perf["ms_qr"]) / 1e3
     achieved = perf["svd_flops"] / svd_s / 1e12 if svd_s > 0 else 0.0
     roof = {
         "bound": "mfma",
-        "kernel": "SVD: QR preconditioner (k_qr_*) + Jacobi steps (k_jacobi_gram_r<2> + "
-                  "k_jacobi_apply_r<2> row-split, k_jacobi_step<8> gate batches)",
+        "kernel": "SVD: QR preconditioner (k_qr_*) + Jacobi steps (k_jacobi_fused_r<4, 2> "
+                  "row-split, k_jacobi_step<8> gate batches)",
         "achieved": round(achieved, 4),
         "peak": FP64_PEAK_TFLOPS,
         "unit": "TFLOP/s",
         "frac": round(achieved / FP64_PEAK_TFLOPS, 5),
-        "traffic": round((TRAFFIC_BYTES["k_jacobi_apply_r<2>"] + TRAFFIC_BYTES["k_jacobi_gram_r<2>"]) / 2),
+        "traffic": round(TRAFFIC_BYTES["k_jacobi_fused_r<4, 2>"]),
         "traffic_source": TRAFFIC_SOURCE,
         "jacobi_only_TFLOPs": round(perf["svd_flops"] / (perf["ms_jacobi"] / 1e3) / 1e12, 4)
         if perf["ms_jacobi"] > 0 else None,
