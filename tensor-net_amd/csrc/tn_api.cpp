@@ -299,6 +299,9 @@ struct tn_engine {
   double2* Xb = nullptr;
   int xb_slots = 0;
   bool split_fused = true;  // one-launch row-split steps (TNMPS_SPLIT_FUSED)
+  // least rows per slice of the one-launch step, whose split doubles past
+  // split_rows while its slices keep at least this many (TNMPS_FUSED_ROWS)
+  int fused_rows = 32;
   int qr_min = 32;       // least C for which a batch takes the QR path (TNMPS_QR_MIN)
   int cus = 0;           // compute units of the device
   int list_min = 64;     // tasks from which the step kernel runs over act (TNMPS_LIST_MIN)
@@ -436,6 +439,7 @@ struct tn_engine {
     split_max = std::max(1, std::min(kMaxSplit, (int)env_double("TNMPS_SPLIT_MAX", kMaxSplit)));
     split_rows = std::max(64, (int)env_double("TNMPS_SPLIT_ROWS", 128));
     split_fused = env_double("TNMPS_SPLIT_FUSED", 1) != 0;
+    fused_rows = std::max(16, (int)env_double("TNMPS_FUSED_ROWS", 32));
     list_min = std::max(1, (int)env_double("TNMPS_LIST_MIN", 64));
     pool_min = std::max(0, (int)env_double("TNMPS_POOL_MIN", 0));
     size_t gp = 0;
@@ -885,7 +889,16 @@ tn_status jacobi_sweeps(tn_engine* e, const Workspace& w, int nt, size_t maxR, s
   // A row split of few tasks takes one launch per step when the workspace
   // has their second X buffers and the panels fit the kernel's registers.
   const bool fused = S > 1 && w.Xb && nt <= w.xb_slots && jacobi_fused_fits(ja, bw);
-  if (fused) ja.Xb = w.Xb;
+  if (fused) {
+    ja.Xb = w.Xb;
+    // its rotation spreads 16-row tiles over the waves, so shorter slices
+    // shorten it: split further while slices keep fused_rows rows and the
+    // launch keeps one workgroup per CU (its LDS holds one per CU)
+    while (S < tu.split_max && (size_t)nt * npairs * S * 2 <= (size_t)e->cus &&
+           maxR >= (size_t)e->fused_rows * S * 2)
+      S *= 2;
+    ja.S = S;
+  }
   // Sweeps are enqueued one ahead while the last sweep read back is still far
   // from convergence (max off-diagonal above kAheadOff: with quadratic
   // convergence the next sweep cannot reach conv_tol, so the one after it is

@@ -735,19 +735,19 @@ __device__ __forceinline__ void mask_panel8_all(int a, int rend, int cend, const
   }
 }
 
-// Rows [64 c0, b) of pn's columns (below cend) written to M unchanged (pn's
+// Rows [lo, b) of pn's columns (below cend) written to M unchanged (pn's
 // chunk c holds rows 64 c).
 template <int NC>
-__device__ __forceinline__ void panel_copy8r(double2* M, size_t ld, int b, int cend,
-                                             const int (&lcol)[4], const Panel8<NC>& pn, int c0) {
+__device__ __forceinline__ void panel_copy8r(double2* M, size_t ld, int lo, int b, int cend,
+                                             const int (&lcol)[4], const Panel8<NC>& pn) {
   const int lane = threadIdx.x & 63;
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
-    if (c < c0) continue;
+    if (64 * c + 64 <= lo) continue;
     const int r = 64 * c + lane;
 #pragma unroll
     for (int q = 0; q < 4; ++q)
-      if (lcol[q] < cend && r < b) M[(size_t)lcol[q] * ld + r] = pn.v[c][q];
+      if (lcol[q] < cend && r >= lo && r < b) M[(size_t)lcol[q] * ld + r] = pn.v[c][q];
   }
 }
 
@@ -817,11 +817,95 @@ __device__ void panel_apply8r(double2* M, size_t ld, int a, int b, int cend, int
   }
 }
 
+// The whole panel of X (chunks 0 .. NCG-1) and the workgroup's slice of W
+// (chunks NCG ..), transposed into LDS once, behind one barrier, for the
+// Gram matrix and both rotations of a row-split step.  Chunk k row r column
+// c at v[k][r][c] (rows padded to 17 entries, as Ps).
+template <int NCG, int NCW>
+struct PanelLds {
+  double2 v[NCG + NCW][64][17];
+};
+
+template <int NC>
+__device__ __forceinline__ void put_panels8(const Panel8<NC>& pn, double2 (*dst)[64][17]) {
+  const int rr = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int c = 0; c < NC; ++c)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) dst[c][rr][wid + 4 * q] = pn.v[c][q];
+}
+
+// sm.Gs[0] = P^H P over rows [0, b) of the panel in chunks[0 ..]: the
+// matrix-core products of panel_gram8r in the same order into the same
+// accumulators (bitwise the same G), with no barrier between chunks.
+template <int NC>
+__device__ void panel_gram8_lds(int b, const double2 (*chunks)[64][17], PairSmem<8>& sm) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  v4d gre[2] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
+  v4d gm[2] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    if (64 * c >= b) break;
+    double2 pv[4];
+#pragma unroll
+    for (int g = 0; g < 4; ++g) pv[g] = chunks[c][16 * wid + 4 * g + (lane >> 4)][lane & 15];
+#pragma unroll
+    for (int g = 0; g < 4; ++g) gram8_acc(pv[g], gre[g & 1], gm[g & 1]);
+  }
+  gram8_finish(gre[0] + gre[1], gm[0] + gm[1], sm);
+}
+
+// Slice rows [xa, xe) of X's panel column oc (into Xo) and [wa, we) of W's
+// <- (copies in LDS) * Wl, in 16-row tiles spread over the four waves (tile
+// i on wave i % 4: the slice of X first, then the slice of W), each tile
+// with panel_apply8r's products.  xa is a multiple of 16 (X's tiles lie in
+// one 64-row chunk each); chunk c of W's copies holds rows wa + 64 c.
+template <int NCG>
+__device__ __forceinline__ void apply8_tiles_lds(double2* Xo, size_t ldx, int xa, int xe,
+                                                 double2* W, size_t ldw, int wa, int we,
+                                                 int cend, int oc,
+                                                 const double2 (*chunks)[64][17],
+                                                 const double2 (&wk)[4], const double (&wks)[4]) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int nxt = xe > xa ? (xe - xa + 15) >> 4 : 0;
+  const int nwt = we > wa ? (we - wa + 15) >> 4 : 0;
+  for (int i = wid; i < nxt + nwt; i += 4) {
+    const bool isx = i < nxt;
+    // tile rows: X's absolute, W's relative to wa
+    const int r0 = isx ? xa + 16 * i : 16 * (i - nxt);
+    const double2(*ck)[17] = chunks[(isx ? 0 : NCG) + (r0 >> 6)];
+    const int rb = r0 & 63;
+    double2 pk[4];
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) pk[kk] = ck[rb + (lane & 15)][4 * kk + (lane >> 4)];
+    v4d ore, oim;
+    apply8_tile(pk, wk, wks, ore, oim);
+    double2* M = isx ? Xo : W;
+    const size_t ld = isx ? ldx : ldw;
+    const int base = isx ? r0 : wa + r0, b = isx ? xe : we;
+    if (oc < cend) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int r = base + (lane >> 4) + 4 * v;
+        if (r < b) M[(size_t)oc * ld + r] = make_double2(ore[v], oim[v]);
+      }
+    }
+  }
+}
+
 // Row range of slice q of S (a power of two) over n rows (slices are
 // multiples of 64 rows).
 __device__ __forceinline__ void split_rows(int n, int S, int q, int& a, int& b) {
   const int sl = __builtin_ctz(S);
   const int per = (((n + S - 1) >> sl) + 63) / 64 * 64;
+  a = min(n, q * per);
+  b = min(n, a + per);
+}
+
+// The same with slices of a multiple of 16 rows (the one-launch kernel's).
+__device__ __forceinline__ void split_rows16(int n, int S, int q, int& a, int& b) {
+  const int sl = __builtin_ctz(S);
+  const int per = (((n + S - 1) >> sl) + 15) / 16 * 16;
   a = min(n, q * per);
   b = min(n, a + per);
 }
@@ -1106,10 +1190,16 @@ __global__ __launch_bounds__(256, 2) void k_jacobi_apply_r(JacobiArgs a, int swe
 }
 
 // Row-split step in one launch: workgroup (p, q) forms the Gram matrix of
-// pair p's whole panel (R <= 64 NCG rows, in registers; every workgroup of
-// the pair forms the same G in the same order, as k_jacobi_step_r does),
-// solves it, and writes its slice q of the rotated rows of X (chunks of the
-// same registers) and of W (at most 64 NCW rows).  X ping-pongs between two
+// pair p's whole panel (R <= 64 NCG rows; every workgroup of the pair forms
+// the same G in the same order, as k_jacobi_step_r does), solves it, and
+// writes its slice q of the rotated rows of X and of W (slices of a
+// multiple of 16 rows, W's at most 64 NCW rows).  The panel and the slice
+// of W are loaded into registers at entry and transposed into LDS once,
+// behind one barrier: the Gram matrix's matrix-core products run back to
+// back from there, and the rotation's 16-row tiles of both slices, spread
+// over the four waves, read the same copies.  PairSmem and PanelLds<4, 2>
+// take about 136 KiB of LDS, one workgroup per CU, which a row-split launch
+// never exceeds (at most one workgroup per CU).  X ping-pongs between two
 // buffers: step g = sweep (nbe - 1) + step reads X (g even) or Xb (g odd)
 // and writes the other, so no workgroup can read rows that another
 // workgroup of its pair has already rotated (a workgroup may start after
@@ -1119,16 +1209,17 @@ __global__ __launch_bounds__(256, 2) void k_jacobi_apply_r(JacobiArgs a, int swe
 // sweeps (a sweep is an odd number of steps), and the host copies it back.
 // Each slice of W is read and written by its own workgroup only.
 template <int NCG, int NCW>
-__global__ __launch_bounds__(256, 2) void k_jacobi_fused_r(JacobiArgs a, int sweep, int step) {
+__global__ __launch_bounds__(256, 1) void k_jacobi_fused_r(JacobiArgs a, int sweep, int step) {
   __shared__ PairSmem<8> sm;
+  __shared__ PanelLds<NCG, NCW> pl;
   const int t = blockIdx.y;
   int p, q;
   split_block(a.S, p, q);
   int lcol[4], ocol[4];
   thread_cols<8>(a.nbe, step, p, lcol, ocol);
   int xa, xb, wa, wb;
-  split_rows(a.Rh, a.S, q, xa, xb);
-  split_rows(a.Ch, a.S, q, wa, wb);
+  split_rows16(a.Rh, a.S, q, xa, xb);
+  split_rows16(a.Ch, a.S, q, wa, wb);
   const int g = sweep * (a.nbe - 1) + step;
   const double2* Xi = ((g & 1) ? a.Xb : a.X) + (size_t)t * a.xslot;
   double2* Xo = ((g & 1) ? a.X : a.Xb) + (size_t)t * a.xslot;
@@ -1146,15 +1237,25 @@ __global__ __launch_bounds__(256, 2) void k_jacobi_fused_r(JacobiArgs a, int swe
   const int xe = min(xb, R), we = min(wb, C);
   mask_panel8_all<NCG>(0, R, C, lcol, px);
   mask_panel8_all<NCW>(wa, we, C, lcol, pw);
-  panel_gram8r<NCG>(0, R, px, sm);
+  put_panels8<NCG>(px, pl.v);
+  put_panels8<NCW>(pw, pl.v + NCG);
+  __syncthreads();
+  panel_gram8_lds<NCG>(R, pl.v, sm);
   int wsel = 0;
   if (panel_solve<8, true>(q == 0 ? a.offs + (size_t)sweep * a.nt + t : nullptr, a.tol_rot,
                            a.tiny2, sm, wsel)) {
-    // slices start on 64-row boundaries: rows [xa, xe) are chunks xa / 64 ..
-    panel_apply8r<NCG>(Xo, (size_t)a.ldx, 0, xe, C, ocol[0], px, sm, wsel, xa >> 6);
-    panel_apply8r<NCW>(W, (size_t)a.ldw, wa, we, C, ocol[0], pw, sm, wsel);
+    double2 wk[4];
+    double wks[4];
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) {
+      wk[kk] = sm.Ws(wsel, 4 * kk + (lane >> 4), lane & 15);
+      wks[kk] = wk[kk].x + wk[kk].y;
+    }
+    apply8_tiles_lds<NCG>(Xo, (size_t)a.ldx, xa, xe, W, (size_t)a.ldw, wa, we, C, ocol[0], pl.v,
+                          wk, wks);
   } else {
-    panel_copy8r<NCG>(Xo, (size_t)a.ldx, xe, C, lcol, px, xa >> 6);
+    panel_copy8r<NCG>(Xo, (size_t)a.ldx, xa, xe, C, lcol, px);
   }
 }
 
@@ -1196,16 +1297,17 @@ void launch_jacobi_gram(const JacobiArgs& a, int npairs, int bw, int sweep, int 
   else
     hipLaunchKernelGGL(k_jacobi_gram<8>, grid, blk, 0, s, a, sweep, step);
 }
+// Rows of the widest slice of the one-launch kernel (multiples of 16).
+static int slice_rows16(int n, int S) { return (((n + S - 1) / S) + 15) / 16 * 16; }
+
 bool jacobi_fused_fits(const JacobiArgs& a, int bw) {
-  // (panels of up to 256 rows: a 512-row one spills from the registers)
-  return bw == 8 && a.S > 1 && a.Rh <= 256 && slice_chunks(a.Ch, a.S) <= 4;
+  // (panels of up to 256 rows: a 512-row one spills from the registers;
+  // slices of W of up to 128 rows, the kernel's two chunks)
+  return bw == 8 && a.S > 1 && a.Rh <= 256 && slice_rows16(a.Ch, a.S) <= 128;
 }
 void launch_jacobi_fused(const JacobiArgs& a, int npairs, int sweep, int step, hipStream_t s) {
   const dim3 grid(npairs * a.S, a.nt), blk(256);
-  if (slice_chunks(a.Ch, a.S) <= 2)
-    hipLaunchKernelGGL((k_jacobi_fused_r<4, 2>), grid, blk, 0, s, a, sweep, step);
-  else
-    hipLaunchKernelGGL((k_jacobi_fused_r<4, 4>), grid, blk, 0, s, a, sweep, step);
+  hipLaunchKernelGGL((k_jacobi_fused_r<4, 2>), grid, blk, 0, s, a, sweep, step);
 }
 void launch_jacobi_apply(const JacobiArgs& a, int npairs, int bw, int sweep, int step,
                          hipStream_t s) {
