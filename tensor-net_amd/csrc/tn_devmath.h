@@ -54,6 +54,22 @@ __device__ __forceinline__ double block_max(double v, double* buf) {
   return s;
 }
 
+// *p when ok, else 0.  The load issues either way (from `safe`, a valid
+// address, when not ok) and the result is masked bitwise with a mask the
+// compiler cannot see through, so a run of these issues back to back: as a
+// conditional load (which the compiler makes of a plain select), each got a
+// branch (and in some kernels a wait for its data) of its own.
+__device__ __forceinline__ double2 ld_or_zero(const double2* p, bool ok, const double2* safe) {
+  const double2 v = *(ok ? p : safe);
+  // one 32-bit mask for all four words (one register live while the load
+  // is in flight)
+  unsigned m = ok ? ~0u : 0u;
+  asm volatile("" : "+v"(m));
+  const unsigned long long m2 = ((unsigned long long)m << 32) | m;
+  return make_double2(__longlong_as_double(__double_as_longlong(v.x) & (long long)m2),
+                      __longlong_as_double(__double_as_longlong(v.y) & (long long)m2));
+}
+
 // Cross-lane moves and wave sums on the DPP network (no LDS): CTRL is a
 // DPP control word (quad_perm 0x00-0xff, row_half_mirror 0x141, row_mirror
 // 0x140).  A double moves as its two 32-bit halves.
@@ -82,6 +98,17 @@ __device__ __forceinline__ double wave_sum_dpp(double v) {
   v += dpp_mov<0x141>(v);
   v += dpp_mov<0x140>(v);
   return (read_lane(v, 0) + read_lane(v, 16)) + (read_lane(v, 32) + read_lane(v, 48));
+}
+
+// Maximum over the 64 lanes of a wave, returned to every lane, by the same
+// moves (fmax is exact: any order gives the same value).  Every lane of the
+// wave must be active.
+__device__ __forceinline__ double wave_max_dpp(double v) {
+  v = fmax(v, dpp_mov<0xB1>(v));
+  v = fmax(v, dpp_mov<0x4E>(v));
+  v = fmax(v, dpp_mov<0x141>(v));
+  v = fmax(v, dpp_mov<0x140>(v));
+  return fmax(fmax(read_lane(v, 0), read_lane(v, 16)), fmax(read_lane(v, 32), read_lane(v, 48)));
 }
 
 // The same for both parts of a complex value, the two chains interleaved.

@@ -471,16 +471,17 @@ template <bool HOIST>
 __device__ bool solve16_wave0(const Solve16& sv, double* rec, double tol_rot, double tiny2) {
   const int lane = threadIdx.x & 63;
   double r2 = 0.0;
+  // the four ratios straight-line (every division issued, masked after) and
+  // their maximum over the wave on the DPP network
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     const int e = lane + 64 * q, gi = e >> 4, gj = e & 15;
-    if (gi < gj) {
-      const double ga = sv.G[0][gi][gi].x, gb = sv.G[0][gj][gj].x;
-      if (ga > tiny2 && gb > tiny2) r2 = fmax(r2, cabs2(sv.G[0][gi][gj]) / (ga * gb));
-    }
+    const double ga = sv.G[0][gi][gi].x, gb = sv.G[0][gj][gj].x;
+    const bool ok = gi < gj && ga > tiny2 && gb > tiny2;
+    const double rt = cabs2(sv.G[0][gi][gj]) / (ok ? ga * gb : 1.0);
+    r2 = fmax(r2, ok ? rt : 0.0);
   }
-#pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) r2 = fmax(r2, __shfl_xor(r2, o, 64));
+  r2 = wave_max_dpp(r2);
   const double off = sqrt(r2);
   if (rec && lane == 0 && off > 0.0)
     atomicMax((unsigned long long*)rec, (unsigned long long)__double_as_longlong(off));

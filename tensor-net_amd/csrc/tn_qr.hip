@@ -81,19 +81,6 @@ __device__ __forceinline__ TaskQr task_qr(const QrArgs& a, int i) {
   return q;
 }
 
-// *p when ok, else 0.  The load issues either way (from `safe`, a valid
-// address, when not ok) and the result is masked bitwise with a mask the
-// compiler cannot see through, so a run of these issues back to back: as a
-// conditional load (which the compiler makes of a plain select), each got a
-// branch and a wait for its data of its own.
-__device__ __forceinline__ double2 ld_or_zero(const double2* p, bool ok, const double2* safe) {
-  const double2 v = *(ok ? p : safe);
-  unsigned long long m = ok ? ~0ull : 0ull;
-  asm volatile("" : "+v"(m));
-  return make_double2(__longlong_as_double(__double_as_longlong(v.x) & (long long)m),
-                      __longlong_as_double(__double_as_longlong(v.y) & (long long)m));
-}
-
 // Column c of the matrix a panel or trailing update reads: panel 0's pass
 // reads X through the sort order, later ones the factored matrix E.
 __device__ __forceinline__ const double2* src_col(const QrArgs& a, const TaskQr& q, int jp,
