@@ -111,6 +111,20 @@ __device__ __forceinline__ double wave_max_dpp(double v) {
   return fmax(fmax(read_lane(v, 0), read_lane(v, 16)), fmax(read_lane(v, 32), read_lane(v, 48)));
 }
 
+// The value of the first lane of the lane's group of 8 (lane l & ~7), on the
+// DPP network: a quad broadcast (quad_perm [0,0,0,0]), then row_shr:4 into
+// lanes 4-7 of each group only (banks 1 and 3 of each row).  Every lane of
+// the wave must be active.
+__device__ __forceinline__ int bcast8_first(int x) {
+  const int v = __builtin_amdgcn_mov_dpp(x, 0x00, 0xF, 0xF, false);
+  return __builtin_amdgcn_update_dpp(v, v, 0x114, 0xF, 0xA, false);
+}
+__device__ __forceinline__ double bcast8_first(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = bcast8_first((int)b), hi = bcast8_first((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
 // The same for both parts of a complex value, the two chains interleaved.
 __device__ __forceinline__ double2 wave_csum_dpp(double2 v) {
   v.x += dpp_mov<0xB1>(v.x);

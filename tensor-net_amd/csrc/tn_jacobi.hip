@@ -362,11 +362,12 @@ struct Solve16 {
   int* rr;               // last round posted
 };
 
-// Wave 0, round ROUND: lane l computes the rotation of column pair l % 8
-// (the eight lanes of a column pair compute bitwise the same one), takes
-// the rotation of row pair l / 8 from lane l / 8 (which computed it as its
-// column pair's) through a lane shuffle, computes the 2 x 2 block (row pair
-// l / 8, column pair l % 8) of G' = Qa^H G Qb and posts its column pair's
+// Wave 0, round ROUND: lane l = 8 g + i takes row pair g and column pair
+// (i + g) % 8, so every group of 8 lanes covers the 8 column pairs and its
+// first lane's column pair is its row pair.  The lane computes its column
+// pair's rotation (the eight lanes of a pair compute bitwise the same one),
+// takes its row pair's from the first lane of its group on the DPP network,
+// computes its 2 x 2 block of G' = Qa^H G Qb and posts its column pair's
 // rotation.  Reads G[CUR], writes G[CUR ^ 1].
 template <int ROUND, int CUR, bool HOIST>
 __device__ __forceinline__ void g_round16(const Solve16& sv, double tol2, double tiny2) {
@@ -383,9 +384,9 @@ __device__ __forceinline__ void g_round16(const Solve16& sv, double tol2, double
   if constexpr (!HOIST) asm volatile("" : "+v"(lane));
   double2 (*G)[17] = sv.G[CUR];
   double2 (*Gn)[17] = sv.G[NXT];
-  const int pr = lane >> 3;
+  const int pr = lane >> 3, pc = (lane + pr) & 7;
   const int x1 = first(pr), x2 = x1 ^ ROUND;
-  const int y1 = first(lane & 7), y2 = y1 ^ ROUND;
+  const int y1 = first(pc), y2 = y1 ^ ROUND;
   const double ay = G[y1][y1].x, by = G[y2][y2].x;
   const double2 cy = G[y1][y2];
   const double2 g11 = G[x1][y1], g12 = G[x1][y2];
@@ -395,8 +396,8 @@ __device__ __forceinline__ void g_round16(const Solve16& sv, double tol2, double
   double csb;
   double2 snb;
   rot2(ay, by, cy, tol2, tiny2, csb, snb);
-  const double csa = __shfl(csb, pr, 64);
-  const double2 sna = make_double2(__shfl(snb.x, pr, 64), __shfl(snb.y, pr, 64));
+  const double csa = bcast8_first(csb);
+  const double2 sna = make_double2(bcast8_first(snb.x), bcast8_first(snb.y));
   // T = G[block] Qb, then G' = Qa^H T, with Q = [[cs, sn], [-conj(sn), cs]]
   const double2 msb = make_double2(-snb.x, snb.y);  // -conj(sn_b)
   const double2 t11 = cadd(cscale(g11, csb), cmul(g12, msb));
@@ -409,8 +410,8 @@ __device__ __forceinline__ void g_round16(const Solve16& sv, double tol2, double
   Gn[x1][y2] = cadd(cscale(t12, csa), cmul(nsa, t22));
   Gn[x2][y1] = cadd(cmul(csa_, t11), cscale(t21, csa));
   Gn[x2][y2] = cadd(cmul(csa_, t12), cscale(t22, csa));
-  sv.rcs[8 * ROUND + (lane & 7)] = csb;
-  sv.rsn[8 * ROUND + (lane & 7)] = snb;
+  sv.rcs[8 * ROUND + pc] = csb;
+  sv.rsn[8 * ROUND + pc] = snb;
 }
 
 template <int ROUND, int CUR, bool HOIST>
