@@ -912,10 +912,15 @@ __device__ __forceinline__ void split_rows16(int n, int S, int q, int& a, int& b
   b = min(n, a + per);
 }
 
-// Block pair p and row slice q of a row-split workgroup.
+// Block pair p and row slice q of a row-split workgroup (grid npairs x S
+// workgroups): block p + npairs q.  Workgroups are dealt to the eight XCDs
+// in turn, so for npairs a multiple of 8 every workgroup of a pair runs on
+// one XCD, and their loads of the pair's whole panel (and of its partial
+// Gram matrices) share one L2.
 __device__ __forceinline__ void split_block(int S, int& p, int& q) {
-  p = (int)blockIdx.x >> __builtin_ctz(S);
-  q = (int)blockIdx.x & (S - 1);
+  const int npairs = (int)gridDim.x >> __builtin_ctz(S);
+  p = (int)blockIdx.x % npairs;
+  q = (int)blockIdx.x / npairs;
 }
 
 // Convergence state of task t before `sweep`: true when its previous sweep
