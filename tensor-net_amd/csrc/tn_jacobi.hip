@@ -521,7 +521,7 @@ __device__ void solve16_wave1(const Solve16& sv) {
 // (for the whole block) when there is nothing to apply.  A function of
 // sm.Gs[0] alone: blocks holding the same G compute bitwise the same
 // rotation.
-template <int BW, bool HOIST = false>
+template <int BW, bool HOIST = false, bool RR_READY = false>
 __device__ bool panel_solve(double* rec, double tol_rot, double tiny2, PairSmem<BW>& sm,
                             int& wsel) {
   constexpr int PW = 2 * BW;
@@ -531,8 +531,12 @@ __device__ bool panel_solve(double* rec, double tol_rot, double tiny2, PairSmem<
     // chunks are in registers meanwhile); W lands in Ws[0]
     const Solve16 sv{sm.Gs, (double2(*)[17]) & sm.Ws(0, 0, 0), (double*)&sm.Ps(0, 0),
                      &sm.Ps(8, 0), &sm.rr};
-    if (tid == 0) sm.rr = 0;
-    __syncthreads();
+    // (RR_READY: the caller zeroed sm.rr, and a barrier after that and after
+    // its last use of sm.Gs[0] and sm.buf precedes this call)
+    if constexpr (!RR_READY) {
+      if (tid == 0) sm.rr = 0;
+      __syncthreads();
+    }
     const int wid = tid >> 6;
     if (wid == 0) {
       const bool go = solve16_wave0<HOIST>(sv, rec, tol_rot, tiny2);
@@ -1204,9 +1208,9 @@ __global__ __launch_bounds__(256, 2) void k_jacobi_apply_r(JacobiArgs a, int swe
 // of W are loaded into registers at entry and transposed into LDS once,
 // behind one barrier: the Gram matrix's matrix-core products run back to
 // back from there, and the rotation's 16-row tiles of both slices, spread
-// over the four waves, read the same copies.  PairSmem and PanelLds<4, 2>
-// take about 136 KiB of LDS, one workgroup per CU, which a row-split launch
-// never exceeds (at most one workgroup per CU).  X ping-pongs between two
+// over the four waves, read the same copies.  PairSmem and PanelLds take
+// about 136 KiB of LDS (NCW = 2; 119 KiB for NCW = 1), one workgroup per CU,
+// which a row-split launch never exceeds (at most one workgroup per CU).  X ping-pongs between two
 // buffers: step g = sweep (nbe - 1) + step reads X (g even) or Xb (g odd)
 // and writes the other, so no workgroup can read rows that another
 // workgroup of its pair has already rotated (a workgroup may start after
@@ -1244,13 +1248,14 @@ __global__ __launch_bounds__(256, 1) void k_jacobi_fused_r(JacobiArgs a, int swe
   const int xe = min(xb, R), we = min(wb, C);
   mask_panel8_all<NCG>(0, R, C, lcol, px);
   mask_panel8_all<NCW>(wa, we, C, lcol, pw);
+  if (threadIdx.x == 0) sm.rr = 0;  // the solve's round word, ordered by the barrier below
   put_panels8<NCG>(px, pl.v);
   put_panels8<NCW>(pw, pl.v + NCG);
   __syncthreads();
-  panel_gram8_lds<NCG>(R, pl.v, sm);
+  panel_gram8_lds<NCG>(R, pl.v, sm);  // ends with a barrier after its last use of sm.buf
   int wsel = 0;
-  if (panel_solve<8, true>(q == 0 ? a.offs + (size_t)sweep * a.nt + t : nullptr, a.tol_rot,
-                           a.tiny2, sm, wsel)) {
+  if (panel_solve<8, true, true>(q == 0 ? a.offs + (size_t)sweep * a.nt + t : nullptr,
+                                 a.tol_rot, a.tiny2, sm, wsel)) {
     double2 wk[4];
     double wks[4];
     const int lane = threadIdx.x & 63;
@@ -1314,7 +1319,10 @@ bool jacobi_fused_fits(const JacobiArgs& a, int bw) {
 }
 void launch_jacobi_fused(const JacobiArgs& a, int npairs, int sweep, int step, hipStream_t s) {
   const dim3 grid(npairs * a.S, a.nt), blk(256);
-  hipLaunchKernelGGL((k_jacobi_fused_r<4, 2>), grid, blk, 0, s, a, sweep, step);
+  if (slice_rows16(a.Ch, a.S) <= 64)
+    hipLaunchKernelGGL((k_jacobi_fused_r<4, 1>), grid, blk, 0, s, a, sweep, step);
+  else
+    hipLaunchKernelGGL((k_jacobi_fused_r<4, 2>), grid, blk, 0, s, a, sweep, step);
 }
 void launch_jacobi_apply(const JacobiArgs& a, int npairs, int bw, int sweep, int step,
                          hipStream_t s) {
