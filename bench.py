@@ -53,7 +53,7 @@ METRIC = "gate+SVD updates/s @chi=256 FP64 vs mixed; trajectories/s + p99 latenc
 # rocprofv3 FETCH_SIZE and WRITE_SIZE (KiB per dispatch, one counter per run)
 # scaled by what the same counters read for a plain 1 GiB device copy
 # (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0); profiles/jacobi_single_traj_chi256_hbm_traffic.md.
-PMC_KIB = {"k_jacobi_fused_r<4, 2>": (2723.5, 621.6)}
+PMC_KIB = {"k_jacobi_fused_r<4, 1>": (1160.3, 621.6)}
 PMC_SCALE = (2.0, 1.0)
 TRAFFIC_BYTES = {k: 1024.0 * (f * PMC_SCALE[0] + w * PMC_SCALE[1]) for k, (f, w) in PMC_KIB.items()}
 TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of the Jacobi kernels on this workload, "
@@ -681,13 +681,13 @@ def run_single(args, pool, workers: int, host: dict) -> dict:
     achieved = perf["svd_flops"] / svd_s / 1e12 if svd_s > 0 else 0.0
     roof = {
         "bound": "mfma",
-        "kernel": "SVD: QR preconditioner (k_qr_*) + Jacobi steps (k_jacobi_fused_r<4, 2> "
+        "kernel": "SVD: QR preconditioner (k_qr_*) + Jacobi steps (k_jacobi_fused_r<4, 1> "
                   "row-split, k_jacobi_step<8> gate batches)",
         "achieved": round(achieved, 4),
         "peak": FP64_PEAK_TFLOPS,
         "unit": "TFLOP/s",
         "frac": round(achieved / FP64_PEAK_TFLOPS, 5),
-        "traffic": round(TRAFFIC_BYTES["k_jacobi_fused_r<4, 2>"]),
+        "traffic": round(TRAFFIC_BYTES["k_jacobi_fused_r<4, 1>"]),
         "traffic_source": TRAFFIC_SOURCE,
         "jacobi_only_TFLOPs": round(perf["svd_flops"] / (perf["ms_jacobi"] / 1e3) / 1e12, 4)
         if perf["ms_jacobi"] > 0 else None,
