@@ -302,6 +302,10 @@ struct tn_engine {
   // least rows per slice of the one-launch step, whose split doubles past
   // split_rows while its slices keep at least this many (TNMPS_FUSED_ROWS)
   int fused_rows = 32;
+  // the 16 x 16 panel solves of FP64 row-split launches rotate the pairs
+  // inside each 8-column block on every inner_every-th step of a sweep only,
+  // and on at least four steps of every sweep (TNMPS_INNER_EVERY; 1: every step)
+  int inner_every = 8;
   int qr_min = 32;       // least C for which a batch takes the QR path (TNMPS_QR_MIN)
   int cus = 0;           // compute units of the device
   int list_min = 64;     // tasks from which the step kernel runs over act (TNMPS_LIST_MIN)
@@ -440,6 +444,7 @@ struct tn_engine {
     split_rows = std::max(64, (int)env_double("TNMPS_SPLIT_ROWS", 128));
     split_fused = env_double("TNMPS_SPLIT_FUSED", 1) != 0;
     fused_rows = std::max(16, (int)env_double("TNMPS_FUSED_ROWS", 32));
+    inner_every = std::max(1, (int)env_double("TNMPS_INNER_EVERY", 8));
     list_min = std::max(1, (int)env_double("TNMPS_LIST_MIN", 64));
     pool_min = std::max(0, (int)env_double("TNMPS_POOL_MIN", 0));
     size_t gp = 0;
@@ -884,6 +889,15 @@ tn_status jacobi_sweeps(tn_engine* e, const Workspace& w, int nt, size_t maxR, s
   ja.Rh = (int)maxR;
   ja.Ch = (int)maxC;
   ja.tol_rot = tol.tol_rot;
+  // Row-split launches (a lone SVD or a few: steps bound by the panel
+  // solve's latency) at the FP64 tolerance rotate the pairs inside each
+  // 8-column block on every inner_every-th step only.  Other launches keep
+  // every step: one workgroup per pair over many tasks hides the solve behind
+  // other workgroups' traffic, where more sweeps would cost more, and the
+  // looser FP32 / BF16-split tolerances end with a residual near conv_tol^2
+  // that the sparser schedule did not keep (it moved the FP32 path's entropy
+  // error against FP64 over four N=64, chi=256 layers past 1e-6).
+  ja.inner_every = (S > 1 && conv_tol <= tol.conv_tol) ? e->inner_every : 1;
   ja.conv_tol = conv_tol;
   ja.tiny2 = tol.tiny_norm * tol.tiny_norm;
   // A row split of few tasks takes one launch per step when the workspace
@@ -1578,6 +1592,7 @@ tn_status pool_sweep(tn_engine* e, const Workspace& w, const std::vector<PoolSlo
   ja.Rh = (int)maxRJ;
   ja.Ch = (int)maxC;
   ja.tol_rot = e->tol.tol_rot;
+  ja.inner_every = 1;  // one workgroup per pair (see jacobi_sweeps)
   ja.conv_tol = 0.0;  // (sweep index 0: no task counts as converged in the kernels)
   ja.tiny2 = e->tol.tiny_norm * e->tol.tiny_norm;
   PerfRec* pr = nullptr;

@@ -106,6 +106,10 @@ struct JacobiArgs {
   int nbe;               // round-robin blocks (even) of the widest task
   int S;                 // workgroups per block pair, a power of two (row split; 1 otherwise)
   int Rh, Ch;            // bounds on every task's R and C
+  int inner_every;       // 16 x 16 panel solves rotate the pairs inside each 8-column
+                         // block on steps that are multiples of this, capped at a
+                         // quarter of the sweep's steps (the pairs across the two
+                         // blocks on every step)
   double tol_rot;        // rotate a pair when |g_ij| > tol_rot sqrt(g_ii g_jj)
   double conv_tol;       // converged after a sweep whose max ratio <= conv_tol
   double tiny2;          // columns with squared norm below take no part

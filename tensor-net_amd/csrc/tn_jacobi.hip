@@ -467,9 +467,12 @@ __device__ __forceinline__ void w_rounds16(const Solve16& sv) {
 
 // Wave 0's part: the largest relative off-diagonal of G[0] (recorded into
 // *rec when not null and positive), then the rounds unless it is at most
-// tol_rot.  Returns whether there is a rotation to apply.
+// tol_rot: rounds 1 .. 15 when `inner`, else rounds 8 .. 15 only (the pairs
+// across the two 8-column blocks; rounds 1 .. 7 pair columns inside a
+// block).  Returns whether there is a rotation to apply.
 template <bool HOIST>
-__device__ bool solve16_wave0(const Solve16& sv, double* rec, double tol_rot, double tiny2) {
+__device__ bool solve16_wave0(const Solve16& sv, double* rec, double tol_rot, double tiny2,
+                              bool inner) {
   const int lane = threadIdx.x & 63;
   double r2 = 0.0;
   // the four ratios straight-line (every division issued, masked after) and
@@ -490,13 +493,17 @@ __device__ bool solve16_wave0(const Solve16& sv, double* rec, double tol_rot, do
     __hip_atomic_store(sv.rr, kRoundsNone, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     return false;
   }
-  g_rounds16<1, 0, HOIST>(sv, tol_rot * tol_rot, tiny2);
+  if (inner)
+    g_rounds16<1, 0, HOIST>(sv, tol_rot * tol_rot, tiny2);
+  else
+    g_rounds16<8, 0, HOIST>(sv, tol_rot * tol_rot, tiny2);
   return true;
 }
 
-// Wave 1's part: W from the identity through the posted rounds.
+// Wave 1's part: W from the identity through the posted rounds (from round
+// 1 when `inner`, else from round 8, as wave 0).
 template <bool HOIST>
-__device__ void solve16_wave1(const Solve16& sv) {
+__device__ void solve16_wave1(const Solve16& sv, bool inner) {
   const int lane = threadIdx.x & 63;
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
@@ -504,10 +511,13 @@ __device__ void solve16_wave1(const Solve16& sv) {
     sv.W[e >> 4][e & 15] = make_double2((e >> 4) == (e & 15) ? 1.0 : 0.0, 0.0);
   }
   wave_sync();
-  wait_round(sv.rr, 1);
+  wait_round(sv.rr, inner ? 1 : 8);
   if (__hip_atomic_load(sv.rr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == kRoundsNone)
     return;
-  w_rounds16<1, HOIST>(sv);
+  if (inner)
+    w_rounds16<1, HOIST>(sv);
+  else
+    w_rounds16<8, HOIST>(sv);
 }
 
 // From the panel Gram matrix in sm.Gs[0]: record the largest relative
@@ -520,10 +530,11 @@ __device__ void solve16_wave1(const Solve16& sv) {
 // needs from the round's Gram buffer, one barrier per round.  Returns false
 // (for the whole block) when there is nothing to apply.  A function of
 // sm.Gs[0] alone: blocks holding the same G compute bitwise the same
-// rotation.
+// rotation.  `inner` false leaves out the 16 x 16 solve's rounds inside each
+// 8-column block (see solve16_wave0).
 template <int BW, bool HOIST = false, bool RR_READY = false>
 __device__ bool panel_solve(double* rec, double tol_rot, double tiny2, PairSmem<BW>& sm,
-                            int& wsel) {
+                            int& wsel, bool inner = true) {
   constexpr int PW = 2 * BW;
   const int tid = threadIdx.x;
   if constexpr (BW == 8) {
@@ -539,10 +550,10 @@ __device__ bool panel_solve(double* rec, double tol_rot, double tiny2, PairSmem<
     }
     const int wid = tid >> 6;
     if (wid == 0) {
-      const bool go = solve16_wave0<HOIST>(sv, rec, tol_rot, tiny2);
+      const bool go = solve16_wave0<HOIST>(sv, rec, tol_rot, tiny2, inner);
       if (tid == 0) sm.go = go ? 1 : 0;
     } else if (wid == 1) {
-      solve16_wave1<HOIST>(sv);
+      solve16_wave1<HOIST>(sv, inner);
     }
     __syncthreads();
     wsel = 0;
@@ -931,6 +942,17 @@ __device__ __forceinline__ void split_block(int S, int& p, int& q) {
 // stayed at or below conv_tol.  The load is unconditional (row 0 at sweep
 // 0, where it is not used), so no branch separates it from the kernel's
 // other entry loads.
+// Whether step's 16 x 16 panel solves rotate the pairs inside each 8-column
+// block too (the pairs across the two blocks on every step): every k-th step
+// of a sweep, k = inner_every but at most a quarter of the sweep's nbe - 1
+// steps, so every sweep keeps at least four such steps.  Every block takes
+// part in every step, so a block's own columns are rotated against each
+// other every k steps.
+__device__ __forceinline__ bool inner_step(const JacobiArgs& a, int step) {
+  const int k = min(a.inner_every, max(1, (a.nbe - 1) >> 2));
+  return step % k == 0;
+}
+
 __device__ __forceinline__ bool converged(const JacobiArgs& a, int t, int sweep) {
   const double o = a.offs[(size_t)(sweep > 0 ? sweep - 1 : 0) * a.nt + t];
   return sweep > 0 && o <= a.conv_tol;
@@ -962,7 +984,8 @@ __device__ __forceinline__ void step_task(const JacobiArgs& a, int t, int p, int
   load_chunk<BW>(X, (size_t)a.ldx, 0, R, C, lcol, cx);
   load_chunk<BW>(W, (size_t)a.ldw, 0, C, C, lcol, cw);
   int wsel = 0;
-  if (!panel_solve<BW>(a.offs + (size_t)sweep * a.nt + t, a.tol_rot, a.tiny2, sm, wsel))
+  if (!panel_solve<BW>(a.offs + (size_t)sweep * a.nt + t, a.tol_rot, a.tiny2, sm, wsel,
+                       inner_step(a, step)))
     return;
   panel_apply<BW>(X, (size_t)a.ldx, 0, R, C, lcol, ocol, cx, sm, wsel);
   panel_apply<BW>(W, (size_t)a.ldw, 0, C, C, lcol, ocol, cw, sm, wsel);
@@ -1087,7 +1110,7 @@ __global__ __launch_bounds__(256, BW == 8 ? 2 : 1) void k_jacobi_apply(JacobiArg
   mask_chunk<BW>(wa, we, C, lcol, cw);
   int wsel = 0;
   if (!panel_solve<BW>(q == 0 ? a.offs + (size_t)sweep * a.nt + t : nullptr, a.tol_rot,
-                       a.tiny2, sm, wsel))
+                       a.tiny2, sm, wsel, inner_step(a, step)))
     return;
   panel_apply<BW>(X, (size_t)a.ldx, xa, xe, C, lcol, ocol, cx, sm, wsel);
   panel_apply<BW>(W, (size_t)a.ldw, wa, we, C, lcol, ocol, cw, sm, wsel);
@@ -1120,7 +1143,9 @@ __global__ __launch_bounds__(256, 2) void k_jacobi_step_r(JacobiArgs a, int swee
   Chunk<8> cw;
   load_chunk<8>(W, (size_t)a.ldw, 0, C, C, lcol, cw);
   int wsel = 0;
-  if (!panel_solve<8>(a.offs + (size_t)sweep * a.nt + t, a.tol_rot, a.tiny2, sm, wsel)) return;
+  if (!panel_solve<8>(a.offs + (size_t)sweep * a.nt + t, a.tol_rot, a.tiny2, sm, wsel,
+                      inner_step(a, step)))
+    return;
   panel_apply8r<NCX>(X, (size_t)a.ldx, 0, R, C, ocol[0], px, sm, wsel);
   panel_apply<8>(W, (size_t)a.ldw, 0, C, C, lcol, ocol, cw, sm, wsel);
 }
@@ -1194,7 +1219,7 @@ __global__ __launch_bounds__(256, 2) void k_jacobi_apply_r(JacobiArgs a, int swe
   mask_panel8<NC>(wa, we, C, lcol, pw);
   int wsel = 0;
   if (!panel_solve<8, true>(q == 0 ? a.offs + (size_t)sweep * a.nt + t : nullptr, a.tol_rot,
-                            a.tiny2, sm, wsel))
+                            a.tiny2, sm, wsel, inner_step(a, step)))
     return;
   panel_apply8r<NC>(X, (size_t)a.ldx, xa, xe, C, ocol[0], px, sm, wsel);
   panel_apply8r<NC>(W, (size_t)a.ldw, wa, we, C, ocol[0], pw, sm, wsel);
@@ -1255,7 +1280,7 @@ __global__ __launch_bounds__(256, 1) void k_jacobi_fused_r(JacobiArgs a, int swe
   panel_gram8_lds<NCG>(R, pl.v, sm);  // ends with a barrier after its last use of sm.buf
   int wsel = 0;
   if (panel_solve<8, true, true>(q == 0 ? a.offs + (size_t)sweep * a.nt + t : nullptr,
-                                 a.tol_rot, a.tiny2, sm, wsel)) {
+                                 a.tol_rot, a.tiny2, sm, wsel, inner_step(a, step))) {
     double2 wk[4];
     double wks[4];
     const int lane = threadIdx.x & 63;

@@ -9,7 +9,7 @@ mps.rs:236-303) to the parity tolerances of the other GPU tests."""
 import numpy as np
 import pytest
 
-from helpers import assert_spectra_equiv, assert_spectrum_close
+from helpers import assert_spectra_equiv
 from oracle.mps_oracle import run_trajectory
 from tnmps import MPS, Engine
 from tnmps.workload import haar_circuit
@@ -42,9 +42,11 @@ def test_fused_matches_two_launch_and_oracle(gpu, monkeypatch, n, depth, p, chi,
     b, gb = _solo(n, chi, c, ref["record"], False, monkeypatch)
     # the one-launch steps ran: about the same sweeps in fewer launches
     assert a.perf()["jacobi_launches"] < 0.8 * b.perf()["jacobi_launches"]
-    for bd in range(n - 1):
-        assert_spectrum_close(a.schmidt(bd), b.schmidt(bd), rtol=1e-12,
-                              msg=f"one launch vs two, bond {bd}")
+    # the two forms sum the panel Gram matrices in different orders, so they
+    # agree to round-off (a singular value at the rank rule's EPSILON
+    # threshold may be kept by one and dropped by the other)
+    assert_spectra_equiv([a.schmidt(bd) for bd in range(n - 1)],
+                         [b.schmidt(bd) for bd in range(n - 1)], rtol=1e-12)
     assert_spectra_equiv([a.schmidt(bd) for bd in range(n - 1)],
                          [ref["state"].L[bd] for bd in range(n - 1)])
     for x, y, z in zip(ga["entropy"], gb["entropy"], ref["entropy"]):
@@ -71,8 +73,9 @@ def test_fused_batches_match_solo(gpu, monkeypatch, count):
     for i in range(count):
         solo = MPS.new_qubits(n, ("const", chi))
         solo.run_circuit(circuits[i].layers, forced=refs[i]["record"])
-        for bd in range(n - 1):
-            assert_spectrum_close(hs[i].schmidt(bd), solo.schmidt(bd), rtol=1e-12,
-                                  msg=f"handle {i} bond {bd}")
+        # (a batch of more than 4 takes the two-launch form, its solo run the
+        # one-launch form: round-off apart, as above)
+        assert_spectra_equiv([hs[i].schmidt(bd) for bd in range(n - 1)],
+                             [solo.schmidt(bd) for bd in range(n - 1)], rtol=1e-12)
         assert_spectra_equiv([hs[i].schmidt(bd) for bd in range(n - 1)],
                              [refs[i]["state"].L[bd] for bd in range(n - 1)])
