@@ -302,8 +302,8 @@ struct tn_engine {
   // least rows per slice of the one-launch step, whose split doubles past
   // split_rows while its slices keep at least this many (TNMPS_FUSED_ROWS)
   int fused_rows = 32;
-  // the 16 x 16 panel solves of FP64 row-split launches rotate the pairs
-  // inside each 8-column block on every inner_every-th step of a sweep only,
+  // the 16 x 16 panel solves of FP64 one-launch row-split steps rotate the
+  // pairs inside each 8-column block on every inner_every-th step only,
   // and on at least four steps of every sweep (TNMPS_INNER_EVERY; 1: every step)
   int inner_every = 8;
   int qr_min = 32;       // least C for which a batch takes the QR path (TNMPS_QR_MIN)
@@ -889,15 +889,7 @@ tn_status jacobi_sweeps(tn_engine* e, const Workspace& w, int nt, size_t maxR, s
   ja.Rh = (int)maxR;
   ja.Ch = (int)maxC;
   ja.tol_rot = tol.tol_rot;
-  // Row-split launches (a lone SVD or a few: steps bound by the panel
-  // solve's latency) at the FP64 tolerance rotate the pairs inside each
-  // 8-column block on every inner_every-th step only.  Other launches keep
-  // every step: one workgroup per pair over many tasks hides the solve behind
-  // other workgroups' traffic, where more sweeps would cost more, and the
-  // looser FP32 / BF16-split tolerances end with a residual near conv_tol^2
-  // that the sparser schedule did not keep (it moved the FP32 path's entropy
-  // error against FP64 over four N=64, chi=256 layers past 1e-6).
-  ja.inner_every = (S > 1 && conv_tol <= tol.conv_tol) ? e->inner_every : 1;
+  ja.inner_every = 1;
   ja.conv_tol = conv_tol;
   ja.tiny2 = tol.tiny_norm * tol.tiny_norm;
   // A row split of few tasks takes one launch per step when the workspace
@@ -905,6 +897,18 @@ tn_status jacobi_sweeps(tn_engine* e, const Workspace& w, int nt, size_t maxR, s
   const bool fused = S > 1 && w.Xb && nt <= w.xb_slots && jacobi_fused_fits(ja, bw);
   if (fused) {
     ja.Xb = w.Xb;
+    // One-launch row-split steps (a lone SVD or a few, panels of up to 256
+    // rows: steps bound by the panel solve's latency) at the FP64 tolerance
+    // rotate the pairs inside each 8-column block on every inner_every-th
+    // step only.  Other launches keep every step: one workgroup per pair over
+    // many tasks hides the solve behind other workgroups' traffic, and more
+    // sweeps cost more there; the two-launch steps of 512-row panels lost
+    // more to extra sweeps than they saved (FP64 N=48, chi=512: 1,020 -> 899
+    // updates/s); and the looser FP32 / BF16-split tolerances end with a
+    // residual near conv_tol^2 that the sparser schedule did not keep (it
+    // moved the FP32 path's entropy error against FP64 over four N=64,
+    // chi=256 layers past 1e-6).
+    if (conv_tol <= tol.conv_tol) ja.inner_every = e->inner_every;
     // its rotation spreads 16-row tiles over the waves, so shorter slices
     // shorten it: split further while slices keep fused_rows rows and the
     // launch keeps one workgroup per CU (its LDS holds one per CU)
