@@ -93,7 +93,10 @@ typedef struct {
     int bw;          /* 8 or 16: Jacobi block width (columns) */
     int split_max;   /* 1..8: most workgroups per block pair (row-split steps) */
     int split_rows;  /* >= 64: the row split S doubles while the rows are at
-                        least split_rows x S (slices of split_rows / 2 rows or more) */
+                        least split_rows x S (slices of split_rows / 2 rows or more);
+                        a one-launch row-split step (a lone SVD or up to 4, at most
+                        256 rows) doubles it further, up to split_max, while its
+                        slices keep 32 rows and its launch one workgroup per CU */
     int ahead;       /* 1..3: sweeps enqueued ahead while far from convergence */
     int qr;          /* 0 or 1: precondition the SVD with a column-sorted
                         Householder QR and run the Jacobi sweeps on R^H
