@@ -44,6 +44,7 @@ enum {
   kInfoM = 0, kInfoN, kInfoR, kInfoC, kInfoOrient, kInfoRank, kInfoFlags,
   kInfoSweeps, kInfoChiL, kInfoChiM, kInfoChiR, kInfoMode,
   kInfoRJ,  // rows of the Jacobi problem: R, or C after the QR preconditioner
+  kInfoNorms,  // 1 when X's column norms are already in sraw (k_qr_final)
   kInfoLen = 16
 };
 

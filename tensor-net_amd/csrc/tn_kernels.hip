@@ -336,6 +336,7 @@ __global__ __launch_bounds__(256) void k_theta(const UpdDesc* __restrict__ descs
       d.info[kInfoN] = ts.n;
       d.info[kInfoR] = ts.R;
       d.info[kInfoRJ] = ts.R;
+      d.info[kInfoNorms] = 0;
       d.info[kInfoC] = ts.C;
       d.info[kInfoOrient] = orient;
       d.info[kInfoMode] = ts.mode;
@@ -408,11 +409,15 @@ __global__ __launch_bounds__(1024) void k_finish(const UpdDesc* __restrict__ des
 
   if (tid == 0) d.info[kInfoSweeps] = sweeps_run;
 
-  for (int j = wid; j < C; j += 16) {
-    double s = 0.0;
-    for (int r = lane; r < R; r += 64) s += cabs2(d.X[(size_t)j * d.ldx + r]);
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-    if (lane == 0) d.sraw[j] = sqrt(s);
+  // the column norms, unless the QR preconditioner's last kernel has formed
+  // them by this arithmetic (bitwise the same) while X was in its cache
+  if (d.info[kInfoNorms] == 0) {
+    for (int j = wid; j < C; j += 16) {
+      double s = 0.0;
+      for (int r = lane; r < R; r += 64) s += cabs2(d.X[(size_t)j * d.ldx + r]);
+      for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+      if (lane == 0) d.sraw[j] = sqrt(s);
+    }
   }
   __syncthreads();
   __threadfence_block();

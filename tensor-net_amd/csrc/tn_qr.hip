@@ -659,6 +659,23 @@ __global__ __launch_bounds__(512) void k_qr_final(QrArgs a) {
       }
     }
   }
+  // The column norms of X for the singular values (k_finish), while this
+  // block's columns are in its cache, with k_finish's arithmetic: wave w
+  // sums rows lane, lane + 64, ... of columns j0 + w and j0 + w + 8 in that
+  // order, then the same lane tree, so the norms are bitwise those k_finish
+  // would form (it skips its pass for the task).  sraw shares cn's slots.
+  __syncthreads();  // every column of X above written
+  for (int h = 0; h < 2; ++h) {
+    const int j = j0 + wid + (kQrNb / 2) * h;
+    if (j < q.C) {
+      double s = 0.0;
+      for (int r = lane; r < q.R; r += 64) s += cabs2(q.X[(size_t)j * a.ldx + r]);
+      for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+      if (lane == 0) q.cn[j] = sqrt(s);
+    }
+  }
+  if (blockIdx.x == 0 && tid == 0)
+    a.info[(size_t)qr_slot(a, blockIdx.y) * kInfoLen + kInfoNorms] = 1;
 }
 
 // ------------------------------------------------------------- launch
