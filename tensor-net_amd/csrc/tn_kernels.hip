@@ -497,27 +497,33 @@ __global__ __launch_bounds__(1024) void k_finish(const UpdDesc* __restrict__ des
 //           = Gamma[k] of lamL Gamma[k] U_M (already divided by lamL)
 //   mode 2: T[j][c]   = sum_u conj(W[u, perm_j]) Gk1[u, c]     (c < n)
 //           = rows of V_A^H Gamma[k+1] (already divided by lamR)
-// Tile: 64 values of the free index x 32 values of j per 256-thread block,
-// u streamed through LDS 16 at a time.  Grid (x tiles, j tiles, task).
+// Tile: 64 values of the free index x 4 JPT values of j per 256-thread
+// block (JPT per thread), u streamed through LDS 16 at a time.  Grid (x
+// tiles, j tiles, task).  Every output sums over u in the same order for
+// any JPT: launches of few tasks take JPT = 2 (four times the workgroups of
+// JPT = 8 for a lone update's product), batches JPT = 8 (a quarter of the
+// reloads of the factor).
+template <int JPT>
 __global__ __launch_bounds__(256) void k_fmul(const UpdDesc* __restrict__ descs) {
+  constexpr int JT = 4 * JPT;  // values of j per block
   const UpdDesc& d = descs[blockIdx.z];
   const int mode = d.info[kInfoMode];
   if (mode == 0) return;
   const int K = d.info[kInfoChiM];
   const int rank = d.info[kInfoRank];
   const int nx = (mode == 1) ? d.info[kInfoM] : d.info[kInfoN];
-  const int x0 = blockIdx.x * 64, j0 = blockIdx.y * 32;
+  const int x0 = blockIdx.x * 64, j0 = blockIdx.y * JT;
   if (x0 >= nx || j0 >= rank) return;
-  __shared__ double2 Ps[16][65];  // [u][x]
-  __shared__ double2 Ws[16][33];  // [u][j]
-  __shared__ int pj[32];
+  __shared__ double2 Ps[16][65];      // [u][x]
+  __shared__ double2 Ws[16][JT + 1];  // [u][j]
+  __shared__ int pj[JT];
   const int tid = threadIdx.x;
-  if (tid < 32) pj[tid] = (j0 + tid < rank) ? d.perm[j0 + tid] : -1;
+  if (tid < JT) pj[tid] = (j0 + tid < rank) ? d.perm[j0 + tid] : -1;
   __syncthreads();
   const int tx = tid & 63, ty = tid >> 6;
-  double2 acc[8];
+  double2 acc[JPT];
 #pragma unroll
-  for (int k = 0; k < 8; ++k) acc[k] = make_double2(0.0, 0.0);
+  for (int k = 0; k < JPT; ++k) acc[k] = make_double2(0.0, 0.0);
   const double2* __restrict__ G = (mode == 1) ? d.gk : d.gk1;
   for (int u0 = 0; u0 < K; u0 += 16) {
 #pragma unroll
@@ -537,28 +543,30 @@ __global__ __launch_bounds__(256) void k_fmul(const UpdDesc* __restrict__ descs)
       Ps[uu][xx] = v;
     }
 #pragma unroll
-    for (int q = 0; q < 2; ++q) {
+    for (int q = 0; q < (16 * JT + 255) / 256; ++q) {
       const int e = tid + 256 * q;
-      const int uu = e & 15, jj = e >> 4;
-      const int u = u0 + uu, c = pj[jj];
-      double2 w = make_double2(0.0, 0.0);
-      if (c >= 0 && u < K) w = d.W[(size_t)c * d.ldw + u];
-      Ws[uu][jj] = (mode == 2) ? cconj(w) : w;
+      if (e < 16 * JT) {
+        const int uu = e & 15, jj = e >> 4;
+        const int u = u0 + uu, c = pj[jj];
+        double2 w = make_double2(0.0, 0.0);
+        if (c >= 0 && u < K) w = d.W[(size_t)c * d.ldw + u];
+        Ws[uu][jj] = (mode == 2) ? cconj(w) : w;
+      }
     }
     __syncthreads();
 #pragma unroll 4
     for (int uu = 0; uu < 16; ++uu) {
       const double2 pv = Ps[uu][tx];
 #pragma unroll
-      for (int k = 0; k < 8; ++k) acc[k] = cadd(acc[k], cmul(pv, Ws[uu][ty * 8 + k]));
+      for (int k = 0; k < JPT; ++k) acc[k] = cadd(acc[k], cmul(pv, Ws[uu][ty * JPT + k]));
     }
     __syncthreads();
   }
   const int x = x0 + tx;
   if (x < nx) {
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      const int j = j0 + ty * 8 + k;
+    for (int k = 0; k < JPT; ++k) {
+      const int j = j0 + ty * JPT + k;
       if (j < rank) d.T[(size_t)j * d.ldx + x] = acc[k];
     }
   }
@@ -963,8 +971,12 @@ void launch_finish(const UpdDesc* descs, int nt, int sweeps_run, hipStream_t s) 
   hipLaunchKernelGGL(k_finish, dim3(nt), dim3(1024), 0, s, descs, nt, sweeps_run);
 }
 void launch_fmul(const UpdDesc* descs, int nt, int maxX, int maxRank, hipStream_t s) {
-  hipLaunchKernelGGL(k_fmul, dim3((maxX + 63) / 64, (maxRank + 31) / 32, nt), dim3(256), 0, s,
-                     descs);
+  if (nt <= 4)
+    hipLaunchKernelGGL(k_fmul<2>, dim3((maxX + 63) / 64, (maxRank + 7) / 8, nt), dim3(256), 0,
+                       s, descs);
+  else
+    hipLaunchKernelGGL(k_fmul<8>, dim3((maxX + 63) / 64, (maxRank + 31) / 32, nt), dim3(256), 0,
+                       s, descs);
 }
 void launch_split(const UpdDesc* descs, int nt, hipStream_t s) {
   hipLaunchKernelGGL(k_split, dim3(kNPart, nt), dim3(256), 0, s, descs);
