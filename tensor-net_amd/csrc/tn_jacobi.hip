@@ -1271,8 +1271,11 @@ __global__ __launch_bounds__(256, 1) void k_jacobi_fused_r(JacobiArgs a, int swe
   const int R = inf[kInfoRJ], C = inf[kInfoC];
   if (converged(a, t, sweep)) return;
   const int xe = min(xb, R), we = min(wb, C);
-  mask_panel8_all<NCG>(0, R, C, lcol, px);
-  mask_panel8_all<NCW>(wa, we, C, lcol, pw);
+  // (the loads took the host's bounds: nothing to mask when the task fills
+  // them and the wave's columns are the task's; uniform over the wave)
+  const bool cols_in = lcol[0] < C && lcol[1] < C && lcol[2] < C && lcol[3] < C;
+  if (!(cols_in && R >= a.Rh)) mask_panel8_all<NCG>(0, R, C, lcol, px);
+  if (!(cols_in && we >= wb)) mask_panel8_all<NCW>(wa, we, C, lcol, pw);
   if (threadIdx.x == 0) sm.rr = 0;  // the solve's round word, ordered by the barrier below
   put_panels8<NCG>(px, pl.v);
   put_panels8<NCW>(pw, pl.v + NCG);
